@@ -1,0 +1,211 @@
+"""Cloth vertex-substeps/s of the batched implicit cloth step on one MI355X.
+
+Workload (`batched`, seed 2): 512 independent cloth grids, sides drawn
+log-uniform in [16, 128] vertices, E ~ U[1e4, 1e6] Pa (BW98 k = E*h,
+h = 1 mm), density ~ U[0.1, 0.4] kg/m^2, two random corners pinned,
+bending 1e-4, dt = 1/600 s, block-Jacobi PCG to relative residual 1e-6
+(cap 100), FP32 with FP64 reductions.  One step = one substep of every
+instance (element pass + gather assembly + one persistent PCG launch).
+
+value        = sum(N_i) * K / wall-seconds of K timed substeps (inputs in HBM)
+plain        = the same instances stepped one ClothSimulator at a time
+roofline     = persistent PCG kernel: algorithmic bytes per launch
+               sum_i [B_setup_i + k_i B_iter_i + 48 N_i] / mean launch time
+               (HIP events on the stream the kernel runs on)
+cpu_baseline = the CPU oracle (a port of the reference step) on a bounded
+               sample of the same instances, all host threads
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "newton-chysx_amd"))
+
+METRIC = "cloth vertex-substeps/s (batched vs plain) at 1e-6 PCG tol; p99 req latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def b_spmv(n, nnz):
+    return 40 * nnz + 64 * n
+
+
+def b_iter(n, nnz):
+    return b_spmv(n, nnz) + 156 * n
+
+
+def b_setup(n, nnz):
+    return b_spmv(n, nnz) + 168 * n
+
+
+def load_traffic():
+    """HBM bytes per PCG launch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(REPO, "profiles", "pcg_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(specs, budget_s=12.0):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import helpers
+    import pyoracle
+
+    threads = pyoracle.num_threads()
+    done_vs, secs, used = 0, 0.0, 0
+    t_all = time.perf_counter()
+    for spec in specs:
+        o = helpers.oracle_from_spec(spec, precision=32)
+        o.step(spec.dt)  # warm-up substep (untimed)
+        s, _ = o.run(spec.dt, 1)
+        secs += s
+        done_vs += spec.n
+        used += 1
+        if secs >= budget_s or time.perf_counter() - t_all > 3 * budget_s:
+            break
+    return {
+        "value": done_vs / secs if secs > 0 else None,
+        "unit": "vertex-substeps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{used} of {len(specs)} batched instances (seeded order), 1 timed substep each after 1 warm-up, "
+                  f"FP32 oracle, {secs:.1f} s of timed CPU work",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import chysx
+    from chysx import scenes
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    specs = scenes.batched(seed=2, count=args.instances)
+    t0 = time.perf_counter()
+    cloths = [scenes.DeviceCloth(s, lanes_per_row=args.lanes) for s in specs]
+    batch = chysx.ClothBatch()
+    batch.set_lanes_per_row(args.lanes)
+    for c in cloths:
+        batch.add_instance(c.sim)
+    batch.finalize()
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    sizes = np.array(batch.instance_sizes(), np.int64)
+    n_rows = int(sizes[:, 0].sum())
+    dt = specs[0].dt
+
+    with torch.cuda.stream(stream):
+        for _ in range(args.warmup):
+            batch.step(dt, sh)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.step(dt, sh)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    iters = np.array(batch.last_pcg_iterations(sh))
+    res = np.array(batch.last_relative_residuals(sh))
+    value = n_rows * args.steps / wall
+
+    # PCG kernel alone: assemble, then time the solve launch with events on `stream`.
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    pcg_ms, pcg_bytes = [], []
+    with torch.cuda.stream(stream):
+        for _ in range(max(3, min(args.steps, 10))):
+            batch.assemble(dt, sh)
+            ev0.record(stream)
+            batch.solve(sh)
+            ev1.record(stream)
+            ev1.synchronize()
+            pcg_ms.append(ev0.elapsed_time(ev1))
+            k = np.array(batch.last_pcg_iterations(sh), np.int64)
+            n, nnz = sizes[:, 0], sizes[:, 1]
+            pcg_bytes.append(float(np.sum(b_setup(n, nnz) + k * b_iter(n, nnz) + 48 * n)))
+    mean_ms = float(np.mean(pcg_ms))
+    achieved = float(np.mean(pcg_bytes)) / (mean_ms * 1e-3) / 1e9
+    traffic = load_traffic()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                "kernel": "pcg_kernel<%d> (persistent block-Jacobi PCG + finalize)" % args.lanes,
+                "mean_launch_ms": round(mean_ms, 3), "algorithmic_bytes_per_launch": float(np.mean(pcg_bytes))}
+
+    plain = None
+    if not args.no_plain:
+        # The same instances, one ClothSimulator.step at a time (each packs
+        # its own single-instance system on first use, untimed).
+        with torch.cuda.stream(stream):
+            for c in cloths:
+                c.sim.step(dt, sh)
+            torch.cuda.synchronize()
+            reps = 2
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                for c in cloths:
+                    c.sim.step(dt, sh)
+            torch.cuda.synchronize()
+            pw = time.perf_counter() - t0
+        plain = {"value": n_rows * reps / pw, "unit": "vertex-substeps/s", "ms_per_substep_all": pw / reps * 1e3,
+                 "method": "sequential ClothSimulator.step over the same instances, 2 timed substeps"}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(specs, args.cpu_budget)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "vertex-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched: 512 cloth grids, side logU[16,128] vertices (seed 2), BW98 + bending, "
+                               "PCG tol 1e-6 cap 100, one fused substep per step",
+                   "instances": len(specs), "vertices": n_rows, "off_diag_blocks": int(sizes[:, 1].sum()),
+                   "pcg_tol": 1e-6, "pcg_cap": 100, "lanes_per_row": args.lanes, "parallelism": "single GPU"},
+        "pcg_iterations": {"mean": float(iters.mean()), "max": int(iters.max()), "min": int(iters.min())},
+        "max_relative_residual": float(res.max()),
+        "plain": plain,
+        "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "setup_s": round(setup_s, 2),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

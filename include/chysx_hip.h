@@ -1,0 +1,223 @@
+/*
+ * chysx_hip.h — C ABI of the MI355X (gfx950) implementation of ChysX's
+ * implicit cloth time-step.
+ *
+ * The C++ host layer (ClothSimulator, BlockCSR3, PCGSolver, ClothBatch) and
+ * any foreign-language binding call the device only through these entry
+ * points.  Signatures use plain pointers, sizes and integer stream handles
+ * (a hipStream_t cast to uintptr_t, 0 = the default stream); no C++ or
+ * framework types cross this boundary.
+ *
+ * Every entry point returns CHX_OK (0) or an error code; the message of the
+ * most recent error on the calling thread is chx_last_error_message().  The
+ * code maps onto the reference's exception types:
+ *   CHX_ERR_INVALID_ARGUMENT -> std::invalid_argument (Python ValueError)
+ *   CHX_ERR_OUT_OF_RANGE     -> std::out_of_range     (Python IndexError)
+ *   CHX_ERR_RUNTIME / DEVICE -> std::runtime_error    (Python RuntimeError)
+ * (ChysX/src/cloth/cloth_simulator.cu:46-51, ChysX/src/sparse/block_csr.cu:197-201).
+ *
+ * Memory layout shared with the kernels:
+ *   positions / velocities / vectors : AoS float3, 12 bytes per vertex
+ *   3x3 blocks                        : 9 floats, row-major, 36 bytes
+ * which is the reference's Vec3f / Mat3f layout (ChysX/src/math/vec.cuh,
+ * ChysX/src/math/matrix.cuh:130-133), so caller buffers are used in place.
+ */
+#ifndef CHYSX_HIP_H_
+#define CHYSX_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CHX_OK = 0,
+    CHX_ERR_INVALID_ARGUMENT = 1,
+    CHX_ERR_OUT_OF_RANGE = 2,
+    CHX_ERR_RUNTIME = 3,
+    CHX_ERR_DEVICE = 4
+};
+
+/* Contribution-list entry encoding (gather map of the deterministic
+ * assembly): bits 31..30 element kind, 29..28 local row a, 27..26 local
+ * column b, 25..0 element index in the packed element array. */
+#define CHX_CONTRIB_PIN 0u
+#define CHX_CONTRIB_TRI 1u
+#define CHX_CONTRIB_DIH 2u
+#define CHX_CONTRIB_MAX_ELEMENT ((1u << 26) - 1u)
+
+/* Per-instance description (device array, one entry per cloth). */
+typedef struct chx_instance {
+    float* pos;             /* external AoS float3 positions (x_n on entry, x_{n+1} on exit) */
+    float* vel;             /* external AoS float3 velocities                                 */
+    const float* inv_mass;  /* external per-vertex inverse mass, or NULL (mass 1 kg)           */
+    int row_begin;          /* first packed row (vertex) of this instance                      */
+    int n_rows;             /* vertex count                                                     */
+    int tile_begin;         /* first row tile of this instance                                  */
+    int tile_end;           /* one past its last row tile                                       */
+    float dt;               /* substep                                                          */
+    float damping;          /* v *= exp(-damping*dt) when > 0                                   */
+    float gx, gy, gz;       /* gravity                                                          */
+    int max_iter;           /* PCG iteration cap                                                */
+    int pad0;
+    double tol;             /* ||r|| <= tol*||b|| stops the solve; <= 0: run max_iter (reference) */
+} chx_instance;
+
+/* Packed element records (device arrays). Vertex indices are local to the
+ * element's instance. */
+typedef struct chx_pin { int v; int inst; float tx, ty, tz; float k; } chx_pin;
+typedef struct chx_tri { int v0, v1, v2, inst; float dm00, dm01, dm10, dm11; float kA; } chx_tri;
+typedef struct chx_dih { int v0, v1, v2, v3; int inst; float rest; float k; int pad; } chx_dih;
+
+/* Record sizes (floats) written per element by the element pass. */
+#define CHX_PIN_REC 4
+#define CHX_TRI_REC 24
+#define CHX_DIH_REC 16
+
+/* Number of float64 partial sums kept per row tile by the PCG kernel. */
+#define CHX_TILE_PARTIALS 8
+
+/* One packed batch of cloth instances: a block-diagonal system whose
+ * per-instance blocks have exactly the reference's BSR pattern
+ * (ChysX/src/sparse/block_csr.h:59-135).  All pointers are device pointers. */
+typedef struct chx_cloth_batch {
+    int n_instances;
+    int n_rows;              /* total vertices                                   */
+    int n_tiles;             /* row tiles                                        */
+    int nnz;                 /* total off-diagonal blocks                        */
+    int n_pins, n_tris, n_dihs;
+    int lanes_per_row;       /* SpMV lanes per block row: 4, 8 or 16             */
+    int grid_blocks;         /* persistent PCG workgroups (0 = auto)             */
+    int pad0;
+    chx_instance single;     /* the instance when inst == NULL (n_instances == 1);
+                                travels in the kernel arguments, so a caller that
+                                swaps state buffers every step needs no upload   */
+    const chx_instance* inst;/* device instance table, or NULL                    */
+    const int* row_inst;     /* [n_rows] owning instance (NULL: all instance 0)   */
+    const int* tiles;        /* [n_tiles*4] {instance, row_begin, row_end, 0}     */
+    const chx_pin* pins;
+    const chx_tri* tris;
+    const chx_dih* dihs;
+    float* pin_rec;          /* [n_pins*CHX_PIN_REC]                              */
+    float* tri_rec;          /* [n_tris*CHX_TRI_REC]                              */
+    float* dih_rec;          /* [n_dihs*CHX_DIH_REC]                              */
+    const int* row_offsets;  /* [n_rows+1] packed                                 */
+    const int* col_indices;  /* [nnz] packed (global) column                      */
+    float* diag;             /* [n_rows*9]                                        */
+    float* values;           /* [nnz*9]                                           */
+    const unsigned* contrib_offsets; /* [n_rows+nnz+1] diag slots then off-diag  */
+    const unsigned* contribs;
+    float* rhs;              /* b  [n_rows*3]                                     */
+    float* x;                /* dx, warm start and solution [n_rows*3]            */
+    float* r;
+    float* z;
+    float* p0;
+    float* p1;
+    float* q;
+    float* minv;             /* [n_rows*9] block-Jacobi preconditioner            */
+    double* partials;        /* [n_tiles*CHX_TILE_PARTIALS]                       */
+    int* inst_iters;         /* [n_instances] PCG iterations of the last solve    */
+    double* inst_rr;         /* [n_instances] final ||r||^2                       */
+    double* inst_bb;         /* [n_instances] ||b||^2                             */
+    int* inst_done;          /* [n_instances] scratch                             */
+    unsigned* sync;          /* [8] grid barrier + control words, zeroed once     */
+} chx_cloth_batch;
+
+/* ---- errors and runtime ------------------------------------------------- */
+const char* chx_last_error_message(void);
+/* Number of visible devices; 0 when the HIP runtime finds none. */
+int chx_device_count(int* count);
+/* Compute units and max co-resident PCG workgroups on the current device. */
+int chx_device_info(int* cu_count, int* max_pcg_blocks);
+
+/* Device memory: replaces CudaArray<T> (ChysX/src/memory/cuda_array.h:64-284,
+ * cuda_array.cu:41-90). */
+int chx_malloc(void** ptr, size_t bytes);
+int chx_free(void* ptr);
+/* Host-to-device copy ordered on `stream`; returns once `src` may be reused. */
+int chx_memcpy_h2d(void* dst, const void* src, size_t bytes, uintptr_t stream);
+/* Device-to-host copy; returns after the data is on the host. */
+int chx_memcpy_d2h(void* dst, const void* src, size_t bytes, uintptr_t stream);
+int chx_memcpy_d2d(void* dst, const void* src, size_t bytes, uintptr_t stream);
+/* Zero-fill with a kernel (safe inside stream capture). */
+int chx_zero(void* dst, size_t bytes, uintptr_t stream);
+int chx_stream_synchronize(uintptr_t stream);
+int chx_device_synchronize(void);
+
+/* ---- setup kernels -------------------------------------------------------- */
+/* Rest shape (Dm^-1 rotated by theta, area) of n triangles from current
+ * positions: replaces triangle_rest_shape_kernel
+ * (ChysX/src/constraint/cloth/triangle_stretch_constraint.cu:50-110,319-356).
+ * tris: device int[3n]; pos: device float3; dm_inv: device float[4n]; area: float[n]. */
+int chx_triangle_rest_shape(const int* tris, const float* pos, int n, float cos_theta, float sin_theta,
+                            float* dm_inv, float* area, uintptr_t stream);
+
+/* Rest dihedral angles of n dihedrals (v0,v1,v2,v3): replaces
+ * bending_rest_angle_kernel (ChysX/src/constraint/cloth/bending_constraint.cu:132-148,253-282). */
+int chx_dihedral_rest_angle(const int* dihedrals, const float* pos, int n, float* rest, uintptr_t stream);
+
+/* Area-weighted lumped inverse mass without atomics: each vertex sums
+ * density/3 * area over its incident triangles (vertex_tri_offsets[n+1],
+ * vertex_tris[] in ascending triangle order).  Replaces
+ * scatter_triangle_mass_kernel + mass_to_inv_mass_kernel
+ * (ChysX/src/cloth/cloth_simulator.cu:151-190,516-575). */
+int chx_mass_area_weighted(const int* tris, const float* pos, const int* vertex_tri_offsets,
+                           const int* vertex_tris, int n_vertices, float density_over_3,
+                           float* inv_mass, uintptr_t stream);
+
+/* Elastic energy of the batch's elements at the instances' current positions
+ * (FP64 sum, synchronous).  Replaces Constraint::compute_energy
+ * (pin_constraint.cu:120-147, triangle_stretch_constraint.cu:358-384,
+ * bending_constraint.cu:284-308). kind_mask: 1 pins, 2 triangles, 4 dihedrals. */
+int chx_batch_energy(const chx_cloth_batch* batch, int kind_mask, double* energy, uintptr_t stream);
+
+/* ---- sparse operator and solver -------------------------------------------- */
+/* y = alpha*A*x + beta*y on a block-CSR matrix with separate diagonal
+ * (beta == 0: y is not read).  Replaces sparse::spmv
+ * (ChysX/src/sparse/block_csr.h:143-148, block_csr.cu:54-83,210-243). */
+int chx_spmv_bsr3(int n_rows, const float* diag, const int* row_offsets, const int* col_indices,
+                  const float* values, const float* x, float* y, float alpha, float beta,
+                  int lanes_per_row, uintptr_t stream);
+
+/* Bytes of device workspace chx_pcg_solve_bsr3 needs for n_rows block rows. */
+int chx_pcg_workspace_size(int n_rows, size_t* bytes);
+
+/* Block-Jacobi PCG on a caller-owned block-CSR system, warm-started from x:
+ * replaces PCGSolver::solve (ChysX/src/solver/pcg_solver.h:126-131,
+ * pcg_solver.cu:282-318).  tol <= 0 runs exactly max_iterations (reference
+ * semantics); otherwise stops when ||r||_2 <= tol*||b||_2 (FP64).  The
+ * caller passes `workspace` (device memory of at least
+ * chx_pcg_workspace_size bytes, no initialisation needed).  Iteration count
+ * and FP64 ||r||^2, ||b||^2 are written to the host outputs (any may be
+ * NULL); when all are NULL the call is asynchronous. */
+int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, const int* col_indices,
+                       const float* values, const float* b, float* x, int max_iterations, double tol,
+                       int lanes_per_row, void* workspace, size_t workspace_bytes, int* iterations,
+                       double* rr, double* bb, uintptr_t stream);
+
+/* ---- the implicit cloth substep -------------------------------------------- */
+/* One implicit-Euler substep of every instance of a packed batch:
+ * element force/Hessian pass, deterministic gather assembly into the fixed
+ * BSR pattern, persistent block-Jacobi PCG (warm start in batch->x) and
+ * finalize into the instances' external pos/vel.  Replaces the device work
+ * of ClothSimulator::step (ChysX/src/cloth/cloth_simulator.cu:814-1261).
+ * `batch` is a host pointer to a descriptor of device pointers.
+ * Asynchronous on `stream`. */
+int chx_cloth_step(const chx_cloth_batch* batch, uintptr_t stream);
+
+/* The two halves of chx_cloth_step, for callers that inspect the assembled
+ * system before solving (debug_dump_last_solve) or time the parts. */
+int chx_cloth_assemble(const chx_cloth_batch* batch, uintptr_t stream);
+int chx_cloth_solve(const chx_cloth_batch* batch, int finalize, uintptr_t stream);
+
+/* Reads the device status word of the batch (non-zero after a grid-barrier
+ * timeout inside the persistent solver) and clears it; synchronous. */
+int chx_batch_check(const chx_cloth_batch* batch, uintptr_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHYSX_HIP_H_ */

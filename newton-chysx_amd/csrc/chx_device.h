@@ -1,0 +1,170 @@
+// Device-side helpers shared by the gfx950 kernels: float3 / 3x3 math in
+// the reference's row-major layout, wave64 reductions, and the grid barrier
+// of the persistent PCG kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "chysx_hip.h"
+
+namespace chx {
+
+constexpr int kWave = 64;
+
+// ---- float3 --------------------------------------------------------------
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float a, float b, float c) { return {a, b, c}; }
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 mul(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float len(f3 a) { return sqrtf(dot(a, a)); }
+
+__device__ __forceinline__ f3 ld3(const float* p, long i) {
+    const float* q = p + 3 * i;
+    return {q[0], q[1], q[2]};
+}
+__device__ __forceinline__ void st3(float* p, long i, f3 v) {
+    float* q = p + 3 * i;
+    q[0] = v.x;
+    q[1] = v.y;
+    q[2] = v.z;
+}
+
+// 3x3 row-major block times vector: m[r*3+c].
+__device__ __forceinline__ f3 mv9(const float* m, f3 v) {
+    return {m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+            m[6] * v.x + m[7] * v.y + m[8] * v.z};
+}
+
+// Adjugate / determinant inverse without a singular guard
+// (ChysX/src/math/matrix.cuh:226-241).
+__device__ __forceinline__ void inverse9(const float* m, float* r) {
+    r[0] = m[4] * m[8] - m[5] * m[7];
+    r[1] = -m[1] * m[8] + m[2] * m[7];
+    r[2] = m[1] * m[5] - m[2] * m[4];
+    r[3] = -m[3] * m[8] + m[5] * m[6];
+    r[4] = m[0] * m[8] - m[2] * m[6];
+    r[5] = -m[0] * m[5] + m[2] * m[3];
+    r[6] = m[3] * m[7] - m[4] * m[6];
+    r[7] = -m[0] * m[7] + m[1] * m[6];
+    r[8] = m[0] * m[4] - m[1] * m[3];
+    const float det = m[0] * r[0] + m[1] * r[3] + m[2] * r[6];
+    const float inv = 1.0f / det;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r[i] *= inv;
+}
+
+__device__ __forceinline__ double dot64(f3 a, f3 b) {
+    return double(a.x) * double(b.x) + double(a.y) * double(b.y) + double(a.z) * double(b.z);
+}
+
+// Instance descriptor i of a batch (the inline `single` one when the batch
+// has no device instance table).
+__device__ __forceinline__ const chx_instance& inst_at(const chx_cloth_batch& B, int i) {
+    return B.inst != nullptr ? B.inst[i] : B.single;
+}
+__device__ __forceinline__ int row_instance(const chx_cloth_batch& B, int row) {
+    return B.row_inst != nullptr ? B.row_inst[row] : 0;
+}
+
+// ---- wave64 / workgroup reductions (fixed order => deterministic) ---------
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Sum over a segment of `width` consecutive lanes (width a power of two <= 64).
+template <int W>
+__device__ __forceinline__ float seg_sum(float v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Workgroup sum of up to 4 doubles per thread; result valid in thread 0.
+// `lds` must hold 4 * (blockDim.x / 64) doubles.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double* lds) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[k * 4 + wid] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += lds[k * 4 + w];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// ---- grid barrier ----------------------------------------------------------
+// sync[0] arrival count, sync[1] generation, sync[2] abort flag,
+// sync[3] active-instance counter, sync[4] status (error) word.
+// The barrier is self-resetting: the last arriving workgroup zeroes the
+// count and bumps the generation, so consecutive launches reuse it.  Every
+// wait is bounded: after kSpinLimit polls a workgroup raises the abort flag
+// and all workgroups leave the kernel, so a broken launch cannot hang.
+constexpr unsigned kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ unsigned ld_acquire(unsigned* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_relaxed(unsigned* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks) {
+    __shared__ int s_ok;
+    // Make this workgroup's global writes visible device-wide.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned gen = ld_relaxed(&sync[1]);
+        const unsigned arrived =
+            __hip_atomic_fetch_add(&sync[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(&sync[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&sync[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while (ld_acquire(&sync[1]) == gen) {
+                if (ld_relaxed(&sync[2]) != 0u) {
+                    ok = 0;
+                    break;
+                }
+                if (++spins > kSpinLimit) {
+                    __hip_atomic_store(&sync[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&sync[4], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return s_ok != 0;
+}
+
+}  // namespace chx

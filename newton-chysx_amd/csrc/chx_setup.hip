@@ -1,0 +1,162 @@
+// C ABI: one-time setup kernels (rest shapes, rest angles, lumped mass) and
+// the diagnostic energy evaluation.
+#include <algorithm>
+#include <vector>
+
+#include "chx_device.h"
+#include "chx_elements.h"
+#include "chx_internal.h"
+
+namespace chx {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void rest_shape_kernel(const int* __restrict__ tris, const float* __restrict__ pos,
+                                                            int n, float cos_t, float sin_t, float* __restrict__ dm_inv,
+                                                            float* __restrict__ area) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const f3 a = ld3(pos, tris[3 * t]), b = ld3(pos, tris[3 * t + 1]), c = ld3(pos, tris[3 * t + 2]);
+    float dm[4], ar;
+    tri_rest_shape(a, b, c, cos_t, sin_t, dm, ar);
+    dm_inv[4 * t + 0] = dm[0];
+    dm_inv[4 * t + 1] = dm[1];
+    dm_inv[4 * t + 2] = dm[2];
+    dm_inv[4 * t + 3] = dm[3];
+    area[t] = ar;
+}
+
+__global__ __launch_bounds__(kBlock) void rest_angle_kernel(const int* __restrict__ dih, const float* __restrict__ pos,
+                                                            int n, float* __restrict__ rest) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int* d = dih + 4 * e;
+    const DihFrame F = dihedral_frame(ld3(pos, d[0]), ld3(pos, d[1]), ld3(pos, d[2]), ld3(pos, d[3]));
+    rest[e] = F.valid ? F.theta : 0.0f;
+}
+
+// Vertex-owner lumped mass: m_v = sum over incident triangles (ascending) of
+// density/3 * area(t); inv_mass = m > 1e-12 ? 1/m : 0
+// (cloth_simulator.cu:151-190: same per-triangle term, gathered instead of
+// scattered with atomics).
+__global__ __launch_bounds__(kBlock) void mass_kernel(const int* __restrict__ tris, const float* __restrict__ pos,
+                                                      const int* __restrict__ vt_off, const int* __restrict__ vt,
+                                                      int n, float d3, float* __restrict__ inv_mass) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    float m = 0.0f;
+    for (int k = vt_off[v]; k < vt_off[v + 1]; ++k) {
+        const int t = vt[k];
+        const f3 x0 = ld3(pos, tris[3 * t]), x1 = ld3(pos, tris[3 * t + 1]), x2 = ld3(pos, tris[3 * t + 2]);
+        const float area = 0.5f * len(cross(sub(x1, x0), sub(x2, x0)));
+        m += d3 * area;
+    }
+    inv_mass[v] = (m > 1.0e-12f) ? 1.0f / m : 0.0f;
+}
+
+// Per-element FP64 energies into `out` (one double per element, ordered
+// pins, triangles, dihedrals).
+__global__ __launch_bounds__(kBlock) void energy_kernel(chx_cloth_batch B, int mask, double* __restrict__ out) {
+    const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long P = B.n_pins, T = B.n_tris, D = B.n_dihs;
+    if (e < P) {
+        double E = 0.0;
+        if (mask & 1) {
+            const chx_pin p = B.pins[e];
+            const f3 d = sub(ld3(inst_at(B, p.inst).pos, p.v), mk(p.tx, p.ty, p.tz));
+            E = double(0.5f * p.k * dot(d, d));  // pin_constraint.cu:40-53
+        }
+        out[e] = E;
+    } else if (e < P + T) {
+        double E = 0.0;
+        if (mask & 2) {
+            const chx_tri t = B.tris[e - P];
+            const float* pos = inst_at(B, t.inst).pos;
+            E = tri_energy(ld3(pos, t.v0), ld3(pos, t.v1), ld3(pos, t.v2), t.dm00, t.dm01, t.dm10, t.dm11, t.kA);
+        }
+        out[e] = E;
+    } else if (e < P + T + D) {
+        double E = 0.0;
+        if (mask & 4) {
+            const chx_dih d = B.dihs[e - P - T];
+            const float* pos = inst_at(B, d.inst).pos;
+            const DihFrame F = dihedral_frame(ld3(pos, d.v0), ld3(pos, d.v1), ld3(pos, d.v2), ld3(pos, d.v3));
+            if (F.valid) {  // bending_constraint.cu:154-172
+                const float dt = F.theta - d.rest;
+                E = double(0.5f * d.k * dt * dt);
+            }
+        }
+        out[e] = E;
+    }
+}
+
+}  // namespace
+}  // namespace chx
+
+using namespace chx;
+
+extern "C" {
+
+int chx_triangle_rest_shape(const int* tris, const float* pos, int n, float cos_theta, float sin_theta, float* dm_inv,
+                            float* area, uintptr_t stream) {
+    return guarded([&] {
+        require(n >= 0, "chx_triangle_rest_shape: negative count");
+        if (n == 0) return;
+        require(tris && pos && dm_inv && area, "chx_triangle_rest_shape: null pointer");
+        rest_shape_kernel<<<blocks_for(n, kBlock), kBlock, 0, as_stream(stream)>>>(tris, pos, n, cos_theta, sin_theta,
+                                                                                   dm_inv, area);
+        hip_check(hipGetLastError(), "rest_shape_kernel launch");
+    });
+}
+
+int chx_dihedral_rest_angle(const int* dihedrals, const float* pos, int n, float* rest, uintptr_t stream) {
+    return guarded([&] {
+        require(n >= 0, "chx_dihedral_rest_angle: negative count");
+        if (n == 0) return;
+        require(dihedrals && pos && rest, "chx_dihedral_rest_angle: null pointer");
+        rest_angle_kernel<<<blocks_for(n, kBlock), kBlock, 0, as_stream(stream)>>>(dihedrals, pos, n, rest);
+        hip_check(hipGetLastError(), "rest_angle_kernel launch");
+    });
+}
+
+int chx_mass_area_weighted(const int* tris, const float* pos, const int* vertex_tri_offsets, const int* vertex_tris,
+                           int n_vertices, float density_over_3, float* inv_mass, uintptr_t stream) {
+    return guarded([&] {
+        require(n_vertices >= 0, "chx_mass_area_weighted: negative count");
+        if (n_vertices == 0) return;
+        require(tris && pos && vertex_tri_offsets && vertex_tris && inv_mass, "chx_mass_area_weighted: null pointer");
+        mass_kernel<<<blocks_for(n_vertices, kBlock), kBlock, 0, as_stream(stream)>>>(
+            tris, pos, vertex_tri_offsets, vertex_tris, n_vertices, density_over_3, inv_mass);
+        hip_check(hipGetLastError(), "mass_kernel launch");
+    });
+}
+
+int chx_batch_energy(const chx_cloth_batch* batch, int kind_mask, double* energy, uintptr_t stream) {
+    return guarded([&] {
+        require(batch && energy, "chx_batch_energy: null pointer");
+        const long total = long(batch->n_pins) + batch->n_tris + batch->n_dihs;
+        *energy = 0.0;
+        if (total == 0) return;
+        double* dev = nullptr;
+        hip_check(hipMalloc(&dev, sizeof(double) * total), "hipMalloc(energy)");
+        std::vector<double> host(total);
+        try {
+            energy_kernel<<<blocks_for(total, kBlock), kBlock, 0, as_stream(stream)>>>(*batch, kind_mask, dev);
+            hip_check(hipGetLastError(), "energy_kernel launch");
+            hip_check(hipMemcpyAsync(host.data(), dev, sizeof(double) * total, hipMemcpyDeviceToHost,
+                                     as_stream(stream)),
+                      "hipMemcpyAsync(energy)");
+            hip_check(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize(energy)");
+        } catch (...) {
+            (void)hipFree(dev);
+            throw;
+        }
+        hip_check(hipFree(dev), "hipFree(energy)");
+        double s = 0.0;
+        for (double v : host) s += v;
+        *energy = s;
+    });
+}
+
+}  // extern "C"

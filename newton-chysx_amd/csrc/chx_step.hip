@@ -1,0 +1,708 @@
+// C ABI: the implicit cloth substep on gfx950.
+//
+//   element pass   one thread per pin / triangle / dihedral: gathers x_n from
+//                  the instance's AoS positions and writes a compact force +
+//                  Hessian record (chx_elements.h).
+//   gather pass    one thread per BSR slot (diagonal slots, then off-diagonal
+//                  slots): sums the slot's contributions in the fixed order of
+//                  its contribution list, so the assembled matrix is
+//                  bitwise reproducible with no float atomics.  Diagonal slots
+//                  also form the right-hand side
+//                  b = m/dt^2 (x~ - x_n) - grad E(x_n) and the inertia block
+//                  (cloth_simulator.cu:66-142,199-210).
+//   PCG            one persistent cooperative kernel per solve: block-Jacobi
+//                  PCG with L lanes per block row in the SpMV, FP64
+//                  per-tile partial sums reduced in a fixed order, two grid
+//                  barriers per iteration, per-instance convergence, then the
+//                  finalize x = x_n + dx, v = dx/dt (cloth_simulator.cu:222-244).
+//
+// Row tiles hold at most kTileRows rows of one instance; each tile's partial
+// sums live in batch->partials[tile*8 + field]:
+//   0/1 rho (<r,z>) for even/odd iterations, 2/3 ||r||^2 even/odd,
+//   4 sigma (<p,Ap>), 5 ||b||^2.
+#include <algorithm>
+#include <mutex>
+
+#include "chx_device.h"
+#include "chx_elements.h"
+#include "chx_internal.h"
+
+namespace chx {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTileRows = 256;
+
+enum { P_RHO0 = 0, P_RHO1 = 1, P_RR0 = 2, P_RR1 = 3, P_SIG = 4, P_BB = 5 };
+
+__device__ __forceinline__ void get_tile(const chx_cloth_batch& B, int t, int& inst, int& rb, int& re) {
+    if (B.tiles != nullptr) {
+        const int* p = B.tiles + 4 * t;
+        inst = p[0];
+        rb = p[1];
+        re = p[2];
+    } else {
+        inst = 0;
+        rb = t * kTileRows;
+        re = min(B.n_rows, rb + kTileRows);
+    }
+}
+
+// Divide guard |den| > 1e-37 (ChysX/src/solver/pcg_solver.cu:122-127), with
+// the FP64 inner products rounded to FP32 once.
+__device__ __forceinline__ float guarded_div(double a, double b) {
+    return (b > 1e-37 || b < -1e-37) ? float(a / b) : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Element pass
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void element_kernel(chx_cloth_batch B) {
+    const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long P = B.n_pins, T = B.n_tris, D = B.n_dihs;
+    if (e < P) {
+        const chx_pin p = B.pins[e];
+        const f3 x = ld3(inst_at(B, p.inst).pos, p.v);
+        const f3 g = mul(sub(x, mk(p.tx, p.ty, p.tz)), p.k);  // pin_constraint.cu:61-75
+        float* rec = B.pin_rec + long(CHX_PIN_REC) * e;
+        rec[0] = g.x;
+        rec[1] = g.y;
+        rec[2] = g.z;
+        rec[3] = p.k;
+    } else if (e < P + T) {
+        const long i = e - P;
+        const chx_tri t = B.tris[i];
+        const float* pos = inst_at(B, t.inst).pos;
+        float rec[CHX_TRI_REC];
+        tri_record(ld3(pos, t.v0), ld3(pos, t.v1), ld3(pos, t.v2), t.dm00, t.dm01, t.dm10, t.dm11, t.kA, rec);
+        float* out = B.tri_rec + long(CHX_TRI_REC) * i;
+#pragma unroll
+        for (int k = 0; k < CHX_TRI_REC; ++k) out[k] = rec[k];
+    } else if (e < P + T + D) {
+        const long i = e - P - T;
+        const chx_dih d = B.dihs[i];
+        const float* pos = inst_at(B, d.inst).pos;
+        const DihFrame F = dihedral_frame(ld3(pos, d.v0), ld3(pos, d.v1), ld3(pos, d.v2), ld3(pos, d.v3));
+        float* out = B.dih_rec + long(CHX_DIH_REC) * i;
+        // An invalid frame contributes nothing (bending_constraint.cu:190-191,227-228).
+        const float c = F.valid ? d.k * (F.theta - d.rest) : 0.0f;
+        const float k = F.valid ? d.k : 0.0f;
+        out[0] = c;
+        out[1] = k;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const f3 a = F.valid ? F.a[j] : mk(0.0f, 0.0f, 0.0f);
+            out[2 + 3 * j] = a.x;
+            out[3 + 3 * j] = a.y;
+            out[4 + 3 * j] = a.z;
+        }
+        out[14] = 0.0f;
+        out[15] = 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gather (assembly) pass
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
+    const long s = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long n = B.n_rows;
+    if (s == 0) {
+        // Reset the persistent solver's barrier / control words for the PCG
+        // launch that follows on this stream.
+        for (int k = 0; k < 8; ++k) B.sync[k] = 0u;
+    }
+    if (s >= n + B.nnz) return;
+    const bool is_diag = s < n;
+    const unsigned c0 = B.contrib_offsets[s], c1 = B.contrib_offsets[s + 1];
+    float h[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    f3 grad = mk(0.f, 0.f, 0.f);
+    for (unsigned c = c0; c < c1; ++c) {
+        const unsigned w = B.contribs[c];
+        const unsigned kind = w >> 30, a = (w >> 28) & 3u, b = (w >> 26) & 3u, id = w & CHX_CONTRIB_MAX_ELEMENT;
+        if (kind == CHX_CONTRIB_PIN) {
+            const float* r = B.pin_rec + long(CHX_PIN_REC) * id;
+            h[0] += r[3];
+            h[4] += r[3];
+            h[8] += r[3];
+            grad = add(grad, mk(r[0], r[1], r[2]));
+        } else if (kind == CHX_CONTRIB_TRI) {
+            const float* r = B.tri_rec + long(CHX_TRI_REC) * id;
+            const float su = r[18 + a] * r[18 + b];
+            const float sv = r[21 + a] * r[21 + b];
+            const float m0 = su * r[6] + sv * r[12], m1 = su * r[7] + sv * r[13], m2 = su * r[8] + sv * r[14];
+            const float m3 = su * r[9] + sv * r[15], m4 = su * r[10] + sv * r[16], m5 = su * r[11] + sv * r[17];
+            h[0] += m0; h[1] += m1; h[2] += m2;
+            h[3] += m1; h[4] += m3; h[5] += m4;
+            h[6] += m2; h[7] += m4; h[8] += m5;
+            if (is_diag && a == b) {
+                const float ua = r[18 + a], va = r[21 + a];
+                grad = add(grad, mk(ua * r[0] + va * r[3], ua * r[1] + va * r[4], ua * r[2] + va * r[5]));
+            }
+        } else {
+            const float* r = B.dih_rec + long(CHX_DIH_REC) * id;
+            const float k = r[1];
+            const f3 ai = mk(r[2 + 3 * a], r[3 + 3 * a], r[4 + 3 * a]);
+            const f3 aj = mk(r[2 + 3 * b], r[3 + 3 * b], r[4 + 3 * b]);
+            h[0] += (ai.x * aj.x) * k; h[1] += (ai.x * aj.y) * k; h[2] += (ai.x * aj.z) * k;
+            h[3] += (ai.y * aj.x) * k; h[4] += (ai.y * aj.y) * k; h[5] += (ai.y * aj.z) * k;
+            h[6] += (ai.z * aj.x) * k; h[7] += (ai.z * aj.y) * k; h[8] += (ai.z * aj.z) * k;
+            if (is_diag && a == b) grad = add(grad, mul(ai, r[0]));
+        }
+    }
+    if (is_diag) {
+        const int row = static_cast<int>(s);
+        const chx_instance& I = inst_at(B, row_instance(B, row));
+        const int local = row - I.row_begin;
+        const f3 p = ld3(I.pos, local), v = ld3(I.vel, local);
+        float m = 1.0f;  // mass_from_inv_mass_kernel, cloth_simulator.cu:66-77
+        if (I.inv_mass != nullptr) {
+            const float w = I.inv_mass[local];
+            m = (w > 1.0e-12f) ? (1.0f / w) : 1.0e8f;
+        }
+        const float dt = I.dt;
+        const float inv_dt2 = 1.0f / (dt * dt);
+        // x~ = x + dt v + dt^2 g (compute_x_tilde_kernel, :83-106)
+        const f3 xt = mk(p.x + dt * v.x + dt * dt * I.gx, p.y + dt * v.y + dt * dt * I.gy,
+                         p.z + dt * v.z + dt * dt * I.gz);
+        const float mi = m * inv_dt2;
+        st3(B.rhs, row, mk(mi * (xt.x - p.x) - grad.x, mi * (xt.y - p.y) - grad.y, mi * (xt.z - p.z) - grad.z));
+        h[0] += mi;
+        h[4] += mi;
+        h[8] += mi;
+        float* d = B.diag + 9L * row;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) d[k] = h[k];
+    } else {
+        float* d = B.values + 9L * (s - n);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) d[k] = h[k];
+    }
+}
+
+__global__ void reset_sync_kernel(unsigned* sync) {
+    if (threadIdx.x < 8) sync[threadIdx.x] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// SpMV row loop: L lanes per block row, 64/L rows per wave and pass.
+// src(j) returns the source vector entry j; fin(row, Ax) runs on the first
+// lane of each row after the segmented reduction.
+// ---------------------------------------------------------------------------
+template <int L, class Src, class Fin>
+__device__ __forceinline__ void spmv_rows(const int* __restrict__ ro, const int* __restrict__ ci,
+                                          const float* __restrict__ diag, const float* __restrict__ val, int rb, int re,
+                                          Src src, Fin fin) {
+    constexpr int RPW = kWave / L;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const int g = lane / L, sub_ = lane % L;
+    for (int base = rb + wid * RPW; base < re; base += nw * RPW) {
+        const int row = base + g;
+        const bool valid = row < re;
+        f3 acc = mk(0.f, 0.f, 0.f);
+        if (valid) {
+            const int beg = ro[row], end = ro[row + 1];
+            if (sub_ == 0) acc = mv9(diag + 9L * row, src(row));
+            for (int k = beg + sub_; k < end; k += L) acc = add(acc, mv9(val + 9L * k, src(ci[k])));
+        }
+        acc.x = seg_sum<L>(acc.x);
+        acc.y = seg_sum<L>(acc.y);
+        acc.z = seg_sum<L>(acc.z);
+        if (valid && sub_ == 0) fin(row, acc);
+    }
+}
+
+// Per-instance sums of up to four partial fields over the instance's tiles,
+// in a fixed order (lane-strided then butterfly), identical in every
+// workgroup.  Result in lds[0..3] for all threads.
+__device__ __forceinline__ void instance_sums(const double* __restrict__ part, int tb, int te, int f0, int f1, int f2,
+                                              int f3_, double* lds) {
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        for (int t = tb + int(threadIdx.x); t < te; t += kWave) {
+            const double* p = part + long(CHX_TILE_PARTIALS) * t;
+            s0 += p[f0];
+            s1 += p[f1];
+            s2 += p[f2];
+            s3 += p[f3_];
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+        s3 = wave_sum(s3);
+        if (threadIdx.x == 0) {
+            lds[0] = s0;
+            lds[1] = s1;
+            lds[2] = s2;
+            lds[3] = s3;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool instance_active(const chx_instance& I, int k, double rr, double bb) {
+    if (k >= I.max_iter) return false;
+    if (I.tol > 0.0 && rr <= I.tol * I.tol * bb) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent block-Jacobi PCG (ChysX/src/solver/pcg_solver.cu:134-215):
+//   r = b - A x, z = M^-1 r, p = z, rho = <r,z>
+//   loop: q = A p; alpha = rho/<p,q>; x += alpha p; r -= alpha q;
+//         z = M^-1 r; rho' = <r,z>; beta = rho'/rho; p = z + beta p
+// The p update of iteration k is folded into the SpMV of iteration k: the
+// SpMV gathers z_j + beta p_{k-1,j} (double-buffered p), which removes one
+// grid barrier and one pass over p per iteration.
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int finalize) {
+    __shared__ double s_red[16];
+    __shared__ double s_sc[4];
+    const unsigned G = gridDim.x;
+    unsigned* sync = B.sync;
+    double* part = B.partials;
+    const int* ro = B.row_offsets;
+    const int* ci = B.col_indices;
+    const float* diag = B.diag;
+    const float* val = B.values;
+
+    // ---- setup -------------------------------------------------------------
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(&sync[3], unsigned(B.n_instances), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int t = blockIdx.x; t < B.n_tiles; t += G) {
+        int inst, rb, re;
+        get_tile(B, t, inst, rb, re);
+        if (threadIdx.x == 0 && t == inst_at(B, inst).tile_begin) B.inst_done[inst] = 0;
+        double acc[3] = {0.0, 0.0, 0.0};
+        const float* xv = B.x;
+        spmv_rows<L>(
+            ro, ci, diag, val, rb, re, [&](int j) { return ld3(xv, j); },
+            [&](int row, f3 Ax) {
+                const f3 b = ld3(B.rhs, row);
+                const f3 r = sub(b, Ax);
+                float mi[9];
+                inverse9(diag + 9L * row, mi);
+                float* mo = B.minv + 9L * row;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) mo[k] = mi[k];
+                const f3 z = mv9(mi, r);
+                st3(B.r, row, r);
+                st3(B.z, row, z);
+                st3(B.p0, row, z);
+                acc[0] += dot64(r, z);
+                acc[1] += dot64(r, r);
+                acc[2] += dot64(b, b);
+            });
+        block_sum(acc, s_red);
+        if (threadIdx.x == 0) {
+            double* p = part + long(CHX_TILE_PARTIALS) * t;
+            p[P_RHO0] = acc[0];
+            p[P_RR0] = acc[1];
+            p[P_BB] = acc[2];
+        }
+    }
+    if (!grid_sync(sync, G)) return;
+
+    for (int k = 0;; ++k) {
+        const int cur = k & 1;
+        const float* pprev = cur ? B.p0 : B.p1;  // p_{k-1}
+        float* pk = cur ? B.p1 : B.p0;           // p_k
+        // ---- phase A: p_k = z + beta p_{k-1}; q = A p_k; sigma = <p_k, q> ----
+        int cached = -1;
+        for (int t = blockIdx.x; t < B.n_tiles; t += G) {
+            int inst, rb, re;
+            get_tile(B, t, inst, rb, re);
+            const chx_instance& I = inst_at(B, inst);
+            if (inst != cached) {
+                instance_sums(part, I.tile_begin, I.tile_end, P_RHO0 + cur, P_RHO0 + (cur ^ 1), P_RR0 + cur, P_BB,
+                              s_sc);
+                cached = inst;
+            }
+            const double rho = s_sc[0], rho_prev = s_sc[1], rr = s_sc[2], bb = s_sc[3];
+            if (!instance_active(I, k, rr, bb)) {
+                if (threadIdx.x == 0 && t == I.tile_begin && B.inst_done[inst] == 0) {
+                    B.inst_done[inst] = 1;
+                    B.inst_iters[inst] = k;
+                    B.inst_rr[inst] = rr;
+                    B.inst_bb[inst] = bb;
+                    __hip_atomic_fetch_sub(&sync[3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                continue;
+            }
+            double acc[1] = {0.0};
+            const float* z = B.z;
+            float* q = B.q;
+            if (k == 0) {
+                spmv_rows<L>(
+                    ro, ci, diag, val, rb, re, [&](int j) { return ld3(pk, j); },
+                    [&](int row, f3 Ap) {
+                        st3(q, row, Ap);
+                        acc[0] += dot64(ld3(pk, row), Ap);
+                    });
+            } else {
+                const float beta = guarded_div(rho, rho_prev);
+                auto pj = [&](int j) {
+                    const f3 zj = ld3(z, j), pp = ld3(pprev, j);
+                    return mk(fmaf(beta, pp.x, zj.x), fmaf(beta, pp.y, zj.y), fmaf(beta, pp.z, zj.z));
+                };
+                spmv_rows<L>(ro, ci, diag, val, rb, re, pj, [&](int row, f3 Ap) {
+                    const f3 pr = pj(row);
+                    st3(pk, row, pr);
+                    st3(q, row, Ap);
+                    acc[0] += dot64(pr, Ap);
+                });
+            }
+            block_sum(acc, s_red);
+            if (threadIdx.x == 0) part[long(CHX_TILE_PARTIALS) * t + P_SIG] = acc[0];
+        }
+        if (!grid_sync(sync, G)) return;
+        if (ld_acquire(&sync[3]) == 0u) break;
+
+        // ---- phase B: alpha; x += alpha p; r -= alpha q; z = M^-1 r; rho', rr' --
+        cached = -1;
+        const int nxt = cur ^ 1;
+        for (int t = blockIdx.x; t < B.n_tiles; t += G) {
+            int inst, rb, re;
+            get_tile(B, t, inst, rb, re);
+            const chx_instance& I = inst_at(B, inst);
+            if (inst != cached) {
+                instance_sums(part, I.tile_begin, I.tile_end, P_RHO0 + cur, P_SIG, P_RR0 + cur, P_BB, s_sc);
+                cached = inst;
+            }
+            const double rho = s_sc[0], sigma = s_sc[1], rr = s_sc[2], bb = s_sc[3];
+            if (!instance_active(I, k, rr, bb)) continue;
+            const float alpha = guarded_div(rho, sigma);
+            double acc[2] = {0.0, 0.0};
+            for (int row = rb + int(threadIdx.x); row < re; row += blockDim.x) {
+                const f3 p = ld3(pk, row), qq = ld3(B.q, row);
+                f3 x = ld3(B.x, row), r = ld3(B.r, row);
+                x = mk(fmaf(alpha, p.x, x.x), fmaf(alpha, p.y, x.y), fmaf(alpha, p.z, x.z));
+                r = mk(fmaf(-alpha, qq.x, r.x), fmaf(-alpha, qq.y, r.y), fmaf(-alpha, qq.z, r.z));
+                const f3 z = mv9(B.minv + 9L * row, r);
+                st3(B.x, row, x);
+                st3(B.r, row, r);
+                st3(B.z, row, z);
+                acc[0] += dot64(r, z);
+                acc[1] += dot64(r, r);
+            }
+            block_sum(acc, s_red);
+            if (threadIdx.x == 0) {
+                double* p = part + long(CHX_TILE_PARTIALS) * t;
+                p[P_RHO0 + nxt] = acc[0];
+                p[P_RR0 + nxt] = acc[1];
+            }
+        }
+        if (!grid_sync(sync, G)) return;
+    }
+
+    // ---- finalize: x_{n+1} = x_n + dx, v = dx/dt (cloth_simulator.cu:222-244) --
+    if (finalize) {
+        for (int t = blockIdx.x; t < B.n_tiles; t += G) {
+            int inst, rb, re;
+            get_tile(B, t, inst, rb, re);
+            const chx_instance& I = inst_at(B, inst);
+            const float inv_dt = 1.0f / I.dt;
+            const float decay = (I.damping > 0.0f) ? expf(-I.damping * I.dt) : 1.0f;
+            for (int row = rb + int(threadIdx.x); row < re; row += blockDim.x) {
+                const int local = row - I.row_begin;
+                const f3 d = ld3(B.x, row);
+                const f3 p = ld3(I.pos, local);
+                st3(I.pos, local, add(p, d));
+                f3 v = mul(d, inv_dt);
+                if (I.damping > 0.0f) v = mul(v, decay);
+                st3(I.vel, local, v);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Standalone SpMV: y = alpha A x + beta y (block_csr.cu:54-83).
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const float* __restrict__ diag, const int* __restrict__ ro,
+                                                      const int* __restrict__ ci, const float* __restrict__ val,
+                                                      const float* __restrict__ x, float* __restrict__ y, float alpha,
+                                                      float beta) {
+    constexpr int RPB = (kBlock / kWave) * (kWave / L);  // rows per block
+    const int rb = blockIdx.x * RPB;
+    const int re = min(n, rb + RPB);
+    spmv_rows<L>(
+        ro, ci, diag, val, rb, re, [&](int j) { return ld3(x, j); },
+        [&](int row, f3 Ax) {
+            if (beta == 0.0f) st3(y, row, mul(Ax, alpha));
+            else st3(y, row, add(mul(ld3(y, row), beta), mul(Ax, alpha)));
+        });
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+using PcgFn = void (*)(chx_cloth_batch, int);
+
+PcgFn pcg_fn(int L) {
+    switch (L) {
+        case 4: return pcg_kernel<4>;
+        case 8: return pcg_kernel<8>;
+        case 16: return pcg_kernel<16>;
+        default: throw Error(CHX_ERR_INVALID_ARGUMENT, "chysx: lanes_per_row must be 4, 8 or 16");
+    }
+}
+
+struct DeviceLimits {
+    int device = -1;
+    int cus = 0;
+    int max_blocks[17] = {0};
+};
+
+std::mutex g_lim_mu;
+DeviceLimits g_lim[16];
+
+int max_coresident(int L) {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_lim_mu);
+    DeviceLimits& d = g_lim[dev & 15];
+    if (d.device != dev) {
+        d = DeviceLimits{};
+        d.device = dev;
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        d.cus = prop.multiProcessorCount;
+    }
+    if (d.max_blocks[L] == 0) {
+        int per_cu = 0;
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(pcg_fn(L)),
+                                                               kBlock, 0),
+                  "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        if (per_cu < 1) per_cu = 1;
+        d.max_blocks[L] = per_cu * d.cus;
+    }
+    return d.max_blocks[L];
+}
+
+void launch_pcg(const chx_cloth_batch& B, int finalize, hipStream_t stream) {
+    if (B.n_tiles <= 0) return;
+    const int L = B.lanes_per_row;
+    PcgFn fn = pcg_fn(L);
+    int grid = max_coresident(L);
+    if (B.grid_blocks > 0) grid = std::min(grid, B.grid_blocks);
+    grid = std::min(grid, B.n_tiles);
+    chx_cloth_batch arg = B;
+    int fin = finalize;
+    void* args[] = {&arg, &fin};
+    hip_check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(fn), dim3(grid), dim3(kBlock), args, 0, stream),
+              "hipLaunchCooperativeKernel(pcg)");
+}
+
+void validate_batch(const chx_cloth_batch* b) {
+    require(b != nullptr, "chysx: null batch descriptor");
+    require(b->n_instances >= 0 && b->n_rows >= 0 && b->n_tiles >= 0 && b->nnz >= 0 && b->n_pins >= 0 &&
+                b->n_tris >= 0 && b->n_dihs >= 0,
+            "chysx: negative size in batch descriptor");
+    require(b->lanes_per_row == 4 || b->lanes_per_row == 8 || b->lanes_per_row == 16,
+            "chysx: lanes_per_row must be 4, 8 or 16");
+    if (b->n_rows > 0) {
+        require(b->row_offsets && b->diag && b->rhs && b->x && b->sync, "chysx: null array in batch descriptor");
+        require(b->inst != nullptr || b->n_instances == 1, "chysx: multi-instance batch needs an instance table");
+        require(b->n_instances == 1 || (b->row_inst != nullptr && b->tiles != nullptr),
+                "chysx: multi-instance batch needs row_inst and tiles");
+    }
+}
+
+void assemble(const chx_cloth_batch& B, hipStream_t stream) {
+    const long ne = long(B.n_pins) + B.n_tris + B.n_dihs;
+    if (ne > 0) {
+        element_kernel<<<blocks_for(ne, kBlock), kBlock, 0, stream>>>(B);
+        hip_check(hipGetLastError(), "element_kernel launch");
+    }
+    const long ns = long(B.n_rows) + B.nnz;
+    if (ns > 0) {
+        gather_kernel<<<blocks_for(ns, kBlock), kBlock, 0, stream>>>(B);
+        hip_check(hipGetLastError(), "gather_kernel launch");
+    }
+}
+
+}  // namespace
+}  // namespace chx
+
+using namespace chx;
+
+extern "C" {
+
+int chx_device_info(int* cu_count, int* max_pcg_blocks) {
+    return guarded([&] {
+        const int mb = max_coresident(8);
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        if (cu_count) *cu_count = g_lim[dev & 15].cus;
+        if (max_pcg_blocks) *max_pcg_blocks = mb;
+    });
+}
+
+int chx_spmv_bsr3(int n_rows, const float* diag, const int* row_offsets, const int* col_indices, const float* values,
+                  const float* x, float* y, float alpha, float beta, int lanes_per_row, uintptr_t stream) {
+    return guarded([&] {
+        require(n_rows >= 0, "spmv: negative row count");
+        if (n_rows == 0) return;
+        require(diag && row_offsets && x && y, "spmv: null pointer");
+        const hipStream_t s = as_stream(stream);
+        auto go = [&](auto kern, int L) {
+            const int rpb = (kBlock / kWave) * (kWave / L);
+            kern<<<blocks_for(n_rows, rpb), kBlock, 0, s>>>(n_rows, diag, row_offsets, col_indices, values, x, y,
+                                                             alpha, beta);
+        };
+        switch (lanes_per_row) {
+            case 4: go(spmv_kernel<4>, 4); break;
+            case 8: go(spmv_kernel<8>, 8); break;
+            case 16: go(spmv_kernel<16>, 16); break;
+            default: throw Error(CHX_ERR_INVALID_ARGUMENT, "spmv: lanes_per_row must be 4, 8 or 16");
+        }
+        hip_check(hipGetLastError(), "spmv_kernel launch");
+    });
+}
+
+int chx_pcg_workspace_size(int n_rows, size_t* bytes) {
+    return guarded([&] {
+        require(n_rows >= 0 && bytes, "chx_pcg_workspace_size: bad argument");
+        const size_t n = size_t(n_rows);
+        const size_t tiles = (n + kTileRows - 1) / kTileRows;
+        auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+        *bytes = 5 * al(12 * n) + al(36 * n) + al(8 * CHX_TILE_PARTIALS * tiles) +
+                 al(sizeof(int)) + 2 * al(sizeof(double)) + al(sizeof(int)) + al(8 * sizeof(unsigned));
+    });
+}
+
+int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, const int* col_indices,
+                       const float* values, const float* b, float* x, int max_iterations, double tol, int lanes_per_row,
+                       void* workspace, size_t workspace_bytes, int* iterations, double* rr, double* bb,
+                       uintptr_t stream) {
+    return guarded([&] {
+        require(n_rows >= 0, "pcg: negative row count");
+        require(max_iterations >= 0, "pcg: negative iteration cap");
+        if (n_rows == 0) {
+            if (iterations) *iterations = 0;
+            if (rr) *rr = 0.0;
+            if (bb) *bb = 0.0;
+            return;
+        }
+        require(diag && row_offsets && b && x && workspace, "pcg: null pointer");
+        size_t need = 0;
+        if (chx_pcg_workspace_size(n_rows, &need) != CHX_OK) throw Error(CHX_ERR_RUNTIME, chx_last_error_message());
+        require(workspace_bytes >= need, "pcg: workspace too small (see chx_pcg_workspace_size)");
+        const size_t n = size_t(n_rows);
+        const int tiles = static_cast<int>((n + kTileRows - 1) / kTileRows);
+        auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+        char* w = static_cast<char*>(workspace);
+        auto take = [&](size_t bytes) {
+            char* p = w;
+            w += al(bytes);
+            return p;
+        };
+        chx_cloth_batch B{};
+        B.n_instances = 1;
+        B.n_rows = n_rows;
+        B.n_tiles = tiles;
+        B.nnz = 0;
+        B.lanes_per_row = lanes_per_row;
+        B.inst = nullptr;
+        B.row_inst = nullptr;
+        B.single.row_begin = 0;
+        B.single.n_rows = n_rows;
+        B.single.tile_begin = 0;
+        B.single.tile_end = tiles;
+        B.single.dt = 1.0f;
+        B.single.max_iter = max_iterations;
+        B.single.tol = tol;
+        B.tiles = nullptr;
+        B.row_offsets = row_offsets;
+        B.col_indices = col_indices;
+        B.diag = const_cast<float*>(diag);
+        B.values = const_cast<float*>(values);
+        B.rhs = const_cast<float*>(b);
+        B.x = x;
+        B.r = reinterpret_cast<float*>(take(12 * n));
+        B.z = reinterpret_cast<float*>(take(12 * n));
+        B.p0 = reinterpret_cast<float*>(take(12 * n));
+        B.p1 = reinterpret_cast<float*>(take(12 * n));
+        B.q = reinterpret_cast<float*>(take(12 * n));
+        B.minv = reinterpret_cast<float*>(take(36 * n));
+        B.partials = reinterpret_cast<double*>(take(8 * CHX_TILE_PARTIALS * size_t(tiles)));
+        B.inst_iters = reinterpret_cast<int*>(take(sizeof(int)));
+        B.inst_rr = reinterpret_cast<double*>(take(sizeof(double)));
+        B.inst_bb = reinterpret_cast<double*>(take(sizeof(double)));
+        B.inst_done = reinterpret_cast<int*>(take(sizeof(int)));
+        B.sync = reinterpret_cast<unsigned*>(take(8 * sizeof(unsigned)));
+        const hipStream_t s = as_stream(stream);
+        pcg_fn(lanes_per_row);  // validates L
+        reset_sync_kernel<<<1, 64, 0, s>>>(B.sync);
+        hip_check(hipGetLastError(), "reset_sync_kernel launch");
+        launch_pcg(B, 0, s);
+        if (iterations || rr || bb) {
+            int it = 0;
+            double r2 = 0.0, b2 = 0.0;
+            unsigned status = 0;
+            hip_check(hipMemcpyAsync(&it, B.inst_iters, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(iters)");
+            hip_check(hipMemcpyAsync(&r2, B.inst_rr, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(rr)");
+            hip_check(hipMemcpyAsync(&b2, B.inst_bb, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(bb)");
+            hip_check(hipMemcpyAsync(&status, B.sync + 4, sizeof(unsigned), hipMemcpyDeviceToHost, s),
+                      "hipMemcpyAsync(status)");
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(pcg)");
+            if (status != 0u) throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out");
+            if (iterations) *iterations = it;
+            if (rr) *rr = r2;
+            if (bb) *bb = b2;
+        }
+    });
+}
+
+int chx_cloth_assemble(const chx_cloth_batch* batch, uintptr_t stream) {
+    return guarded([&] {
+        validate_batch(batch);
+        assemble(*batch, as_stream(stream));
+    });
+}
+
+int chx_cloth_solve(const chx_cloth_batch* batch, int finalize, uintptr_t stream) {
+    return guarded([&] {
+        validate_batch(batch);
+        const hipStream_t s = as_stream(stream);
+        reset_sync_kernel<<<1, 64, 0, s>>>(batch->sync);
+        hip_check(hipGetLastError(), "reset_sync_kernel launch");
+        launch_pcg(*batch, finalize, s);
+    });
+}
+
+int chx_cloth_step(const chx_cloth_batch* batch, uintptr_t stream) {
+    return guarded([&] {
+        validate_batch(batch);
+        const hipStream_t s = as_stream(stream);
+        if (batch->n_rows + batch->nnz == 0) {
+            reset_sync_kernel<<<1, 64, 0, s>>>(batch->sync);
+            hip_check(hipGetLastError(), "reset_sync_kernel launch");
+        }
+        assemble(*batch, s);
+        launch_pcg(*batch, 1, s);
+    });
+}
+
+int chx_batch_check(const chx_cloth_batch* batch, uintptr_t stream) {
+    return guarded([&] {
+        require(batch != nullptr && batch->sync != nullptr, "chx_batch_check: null batch");
+        unsigned words[8];
+        const hipStream_t s = as_stream(stream);
+        hip_check(hipMemcpyAsync(words, batch->sync, sizeof(words), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(sync)");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(check)");
+        if (words[4] != 0u) {
+            reset_sync_kernel<<<1, 64, 0, s>>>(batch->sync);
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(reset)");
+            throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out");
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,261 @@
+// pybind11 module `_chysx_native`: the reference's Python surface for the
+// cloth step (ChysX/src/bindings.cpp:21-595,1068-1075) over the MI355X
+// implementation.  Device buffers arrive as integer pointers (for example
+// torch.Tensor.data_ptr() on ROCm); numpy inputs are force-cast to
+// C-contiguous int32 / float32 like the reference's bindings.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <stdexcept>
+
+#include "cloth_batch.h"
+#include "cloth_simulator.h"
+#include "sparse_solver.h"
+#include "topology.h"
+
+namespace py = pybind11;
+using chysx::cloth::ClothMaterial;
+using chysx::cloth::ClothSimulator;
+
+namespace {
+
+using IArr = py::array_t<int, py::array::c_style | py::array::forcecast>;
+using FArr = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+const chysx::math::Vec3i* as_tris(const IArr& t, const char* who) {
+    if (t.ndim() != 2 || t.shape(1) != 3)
+        throw std::invalid_argument(std::string(who) + ": triangles must have shape (M, 3) and dtype int32");
+    return reinterpret_cast<const chysx::math::Vec3i*>(t.data());
+}
+
+py::array_t<int> to_array(const std::vector<int>& v) {
+    py::array_t<int> a(static_cast<py::ssize_t>(v.size()));
+    std::copy(v.begin(), v.end(), a.mutable_data());
+    return a;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_chysx_native, m) {
+    m.doc() = "ChysX cloth step on AMD Instinct MI355X (gfx950 HIP kernels behind a C ABI)";
+
+    py::class_<ClothMaterial>(m, "ClothMaterial")
+        .def(py::init<>())
+        .def_readwrite("lame_mu", &ClothMaterial::lame_mu)
+        .def_readwrite("lame_lambda", &ClothMaterial::lame_lambda)
+        .def_readwrite("bending", &ClothMaterial::bending)
+        .def_readwrite("density", &ClothMaterial::density)
+        .def_readwrite("damping", &ClothMaterial::damping)
+        .def_readwrite("gx", &ClothMaterial::gx)
+        .def_readwrite("gy", &ClothMaterial::gy)
+        .def_readwrite("gz", &ClothMaterial::gz);
+
+    py::class_<ClothSimulator>(m, "ClothSimulator")
+        .def(py::init<>())
+        .def("set_material", &ClothSimulator::set_material, py::arg("material"))
+        .def("set_external_buffers", &ClothSimulator::set_external_buffers, py::arg("pos_ptr"), py::arg("vel_ptr"),
+             py::arg("particle_count"), py::arg("inv_mass_ptr") = 0, py::arg("force_ptr") = 0)
+        .def("step", &ClothSimulator::step, py::arg("dt"), py::arg("cuda_stream") = 0)
+        .def(
+            "set_pins",
+            [](ClothSimulator& self, IArr indices, FArr targets, float stiffness) {
+                if (indices.ndim() != 1) throw std::invalid_argument("ClothSimulator.set_pins: indices must be 1-D");
+                if (targets.ndim() != 2 || targets.shape(1) != 3)
+                    throw std::invalid_argument("ClothSimulator.set_pins: targets must have shape (N, 3)");
+                if (indices.shape(0) != targets.shape(0))
+                    throw std::invalid_argument(
+                        "ClothSimulator.set_pins: indices and targets must have the same length");
+                self.set_pins(indices.data(), reinterpret_cast<const chysx::math::Vec3f*>(targets.data()),
+                              static_cast<int>(indices.shape(0)), stiffness);
+            },
+            py::arg("indices"), py::arg("targets"), py::arg("stiffness") = 1.0e6f)
+        .def("clear_pins", &ClothSimulator::clear_pins)
+        .def("num_pins", [](const ClothSimulator& s) { return s.pins().size(); })
+        .def(
+            "update_pin_targets",
+            [](ClothSimulator& self, FArr targets, std::uintptr_t stream) {
+                if (targets.ndim() != 2 || targets.shape(1) != 3)
+                    throw std::invalid_argument(
+                        "ClothSimulator.update_pin_targets: targets must have shape (N, 3) and dtype float32");
+                self.update_pin_targets(reinterpret_cast<const chysx::math::Vec3f*>(targets.data()),
+                                        static_cast<int>(targets.shape(0)), stream);
+            },
+            py::arg("targets"), py::arg("cuda_stream") = 0)
+        .def(
+            "set_mesh",
+            [](ClothSimulator& self, IArr tris) {
+                self.set_mesh(as_tris(tris, "ClothSimulator.set_mesh"), static_cast<int>(tris.shape(0)));
+            },
+            py::arg("triangles"))
+        .def("build_springs_from_current_positions", &ClothSimulator::build_springs_from_current_positions,
+             py::arg("stiffness"), py::arg("cuda_stream") = 0)
+        .def("num_springs", [](ClothSimulator& s) { return s.springs().size(); })
+        .def("redistribute_mass_area_weighted", &ClothSimulator::redistribute_mass_area_weighted,
+             py::arg("surface_density"), py::arg("inv_mass_ptr"), py::arg("particle_count"),
+             py::arg("cuda_stream") = 0)
+        .def("build_fem_stretch_from_current_positions", &ClothSimulator::build_fem_stretch_from_current_positions,
+             py::arg("stiffness"), py::arg("cuda_stream") = 0)
+        .def("num_fem_stretch_triangles", [](const ClothSimulator& s) { return s.fem_stretch().size(); })
+        .def("build_fem_shear_from_current_positions", &ClothSimulator::build_fem_shear_from_current_positions,
+             py::arg("stiffness"), py::arg("cuda_stream") = 0)
+        .def("num_fem_shear_triangles", [](const ClothSimulator& s) { return s.fem_shear().size(); })
+        .def("build_bending_from_current_positions", &ClothSimulator::build_bending_from_current_positions,
+             py::arg("stiffness"), py::arg("cuda_stream") = 0)
+        .def("num_bending_dihedrals", [](const ClothSimulator& s) { return s.bending().size(); })
+        .def(
+            "set_solver_type",
+            [](ClothSimulator& self, int t) {
+                self.set_solver_type(t == 1 ? chysx::cloth::SolverType::VBD : chysx::cloth::SolverType::PCG);
+            },
+            py::arg("solver_type"))
+        .def("set_pcg_iterations", &ClothSimulator::set_pcg_iterations, py::arg("max_iter"))
+        .def("pcg_iterations", &ClothSimulator::pcg_iterations)
+        .def("set_pcg_tolerance", &ClothSimulator::set_pcg_tolerance, py::arg("tolerance"))
+        .def("pcg_tolerance", &ClothSimulator::pcg_tolerance)
+        .def("set_lanes_per_row", &ClothSimulator::set_lanes_per_row, py::arg("lanes"))
+        .def("lanes_per_row", &ClothSimulator::lanes_per_row)
+        .def("last_pcg_iterations", &ClothSimulator::last_pcg_iterations, py::arg("cuda_stream") = 0)
+        .def("last_relative_residual", &ClothSimulator::last_relative_residual, py::arg("cuda_stream") = 0)
+        .def("num_particles", &ClothSimulator::num_particles)
+        .def("num_off_diag_blocks", &ClothSimulator::num_off_diag_blocks)
+        .def("debug_dump_last_solve",
+             [](const ClothSimulator& s) {
+                 const int n = s.num_particles();
+                 const int nnz = s.num_off_diag_blocks();
+                 py::array_t<float> diag({n, 3, 3});
+                 py::array_t<int> row_offsets({n + 1});
+                 py::array_t<int> col_indices({nnz});
+                 py::array_t<float> values({nnz, 3, 3});
+                 py::array_t<float> rhs({n, 3});
+                 py::array_t<float> dx({n, 3});
+                 if (n == 0) return py::make_tuple(diag, row_offsets, col_indices, values, rhs, dx);
+                 s.debug_copy_hessian_diag(diag.mutable_data());
+                 s.debug_copy_hessian_csr(row_offsets.mutable_data(), nnz > 0 ? col_indices.mutable_data() : nullptr,
+                                          nnz > 0 ? values.mutable_data() : nullptr);
+                 s.debug_copy_last_rhs(rhs.mutable_data());
+                 s.debug_copy_last_dx(dx.mutable_data());
+                 return py::make_tuple(diag, row_offsets, col_indices, values, rhs, dx);
+             })
+        .def(
+            "debug_set_warm_start",
+            [](ClothSimulator& s, FArr dx) {
+                if (dx.ndim() != 2 || dx.shape(1) != 3 || dx.shape(0) != s.buffers().particle_count())
+                    throw std::invalid_argument("ClothSimulator.debug_set_warm_start: dx must have shape (N, 3)");
+                s.debug_set_warm_start(dx.data());
+            },
+            py::arg("dx"))
+        .def("compute_energy", &ClothSimulator::compute_energy, py::arg("kind_mask") = 7)
+        .def(
+            "debug_slot_tables",
+            [](const ClothSimulator& s) {
+                py::dict d;
+                d["pins"] = to_array(s.pins().hessian_slots());
+                d["stretch"] = to_array(s.fem_stretch().hessian_slots());
+                d["shear"] = to_array(s.fem_shear().hessian_slots());
+                d["bending"] = to_array(s.bending().hessian_slots());
+                return d;
+            })
+        .def("debug_dihedrals",
+             [](const ClothSimulator& s) {
+                 const auto& d = s.bending().indices();
+                 py::array_t<int> a({static_cast<py::ssize_t>(d.size()), static_cast<py::ssize_t>(4)});
+                 std::memcpy(a.mutable_data(), d.data(), d.size() * sizeof(chysx::math::Vec4i));
+                 py::array_t<float> r(static_cast<py::ssize_t>(d.size()));
+                 std::copy(s.bending().rest_angles().begin(), s.bending().rest_angles().end(), r.mutable_data());
+                 return py::make_tuple(a, r);
+             })
+        .def("debug_edges",
+             [](const ClothSimulator& s) {
+                 const auto& e = s.buffers().edges;
+                 py::array_t<int> a({static_cast<py::ssize_t>(e.size()), static_cast<py::ssize_t>(2)});
+                 std::memcpy(a.mutable_data(), e.data(), e.size() * sizeof(chysx::math::Vec2i));
+                 return a;
+             })
+        .def_property_readonly(
+            "material", [](ClothSimulator& s) -> ClothMaterial& { return s.material(); },
+            py::return_value_policy::reference_internal);
+
+    // ---- batched form (many instances, one fused substep) -------------------
+    py::class_<chysx::cloth::ClothBatch>(m, "ClothBatch")
+        .def(py::init<>())
+        .def(
+            "add_instance",
+            [](chysx::cloth::ClothBatch& b, ClothSimulator& sim) { return b.add_instance(sim); }, py::arg("simulator"),
+            py::keep_alive<1, 2>())
+        .def("num_instances", &chysx::cloth::ClothBatch::num_instances)
+        .def("num_rows", &chysx::cloth::ClothBatch::num_rows)
+        .def("device_bytes", &chysx::cloth::ClothBatch::device_bytes)
+        .def("set_lanes_per_row", &chysx::cloth::ClothBatch::set_lanes_per_row, py::arg("lanes"))
+        .def("set_grid_blocks", &chysx::cloth::ClothBatch::set_grid_blocks, py::arg("blocks"))
+        .def("finalize", &chysx::cloth::ClothBatch::finalize)
+        .def("step", &chysx::cloth::ClothBatch::step, py::arg("dt"), py::arg("cuda_stream") = 0)
+        .def("assemble", &chysx::cloth::ClothBatch::assemble, py::arg("dt"), py::arg("cuda_stream") = 0)
+        .def("solve", &chysx::cloth::ClothBatch::solve, py::arg("cuda_stream") = 0)
+        .def("instance_sizes", &chysx::cloth::ClothBatch::instance_sizes)
+        .def("last_pcg_iterations", &chysx::cloth::ClothBatch::last_pcg_iterations, py::arg("cuda_stream") = 0)
+        .def("last_relative_residuals", &chysx::cloth::ClothBatch::last_relative_residuals,
+             py::arg("cuda_stream") = 0);
+
+    // ---- host topology (no device needed) ------------------------------------
+    py::module_ topo = m.def_submodule("topology", "Host topology of the cloth step (bit-exact integer work)");
+    topo.def(
+        "build_edges",
+        [](IArr tris) {
+            const auto e = chysx::topology::build_edges(as_tris(tris, "build_edges"), static_cast<int>(tris.shape(0)));
+            py::array_t<int> a({static_cast<py::ssize_t>(e.size()), static_cast<py::ssize_t>(2)});
+            std::memcpy(a.mutable_data(), e.data(), e.size() * sizeof(chysx::math::Vec2i));
+            return a;
+        },
+        py::arg("triangles"));
+    topo.def(
+        "detect_dihedrals",
+        [](IArr tris) {
+            const auto d =
+                chysx::topology::detect_dihedrals(as_tris(tris, "detect_dihedrals"), static_cast<int>(tris.shape(0)));
+            py::array_t<int> a({static_cast<py::ssize_t>(d.size()), static_cast<py::ssize_t>(4)});
+            std::memcpy(a.mutable_data(), d.data(), d.size() * sizeof(chysx::math::Vec4i));
+            return a;
+        },
+        py::arg("triangles"));
+    topo.def(
+        "build_pattern",
+        [](int n, IArr rows, IArr cols) {
+            if (rows.ndim() != 1 || cols.ndim() != 1 || rows.shape(0) != cols.shape(0))
+                throw std::invalid_argument("build_pattern: rows/cols must be 1-D of equal length");
+            const auto p = chysx::topology::build_pattern(n, rows.data(), cols.data(), static_cast<int>(rows.shape(0)));
+            return py::make_tuple(to_array(p.row_offsets), to_array(p.col_indices));
+        },
+        py::arg("n"), py::arg("rows"), py::arg("cols"));
+    topo.def(
+        "cloth_system",
+        [](int n, IArr tris, IArr pins, bool stretch, bool shear, bool bending) {
+            // Pattern + slot tables of a cloth with the given element sets, as
+            // ClothSimulator::ensure_hessian_topology builds them.
+            const auto* t = as_tris(tris, "cloth_system");
+            const int nt = static_cast<int>(tris.shape(0));
+            std::vector<chysx::math::Vec3i> tv(t, t + nt);
+            std::vector<chysx::math::Vec4i> dih;
+            if (bending) dih = chysx::topology::detect_dihedrals(t, nt);
+            std::vector<int> rows, cols;
+            std::vector<chysx::math::Vec3i> none;
+            chysx::topology::append_triangle_pairs(stretch ? tv : none, rows, cols);
+            chysx::topology::append_triangle_pairs(shear ? tv : none, rows, cols);
+            chysx::topology::append_dihedral_pairs(dih, rows, cols);
+            const auto p = chysx::topology::build_pattern(n, rows.data(), cols.data(), static_cast<int>(rows.size()));
+            std::vector<int> pv(pins.data(), pins.data() + pins.size());
+            py::dict d;
+            d["row_offsets"] = to_array(p.row_offsets);
+            d["col_indices"] = to_array(p.col_indices);
+            d["pin_slots"] = to_array(chysx::topology::slot_table_pins(p, pv));
+            d["stretch_slots"] = to_array(stretch ? chysx::topology::slot_table3(p, tv) : std::vector<int>{});
+            d["shear_slots"] = to_array(shear ? chysx::topology::slot_table3(p, tv) : std::vector<int>{});
+            d["bending_slots"] = to_array(chysx::topology::slot_table4(p, dih));
+            py::array_t<int> da({static_cast<py::ssize_t>(dih.size()), static_cast<py::ssize_t>(4)});
+            std::memcpy(da.mutable_data(), dih.data(), dih.size() * sizeof(chysx::math::Vec4i));
+            d["dihedrals"] = da;
+            return d;
+        },
+        py::arg("n"), py::arg("triangles"), py::arg("pins"), py::arg("stretch") = true, py::arg("shear") = true,
+        py::arg("bending") = true);
+}

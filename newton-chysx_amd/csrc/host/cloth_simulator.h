@@ -1,0 +1,157 @@
+// chysx::cloth::ClothSimulator — drop-in for the reference class
+// (ChysX/src/cloth/cloth_simulator.h:59-719), PCG branch.
+//
+// Same lifecycle and contracts: the material is copied, particle buffers are
+// borrowed device pointers (AoS float3 positions / velocities, optional
+// inverse mass), set-up builds constraint sets from the current positions,
+// and step(dt, stream) runs one implicit-Euler substep asynchronously on the
+// caller's stream.  The device work goes through the C ABI in chysx_hip.h.
+//
+// Out of scope here (the reference's contact, self-collision, tetrahedral
+// FEM and VBD branches are off by default there): the corresponding
+// methods are absent, and set_solver_type(VBD) makes step() throw.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "chysx_common.h"
+#include "constraints.h"
+#include "packed_batch.h"
+#include "topology.h"
+
+namespace chysx {
+namespace cloth {
+
+// ChysX/src/cloth/cloth_material.h:18-43.
+struct ClothMaterial {
+    float lame_mu = 0.0f;
+    float lame_lambda = 0.0f;
+    float bending = 0.0f;
+    float density = 0.05f;
+    float damping = 0.0f;
+    float gx = 0.0f;
+    float gy = 0.0f;
+    float gz = -9.81f;
+};
+
+// ChysX/src/cloth/cloth_buffers.h:31-70.
+struct ClothBuffers {
+    DeviceSpan<math::Vec3f> pos;
+    DeviceSpan<math::Vec3f> vel;
+    DeviceSpan<float> inv_mass;
+    DeviceSpan<math::Vec3f> ext_force;
+    std::vector<math::Vec3i> tris;
+    std::vector<math::Vec2i> edges;
+    int particle_count() const noexcept { return static_cast<int>(pos.size()); }
+};
+
+enum class SolverType { PCG = 0, VBD = 1 };
+
+class ClothSimulator {
+public:
+    ClothSimulator() = default;
+    ClothSimulator(const ClothSimulator&) = delete;
+    ClothSimulator& operator=(const ClothSimulator&) = delete;
+
+    // ---- material ------------------------------------------------------
+    void set_material(const ClothMaterial& m) noexcept { material_ = m; }
+    ClothMaterial& material() noexcept { return material_; }
+    const ClothMaterial& material() const noexcept { return material_; }
+
+    // ---- buffers -------------------------------------------------------
+    void set_external_buffers(std::uintptr_t pos_ptr, std::uintptr_t vel_ptr, int particle_count,
+                              std::uintptr_t inv_mass_ptr = 0, std::uintptr_t force_ptr = 0) noexcept;
+    ClothBuffers& buffers() noexcept { return buffers_; }
+    const ClothBuffers& buffers() const noexcept { return buffers_; }
+
+    // ---- pins -----------------------------------------------------------
+    void set_pins(const int* host_indices, const math::Vec3f* host_targets, int n, float stiffness = 1.0e6f);
+    void clear_pins() noexcept;
+    void update_pin_targets(const math::Vec3f* host_targets, int n_pins, std::uintptr_t cuda_stream = 0);
+    constraint::PinConstraint& pins() noexcept { return pins_; }
+    const constraint::PinConstraint& pins() const noexcept { return pins_; }
+
+    // ---- mesh and elements --------------------------------------------------
+    void set_mesh(const math::Vec3i* host_triangles, int n_triangles);
+    void build_springs_from_current_positions(float stiffness, std::uintptr_t cuda_stream = 0);
+    void build_fem_stretch_from_current_positions(float stiffness, std::uintptr_t cuda_stream = 0);
+    void build_fem_shear_from_current_positions(float stiffness, std::uintptr_t cuda_stream = 0);
+    void build_bending_from_current_positions(float stiffness, std::uintptr_t cuda_stream = 0);
+    constraint::SpringConstraint& springs() noexcept { return springs_; }
+    constraint::TriangleStretchConstraint& fem_stretch() noexcept { return fem_stretch_; }
+    const constraint::TriangleStretchConstraint& fem_stretch() const noexcept { return fem_stretch_; }
+    constraint::TriangleStretchConstraint& fem_shear() noexcept { return fem_shear_; }
+    const constraint::TriangleStretchConstraint& fem_shear() const noexcept { return fem_shear_; }
+    constraint::BendingConstraint& bending() noexcept { return bending_; }
+    const constraint::BendingConstraint& bending() const noexcept { return bending_; }
+
+    void redistribute_mass_area_weighted(float surface_density, std::uintptr_t inv_mass_ptr, int particle_count,
+                                         std::uintptr_t cuda_stream = 0);
+
+    // ---- stepping --------------------------------------------------------------
+    void step(float dt, std::uintptr_t cuda_stream = 0);
+
+    // ---- solver knobs -------------------------------------------------------------
+    void set_pcg_iterations(int max_iter) noexcept { pcg_max_iterations_ = (max_iter > 0) ? max_iter : 1; }
+    int pcg_iterations() const noexcept { return pcg_max_iterations_; }
+    void set_solver_type(SolverType t) noexcept { solver_type_ = t; }
+    SolverType solver_type() const noexcept { return solver_type_; }
+    // Additions: convergence tolerance (<= 0: the reference's fixed
+    // iteration count), SpMV lanes per block row, and read-backs of the last
+    // solve (synchronous).
+    void set_pcg_tolerance(double tol) noexcept { pcg_tolerance_ = tol; }
+    double pcg_tolerance() const noexcept { return pcg_tolerance_; }
+    void set_lanes_per_row(int lanes);
+    int lanes_per_row() const noexcept { return lanes_per_row_; }
+    int last_pcg_iterations(std::uintptr_t cuda_stream = 0);
+    double last_relative_residual(std::uintptr_t cuda_stream = 0);
+
+    // ---- diagnostics -----------------------------------------------------------------
+    int num_particles() const noexcept { return H_num_block_rows_; }
+    int num_off_diag_blocks() const noexcept { return pattern_.nnz(); }
+    void debug_copy_hessian_diag(float* out) const;
+    void debug_copy_hessian_csr(int* out_row_offsets, int* out_col_indices, float* out_values) const;
+    void debug_copy_last_rhs(float* out) const;
+    void debug_copy_last_dx(float* out) const;
+    // Overwrites the PCG warm start (the previous solve's dx); n*3 floats.
+    void debug_set_warm_start(const float* in);
+    // Elastic energy at the current positions (FP64, synchronous);
+    // kind_mask 1 pins, 2 triangles, 4 dihedrals.
+    double compute_energy(int kind_mask = 7);
+    const topology::Pattern& pattern() const noexcept { return pattern_; }
+
+    // Host-only part of the topology build (pattern + slot tables), used by
+    // ClothBatch to pack this simulator as one instance of a batch.
+    void prepare_topology();
+    engine::InstanceSource instance_source() const { return source(); }
+
+private:
+    void resize_work_buffers(int n);
+    void ensure_hessian_topology();
+    engine::InstanceSource source() const;
+    void check_elements_current();
+
+    ClothMaterial material_;
+    ClothBuffers buffers_;
+    constraint::PinConstraint pins_;
+    constraint::SpringConstraint springs_;
+    constraint::TriangleStretchConstraint fem_stretch_;
+    constraint::TriangleStretchConstraint fem_shear_;
+    constraint::BendingConstraint bending_;
+
+    DeviceArray<math::Vec3f> dx_;
+    topology::Pattern pattern_;
+    engine::PackedBatch batch_;
+    int H_num_block_rows_ = 0;
+    bool topology_dirty_ = true;
+    bool pins_dirty_ = false;
+    float packed_k_[4] = {0.f, 0.f, 0.f, 0.f};
+    int pcg_max_iterations_ = 50;
+    double pcg_tolerance_ = 0.0;
+    int lanes_per_row_ = 8;
+    SolverType solver_type_ = SolverType::PCG;
+};
+
+}  // namespace cloth
+}  // namespace chysx
