@@ -82,6 +82,10 @@ def cpu_baseline(specs, budget_s=12.0):
     }
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +121,7 @@ def main():
     batch.finalize()
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
+    log(f"set up {len(specs)} instances in {setup_s:.1f} s")
     sizes = np.array(batch.instance_sizes(), np.int64)
     n_rows = int(sizes[:, 0].sum())
     dt = specs[0].dt
@@ -130,6 +135,7 @@ def main():
             batch.step(dt, sh)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+    log(f"timed {args.steps} batched substeps: {wall / args.steps * 1e3:.2f} ms each")
     iters = np.array(batch.last_pcg_iterations(sh))
     res = np.array(batch.last_relative_residuals(sh))
     value = n_rows * args.steps / wall
@@ -157,6 +163,7 @@ def main():
                 "kernel": "pcg_kernel<%d> (persistent block-Jacobi PCG + finalize)" % args.lanes,
                 "mean_launch_ms": round(mean_ms, 3), "algorithmic_bytes_per_launch": float(np.mean(pcg_bytes))}
 
+    log(f"PCG launch {mean_ms:.3f} ms, {achieved:.0f} GB/s algorithmic")
     plain = None
     if not args.no_plain:
         # The same instances, one ClothSimulator.step at a time (each packs
@@ -175,6 +182,8 @@ def main():
         plain = {"value": n_rows * reps / pw, "unit": "vertex-substeps/s", "ms_per_substep_all": pw / reps * 1e3,
                  "method": "sequential ClothSimulator.step over the same instances, 2 timed substeps"}
 
+    if plain:
+        log(f"plain sequential: {plain['value']:.3e} vertex-substeps/s")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(specs, args.cpu_budget)
