@@ -79,6 +79,9 @@ typedef struct chx_dih { int v0, v1, v2, v3; int inst; float rest; float k; int 
 /* Number of float64 partial sums kept per row tile by the PCG kernel. */
 #define CHX_TILE_PARTIALS 8
 
+/* 32-bit words of the grid-barrier / control area `sync` of a batch. */
+#define CHX_SYNC_WORDS 320
+
 /* One packed batch of cloth instances: a block-diagonal system whose
  * per-instance blocks have exactly the reference's BSR pattern
  * (ChysX/src/sparse/block_csr.h:59-135).  All pointers are device pointers. */
@@ -122,15 +125,16 @@ typedef struct chx_cloth_batch {
     double* inst_rr;         /* [n_instances] final ||r||^2                       */
     double* inst_bb;         /* [n_instances] ||b||^2                             */
     int* inst_done;          /* [n_instances] scratch                             */
-    unsigned* sync;          /* [8] grid barrier + control words, zeroed once     */
+    unsigned* sync;          /* [CHX_SYNC_WORDS] grid barrier + control words     */
 } chx_cloth_batch;
 
 /* ---- errors and runtime ------------------------------------------------- */
 const char* chx_last_error_message(void);
 /* Number of visible devices; 0 when the HIP runtime finds none. */
 int chx_device_count(int* count);
-/* Compute units and max co-resident PCG workgroups on the current device. */
-int chx_device_info(int* cu_count, int* max_pcg_blocks);
+/* Compute units, the persistent PCG grid cap, and the runtime's occupancy
+ * figure (workgroups per CU) for the PCG kernel on the current device. */
+int chx_device_info(int* cu_count, int* max_pcg_blocks, int* occupancy_blocks_per_cu);
 
 /* Device memory: replaces CudaArray<T> (ChysX/src/memory/cuda_array.h:64-284,
  * cuda_array.cu:41-90). */

@@ -24,7 +24,7 @@ _S = ctypes.c_size_t
 _SIGS = {
     "chx_last_error_message": (ctypes.c_char_p, []),
     "chx_device_count": (_I, [_P]),
-    "chx_device_info": (_I, [_P, _P]),
+    "chx_device_info": (_I, [_P, _P, _P]),
     "chx_malloc": (_I, [_P, _S]),
     "chx_free": (_I, [_P]),
     "chx_memcpy_h2d": (_I, [_P, _P, _S, _U]),
@@ -110,6 +110,7 @@ def pcg_solve_bsr3(n, diag, row_offsets, col_indices, values, b, x, max_iteratio
 
 
 def device_info():
-    cu, mb = ctypes.c_int(), ctypes.c_int()
-    check(lib().chx_device_info(ctypes.byref(cu), ctypes.byref(mb)), "device_info")
-    return cu.value, mb.value
+    """(compute units, persistent PCG grid cap, runtime occupancy per CU)."""
+    cu, mb, occ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib().chx_device_info(ctypes.byref(cu), ctypes.byref(mb), ctypes.byref(occ)), "device_info")
+    return cu.value, mb.value, occ.value

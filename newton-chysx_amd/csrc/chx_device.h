@@ -66,9 +66,11 @@ __device__ __forceinline__ double dot64(f3 a, f3 b) {
 }
 
 // Instance descriptor i of a batch (the inline `single` one when the batch
-// has no device instance table).
+// has no device instance table).  Both cases go through one pointer so the
+// code that reads the descriptor is the same instruction sequence.
 __device__ __forceinline__ const chx_instance& inst_at(const chx_cloth_batch& B, int i) {
-    return B.inst != nullptr ? B.inst[i] : B.single;
+    const chx_instance* p = (B.inst != nullptr) ? B.inst + i : &B.single;
+    return *p;
 }
 __device__ __forceinline__ int row_instance(const chx_cloth_batch& B, int row) {
     return B.row_inst != nullptr ? B.row_inst[row] : 0;
@@ -116,19 +118,51 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double* lds) {
 }
 
 // ---- grid barrier ----------------------------------------------------------
-// sync[0] arrival count, sync[1] generation, sync[2] abort flag,
-// sync[3] active-instance counter, sync[4] status (error) word.
+// Control words, each on its own 256-byte line so that polling the
+// generation does not contend with arrivals:
+//   sync[SYNC_COUNT] arrivals, sync[SYNC_GEN] generation, sync[SYNC_ABORT]
+//   abort flag, sync[SYNC_ACTIVE] active-instance counter, sync[SYNC_STATUS]
+//   status word, then the arrival count seen by a workgroup that gave up and
+//   the grid size.
+enum : int {
+    SYNC_COUNT = 0,
+    SYNC_GEN = 64,
+    SYNC_ABORT = 128,
+    SYNC_ACTIVE = 192,
+    SYNC_STATUS = 256,
+    SYNC_SEEN = 257,
+    SYNC_GRID = 258,
+    SYNC_WORDS = CHX_SYNC_WORDS
+};
+static_assert(SYNC_GRID < CHX_SYNC_WORDS, "sync area too small");
 // The barrier is self-resetting: the last arriving workgroup zeroes the
 // count and bumps the generation, so consecutive launches reuse it.  Every
-// wait is bounded: after kSpinLimit polls a workgroup raises the abort flag
-// and all workgroups leave the kernel, so a broken launch cannot hang.
-constexpr unsigned kSpinLimit = 1u << 22;
+// wait is bounded in time: a workgroup that has waited kWaitTicks of the
+// 100 MHz real-time counter raises the abort flag and all workgroups leave
+// the kernel, so a launch whose workgroups are not all resident ends with an
+// error instead of hanging.  Waiters back off exponentially so ~1000
+// workgroups polling one word do not starve the arrivals.
+constexpr unsigned long long kWaitTicks = 100000000ull;  // 1 s at 100 MHz
 
 __device__ __forceinline__ unsigned ld_acquire(unsigned* p) {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ unsigned ld_relaxed(unsigned* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(unsigned* p, unsigned v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void backoff(unsigned& level) {
+    switch (level) {
+        case 0: __builtin_amdgcn_s_sleep(1); break;
+        case 1: __builtin_amdgcn_s_sleep(2); break;
+        case 2: __builtin_amdgcn_s_sleep(4); break;
+        case 3: __builtin_amdgcn_s_sleep(8); break;
+        default: __builtin_amdgcn_s_sleep(16); break;
+    }
+    if (level < 4) ++level;
 }
 
 __device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks) {
@@ -138,26 +172,29 @@ __device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks) {
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
-        const unsigned gen = ld_relaxed(&sync[1]);
+        const unsigned gen = ld_relaxed(&sync[SYNC_GEN]);
         const unsigned arrived =
-            __hip_atomic_fetch_add(&sync[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&sync[SYNC_COUNT], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == nblocks - 1) {
-            __hip_atomic_store(&sync[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&sync[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            st_relaxed(&sync[SYNC_COUNT], 0u);
+            __hip_atomic_fetch_add(&sync[SYNC_GEN], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            unsigned spins = 0;
-            while (ld_acquire(&sync[1]) == gen) {
-                if (ld_relaxed(&sync[2]) != 0u) {
+            const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+            unsigned level = 0;
+            while (ld_acquire(&sync[SYNC_GEN]) == gen) {
+                if (ld_relaxed(&sync[SYNC_ABORT]) != 0u) {
                     ok = 0;
                     break;
                 }
-                if (++spins > kSpinLimit) {
-                    __hip_atomic_store(&sync[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(&sync[4], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                    st_relaxed(&sync[SYNC_SEEN], ld_relaxed(&sync[SYNC_COUNT]));
+                    st_relaxed(&sync[SYNC_GRID], nblocks);
+                    st_relaxed(&sync[SYNC_ABORT], 1u);
+                    st_relaxed(&sync[SYNC_STATUS], 1u);
                     ok = 0;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(1);
+                backoff(level);
             }
         }
         s_ok = ok;

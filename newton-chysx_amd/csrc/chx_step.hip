@@ -107,10 +107,10 @@ __global__ __launch_bounds__(kBlock) void element_kernel(chx_cloth_batch B) {
 __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
     const long s = long(blockIdx.x) * blockDim.x + threadIdx.x;
     const long n = B.n_rows;
-    if (s == 0) {
+    if (blockIdx.x == 0) {
         // Reset the persistent solver's barrier / control words for the PCG
         // launch that follows on this stream.
-        for (int k = 0; k < 8; ++k) B.sync[k] = 0u;
+        for (int k = threadIdx.x; k < SYNC_WORDS; k += blockDim.x) B.sync[k] = 0u;
     }
     if (s >= n + B.nnz) return;
     const bool is_diag = s < n;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
 }
 
 __global__ void reset_sync_kernel(unsigned* sync) {
-    if (threadIdx.x < 8) sync[threadIdx.x] = 0u;
+    for (int k = threadIdx.x; k < SYNC_WORDS; k += blockDim.x) sync[k] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
 
     // ---- setup -------------------------------------------------------------
     if (blockIdx.x == 0 && threadIdx.x == 0)
-        __hip_atomic_store(&sync[3], unsigned(B.n_instances), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        st_relaxed(&sync[SYNC_ACTIVE], unsigned(B.n_instances));
     for (int t = blockIdx.x; t < B.n_tiles; t += G) {
         int inst, rb, re;
         get_tile(B, t, inst, rb, re);
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                     B.inst_iters[inst] = k;
                     B.inst_rr[inst] = rr;
                     B.inst_bb[inst] = bb;
-                    __hip_atomic_fetch_sub(&sync[3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_sub(&sync[SYNC_ACTIVE], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 continue;
             }
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
             if (threadIdx.x == 0) part[long(CHX_TILE_PARTIALS) * t + P_SIG] = acc[0];
         }
         if (!grid_sync(sync, G)) return;
-        if (ld_acquire(&sync[3]) == 0u) break;
+        if (ld_acquire(&sync[SYNC_ACTIVE]) == 0u) break;
 
         // ---- phase B: alpha; x += alpha p; r -= alpha q; z = M^-1 r; rho', rr' --
         cached = -1;
@@ -457,10 +457,17 @@ struct DeviceLimits {
     int device = -1;
     int cus = 0;
     int max_blocks[17] = {0};
+    int api_blocks[17] = {0};
 };
 
 std::mutex g_lim_mu;
 DeviceLimits g_lim[16];
+
+// Workgroups of the persistent PCG kernel that may be in flight at once.
+// The grid is capped at kBlocksPerCu per compute unit, well under the
+// kernel's register-limited occupancy (7 four-wave workgroups per CU on
+// gfx950), so every workgroup of the cooperative launch is resident.
+constexpr int kBlocksPerCu = 4;
 
 int max_coresident(int L) {
     int dev = 0;
@@ -479,8 +486,9 @@ int max_coresident(int L) {
         hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(pcg_fn(L)),
                                                                kBlock, 0),
                   "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        d.api_blocks[L] = per_cu;
         if (per_cu < 1) per_cu = 1;
-        d.max_blocks[L] = per_cu * d.cus;
+        d.max_blocks[L] = std::min(per_cu, kBlocksPerCu) * d.cus;
     }
     return d.max_blocks[L];
 }
@@ -534,13 +542,14 @@ using namespace chx;
 
 extern "C" {
 
-int chx_device_info(int* cu_count, int* max_pcg_blocks) {
+int chx_device_info(int* cu_count, int* max_pcg_blocks, int* occupancy_blocks_per_cu) {
     return guarded([&] {
         const int mb = max_coresident(8);
         int dev = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         if (cu_count) *cu_count = g_lim[dev & 15].cus;
         if (max_pcg_blocks) *max_pcg_blocks = mb;
+        if (occupancy_blocks_per_cu) *occupancy_blocks_per_cu = g_lim[dev & 15].api_blocks[8];
     });
 }
 
@@ -573,7 +582,7 @@ int chx_pcg_workspace_size(int n_rows, size_t* bytes) {
         const size_t tiles = (n + kTileRows - 1) / kTileRows;
         auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
         *bytes = 5 * al(12 * n) + al(36 * n) + al(8 * CHX_TILE_PARTIALS * tiles) +
-                 al(sizeof(int)) + 2 * al(sizeof(double)) + al(sizeof(int)) + al(8 * sizeof(unsigned));
+                 al(sizeof(int)) + 2 * al(sizeof(double)) + al(sizeof(int)) + al(SYNC_WORDS * sizeof(unsigned));
     });
 }
 
@@ -636,7 +645,7 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
         B.inst_rr = reinterpret_cast<double*>(take(sizeof(double)));
         B.inst_bb = reinterpret_cast<double*>(take(sizeof(double)));
         B.inst_done = reinterpret_cast<int*>(take(sizeof(int)));
-        B.sync = reinterpret_cast<unsigned*>(take(8 * sizeof(unsigned)));
+        B.sync = reinterpret_cast<unsigned*>(take(SYNC_WORDS * sizeof(unsigned)));
         const hipStream_t s = as_stream(stream);
         pcg_fn(lanes_per_row);  // validates L
         reset_sync_kernel<<<1, 64, 0, s>>>(B.sync);
@@ -649,7 +658,7 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
             hip_check(hipMemcpyAsync(&it, B.inst_iters, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(iters)");
             hip_check(hipMemcpyAsync(&r2, B.inst_rr, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(rr)");
             hip_check(hipMemcpyAsync(&b2, B.inst_bb, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(bb)");
-            hip_check(hipMemcpyAsync(&status, B.sync + 4, sizeof(unsigned), hipMemcpyDeviceToHost, s),
+            hip_check(hipMemcpyAsync(&status, B.sync + SYNC_STATUS, sizeof(unsigned), hipMemcpyDeviceToHost, s),
                       "hipMemcpyAsync(status)");
             hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(pcg)");
             if (status != 0u) throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out");
@@ -693,14 +702,16 @@ int chx_cloth_step(const chx_cloth_batch* batch, uintptr_t stream) {
 int chx_batch_check(const chx_cloth_batch* batch, uintptr_t stream) {
     return guarded([&] {
         require(batch != nullptr && batch->sync != nullptr, "chx_batch_check: null batch");
-        unsigned words[8];
+        unsigned words[SYNC_WORDS];
         const hipStream_t s = as_stream(stream);
         hip_check(hipMemcpyAsync(words, batch->sync, sizeof(words), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(sync)");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(check)");
-        if (words[4] != 0u) {
+        if (words[SYNC_STATUS] != 0u) {
             reset_sync_kernel<<<1, 64, 0, s>>>(batch->sync);
             hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(reset)");
-            throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out");
+            throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out (" +
+                                            std::to_string(words[SYNC_SEEN]) + " of " + std::to_string(words[SYNC_GRID]) +
+                                            " workgroups arrived)");
         }
     });
 }

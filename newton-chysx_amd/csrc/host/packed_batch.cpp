@@ -229,7 +229,7 @@ void PackedBatch::build(const std::vector<InstanceSource>& src, int lanes_per_ro
     alloc(inst_done_, std::max(I, 1), device_bytes_);
     alloc(inst_rr_, std::max(I, 1), device_bytes_);
     alloc(inst_bb_, std::max(I, 1), device_bytes_);
-    alloc(sync_, 8, device_bytes_);
+    alloc(sync_, CHX_SYNC_WORDS, device_bytes_);
     sync_.zero_device();
     inst_iters_.zero_device();
     inst_rr_.zero_device();
