@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--autotune", action="store_true", help="sweep lanes per row x grid size first")
+    ap.add_argument("--no-plain-large", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -118,13 +120,20 @@ def main():
     batch.set_lanes_per_row(args.lanes)
     for c in cloths:
         batch.add_instance(c.sim)
-    batch.finalize()
+    batch.finalize(specs[0].dt)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     log(f"set up {len(specs)} instances in {setup_s:.1f} s")
     sizes = np.array(batch.instance_sizes(), np.int64)
     n_rows = int(sizes[:, 0].sum())
     dt = specs[0].dt
+    tuned = None
+    if args.autotune:
+        from chysx import autotune
+
+        tuned = autotune.tune_batch(batch, dt, stream, log=log)
+        args.lanes = tuned["lanes_per_row"]
+        log(f"autotune picked lanes={tuned['lanes_per_row']} grid={tuned['grid_blocks']}")
 
     with torch.cuda.stream(stream):
         for _ in range(args.warmup):
@@ -184,6 +193,24 @@ def main():
 
     if plain:
         log(f"plain sequential: {plain['value']:.3e} vertex-substeps/s")
+    plain_large = None
+    if not args.no_plain_large:
+        # One large instance (256x256 grid, E ~ U[1e4, 1e6], seed 1) through
+        # ClothSimulator.step: the single-instance path at full occupancy.
+        big = scenes.DeviceCloth(scenes.plain_large(), lanes_per_row=args.lanes)
+        with torch.cuda.stream(stream):
+            for _ in range(3):
+                big.sim.step(dt, sh)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = 10
+            for _ in range(reps):
+                big.sim.step(dt, sh)
+            torch.cuda.synchronize()
+            bw = time.perf_counter() - t0
+        plain_large = {"value": big.spec.n * reps / bw, "unit": "vertex-substeps/s", "vertices": big.spec.n,
+                       "ms_per_substep": bw / reps * 1e3, "pcg_iterations": big.sim.last_pcg_iterations(sh)}
+        log(f"plain 256x256: {plain_large['ms_per_substep']:.2f} ms/substep")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(specs, args.cpu_budget)
@@ -208,6 +235,8 @@ def main():
         "pcg_iterations": {"mean": float(iters.mean()), "max": int(iters.max()), "min": int(iters.min())},
         "max_relative_residual": float(res.max()),
         "plain": plain,
+        "plain_large": plain_large,
+        "autotune": tuned,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -48,28 +48,11 @@ enum {
 #define CHX_CONTRIB_DIH 2u
 #define CHX_CONTRIB_MAX_ELEMENT ((1u << 26) - 1u)
 
-/* Per-instance description (device array, one entry per cloth). */
-typedef struct chx_instance {
-    float* pos;             /* external AoS float3 positions (x_n on entry, x_{n+1} on exit) */
-    float* vel;             /* external AoS float3 velocities                                 */
-    const float* inv_mass;  /* external per-vertex inverse mass, or NULL (mass 1 kg)           */
-    int row_begin;          /* first packed row (vertex) of this instance                      */
-    int n_rows;             /* vertex count                                                     */
-    int tile_begin;         /* first row tile of this instance                                  */
-    int tile_end;           /* one past its last row tile                                       */
-    float dt;               /* substep                                                          */
-    float damping;          /* v *= exp(-damping*dt) when > 0                                   */
-    float gx, gy, gz;       /* gravity                                                          */
-    int max_iter;           /* PCG iteration cap                                                */
-    int pad0;
-    double tol;             /* ||r|| <= tol*||b|| stops the solve; <= 0: run max_iter (reference) */
-} chx_instance;
-
 /* Packed element records (device arrays). Vertex indices are local to the
- * element's instance. */
-typedef struct chx_pin { int v; int inst; float tx, ty, tz; float k; } chx_pin;
-typedef struct chx_tri { int v0, v1, v2, inst; float dm00, dm01, dm10, dm11; float kA; } chx_tri;
-typedef struct chx_dih { int v0, v1, v2, v3; int inst; float rest; float k; int pad; } chx_dih;
+ * element's cloth instance. */
+typedef struct chx_pin { int v; int pad; float tx, ty, tz; float k; } chx_pin;
+typedef struct chx_tri { int v0, v1, v2, pad; float dm00, dm01, dm10, dm11; float kA; } chx_tri;
+typedef struct chx_dih { int v0, v1, v2, v3; float rest; float k; int pad0, pad1; } chx_dih;
 
 /* Record sizes (floats) written per element by the element pass. */
 #define CHX_PIN_REC 4
@@ -80,52 +63,81 @@ typedef struct chx_dih { int v0, v1, v2, v3; int inst; float rest; float k; int 
 #define CHX_TILE_PARTIALS 8
 
 /* 32-bit words of the grid-barrier / control area `sync` of a batch. */
-#define CHX_SYNC_WORDS 320
+#define CHX_SYNC_WORDS 448
 
-/* One packed batch of cloth instances: a block-diagonal system whose
- * per-instance blocks have exactly the reference's BSR pattern
- * (ChysX/src/sparse/block_csr.h:59-135).  All pointers are device pointers. */
-typedef struct chx_cloth_batch {
-    int n_instances;
-    int n_rows;              /* total vertices                                   */
-    int n_tiles;             /* row tiles                                        */
-    int nnz;                 /* total off-diagonal blocks                        */
-    int n_pins, n_tris, n_dihs;
-    int lanes_per_row;       /* SpMV lanes per block row: 4, 8 or 16             */
-    int grid_blocks;         /* persistent PCG workgroups (0 = auto)             */
-    int pad0;
-    chx_instance single;     /* the instance when inst == NULL (n_instances == 1);
-                                travels in the kernel arguments, so a caller that
-                                swaps state buffers every step needs no upload   */
-    const chx_instance* inst;/* device instance table, or NULL                    */
-    const int* row_inst;     /* [n_rows] owning instance (NULL: all instance 0)   */
-    const int* tiles;        /* [n_tiles*4] {instance, row_begin, row_end, 0}     */
+/* Rows per PCG row tile (one workgroup pass) and elements / BSR slots per
+ * element-pass and gather-pass work item. */
+#define CHX_TILE_ROWS 256
+#define CHX_ITEM_SIZE 256
+
+/* One cloth instance: every array is the instance's own device memory, with
+ * indices local to the instance, so the off-diagonal pattern is exactly the
+ * reference's (ChysX/src/sparse/block_csr.h:59-135).  Slots of the gather
+ * map: [0, n_rows) diagonal blocks, [n_rows, n_rows + nnz) off-diagonal. */
+typedef struct chx_instance {
+    float* pos;                      /* external AoS float3 positions (x_n in, x_{n+1} out) */
+    float* vel;                      /* external AoS float3 velocities                      */
+    const float* inv_mass;           /* external inverse masses, or NULL (mass 1 kg)         */
+    const int* row_offsets;          /* [n_rows+1]                                           */
+    const int* col_indices;          /* [nnz]                                                */
+    float* diag;                     /* [n_rows*9] row-major 3x3                              */
+    float* values;                   /* [nnz*9]                                              */
     const chx_pin* pins;
-    const chx_tri* tris;
+    const chx_tri* tris;             /* stretch elements, then shear elements                */
     const chx_dih* dihs;
-    float* pin_rec;          /* [n_pins*CHX_PIN_REC]                              */
-    float* tri_rec;          /* [n_tris*CHX_TRI_REC]                              */
-    float* dih_rec;          /* [n_dihs*CHX_DIH_REC]                              */
-    const int* row_offsets;  /* [n_rows+1] packed                                 */
-    const int* col_indices;  /* [nnz] packed (global) column                      */
-    float* diag;             /* [n_rows*9]                                        */
-    float* values;           /* [nnz*9]                                           */
-    const unsigned* contrib_offsets; /* [n_rows+nnz+1] diag slots then off-diag  */
-    const unsigned* contribs;
-    float* rhs;              /* b  [n_rows*3]                                     */
-    float* x;                /* dx, warm start and solution [n_rows*3]            */
+    float* pin_rec;                  /* [n_pins*CHX_PIN_REC]                                 */
+    float* tri_rec;                  /* [n_tris*CHX_TRI_REC]                                 */
+    float* dih_rec;                  /* [n_dihs*CHX_DIH_REC]                                 */
+    const unsigned* contrib_offsets; /* [n_rows+nnz+1]                                       */
+    const unsigned* contribs;        /* gather-map entries                                   */
+    float* rhs;                      /* b [n_rows*3]                                         */
+    float* x;                        /* dx: warm start in, solution out [n_rows*3]           */
     float* r;
     float* z;
     float* p0;
     float* p1;
     float* q;
-    float* minv;             /* [n_rows*9] block-Jacobi preconditioner            */
-    double* partials;        /* [n_tiles*CHX_TILE_PARTIALS]                       */
-    int* inst_iters;         /* [n_instances] PCG iterations of the last solve    */
-    double* inst_rr;         /* [n_instances] final ||r||^2                       */
-    double* inst_bb;         /* [n_instances] ||b||^2                             */
-    int* inst_done;          /* [n_instances] scratch                             */
-    unsigned* sync;          /* [CHX_SYNC_WORDS] grid barrier + control words     */
+    float* minv;                     /* [n_rows*9] block-Jacobi preconditioner               */
+    uint16_t* diag16;                /* BF16 copies of diag / values for the inner SpMV of   */
+    uint16_t* values16;              /* the BF16 precision mode, or NULL                     */
+    int n_rows, nnz, n_pins, n_tris, n_dihs;
+    int tile_begin;                  /* this instance's row tiles in the batch tile list     */
+    int tile_end;
+    int pad0;
+    float dt;                        /* substep                                              */
+    float damping;                   /* v *= exp(-damping*dt) when > 0                       */
+    float gx, gy, gz;                /* gravity                                              */
+    int max_iter;                    /* PCG iteration cap                                    */
+    double tol;                      /* stop at ||r|| <= tol*||b||; <= 0: run max_iter       */
+} chx_instance;
+
+/* A batch of instances advanced together.  Work lists are device int4
+ * arrays: row tiles {instance, row_begin, row_end, 0} (local rows, at most
+ * CHX_TILE_ROWS each, grouped by instance), element items {instance, kind
+ * (0 pin, 1 triangle, 2 dihedral), begin, end} and slot items {instance,
+ * begin, end, 0} (at most CHX_ITEM_SIZE each).  A batch with inst == NULL
+ * holds one instance in `single` (passed by value in the kernel arguments)
+ * and its work lists are implied by the counts. */
+typedef struct chx_cloth_batch {
+    int n_instances;
+    int n_tiles;
+    int n_elem_items;
+    int n_slot_items;
+    int lanes_per_row;               /* SpMV lanes per block row: 4, 8 or 16          */
+    int grid_blocks;                 /* persistent PCG workgroups (0 = auto)          */
+    int bf16_blocks;                 /* 1: inner SpMV on BF16 blocks (FP32 accumulate) */
+    int ir_period;                   /* BF16 mode: true-residual refinement period    */
+    chx_instance single;
+    const chx_instance* inst;        /* device instance table, or NULL                */
+    const int* tiles;
+    const int* elem_items;
+    const int* slot_items;
+    double* partials;                /* [n_tiles*CHX_TILE_PARTIALS]                   */
+    int* inst_iters;                 /* [n_instances] PCG iterations of the last solve */
+    double* inst_rr;                 /* [n_instances] final ||r||^2                   */
+    double* inst_bb;                 /* [n_instances] ||b||^2                         */
+    int* inst_done;                  /* [n_instances] scratch                         */
+    unsigned* sync;                  /* [CHX_SYNC_WORDS] grid barrier + control words */
 } chx_cloth_batch;
 
 /* ---- errors and runtime ------------------------------------------------- */

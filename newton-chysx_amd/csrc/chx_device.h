@@ -61,6 +61,37 @@ __device__ __forceinline__ void inverse9(const float* m, float* r) {
     for (int i = 0; i < 9; ++i) r[i] *= inv;
 }
 
+// BF16 <-> FP32 (round to nearest even; NaN stays NaN).
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<uint16_t>((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<unsigned>(h) << 16); }
+
+// 3x3 block stored as BF16, applied in FP32.
+__device__ __forceinline__ f3 mv9_bf16(const uint16_t* m, f3 v) {
+    float a[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a[i] = bf16_to_f32(m[i]);
+    return mv9(a, v);
+}
+
+// Matrix views for the SpMV: FP32 blocks, or BF16 copies with FP32 math.
+struct Mat32 {
+    const float* d;
+    const float* v;
+    __device__ __forceinline__ f3 diag(long row, f3 x) const { return mv9(d + 9 * row, x); }
+    __device__ __forceinline__ f3 off(long k, f3 x) const { return mv9(v + 9 * k, x); }
+};
+struct Mat16 {
+    const uint16_t* d;
+    const uint16_t* v;
+    __device__ __forceinline__ f3 diag(long row, f3 x) const { return mv9_bf16(d + 9 * row, x); }
+    __device__ __forceinline__ f3 off(long k, f3 x) const { return mv9_bf16(v + 9 * k, x); }
+};
+
 __device__ __forceinline__ double dot64(f3 a, f3 b) {
     return double(a.x) * double(b.x) + double(a.y) * double(b.y) + double(a.z) * double(b.z);
 }
@@ -72,8 +103,64 @@ __device__ __forceinline__ const chx_instance& inst_at(const chx_cloth_batch& B,
     const chx_instance* p = (B.inst != nullptr) ? B.inst + i : &B.single;
     return *p;
 }
-__device__ __forceinline__ int row_instance(const chx_cloth_batch& B, int row) {
-    return B.row_inst != nullptr ? B.row_inst[row] : 0;
+
+// Work-list entries (int4 arrays of the batch, or implied by the counts of
+// the single instance).
+__device__ __forceinline__ void get_tile(const chx_cloth_batch& B, int t, int& inst, int& rb, int& re) {
+    if (B.tiles != nullptr) {
+        const int* p = B.tiles + 4 * t;
+        inst = p[0];
+        rb = p[1];
+        re = p[2];
+    } else {
+        inst = 0;
+        rb = t * CHX_TILE_ROWS;
+        re = min(B.single.n_rows, rb + CHX_TILE_ROWS);
+    }
+}
+
+__device__ __forceinline__ int blocks_of(int n) { return (n + CHX_ITEM_SIZE - 1) / CHX_ITEM_SIZE; }
+
+__device__ __forceinline__ void get_elem_item(const chx_cloth_batch& B, int w, int& inst, int& kind, int& b, int& e) {
+    if (B.elem_items != nullptr) {
+        const int* p = B.elem_items + 4 * w;
+        inst = p[0];
+        kind = p[1];
+        b = p[2];
+        e = p[3];
+        return;
+    }
+    const chx_instance& I = B.single;
+    inst = 0;
+    const int np = blocks_of(I.n_pins), nt = blocks_of(I.n_tris);
+    int n;
+    if (w < np) {
+        kind = 0;
+        n = I.n_pins;
+    } else if (w < np + nt) {
+        kind = 1;
+        w -= np;
+        n = I.n_tris;
+    } else {
+        kind = 2;
+        w -= np + nt;
+        n = I.n_dihs;
+    }
+    b = w * CHX_ITEM_SIZE;
+    e = min(n, b + CHX_ITEM_SIZE);
+}
+
+__device__ __forceinline__ void get_slot_item(const chx_cloth_batch& B, int w, int& inst, int& b, int& e) {
+    if (B.slot_items != nullptr) {
+        const int* p = B.slot_items + 4 * w;
+        inst = p[0];
+        b = p[1];
+        e = p[2];
+        return;
+    }
+    inst = 0;
+    b = w * CHX_ITEM_SIZE;
+    e = min(B.single.n_rows + B.single.nnz, b + CHX_ITEM_SIZE);
 }
 
 // ---- wave64 / workgroup reductions (fixed order => deterministic) ---------
@@ -132,9 +219,10 @@ enum : int {
     SYNC_STATUS = 256,
     SYNC_SEEN = 257,
     SYNC_GRID = 258,
+    SYNC_SNAP = 320,  // two snapshots of SYNC_ACTIVE, 64 words apart
     SYNC_WORDS = CHX_SYNC_WORDS
 };
-static_assert(SYNC_GRID < CHX_SYNC_WORDS, "sync area too small");
+static_assert(SYNC_SNAP + 64 < CHX_SYNC_WORDS, "sync area too small");
 // The barrier is self-resetting: the last arriving workgroup zeroes the
 // count and bumps the generation, so consecutive launches reuse it.  Every
 // wait is bounded in time: a workgroup that has waited kWaitTicks of the
@@ -144,9 +232,6 @@ static_assert(SYNC_GRID < CHX_SYNC_WORDS, "sync area too small");
 // workgroups polling one word do not starve the arrivals.
 constexpr unsigned long long kWaitTicks = 100000000ull;  // 1 s at 100 MHz
 
-__device__ __forceinline__ unsigned ld_acquire(unsigned* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ unsigned ld_relaxed(unsigned* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -165,23 +250,44 @@ __device__ __forceinline__ void backoff(unsigned& level) {
     if (level < 4) ++level;
 }
 
-__device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks) {
+// Fence placement matters on gfx950: each XCD has its own L2, so an
+// agent-scope release writes the L2 back and an agent-scope acquire
+// invalidates it.  The barrier therefore issues exactly one release and one
+// acquire per workgroup, both from thread 0 (the workgroup barrier orders the
+// other waves' accesses around them), arrives with a relaxed atomic, and
+// polls with relaxed loads -- an acquiring poll would invalidate the L2 on
+// every probe.  Each wave first drains its own outstanding stores.  The last
+// arriver acquires the other arrivals before it releases the generation, so
+// every workgroup's writes reach every other one.
+//
+// With snap = 0 or 1 the last arriver also copies SYNC_ACTIVE into snapshot
+// slot `snap` before it releases the generation, and every workgroup reads
+// that slot back into *active (uniform across the workgroup).  The live
+// counter may already be changing in the next phase when a slow workgroup
+// leaves the barrier; the snapshot of this barrier is only rewritten two
+// barriers later, which needs that workgroup's arrival first.
+__device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks, int snap = -1,
+                                          unsigned* active = nullptr) {
     __shared__ int s_ok;
-    // Make this workgroup's global writes visible device-wide.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __shared__ unsigned s_active;
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const unsigned gen = ld_relaxed(&sync[SYNC_GEN]);
         const unsigned arrived =
-            __hip_atomic_fetch_add(&sync[SYNC_COUNT], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&sync[SYNC_COUNT], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == nblocks - 1) {
             st_relaxed(&sync[SYNC_COUNT], 0u);
-            __hip_atomic_fetch_add(&sync[SYNC_GEN], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (snap >= 0) st_relaxed(&sync[SYNC_SNAP + 64 * snap], ld_relaxed(&sync[SYNC_ACTIVE]));
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(&sync[SYNC_GEN], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
             unsigned level = 0;
-            while (ld_acquire(&sync[SYNC_GEN]) == gen) {
+            while (ld_relaxed(&sync[SYNC_GEN]) == gen) {
                 if (ld_relaxed(&sync[SYNC_ABORT]) != 0u) {
                     ok = 0;
                     break;
@@ -196,11 +302,13 @@ __device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks) {
                 }
                 backoff(level);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         s_ok = ok;
+        if (snap >= 0) s_active = ld_relaxed(&sync[SYNC_SNAP + 64 * snap]);
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (active != nullptr) *active = s_active;
     return s_ok != 0;
 }
 

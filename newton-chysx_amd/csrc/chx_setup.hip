@@ -55,40 +55,36 @@ __global__ __launch_bounds__(kBlock) void mass_kernel(const int* __restrict__ tr
     inv_mass[v] = (m > 1.0e-12f) ? 1.0f / m : 0.0f;
 }
 
-// Per-element FP64 energies into `out` (one double per element, ordered
-// pins, triangles, dihedrals).
+// FP64 energy of one element-pass work item per workgroup (fixed-order
+// workgroup sum) into out[item].
 __global__ __launch_bounds__(kBlock) void energy_kernel(chx_cloth_batch B, int mask, double* __restrict__ out) {
-    const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    const long P = B.n_pins, T = B.n_tris, D = B.n_dihs;
-    if (e < P) {
-        double E = 0.0;
-        if (mask & 1) {
-            const chx_pin p = B.pins[e];
-            const f3 d = sub(ld3(inst_at(B, p.inst).pos, p.v), mk(p.tx, p.ty, p.tz));
-            E = double(0.5f * p.k * dot(d, d));  // pin_constraint.cu:40-53
-        }
-        out[e] = E;
-    } else if (e < P + T) {
-        double E = 0.0;
-        if (mask & 2) {
-            const chx_tri t = B.tris[e - P];
-            const float* pos = inst_at(B, t.inst).pos;
-            E = tri_energy(ld3(pos, t.v0), ld3(pos, t.v1), ld3(pos, t.v2), t.dm00, t.dm01, t.dm10, t.dm11, t.kA);
-        }
-        out[e] = E;
-    } else if (e < P + T + D) {
-        double E = 0.0;
-        if (mask & 4) {
-            const chx_dih d = B.dihs[e - P - T];
-            const float* pos = inst_at(B, d.inst).pos;
-            const DihFrame F = dihedral_frame(ld3(pos, d.v0), ld3(pos, d.v1), ld3(pos, d.v2), ld3(pos, d.v3));
+    __shared__ double lds[16];
+    int inst, kind, ib, ie;
+    get_elem_item(B, blockIdx.x, inst, kind, ib, ie);
+    const chx_instance& I = inst_at(B, inst);
+    const int i = ib + threadIdx.x;
+    double E[1] = {0.0};
+    if (i < ie) {
+        if (kind == 0 && (mask & 1)) {
+            const chx_pin p = I.pins[i];
+            const f3 d = sub(ld3(I.pos, p.v), mk(p.tx, p.ty, p.tz));
+            E[0] = double(0.5f * p.k * dot(d, d));  // pin_constraint.cu:40-53
+        } else if (kind == 1 && (mask & 2)) {
+            const chx_tri t = I.tris[i];
+            E[0] = tri_energy(ld3(I.pos, t.v0), ld3(I.pos, t.v1), ld3(I.pos, t.v2), t.dm00, t.dm01, t.dm10, t.dm11,
+                              t.kA);
+        } else if (kind == 2 && (mask & 4)) {
+            const chx_dih d = I.dihs[i];
+            const DihFrame F =
+                dihedral_frame(ld3(I.pos, d.v0), ld3(I.pos, d.v1), ld3(I.pos, d.v2), ld3(I.pos, d.v3));
             if (F.valid) {  // bending_constraint.cu:154-172
-                const float dt = F.theta - d.rest;
-                E = double(0.5f * d.k * dt * dt);
+                const float dth = F.theta - d.rest;
+                E[0] = double(0.5f * d.k * dth * dth);
             }
         }
-        out[e] = E;
     }
+    block_sum(E, lds);
+    if (threadIdx.x == 0) out[blockIdx.x] = E[0];
 }
 
 }  // namespace
@@ -135,14 +131,18 @@ int chx_mass_area_weighted(const int* tris, const float* pos, const int* vertex_
 int chx_batch_energy(const chx_cloth_batch* batch, int kind_mask, double* energy, uintptr_t stream) {
     return guarded([&] {
         require(batch && energy, "chx_batch_energy: null pointer");
-        const long total = long(batch->n_pins) + batch->n_tris + batch->n_dihs;
+        const chx_cloth_batch& B = *batch;
+        auto cnt = [](int n) { return (n + CHX_ITEM_SIZE - 1) / CHX_ITEM_SIZE; };
+        const long total = B.inst != nullptr
+                               ? long(B.n_elem_items)
+                               : long(cnt(B.single.n_pins)) + cnt(B.single.n_tris) + cnt(B.single.n_dihs);
         *energy = 0.0;
         if (total == 0) return;
         double* dev = nullptr;
         hip_check(hipMalloc(&dev, sizeof(double) * total), "hipMalloc(energy)");
         std::vector<double> host(total);
         try {
-            energy_kernel<<<blocks_for(total, kBlock), kBlock, 0, as_stream(stream)>>>(*batch, kind_mask, dev);
+            energy_kernel<<<static_cast<int>(total), kBlock, 0, as_stream(stream)>>>(B, kind_mask, dev);
             hip_check(hipGetLastError(), "energy_kernel launch");
             hip_check(hipMemcpyAsync(host.data(), dev, sizeof(double) * total, hipMemcpyDeviceToHost,
                                      as_stream(stream)),

@@ -16,7 +16,9 @@
 //                  barriers per iteration, per-instance convergence, then the
 //                  finalize x = x_n + dx, v = dx/dt (cloth_simulator.cu:222-244).
 //
-// Row tiles hold at most kTileRows rows of one instance; each tile's partial
+// Every instance owns its arrays (local indices); the batch gives work lists
+// of row tiles (CHX_TILE_ROWS rows of one instance), element items and slot
+// items (CHX_ITEM_SIZE entries of one instance).  Each row tile's partial
 // sums live in batch->partials[tile*8 + field]:
 //   0/1 rho (<r,z>) for even/odd iterations, 2/3 ||r||^2 even/odd,
 //   4 sigma (<p,Ap>), 5 ||b||^2.
@@ -31,22 +33,9 @@ namespace chx {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kTileRows = 256;
+static_assert(CHX_ITEM_SIZE == kBlock, "one thread per work-item entry");
 
 enum { P_RHO0 = 0, P_RHO1 = 1, P_RR0 = 2, P_RR1 = 3, P_SIG = 4, P_BB = 5 };
-
-__device__ __forceinline__ void get_tile(const chx_cloth_batch& B, int t, int& inst, int& rb, int& re) {
-    if (B.tiles != nullptr) {
-        const int* p = B.tiles + 4 * t;
-        inst = p[0];
-        rb = p[1];
-        re = p[2];
-    } else {
-        inst = 0;
-        rb = t * kTileRows;
-        re = min(B.n_rows, rb + kTileRows);
-    }
-}
 
 // Divide guard |den| > 1e-37 (ChysX/src/solver/pcg_solver.cu:122-127), with
 // the FP64 inner products rounded to FP32 once.
@@ -58,32 +47,32 @@ __device__ __forceinline__ float guarded_div(double a, double b) {
 // Element pass
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void element_kernel(chx_cloth_batch B) {
-    const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    const long P = B.n_pins, T = B.n_tris, D = B.n_dihs;
-    if (e < P) {
-        const chx_pin p = B.pins[e];
-        const f3 x = ld3(inst_at(B, p.inst).pos, p.v);
+    int inst, kind, ib, ie;
+    get_elem_item(B, blockIdx.x, inst, kind, ib, ie);
+    const int i = ib + threadIdx.x;
+    if (i >= ie) return;
+    const chx_instance& I = inst_at(B, inst);
+    const float* pos = I.pos;
+    if (kind == 0) {
+        const chx_pin p = I.pins[i];
+        const f3 x = ld3(pos, p.v);
         const f3 g = mul(sub(x, mk(p.tx, p.ty, p.tz)), p.k);  // pin_constraint.cu:61-75
-        float* rec = B.pin_rec + long(CHX_PIN_REC) * e;
+        float* rec = I.pin_rec + long(CHX_PIN_REC) * i;
         rec[0] = g.x;
         rec[1] = g.y;
         rec[2] = g.z;
         rec[3] = p.k;
-    } else if (e < P + T) {
-        const long i = e - P;
-        const chx_tri t = B.tris[i];
-        const float* pos = inst_at(B, t.inst).pos;
+    } else if (kind == 1) {
+        const chx_tri t = I.tris[i];
         float rec[CHX_TRI_REC];
         tri_record(ld3(pos, t.v0), ld3(pos, t.v1), ld3(pos, t.v2), t.dm00, t.dm01, t.dm10, t.dm11, t.kA, rec);
-        float* out = B.tri_rec + long(CHX_TRI_REC) * i;
+        float* out = I.tri_rec + long(CHX_TRI_REC) * i;
 #pragma unroll
         for (int k = 0; k < CHX_TRI_REC; ++k) out[k] = rec[k];
-    } else if (e < P + T + D) {
-        const long i = e - P - T;
-        const chx_dih d = B.dihs[i];
-        const float* pos = inst_at(B, d.inst).pos;
+    } else {
+        const chx_dih d = I.dihs[i];
         const DihFrame F = dihedral_frame(ld3(pos, d.v0), ld3(pos, d.v1), ld3(pos, d.v2), ld3(pos, d.v3));
-        float* out = B.dih_rec + long(CHX_DIH_REC) * i;
+        float* out = I.dih_rec + long(CHX_DIH_REC) * i;
         // An invalid frame contributes nothing (bending_constraint.cu:190-191,227-228).
         const float c = F.valid ? d.k * (F.theta - d.rest) : 0.0f;
         const float k = F.valid ? d.k : 0.0f;
@@ -105,29 +94,32 @@ __global__ __launch_bounds__(kBlock) void element_kernel(chx_cloth_batch B) {
 // Gather (assembly) pass
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
-    const long s = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    const long n = B.n_rows;
     if (blockIdx.x == 0) {
         // Reset the persistent solver's barrier / control words for the PCG
         // launch that follows on this stream.
         for (int k = threadIdx.x; k < SYNC_WORDS; k += blockDim.x) B.sync[k] = 0u;
     }
-    if (s >= n + B.nnz) return;
+    int inst, sb, se;
+    get_slot_item(B, blockIdx.x, inst, sb, se);
+    const int s = sb + threadIdx.x;
+    if (s >= se) return;
+    const chx_instance& I = inst_at(B, inst);
+    const int n = I.n_rows;
     const bool is_diag = s < n;
-    const unsigned c0 = B.contrib_offsets[s], c1 = B.contrib_offsets[s + 1];
+    const unsigned c0 = I.contrib_offsets[s], c1 = I.contrib_offsets[s + 1];
     float h[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     f3 grad = mk(0.f, 0.f, 0.f);
     for (unsigned c = c0; c < c1; ++c) {
-        const unsigned w = B.contribs[c];
+        const unsigned w = I.contribs[c];
         const unsigned kind = w >> 30, a = (w >> 28) & 3u, b = (w >> 26) & 3u, id = w & CHX_CONTRIB_MAX_ELEMENT;
         if (kind == CHX_CONTRIB_PIN) {
-            const float* r = B.pin_rec + long(CHX_PIN_REC) * id;
+            const float* r = I.pin_rec + long(CHX_PIN_REC) * id;
             h[0] += r[3];
             h[4] += r[3];
             h[8] += r[3];
             grad = add(grad, mk(r[0], r[1], r[2]));
         } else if (kind == CHX_CONTRIB_TRI) {
-            const float* r = B.tri_rec + long(CHX_TRI_REC) * id;
+            const float* r = I.tri_rec + long(CHX_TRI_REC) * id;
             const float su = r[18 + a] * r[18 + b];
             const float sv = r[21 + a] * r[21 + b];
             const float m0 = su * r[6] + sv * r[12], m1 = su * r[7] + sv * r[13], m2 = su * r[8] + sv * r[14];
@@ -140,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
                 grad = add(grad, mk(ua * r[0] + va * r[3], ua * r[1] + va * r[4], ua * r[2] + va * r[5]));
             }
         } else {
-            const float* r = B.dih_rec + long(CHX_DIH_REC) * id;
+            const float* r = I.dih_rec + long(CHX_DIH_REC) * id;
             const float k = r[1];
             const f3 ai = mk(r[2 + 3 * a], r[3 + 3 * a], r[4 + 3 * a]);
             const f3 aj = mk(r[2 + 3 * b], r[3 + 3 * b], r[4 + 3 * b]);
@@ -151,13 +143,11 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
         }
     }
     if (is_diag) {
-        const int row = static_cast<int>(s);
-        const chx_instance& I = inst_at(B, row_instance(B, row));
-        const int local = row - I.row_begin;
-        const f3 p = ld3(I.pos, local), v = ld3(I.vel, local);
+        const int row = s;
+        const f3 p = ld3(I.pos, row), v = ld3(I.vel, row);
         float m = 1.0f;  // mass_from_inv_mass_kernel, cloth_simulator.cu:66-77
         if (I.inv_mass != nullptr) {
-            const float w = I.inv_mass[local];
+            const float w = I.inv_mass[row];
             m = (w > 1.0e-12f) ? (1.0f / w) : 1.0e8f;
         }
         const float dt = I.dt;
@@ -166,17 +156,27 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
         const f3 xt = mk(p.x + dt * v.x + dt * dt * I.gx, p.y + dt * v.y + dt * dt * I.gy,
                          p.z + dt * v.z + dt * dt * I.gz);
         const float mi = m * inv_dt2;
-        st3(B.rhs, row, mk(mi * (xt.x - p.x) - grad.x, mi * (xt.y - p.y) - grad.y, mi * (xt.z - p.z) - grad.z));
+        st3(I.rhs, row, mk(mi * (xt.x - p.x) - grad.x, mi * (xt.y - p.y) - grad.y, mi * (xt.z - p.z) - grad.z));
         h[0] += mi;
         h[4] += mi;
         h[8] += mi;
-        float* d = B.diag + 9L * row;
+        float* d = I.diag + 9L * row;
 #pragma unroll
         for (int k = 0; k < 9; ++k) d[k] = h[k];
+        if (I.diag16 != nullptr) {
+            uint16_t* d16 = I.diag16 + 9L * row;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) d16[k] = f32_to_bf16(h[k]);
+        }
     } else {
-        float* d = B.values + 9L * (s - n);
+        float* d = I.values + 9L * (s - n);
 #pragma unroll
         for (int k = 0; k < 9; ++k) d[k] = h[k];
+        if (I.values16 != nullptr) {
+            uint16_t* d16 = I.values16 + 9L * (s - n);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) d16[k] = f32_to_bf16(h[k]);
+        }
     }
 }
 
@@ -189,10 +189,9 @@ __global__ void reset_sync_kernel(unsigned* sync) {
 // src(j) returns the source vector entry j; fin(row, Ax) runs on the first
 // lane of each row after the segmented reduction.
 // ---------------------------------------------------------------------------
-template <int L, class Src, class Fin>
-__device__ __forceinline__ void spmv_rows(const int* __restrict__ ro, const int* __restrict__ ci,
-                                          const float* __restrict__ diag, const float* __restrict__ val, int rb, int re,
-                                          Src src, Fin fin) {
+template <int L, class Mat, class Src, class Fin>
+__device__ __forceinline__ void spmv_rows(const int* __restrict__ ro, const int* __restrict__ ci, Mat mat, int rb,
+                                          int re, Src src, Fin fin) {
     constexpr int RPW = kWave / L;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x >> 6;
@@ -204,8 +203,8 @@ __device__ __forceinline__ void spmv_rows(const int* __restrict__ ro, const int*
         f3 acc = mk(0.f, 0.f, 0.f);
         if (valid) {
             const int beg = ro[row], end = ro[row + 1];
-            if (sub_ == 0) acc = mv9(diag + 9L * row, src(row));
-            for (int k = beg + sub_; k < end; k += L) acc = add(acc, mv9(val + 9L * k, src(ci[k])));
+            if (sub_ == 0) acc = mat.diag(row, src(row));
+            for (int k = beg + sub_; k < end; k += L) acc = add(acc, mat.off(k, src(ci[k])));
         }
         acc.x = seg_sum<L>(acc.x);
         acc.y = seg_sum<L>(acc.y);
@@ -216,10 +215,13 @@ __device__ __forceinline__ void spmv_rows(const int* __restrict__ ro, const int*
 
 // Per-instance sums of up to four partial fields over the instance's tiles,
 // in a fixed order (lane-strided then butterfly), identical in every
-// workgroup.  Result in lds[0..3] for all threads.
+// workgroup.  Result in lds[0..3] for all threads; lds[4] is 1.0 when the
+// instance has already finished its solve (*done set in an earlier phase).
 __device__ __forceinline__ void instance_sums(const double* __restrict__ part, int tb, int te, int f0, int f1, int f2,
-                                              int f3_, double* lds) {
+                                              int f3_, double* lds, int* done) {
     __syncthreads();
+    if (threadIdx.x == 0)
+        lds[4] = __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 1.0 : 0.0;
     if (threadIdx.x < kWave) {
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
         for (int t = tb + int(threadIdx.x); t < te; t += kWave) {
@@ -243,9 +245,16 @@ __device__ __forceinline__ void instance_sums(const double* __restrict__ part, i
     __syncthreads();
 }
 
-__device__ __forceinline__ bool instance_active(const chx_instance& I, int k, double rr, double bb) {
+// `checkable`: the residual norm at hand is one the convergence test may use
+// (always in FP32 mode; in BF16 mode only the true residual after a
+// refinement step).  Finishing is final: once an instance stops, its tiles no
+// longer refresh the parity-indexed partials, so the test is never re-run on
+// them (`finished`).
+__device__ __forceinline__ bool instance_active(const chx_instance& I, int k, double rr, double bb, bool checkable,
+                                                bool finished) {
+    if (finished) return false;
     if (k >= I.max_iter) return false;
-    if (I.tol > 0.0 && rr <= I.tol * I.tol * bb) return false;
+    if (checkable && I.tol > 0.0 && rr <= I.tol * I.tol * bb) return false;
     return true;
 }
 
@@ -258,45 +267,60 @@ __device__ __forceinline__ bool instance_active(const chx_instance& I, int k, do
 // SpMV gathers z_j + beta p_{k-1,j} (double-buffered p), which removes one
 // grid barrier and one pass over p per iteration.
 // ---------------------------------------------------------------------------
+// Setup / refinement step of one tile: r = b - A32 x (true residual with
+// the FP32 matrix), M^-1 = inverse(D) (setup only), z = M^-1 r, p = z; tile
+// partials <r,z>, <r,r> (and <b,b> at setup).
 template <int L>
+__device__ __forceinline__ void residual_tile(const chx_instance& I, int rb, int re, bool setup, float* pdst,
+                                              double (&acc)[3]) {
+    const float* xv = I.x;
+    spmv_rows<L>(
+        I.row_offsets, I.col_indices, Mat32{I.diag, I.values}, rb, re, [&](int j) { return ld3(xv, j); },
+        [&](int row, f3 Ax) {
+            const f3 b = ld3(I.rhs, row);
+            const f3 r = sub(b, Ax);
+            float mi[9];
+            if (setup) {
+                inverse9(I.diag + 9L * row, mi);
+                float* mo = I.minv + 9L * row;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) mo[k] = mi[k];
+            } else {
+                const float* mo = I.minv + 9L * row;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) mi[k] = mo[k];
+            }
+            const f3 z = mv9(mi, r);
+            st3(I.r, row, r);
+            st3(I.z, row, z);
+            st3(pdst, row, z);
+            acc[0] += dot64(r, z);
+            acc[1] += dot64(r, r);
+            acc[2] += dot64(b, b);
+        });
+}
+
+template <int L, bool BF16>
 __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int finalize) {
     __shared__ double s_red[16];
-    __shared__ double s_sc[4];
+    __shared__ double s_sc[5];
     const unsigned G = gridDim.x;
     unsigned* sync = B.sync;
     double* part = B.partials;
-    const int* ro = B.row_offsets;
-    const int* ci = B.col_indices;
-    const float* diag = B.diag;
-    const float* val = B.values;
+    // BF16 mode: every `period` iterations the next iteration starts from the
+    // true residual (FP32 matrix) and restarts the search directions.
+    const int period = BF16 ? max(B.ir_period, 1) : 0;
 
-    // ---- setup -------------------------------------------------------------
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        st_relaxed(&sync[SYNC_ACTIVE], unsigned(B.n_instances));
+    // ---- setup: r = b - A x, M^-1, z = M^-1 r, p = z; rho, ||r||^2, ||b||^2 ----
+    if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&sync[SYNC_ACTIVE], unsigned(B.n_instances));
     for (int t = blockIdx.x; t < B.n_tiles; t += G) {
         int inst, rb, re;
         get_tile(B, t, inst, rb, re);
-        if (threadIdx.x == 0 && t == inst_at(B, inst).tile_begin) B.inst_done[inst] = 0;
+        const chx_instance& I = inst_at(B, inst);
+        if (threadIdx.x == 0 && t == I.tile_begin)
+            __hip_atomic_store(&B.inst_done[inst], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         double acc[3] = {0.0, 0.0, 0.0};
-        const float* xv = B.x;
-        spmv_rows<L>(
-            ro, ci, diag, val, rb, re, [&](int j) { return ld3(xv, j); },
-            [&](int row, f3 Ax) {
-                const f3 b = ld3(B.rhs, row);
-                const f3 r = sub(b, Ax);
-                float mi[9];
-                inverse9(diag + 9L * row, mi);
-                float* mo = B.minv + 9L * row;
-#pragma unroll
-                for (int k = 0; k < 9; ++k) mo[k] = mi[k];
-                const f3 z = mv9(mi, r);
-                st3(B.r, row, r);
-                st3(B.z, row, z);
-                st3(B.p0, row, z);
-                acc[0] += dot64(r, z);
-                acc[1] += dot64(r, r);
-                acc[2] += dot64(b, b);
-            });
+        residual_tile<L>(I, rb, re, true, I.p0, acc);
         block_sum(acc, s_red);
         if (threadIdx.x == 0) {
             double* p = part + long(CHX_TILE_PARTIALS) * t;
@@ -309,8 +333,9 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
 
     for (int k = 0;; ++k) {
         const int cur = k & 1;
-        const float* pprev = cur ? B.p0 : B.p1;  // p_{k-1}
-        float* pk = cur ? B.p1 : B.p0;           // p_k
+        const int j = BF16 ? k % period : k;                // iterations since the last (re)start
+        const bool refine = BF16 && (j == period - 1);     // this iteration computes the true residual
+        const bool checkable = !BF16 || j == 0;
         // ---- phase A: p_k = z + beta p_{k-1}; q = A p_k; sigma = <p_k, q> ----
         int cached = -1;
         for (int t = blockIdx.x; t < B.n_tiles; t += G) {
@@ -318,14 +343,17 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
             get_tile(B, t, inst, rb, re);
             const chx_instance& I = inst_at(B, inst);
             if (inst != cached) {
-                instance_sums(part, I.tile_begin, I.tile_end, P_RHO0 + cur, P_RHO0 + (cur ^ 1), P_RR0 + cur, P_BB,
-                              s_sc);
+                // A refinement step rewrites the rho slot of parity cur^1, so
+                // it does not read it (its beta is not needed).
+                instance_sums(part, I.tile_begin, I.tile_end, P_RHO0 + cur, refine ? P_RHO0 + cur : P_RHO0 + (cur ^ 1),
+                              P_RR0 + cur, P_BB, s_sc, &B.inst_done[inst]);
                 cached = inst;
             }
             const double rho = s_sc[0], rho_prev = s_sc[1], rr = s_sc[2], bb = s_sc[3];
-            if (!instance_active(I, k, rr, bb)) {
-                if (threadIdx.x == 0 && t == I.tile_begin && B.inst_done[inst] == 0) {
-                    B.inst_done[inst] = 1;
+            const bool finished = s_sc[4] != 0.0;
+            if (!instance_active(I, k, rr, bb, checkable, finished)) {
+                if (threadIdx.x == 0 && t == I.tile_begin && !finished) {
+                    __hip_atomic_store(&B.inst_done[inst], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     B.inst_iters[inst] = k;
                     B.inst_rr[inst] = rr;
                     B.inst_bb[inst] = bb;
@@ -333,34 +361,54 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 }
                 continue;
             }
-            double acc[1] = {0.0};
-            const float* z = B.z;
-            float* q = B.q;
-            if (k == 0) {
-                spmv_rows<L>(
-                    ro, ci, diag, val, rb, re, [&](int j) { return ld3(pk, j); },
-                    [&](int row, f3 Ap) {
-                        st3(q, row, Ap);
-                        acc[0] += dot64(ld3(pk, row), Ap);
-                    });
-            } else {
-                const float beta = guarded_div(rho, rho_prev);
-                auto pj = [&](int j) {
-                    const f3 zj = ld3(z, j), pp = ld3(pprev, j);
-                    return mk(fmaf(beta, pp.x, zj.x), fmaf(beta, pp.y, zj.y), fmaf(beta, pp.z, zj.z));
-                };
-                spmv_rows<L>(ro, ci, diag, val, rb, re, pj, [&](int row, f3 Ap) {
-                    const f3 pr = pj(row);
-                    st3(pk, row, pr);
-                    st3(q, row, Ap);
-                    acc[0] += dot64(pr, Ap);
-                });
+            const float* pprev = cur ? I.p0 : I.p1;  // p_{k-1}
+            float* pk = cur ? I.p1 : I.p0;           // p_k
+            if (refine) {
+                // True residual of the current x; p_{k+1} = z goes to the
+                // buffer of parity k+1 (p_{k-1}'s, unused this iteration).
+                double acc[3] = {0.0, 0.0, 0.0};
+                residual_tile<L>(I, rb, re, false, const_cast<float*>(pprev), acc);
+                block_sum(acc, s_red);
+                if (threadIdx.x == 0) {
+                    double* p = part + long(CHX_TILE_PARTIALS) * t;
+                    p[P_RHO0 + (cur ^ 1)] = acc[0];
+                    p[P_RR0 + (cur ^ 1)] = acc[1];
+                }
+                continue;
             }
+            const float* z = I.z;
+            float* q = I.q;
+            double acc[1] = {0.0};
+            auto run = [&](auto mat) {
+                if (j == 0) {
+                    spmv_rows<L>(
+                        I.row_offsets, I.col_indices, mat, rb, re, [&](int jj) { return ld3(pk, jj); },
+                        [&](int row, f3 Ap) {
+                            st3(q, row, Ap);
+                            acc[0] += dot64(ld3(pk, row), Ap);
+                        });
+                } else {
+                    const float beta = guarded_div(rho, rho_prev);
+                    auto pj = [&](int jj) {
+                        const f3 zj = ld3(z, jj), pp = ld3(pprev, jj);
+                        return mk(fmaf(beta, pp.x, zj.x), fmaf(beta, pp.y, zj.y), fmaf(beta, pp.z, zj.z));
+                    };
+                    spmv_rows<L>(I.row_offsets, I.col_indices, mat, rb, re, pj, [&](int row, f3 Ap) {
+                        const f3 pr = pj(row);
+                        st3(pk, row, pr);
+                        st3(q, row, Ap);
+                        acc[0] += dot64(pr, Ap);
+                    });
+                }
+            };
+            if (BF16) run(Mat16{I.diag16, I.values16});
+            else run(Mat32{I.diag, I.values});
             block_sum(acc, s_red);
             if (threadIdx.x == 0) part[long(CHX_TILE_PARTIALS) * t + P_SIG] = acc[0];
         }
-        if (!grid_sync(sync, G)) return;
-        if (ld_acquire(&sync[SYNC_ACTIVE]) == 0u) break;
+        unsigned active = 0;
+        if (!grid_sync(sync, G, cur, &active)) return;
+        if (active == 0u) break;
 
         // ---- phase B: alpha; x += alpha p; r -= alpha q; z = M^-1 r; rho', rr' --
         cached = -1;
@@ -370,23 +418,25 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
             get_tile(B, t, inst, rb, re);
             const chx_instance& I = inst_at(B, inst);
             if (inst != cached) {
-                instance_sums(part, I.tile_begin, I.tile_end, P_RHO0 + cur, P_SIG, P_RR0 + cur, P_BB, s_sc);
+                instance_sums(part, I.tile_begin, I.tile_end, P_RHO0 + cur, P_SIG, P_RR0 + cur, P_BB, s_sc,
+                              &B.inst_done[inst]);
                 cached = inst;
             }
             const double rho = s_sc[0], sigma = s_sc[1], rr = s_sc[2], bb = s_sc[3];
-            if (!instance_active(I, k, rr, bb)) continue;
+            if (refine || !instance_active(I, k, rr, bb, checkable, s_sc[4] != 0.0)) continue;
             const float alpha = guarded_div(rho, sigma);
+            const float* pk = cur ? I.p1 : I.p0;
             double acc[2] = {0.0, 0.0};
             for (int row = rb + int(threadIdx.x); row < re; row += blockDim.x) {
-                const f3 p = ld3(pk, row), qq = ld3(B.q, row);
-                f3 x = ld3(B.x, row), r = ld3(B.r, row);
+                const f3 p = ld3(pk, row), qq = ld3(I.q, row);
+                f3 x = ld3(I.x, row), r = ld3(I.r, row);
                 x = mk(fmaf(alpha, p.x, x.x), fmaf(alpha, p.y, x.y), fmaf(alpha, p.z, x.z));
                 r = mk(fmaf(-alpha, qq.x, r.x), fmaf(-alpha, qq.y, r.y), fmaf(-alpha, qq.z, r.z));
-                const f3 z = mv9(B.minv + 9L * row, r);
-                st3(B.x, row, x);
-                st3(B.r, row, r);
-                st3(B.z, row, z);
-                acc[0] += dot64(r, z);
+                const f3 zz = mv9(I.minv + 9L * row, r);
+                st3(I.x, row, x);
+                st3(I.r, row, r);
+                st3(I.z, row, zz);
+                acc[0] += dot64(r, zz);
                 acc[1] += dot64(r, r);
             }
             block_sum(acc, s_red);
@@ -408,13 +458,12 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
             const float inv_dt = 1.0f / I.dt;
             const float decay = (I.damping > 0.0f) ? expf(-I.damping * I.dt) : 1.0f;
             for (int row = rb + int(threadIdx.x); row < re; row += blockDim.x) {
-                const int local = row - I.row_begin;
-                const f3 d = ld3(B.x, row);
-                const f3 p = ld3(I.pos, local);
-                st3(I.pos, local, add(p, d));
+                const f3 d = ld3(I.x, row);
+                const f3 p = ld3(I.pos, row);
+                st3(I.pos, row, add(p, d));
                 f3 v = mul(d, inv_dt);
                 if (I.damping > 0.0f) v = mul(v, decay);
-                st3(I.vel, local, v);
+                st3(I.vel, row, v);
             }
         }
     }
@@ -432,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const float* __rest
     const int rb = blockIdx.x * RPB;
     const int re = min(n, rb + RPB);
     spmv_rows<L>(
-        ro, ci, diag, val, rb, re, [&](int j) { return ld3(x, j); },
+        ro, ci, Mat32{diag, val}, rb, re, [&](int j) { return ld3(x, j); },
         [&](int row, f3 Ax) {
             if (beta == 0.0f) st3(y, row, mul(Ax, alpha));
             else st3(y, row, add(mul(ld3(y, row), beta), mul(Ax, alpha)));
@@ -444,11 +493,11 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const float* __rest
 // ---------------------------------------------------------------------------
 using PcgFn = void (*)(chx_cloth_batch, int);
 
-PcgFn pcg_fn(int L) {
+PcgFn pcg_fn(int L, bool bf16 = false) {
     switch (L) {
-        case 4: return pcg_kernel<4>;
-        case 8: return pcg_kernel<8>;
-        case 16: return pcg_kernel<16>;
+        case 4: return bf16 ? pcg_kernel<4, true> : pcg_kernel<4, false>;
+        case 8: return bf16 ? pcg_kernel<8, true> : pcg_kernel<8, false>;
+        case 16: return bf16 ? pcg_kernel<16, true> : pcg_kernel<16, false>;
         default: throw Error(CHX_ERR_INVALID_ARGUMENT, "chysx: lanes_per_row must be 4, 8 or 16");
     }
 }
@@ -456,8 +505,8 @@ PcgFn pcg_fn(int L) {
 struct DeviceLimits {
     int device = -1;
     int cus = 0;
-    int max_blocks[17] = {0};
-    int api_blocks[17] = {0};
+    int max_blocks[2][17] = {{0}};
+    int api_blocks[2][17] = {{0}};
 };
 
 std::mutex g_lim_mu;
@@ -469,7 +518,7 @@ DeviceLimits g_lim[16];
 // gfx950), so every workgroup of the cooperative launch is resident.
 constexpr int kBlocksPerCu = 4;
 
-int max_coresident(int L) {
+int max_coresident(int L, bool bf16 = false) {
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     std::lock_guard<std::mutex> lk(g_lim_mu);
@@ -481,23 +530,24 @@ int max_coresident(int L) {
         hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
         d.cus = prop.multiProcessorCount;
     }
-    if (d.max_blocks[L] == 0) {
+    const int m = bf16 ? 1 : 0;
+    if (d.max_blocks[m][L] == 0) {
         int per_cu = 0;
-        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(pcg_fn(L)),
-                                                               kBlock, 0),
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                      &per_cu, reinterpret_cast<const void*>(pcg_fn(L, bf16)), kBlock, 0),
                   "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-        d.api_blocks[L] = per_cu;
+        d.api_blocks[m][L] = per_cu;
         if (per_cu < 1) per_cu = 1;
-        d.max_blocks[L] = std::min(per_cu, kBlocksPerCu) * d.cus;
+        d.max_blocks[m][L] = std::min(per_cu, kBlocksPerCu) * d.cus;
     }
-    return d.max_blocks[L];
+    return d.max_blocks[m][L];
 }
 
 void launch_pcg(const chx_cloth_batch& B, int finalize, hipStream_t stream) {
     if (B.n_tiles <= 0) return;
     const int L = B.lanes_per_row;
-    PcgFn fn = pcg_fn(L);
-    int grid = max_coresident(L);
+    PcgFn fn = pcg_fn(L, B.bf16_blocks != 0);
+    int grid = max_coresident(L, B.bf16_blocks != 0);
     if (B.grid_blocks > 0) grid = std::min(grid, B.grid_blocks);
     grid = std::min(grid, B.n_tiles);
     chx_cloth_batch arg = B;
@@ -509,29 +559,58 @@ void launch_pcg(const chx_cloth_batch& B, int finalize, hipStream_t stream) {
 
 void validate_batch(const chx_cloth_batch* b) {
     require(b != nullptr, "chysx: null batch descriptor");
-    require(b->n_instances >= 0 && b->n_rows >= 0 && b->n_tiles >= 0 && b->nnz >= 0 && b->n_pins >= 0 &&
-                b->n_tris >= 0 && b->n_dihs >= 0,
+    require(b->n_instances >= 0 && b->n_tiles >= 0 && b->n_elem_items >= 0 && b->n_slot_items >= 0,
             "chysx: negative size in batch descriptor");
     require(b->lanes_per_row == 4 || b->lanes_per_row == 8 || b->lanes_per_row == 16,
             "chysx: lanes_per_row must be 4, 8 or 16");
-    if (b->n_rows > 0) {
-        require(b->row_offsets && b->diag && b->rhs && b->x && b->sync, "chysx: null array in batch descriptor");
-        require(b->inst != nullptr || b->n_instances == 1, "chysx: multi-instance batch needs an instance table");
-        require(b->n_instances == 1 || (b->row_inst != nullptr && b->tiles != nullptr),
-                "chysx: multi-instance batch needs row_inst and tiles");
+    require(b->sync != nullptr, "chysx: batch descriptor without a sync area");
+    require(b->bf16_blocks == 0 || b->ir_period >= 1, "chysx: BF16 mode needs ir_period >= 1");
+    if (b->inst == nullptr) {
+        require(b->n_instances == 1, "chysx: a batch without an instance table holds exactly one instance");
+        require(b->tiles == nullptr && b->elem_items == nullptr && b->slot_items == nullptr,
+                "chysx: single-instance work lists are implied");
+        const chx_instance& I = b->single;
+        require(I.n_rows >= 0 && I.nnz >= 0 && I.n_pins >= 0 && I.n_tris >= 0 && I.n_dihs >= 0,
+                "chysx: negative size in instance");
+        if (I.n_rows > 0)
+            require(I.row_offsets && I.diag && I.rhs && I.x && I.r && I.z && I.p0 && I.p1 && I.q && I.minv,
+                    "chysx: null array in instance");
+        if (b->bf16_blocks && I.n_rows > 0)
+            require(I.diag16 && (I.values16 || I.nnz == 0), "chysx: BF16 mode needs the BF16 block copies");
+    } else {
+        require(b->tiles != nullptr || b->n_tiles == 0, "chysx: batch without a tile list");
+        require(b->elem_items != nullptr || b->n_elem_items == 0, "chysx: batch without an element list");
+        require(b->slot_items != nullptr || b->n_slot_items == 0, "chysx: batch without a slot list");
     }
+    if (b->n_tiles > 0) require(b->partials && b->inst_iters && b->inst_rr && b->inst_bb && b->inst_done,
+                                "chysx: batch without solver outputs");
+}
+
+int item_count(int n) { return (n + CHX_ITEM_SIZE - 1) / CHX_ITEM_SIZE; }
+
+// Launch counts: the single instance's lists are implied by its sizes.
+int elem_items(const chx_cloth_batch& B) {
+    if (B.inst != nullptr) return B.n_elem_items;
+    return item_count(B.single.n_pins) + item_count(B.single.n_tris) + item_count(B.single.n_dihs);
+}
+int slot_items(const chx_cloth_batch& B) {
+    if (B.inst != nullptr) return B.n_slot_items;
+    return item_count(B.single.n_rows + B.single.nnz);
 }
 
 void assemble(const chx_cloth_batch& B, hipStream_t stream) {
-    const long ne = long(B.n_pins) + B.n_tris + B.n_dihs;
+    const int ne = elem_items(B);
     if (ne > 0) {
-        element_kernel<<<blocks_for(ne, kBlock), kBlock, 0, stream>>>(B);
+        element_kernel<<<ne, kBlock, 0, stream>>>(B);
         hip_check(hipGetLastError(), "element_kernel launch");
     }
-    const long ns = long(B.n_rows) + B.nnz;
+    const int ns = slot_items(B);
     if (ns > 0) {
-        gather_kernel<<<blocks_for(ns, kBlock), kBlock, 0, stream>>>(B);
+        gather_kernel<<<ns, kBlock, 0, stream>>>(B);
         hip_check(hipGetLastError(), "gather_kernel launch");
+    } else {
+        reset_sync_kernel<<<1, kBlock, 0, stream>>>(B.sync);
+        hip_check(hipGetLastError(), "reset_sync_kernel launch");
     }
 }
 
@@ -549,7 +628,7 @@ int chx_device_info(int* cu_count, int* max_pcg_blocks, int* occupancy_blocks_pe
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         if (cu_count) *cu_count = g_lim[dev & 15].cus;
         if (max_pcg_blocks) *max_pcg_blocks = mb;
-        if (occupancy_blocks_per_cu) *occupancy_blocks_per_cu = g_lim[dev & 15].api_blocks[8];
+        if (occupancy_blocks_per_cu) *occupancy_blocks_per_cu = g_lim[dev & 15].api_blocks[0][8];
     });
 }
 
@@ -579,10 +658,10 @@ int chx_pcg_workspace_size(int n_rows, size_t* bytes) {
     return guarded([&] {
         require(n_rows >= 0 && bytes, "chx_pcg_workspace_size: bad argument");
         const size_t n = size_t(n_rows);
-        const size_t tiles = (n + kTileRows - 1) / kTileRows;
+        const size_t tiles = (n + CHX_TILE_ROWS - 1) / CHX_TILE_ROWS;
         auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
-        *bytes = 5 * al(12 * n) + al(36 * n) + al(8 * CHX_TILE_PARTIALS * tiles) +
-                 al(sizeof(int)) + 2 * al(sizeof(double)) + al(sizeof(int)) + al(SYNC_WORDS * sizeof(unsigned));
+        *bytes = 5 * al(12 * n) + al(36 * n) + al(8 * CHX_TILE_PARTIALS * tiles) + al(sizeof(int)) +
+                 2 * al(sizeof(double)) + al(sizeof(int)) + al(SYNC_WORDS * sizeof(unsigned));
     });
 }
 
@@ -604,7 +683,7 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
         if (chx_pcg_workspace_size(n_rows, &need) != CHX_OK) throw Error(CHX_ERR_RUNTIME, chx_last_error_message());
         require(workspace_bytes >= need, "pcg: workspace too small (see chx_pcg_workspace_size)");
         const size_t n = size_t(n_rows);
-        const int tiles = static_cast<int>((n + kTileRows - 1) / kTileRows);
+        const int tiles = static_cast<int>((n + CHX_TILE_ROWS - 1) / CHX_TILE_ROWS);
         auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
         char* w = static_cast<char*>(workspace);
         auto take = [&](size_t bytes) {
@@ -614,32 +693,27 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
         };
         chx_cloth_batch B{};
         B.n_instances = 1;
-        B.n_rows = n_rows;
         B.n_tiles = tiles;
-        B.nnz = 0;
         B.lanes_per_row = lanes_per_row;
-        B.inst = nullptr;
-        B.row_inst = nullptr;
-        B.single.row_begin = 0;
-        B.single.n_rows = n_rows;
-        B.single.tile_begin = 0;
-        B.single.tile_end = tiles;
-        B.single.dt = 1.0f;
-        B.single.max_iter = max_iterations;
-        B.single.tol = tol;
-        B.tiles = nullptr;
-        B.row_offsets = row_offsets;
-        B.col_indices = col_indices;
-        B.diag = const_cast<float*>(diag);
-        B.values = const_cast<float*>(values);
-        B.rhs = const_cast<float*>(b);
-        B.x = x;
-        B.r = reinterpret_cast<float*>(take(12 * n));
-        B.z = reinterpret_cast<float*>(take(12 * n));
-        B.p0 = reinterpret_cast<float*>(take(12 * n));
-        B.p1 = reinterpret_cast<float*>(take(12 * n));
-        B.q = reinterpret_cast<float*>(take(12 * n));
-        B.minv = reinterpret_cast<float*>(take(36 * n));
+        chx_instance& I = B.single;
+        I.n_rows = n_rows;
+        I.row_offsets = row_offsets;
+        I.col_indices = col_indices;
+        I.diag = const_cast<float*>(diag);
+        I.values = const_cast<float*>(values);
+        I.rhs = const_cast<float*>(b);
+        I.x = x;
+        I.r = reinterpret_cast<float*>(take(12 * n));
+        I.z = reinterpret_cast<float*>(take(12 * n));
+        I.p0 = reinterpret_cast<float*>(take(12 * n));
+        I.p1 = reinterpret_cast<float*>(take(12 * n));
+        I.q = reinterpret_cast<float*>(take(12 * n));
+        I.minv = reinterpret_cast<float*>(take(36 * n));
+        I.tile_begin = 0;
+        I.tile_end = tiles;
+        I.dt = 1.0f;
+        I.max_iter = max_iterations;
+        I.tol = tol;
         B.partials = reinterpret_cast<double*>(take(8 * CHX_TILE_PARTIALS * size_t(tiles)));
         B.inst_iters = reinterpret_cast<int*>(take(sizeof(int)));
         B.inst_rr = reinterpret_cast<double*>(take(sizeof(double)));
@@ -648,7 +722,7 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
         B.sync = reinterpret_cast<unsigned*>(take(SYNC_WORDS * sizeof(unsigned)));
         const hipStream_t s = as_stream(stream);
         pcg_fn(lanes_per_row);  // validates L
-        reset_sync_kernel<<<1, 64, 0, s>>>(B.sync);
+        reset_sync_kernel<<<1, kBlock, 0, s>>>(B.sync);
         hip_check(hipGetLastError(), "reset_sync_kernel launch");
         launch_pcg(B, 0, s);
         if (iterations || rr || bb) {
@@ -680,7 +754,7 @@ int chx_cloth_solve(const chx_cloth_batch* batch, int finalize, uintptr_t stream
     return guarded([&] {
         validate_batch(batch);
         const hipStream_t s = as_stream(stream);
-        reset_sync_kernel<<<1, 64, 0, s>>>(batch->sync);
+        reset_sync_kernel<<<1, kBlock, 0, s>>>(batch->sync);
         hip_check(hipGetLastError(), "reset_sync_kernel launch");
         launch_pcg(*batch, finalize, s);
     });
@@ -690,10 +764,6 @@ int chx_cloth_step(const chx_cloth_batch* batch, uintptr_t stream) {
     return guarded([&] {
         validate_batch(batch);
         const hipStream_t s = as_stream(stream);
-        if (batch->n_rows + batch->nnz == 0) {
-            reset_sync_kernel<<<1, 64, 0, s>>>(batch->sync);
-            hip_check(hipGetLastError(), "reset_sync_kernel launch");
-        }
         assemble(*batch, s);
         launch_pcg(*batch, 1, s);
     });
@@ -707,11 +777,11 @@ int chx_batch_check(const chx_cloth_batch* batch, uintptr_t stream) {
         hip_check(hipMemcpyAsync(words, batch->sync, sizeof(words), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(sync)");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(check)");
         if (words[SYNC_STATUS] != 0u) {
-            reset_sync_kernel<<<1, 64, 0, s>>>(batch->sync);
+            reset_sync_kernel<<<1, kBlock, 0, s>>>(batch->sync);
             hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(reset)");
             throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out (" +
-                                            std::to_string(words[SYNC_SEEN]) + " of " + std::to_string(words[SYNC_GRID]) +
-                                            " workgroups arrived)");
+                                            std::to_string(words[SYNC_SEEN]) + " of " +
+                                            std::to_string(words[SYNC_GRID]) + " workgroups arrived)");
         }
     });
 }

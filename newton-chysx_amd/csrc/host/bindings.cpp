@@ -115,6 +115,8 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("pcg_tolerance", &ClothSimulator::pcg_tolerance)
         .def("set_lanes_per_row", &ClothSimulator::set_lanes_per_row, py::arg("lanes"))
         .def("lanes_per_row", &ClothSimulator::lanes_per_row)
+        .def("set_bf16_blocks", &ClothSimulator::set_bf16_blocks, py::arg("enabled"), py::arg("ir_period") = 10)
+        .def("bf16_blocks", &ClothSimulator::bf16_blocks)
         .def("last_pcg_iterations", &ClothSimulator::last_pcg_iterations, py::arg("cuda_stream") = 0)
         .def("last_relative_residual", &ClothSimulator::last_relative_residual, py::arg("cuda_stream") = 0)
         .def("num_particles", &ClothSimulator::num_particles)
@@ -146,6 +148,7 @@ PYBIND11_MODULE(_chysx_native, m) {
             },
             py::arg("dx"))
         .def("compute_energy", &ClothSimulator::compute_energy, py::arg("kind_mask") = 7)
+        .def("device_bytes", &ClothSimulator::device_bytes)
         .def(
             "debug_slot_tables",
             [](const ClothSimulator& s) {
@@ -188,7 +191,12 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("device_bytes", &chysx::cloth::ClothBatch::device_bytes)
         .def("set_lanes_per_row", &chysx::cloth::ClothBatch::set_lanes_per_row, py::arg("lanes"))
         .def("set_grid_blocks", &chysx::cloth::ClothBatch::set_grid_blocks, py::arg("blocks"))
-        .def("finalize", &chysx::cloth::ClothBatch::finalize)
+        .def("set_bf16_blocks", &chysx::cloth::ClothBatch::set_bf16_blocks, py::arg("enabled"),
+             py::arg("ir_period") = 10)
+        .def("finalize", &chysx::cloth::ClothBatch::finalize, py::arg("dt") = 1.0f / 60.0f)
+        .def("remove_instance", &chysx::cloth::ClothBatch::remove_instance, py::arg("simulator"))
+        .def("clear", &chysx::cloth::ClothBatch::clear)
+        .def("lanes_per_row", &chysx::cloth::ClothBatch::lanes_per_row)
         .def("step", &chysx::cloth::ClothBatch::step, py::arg("dt"), py::arg("cuda_stream") = 0)
         .def("assemble", &chysx::cloth::ClothBatch::assemble, py::arg("dt"), py::arg("cuda_stream") = 0)
         .def("solve", &chysx::cloth::ClothBatch::solve, py::arg("cuda_stream") = 0)

@@ -1,5 +1,6 @@
 #include "cloth_batch.h"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -7,61 +8,73 @@ namespace chysx {
 namespace cloth {
 
 int ClothBatch::add_instance(ClothSimulator& sim) {
+    if (std::find(sims_.begin(), sims_.end(), &sim) != sims_.end())
+        throw std::invalid_argument("ClothBatch::add_instance: simulator already in the batch");
     sims_.push_back(&sim);
-    packed_ = false;
+    bound_ = false;
     return static_cast<int>(sims_.size()) - 1;
 }
 
-void ClothBatch::set_lanes_per_row(int lanes) {
-    if (lanes != 4 && lanes != 8 && lanes != 16)
-        throw std::invalid_argument("ClothBatch::set_lanes_per_row: lanes must be 4, 8 or 16");
-    lanes_ = lanes;
-    batch_.set_lanes_per_row(lanes);
+bool ClothBatch::remove_instance(ClothSimulator& sim) {
+    const auto it = std::find(sims_.begin(), sims_.end(), &sim);
+    if (it == sims_.end()) return false;
+    sims_.erase(it);
+    bound_ = false;
+    return true;
 }
 
-void ClothBatch::set_grid_blocks(int blocks) {
-    grid_blocks_ = blocks < 0 ? 0 : blocks;
-    batch_.set_grid_blocks(grid_blocks_);
+void ClothBatch::clear() noexcept {
+    sims_.clear();
+    bound_ = false;
 }
 
-void ClothBatch::finalize() {
-    if (sims_.empty()) throw std::runtime_error("ClothBatch::finalize: no instances");
-    std::vector<engine::InstanceSource> src;
-    src.reserve(sims_.size());
+std::vector<chx_instance> ClothBatch::descriptors(float dt) {
+    std::vector<chx_instance> ds;
+    ds.reserve(sims_.size());
     for (ClothSimulator* s : sims_) {
-        s->prepare_topology();
-        src.push_back(s->instance_source());
+        ds.push_back(s->prepare_instance(dt));
+        if (bf16_ && ds.back().n_rows > 0 && ds.back().diag16 == nullptr)
+            throw std::invalid_argument(
+                "ClothBatch: BF16 mode needs every instance switched with set_bf16_blocks(true)");
     }
-    batch_.build(src, lanes_);
-    batch_.set_grid_blocks(grid_blocks_);
-    packed_ = true;
+    return ds;
 }
 
-void ClothBatch::prepare_step(float dt) {
-    if (!(dt > 0.0f)) throw std::invalid_argument("ClothBatch::step: dt must be positive");
-    if (!packed_) finalize();
-    for (std::size_t i = 0; i < sims_.size(); ++i) {
-        engine::InstanceSource s = sims_[i]->instance_source();
-        if (s.pos == nullptr || s.vel == nullptr)
-            throw std::runtime_error("ClothBatch::step: every instance needs pos / vel buffers");
-        s.dt = dt;
-        batch_.set_instance(static_cast<int>(i), s);
-    }
+void ClothBatch::finalize(float dt) {
+    if (sims_.empty()) throw std::runtime_error("ClothBatch::finalize: no instances");
+    runner_.bind_many(descriptors(dt));
+    bound_ = true;
 }
 
 void ClothBatch::step(float dt, std::uintptr_t cuda_stream) {
-    prepare_step(dt);
-    batch_.step(cuda_stream);
+    if (!(dt > 0.0f)) throw std::invalid_argument("ClothBatch::step: dt must be positive");
+    if (!bound_) finalize(dt);
+    else runner_.update_many(descriptors(dt), cuda_stream);
+    runner_.step(cuda_stream);
 }
 
 void ClothBatch::assemble(float dt, std::uintptr_t cuda_stream) {
-    prepare_step(dt);
-    batch_.assemble(cuda_stream);
+    if (!(dt > 0.0f)) throw std::invalid_argument("ClothBatch::assemble: dt must be positive");
+    if (!bound_) finalize(dt);
+    else runner_.update_many(descriptors(dt), cuda_stream);
+    runner_.assemble(cuda_stream);
 }
 
 void ClothBatch::solve(std::uintptr_t cuda_stream) {
-    if (!packed_) throw std::runtime_error("ClothBatch::solve: call assemble() first");
-    batch_.solve(true, cuda_stream);
+    if (!bound_) throw std::runtime_error("ClothBatch::solve: call assemble() first");
+    runner_.solve(true, cuda_stream);
+}
+
+int ClothBatch::num_rows() const noexcept {
+    int n = 0;
+    for (const ClothSimulator* s : sims_) n += s->buffers().particle_count();
+    return n;
+}
+
+std::size_t ClothBatch::device_bytes() const noexcept {
+    std::size_t b = 0;
+    for (const ClothSimulator* s : sims_) b += s->device_bytes();
+    return b;
 }
 
 std::vector<std::pair<int, int>> ClothBatch::instance_sizes() const {
@@ -73,15 +86,15 @@ std::vector<std::pair<int, int>> ClothBatch::instance_sizes() const {
 std::vector<int> ClothBatch::last_pcg_iterations(std::uintptr_t cuda_stream) {
     std::vector<int> it;
     std::vector<double> rr, bb;
-    if (packed_) batch_.read_stats(it, rr, bb, cuda_stream);
+    if (bound_) runner_.read_stats(it, rr, bb, cuda_stream);
     return it;
 }
 
 std::vector<double> ClothBatch::last_relative_residuals(std::uintptr_t cuda_stream) {
     std::vector<int> it;
     std::vector<double> rr, bb, out;
-    if (!packed_) return out;
-    batch_.read_stats(it, rr, bb, cuda_stream);
+    if (!bound_) return out;
+    runner_.read_stats(it, rr, bb, cuda_stream);
     out.resize(rr.size());
     for (std::size_t i = 0; i < rr.size(); ++i) out[i] = bb[i] > 0.0 ? std::sqrt(rr[i] / bb[i]) : std::sqrt(rr[i]);
     return out;

@@ -4,16 +4,21 @@
 // ClothSimulator per cloth (ChysX/scenes/cloth_freefall.cu:165-170).
 //
 // Each instance is a fully set-up ClothSimulator (buffers, mesh, element
-// sets, pins, material, PCG knobs); the batch packs them block-diagonally.
-// Per-instance results are identical to stepping that simulator alone:
-// every reduction is per instance and in a fixed order.
+// sets, pins, material, PCG knobs) whose device arrays stay where they are;
+// the batch only holds a table of their descriptors and the work lists.
+// Per-instance results are identical to stepping each simulator alone: the
+// same kernels run on the same arrays and every reduction is per instance,
+// over the same tiles, in the same order.  Instances can be added and
+// removed between substeps (continuous batching) at the cost of rebuilding
+// the table.
 #pragma once
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "cloth_simulator.h"
-#include "packed_batch.h"
+#include "engine.h"
 
 namespace chysx {
 namespace cloth {
@@ -24,38 +29,46 @@ public:
     ClothBatch(const ClothBatch&) = delete;
     ClothBatch& operator=(const ClothBatch&) = delete;
 
-    // Registers a simulator (kept by reference; it must outlive the batch).
-    // Returns its instance index.  Invalidates the packing.
+    // Registers a simulator (kept by reference; it must outlive its
+    // membership).  Returns its position.
     int add_instance(ClothSimulator& sim);
-    // Builds every instance's topology and packs the batch (done lazily by
-    // step() when needed).
-    void finalize();
+    // Removes a simulator (by identity); returns false when absent.
+    bool remove_instance(ClothSimulator& sim);
+    void clear() noexcept;
+    // Builds every instance's device arrays and binds the table (done
+    // lazily by step() after membership changes).
+    void finalize(float dt = 1.0f / 60.0f);
     // One substep of every instance; asynchronous on the stream.
     void step(float dt, std::uintptr_t cuda_stream = 0);
     // The two halves of step(): element + gather assembly, then the
     // persistent PCG solve with the fused finalize.
     void assemble(float dt, std::uintptr_t cuda_stream = 0);
     void solve(std::uintptr_t cuda_stream = 0);
-    // (vertices, off-diagonal blocks) of every instance.
-    std::vector<std::pair<int, int>> instance_sizes() const;
 
     int num_instances() const noexcept { return static_cast<int>(sims_.size()); }
-    int num_rows() const noexcept { return packed_ ? batch_.n_rows() : 0; }
-    std::size_t device_bytes() const noexcept { return packed_ ? batch_.device_bytes() : 0; }
-    void set_lanes_per_row(int lanes);
-    void set_grid_blocks(int blocks);
+    int num_rows() const noexcept;
+    std::size_t device_bytes() const noexcept;
+    // (vertices, off-diagonal blocks) of every instance.
+    std::vector<std::pair<int, int>> instance_sizes() const;
+    void set_lanes_per_row(int lanes) { runner_.set_lanes_per_row(lanes); }
+    int lanes_per_row() const noexcept { return runner_.lanes_per_row(); }
+    void set_grid_blocks(int blocks) { runner_.set_grid_blocks(blocks); }
+    // BF16 precision mode for the whole batch; every instance must have been
+    // switched with ClothSimulator::set_bf16_blocks(true) as well.
+    void set_bf16_blocks(bool on, int ir_period = 10) {
+        bf16_ = on;
+        runner_.set_bf16(on, ir_period);
+    }
     std::vector<int> last_pcg_iterations(std::uintptr_t cuda_stream = 0);
     std::vector<double> last_relative_residuals(std::uintptr_t cuda_stream = 0);
-    engine::PackedBatch& packed() noexcept { return batch_; }
 
 private:
-    void prepare_step(float dt);
+    std::vector<chx_instance> descriptors(float dt);
 
     std::vector<ClothSimulator*> sims_;
-    engine::PackedBatch batch_;
-    bool packed_ = false;
-    int lanes_ = 8;
-    int grid_blocks_ = 0;
+    engine::BatchRunner runner_;
+    bool bound_ = false;
+    bool bf16_ = false;
 };
 
 }  // namespace cloth

@@ -159,7 +159,15 @@ void ClothSimulator::set_lanes_per_row(int lanes) {
     if (lanes != 4 && lanes != 8 && lanes != 16)
         throw std::invalid_argument("ClothSimulator::set_lanes_per_row: lanes must be 4, 8 or 16");
     lanes_per_row_ = lanes;
-    batch_.set_lanes_per_row(lanes);
+    runner_.set_lanes_per_row(lanes);
+}
+
+void ClothSimulator::set_bf16_blocks(bool on, int ir_period) {
+    if (ir_period < 1) throw std::invalid_argument("ClothSimulator::set_bf16_blocks: ir_period must be >= 1");
+    if (on != bf16_) topology_dirty_ = true;  // the block copies are (de)allocated with the instance
+    bf16_ = on;
+    ir_period_ = ir_period;
+    runner_.set_bf16(on, ir_period);
 }
 
 // cloth_simulator.cu:795-812: dx (the PCG warm start) is zeroed only when
@@ -191,7 +199,10 @@ void ClothSimulator::ensure_hessian_topology() {
     const int N = buffers_.particle_count();
     if (N <= 0) return;
     prepare_topology();
-    batch_.build({source()}, lanes_per_row_, reinterpret_cast<float*>(dx_.gpu_data()));
+    inst_.build(source());
+    runner_.set_lanes_per_row(lanes_per_row_);
+    runner_.set_bf16(bf16_, ir_period_);
+    runner_.bind_single(inst_.descriptor(source()));
     packed_k_[0] = pins_.stiffness();
     packed_k_[1] = fem_stretch_.stiffness();
     packed_k_[2] = fem_shear_.stiffness();
@@ -208,6 +219,7 @@ engine::InstanceSource ClothSimulator::source() const {
     s.vel = reinterpret_cast<float*>(buffers_.vel.data());
     s.inv_mass = buffers_.inv_mass.data();
     s.pattern = &pattern_;
+    s.x = reinterpret_cast<float*>(dx_.gpu_data());
     s.pins = &pins_;
     s.stretch = &fem_stretch_;
     s.shear = &fem_shear_;
@@ -218,6 +230,7 @@ engine::InstanceSource ClothSimulator::source() const {
     s.gz = material_.gz;
     s.max_iter = pcg_max_iterations_;
     s.tol = pcg_tolerance_;
+    s.bf16_blocks = bf16_;
     return s;
 }
 
@@ -227,7 +240,7 @@ void ClothSimulator::check_elements_current() {
     const bool k_changed = packed_k_[0] != pins_.stiffness() || packed_k_[1] != fem_stretch_.stiffness() ||
                            packed_k_[2] != fem_shear_.stiffness() || packed_k_[3] != bending_.stiffness();
     if (k_changed || pins_dirty_) {
-        batch_.update_elements({source()});
+        inst_.update_elements(source());
         packed_k_[0] = pins_.stiffness();
         packed_k_[1] = fem_stretch_.stiffness();
         packed_k_[2] = fem_shear_.stiffness();
@@ -252,16 +265,29 @@ void ClothSimulator::step(float dt, std::uintptr_t cuda_stream) {
     else check_elements_current();
     engine::InstanceSource s = source();
     s.dt = dt;
-    batch_.set_instance(0, s);
-    batch_.desc().x = reinterpret_cast<float*>(dx_.gpu_data());
-    batch_.step(cuda_stream);
+    runner_.update_single(inst_.descriptor(s));
+    runner_.step(cuda_stream);
+}
+
+chx_instance ClothSimulator::prepare_instance(float dt) {
+    const int n = buffers_.particle_count();
+    if (n <= 0) throw std::runtime_error("ClothSimulator::prepare_instance: no particles bound");
+    if (buffers_.pos.data() == nullptr || buffers_.vel.data() == nullptr)
+        throw std::runtime_error("ClothSimulator::prepare_instance: pos / vel must be set via set_external_buffers()");
+    if (!(dt > 0.0f)) throw std::invalid_argument("ClothSimulator::prepare_instance: dt must be positive");
+    resize_work_buffers(n);
+    if (topology_dirty_ || H_num_block_rows_ != n) ensure_hessian_topology();
+    else check_elements_current();
+    engine::InstanceSource s = source();
+    s.dt = dt;
+    return inst_.descriptor(s);
 }
 
 int ClothSimulator::last_pcg_iterations(std::uintptr_t cuda_stream) {
     if (H_num_block_rows_ == 0) return 0;
     std::vector<int> it;
     std::vector<double> rr, bb;
-    batch_.read_stats(it, rr, bb, cuda_stream);
+    runner_.read_stats(it, rr, bb, cuda_stream);
     return it[0];
 }
 
@@ -269,7 +295,7 @@ double ClothSimulator::last_relative_residual(std::uintptr_t cuda_stream) {
     if (H_num_block_rows_ == 0) return 0.0;
     std::vector<int> it;
     std::vector<double> rr, bb;
-    batch_.read_stats(it, rr, bb, cuda_stream);
+    runner_.read_stats(it, rr, bb, cuda_stream);
     return bb[0] > 0.0 ? std::sqrt(rr[0] / bb[0]) : std::sqrt(rr[0]);
 }
 
@@ -278,7 +304,7 @@ void ClothSimulator::debug_copy_hessian_diag(float* out) const {
     if (n == 0) return;
     if (out == nullptr) throw std::invalid_argument("ClothSimulator::debug_copy_hessian_diag: out must be non-null");
     check(chx_device_synchronize(), "cloth", "device synchronize(debug_copy_hessian_diag)");
-    batch_.copy_diag(out, 0, n);
+    check(chx_memcpy_d2h(out, inst_.diag(), sizeof(float) * 9 * n, 0), "cloth", "diag copy");
 }
 
 void ClothSimulator::debug_copy_hessian_csr(int* out_row_offsets, int* out_col_indices, float* out_values) const {
@@ -294,7 +320,7 @@ void ClothSimulator::debug_copy_hessian_csr(int* out_row_offsets, int* out_col_i
             throw std::invalid_argument(
                 "ClothSimulator::debug_copy_hessian_csr: col_indices/values null but nnz_off > 0");
         std::memcpy(out_col_indices, pattern_.col_indices.data(), sizeof(int) * nnz);
-        batch_.copy_values(out_values, 0, nnz);
+        check(chx_memcpy_d2h(out_values, inst_.values(), sizeof(float) * 9 * nnz, 0), "cloth", "values copy");
     }
 }
 
@@ -303,7 +329,7 @@ void ClothSimulator::debug_copy_last_rhs(float* out) const {
     if (n == 0) return;
     if (out == nullptr) throw std::invalid_argument("ClothSimulator::debug_copy_last_rhs: out must be non-null");
     check(chx_device_synchronize(), "cloth", "device synchronize(debug_copy_last_rhs)");
-    batch_.copy_rhs(out, 0, n);
+    check(chx_memcpy_d2h(out, inst_.rhs(), sizeof(float) * 3 * n, 0), "cloth", "rhs copy");
 }
 
 void ClothSimulator::debug_copy_last_dx(float* out) const {
@@ -328,11 +354,8 @@ double ClothSimulator::compute_energy(int kind_mask) {
     resize_work_buffers(n);
     if (topology_dirty_ || H_num_block_rows_ != n) ensure_hessian_topology();
     else check_elements_current();
-    engine::InstanceSource s = source();
-    batch_.set_instance(0, s);
-    double e = 0.0;
-    check(chx_batch_energy(&batch_.desc(), kind_mask, &e, 0), "cloth", "energy");
-    return e;
+    runner_.update_single(inst_.descriptor(source()));
+    return runner_.energy(kind_mask, 0);
 }
 
 }  // namespace cloth

@@ -17,7 +17,7 @@
 
 #include "chysx_common.h"
 #include "constraints.h"
-#include "packed_batch.h"
+#include "engine.h"
 #include "topology.h"
 
 namespace chysx {
@@ -104,6 +104,13 @@ public:
     double pcg_tolerance() const noexcept { return pcg_tolerance_; }
     void set_lanes_per_row(int lanes);
     int lanes_per_row() const noexcept { return lanes_per_row_; }
+    // Precision of the solve: false = FP32 Hessian blocks (default); true =
+    // BF16 copies of the blocks in the inner SpMV (FP32 accumulation) with
+    // the true residual (FP32 blocks, FP64 norms) recomputed and the
+    // iteration restarted every `ir_period` iterations.
+    void set_bf16_blocks(bool on, int ir_period = 10);
+    bool bf16_blocks() const noexcept { return bf16_; }
+    int ir_period() const noexcept { return ir_period_; }
     int last_pcg_iterations(std::uintptr_t cuda_stream = 0);
     double last_relative_residual(std::uintptr_t cuda_stream = 0);
 
@@ -121,10 +128,13 @@ public:
     double compute_energy(int kind_mask = 7);
     const topology::Pattern& pattern() const noexcept { return pattern_; }
 
-    // Host-only part of the topology build (pattern + slot tables), used by
-    // ClothBatch to pack this simulator as one instance of a batch.
+    // Host part of the topology build (pattern + slot tables).
     void prepare_topology();
-    engine::InstanceSource instance_source() const { return source(); }
+    // Builds (or refreshes) this cloth's device arrays without stepping and
+    // returns the launch descriptor for substep `dt`; ClothBatch uses it to
+    // advance this cloth as one instance of a batch.
+    chx_instance prepare_instance(float dt);
+    std::size_t device_bytes() const noexcept { return inst_.device_bytes() + dx_.gpu_size() * sizeof(math::Vec3f); }
 
 private:
     void resize_work_buffers(int n);
@@ -142,7 +152,8 @@ private:
 
     DeviceArray<math::Vec3f> dx_;
     topology::Pattern pattern_;
-    engine::PackedBatch batch_;
+    engine::InstanceData inst_;
+    engine::BatchRunner runner_;
     int H_num_block_rows_ = 0;
     bool topology_dirty_ = true;
     bool pins_dirty_ = false;
@@ -150,6 +161,8 @@ private:
     int pcg_max_iterations_ = 50;
     double pcg_tolerance_ = 0.0;
     int lanes_per_row_ = 8;
+    bool bf16_ = false;
+    int ir_period_ = 10;
     SolverType solver_type_ = SolverType::PCG;
 };
 

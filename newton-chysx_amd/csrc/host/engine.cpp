@@ -1,0 +1,363 @@
+#include "engine.h"
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+
+namespace chysx {
+namespace engine {
+
+namespace {
+
+unsigned encode(unsigned kind, int a, int b, std::size_t id) {
+    if (id > CHX_CONTRIB_MAX_ELEMENT)
+        throw std::runtime_error("chysx::engine: element index exceeds the 26-bit gather-map field");
+    return (kind << 30) | (unsigned(a) << 28) | (unsigned(b) << 26) | static_cast<unsigned>(id);
+}
+
+template <class T> void upload(DeviceArray<T>& a, std::size_t& bytes) {
+    a.copy_to_device();
+    bytes += a.gpu_size() * sizeof(T);
+}
+template <class T> void alloc(DeviceArray<T>& a, std::size_t n, std::size_t& bytes) {
+    a.ensure_device(n);
+    bytes += n * sizeof(T);
+}
+
+int item_count(int n) { return (n + CHX_ITEM_SIZE - 1) / CHX_ITEM_SIZE; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// InstanceData
+// ---------------------------------------------------------------------------
+
+void InstanceData::pack_elements(const InstanceSource& s) {
+    auto& P = pins_.host();
+    auto& T = tris_.host();
+    auto& D = dihs_.host();
+    P.clear();
+    T.clear();
+    D.clear();
+    if (s.pins) {
+        const float k = s.pins->stiffness();
+        for (int c = 0; c < s.pins->size(); ++c) {
+            const auto& t = s.pins->targets()[c];
+            P.push_back(chx_pin{s.pins->indices()[c], 0, t.x, t.y, t.z, k});
+        }
+    }
+    for (const constraint::TriangleStretchConstraint* ts : {s.stretch, s.shear}) {
+        if (!ts) continue;
+        const float k = ts->stiffness();
+        for (int t = 0; t < ts->size(); ++t) {
+            const auto& v = ts->indices()[t];
+            const auto& dm = ts->Dm_inv()[t].data;
+            // k_eff = areas[t] * k (triangle_stretch_constraint.cu:193).
+            T.push_back(chx_tri{v.x, v.y, v.z, 0, dm[0], dm[1], dm[2], dm[3], ts->areas()[t] * k});
+        }
+    }
+    if (s.bending) {
+        const float k = s.bending->stiffness();
+        for (int e = 0; e < s.bending->size(); ++e) {
+            const auto& v = s.bending->indices()[e];
+            D.push_back(chx_dih{v.x, v.y, v.z, v.w, s.bending->rest_angles()[e], k, 0, 0});
+        }
+    }
+}
+
+void InstanceData::build(const InstanceSource& s) {
+    if (!s.pattern || s.pattern->n != s.n) throw std::invalid_argument("chysx::engine: pattern missing or sized wrong");
+    bytes_ = 0;
+    n_ = s.n;
+    nnz_ = s.pattern->nnz();
+    row_offsets_.host() = s.pattern->row_offsets;
+    col_indices_.host() = s.pattern->col_indices;
+    pack_elements(s);
+
+    // Gather map.  Slots: diagonal block of row r -> r, off-diagonal block k
+    // -> n + k.  Entries are appended in element order (pins, stretch,
+    // shear, bending; element; a; b) — the order of the reference's launches
+    // — so every slot sums its terms in that order.
+    const std::size_t S = static_cast<std::size_t>(n_) + nnz_;
+    auto slot_of = [&](int slot) -> std::size_t {
+        return slot < 0 ? static_cast<std::size_t>(-slot - 1) : static_cast<std::size_t>(n_) + slot;
+    };
+    auto walk = [&](auto&& visit) {
+        if (s.pins) {
+            const auto& sl = s.pins->hessian_slots();
+            if (static_cast<int>(sl.size()) != s.pins->size()) throw std::runtime_error("chysx::engine: pin slots not bound");
+            for (int c = 0; c < s.pins->size(); ++c) visit(slot_of(sl[c]), CHX_CONTRIB_PIN, 0, 0, std::size_t(c));
+        }
+        std::size_t tri_base = 0;
+        for (const constraint::TriangleStretchConstraint* ts : {s.stretch, s.shear}) {
+            if (!ts) continue;
+            const auto& sl = ts->hessian_slots();
+            if (sl.size() != 9u * ts->size()) throw std::runtime_error("chysx::engine: triangle slots not bound");
+            for (int t = 0; t < ts->size(); ++t)
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) visit(slot_of(sl[9 * t + 3 * a + b]), CHX_CONTRIB_TRI, a, b, tri_base + t);
+            tri_base += ts->size();
+        }
+        if (s.bending) {
+            const auto& sl = s.bending->hessian_slots();
+            if (sl.size() != 16u * s.bending->size()) throw std::runtime_error("chysx::engine: dihedral slots not bound");
+            for (int e = 0; e < s.bending->size(); ++e)
+                for (int a = 0; a < 4; ++a)
+                    for (int b = 0; b < 4; ++b)
+                        visit(slot_of(sl[16 * e + 4 * a + b]), CHX_CONTRIB_DIH, a, b, std::size_t(e));
+        }
+    };
+    std::vector<unsigned long long> count(S + 1, 0);
+    walk([&](std::size_t g, unsigned, int, int, std::size_t) { ++count[g + 1]; });
+    for (std::size_t k = 0; k < S; ++k) count[k + 1] += count[k];
+    if (count[S] > std::numeric_limits<unsigned>::max())
+        throw std::runtime_error("chysx::engine: gather map exceeds 2^32 entries");
+    auto& off = contrib_offsets_.host();
+    off.resize(S + 1);
+    for (std::size_t k = 0; k <= S; ++k) off[k] = static_cast<unsigned>(count[k]);
+    auto& ent = contribs_.host();
+    ent.assign(count[S], 0u);
+    std::vector<unsigned> fill(off.begin(), off.end() - 1);
+    walk([&](std::size_t g, unsigned kind, int a, int b, std::size_t id) { ent[fill[g]++] = encode(kind, a, b, id); });
+
+    upload(row_offsets_, bytes_);
+    upload(col_indices_, bytes_);
+    upload(contrib_offsets_, bytes_);
+    upload(contribs_, bytes_);
+    upload(pins_, bytes_);
+    upload(tris_, bytes_);
+    upload(dihs_, bytes_);
+    const std::size_t n = static_cast<std::size_t>(n_);
+    alloc(pin_rec_, pins_.cpu_size() * CHX_PIN_REC, bytes_);
+    alloc(tri_rec_, tris_.cpu_size() * CHX_TRI_REC, bytes_);
+    alloc(dih_rec_, dihs_.cpu_size() * CHX_DIH_REC, bytes_);
+    alloc(diag_, 9 * n, bytes_);
+    alloc(values_, 9 * static_cast<std::size_t>(nnz_), bytes_);
+    alloc(rhs_, 3 * n, bytes_);
+    alloc(r_, 3 * n, bytes_);
+    alloc(z_, 3 * n, bytes_);
+    alloc(p0_, 3 * n, bytes_);
+    alloc(p1_, 3 * n, bytes_);
+    alloc(q_, 3 * n, bytes_);
+    alloc(minv_, 9 * n, bytes_);
+    if (s.bf16_blocks) {
+        alloc(diag16_, 9 * n, bytes_);
+        alloc(values16_, 9 * static_cast<std::size_t>(nnz_), bytes_);
+    } else {
+        diag16_.release();
+        values16_.release();
+    }
+    built_ = true;
+}
+
+void InstanceData::update_elements(const InstanceSource& s) {
+    const std::size_t nP = pins_.cpu_size(), nT = tris_.cpu_size(), nD = dihs_.cpu_size();
+    pack_elements(s);
+    if (pins_.cpu_size() != nP || tris_.cpu_size() != nT || dihs_.cpu_size() != nD)
+        throw std::runtime_error("chysx::engine: element counts changed; rebuild the instance");
+    pins_.copy_to_device();
+    tris_.copy_to_device();
+    dihs_.copy_to_device();
+}
+
+chx_instance InstanceData::descriptor(const InstanceSource& s) const {
+    chx_instance I;
+    std::memset(&I, 0, sizeof(I));  // padding too: descriptors are compared bytewise
+    I.pos = s.pos;
+    I.vel = s.vel;
+    I.inv_mass = s.inv_mass;
+    I.row_offsets = row_offsets_.gpu_data();
+    I.col_indices = col_indices_.gpu_data();
+    I.diag = diag_.gpu_data();
+    I.values = values_.gpu_data();
+    I.pins = pins_.gpu_data();
+    I.tris = tris_.gpu_data();
+    I.dihs = dihs_.gpu_data();
+    I.pin_rec = pin_rec_.gpu_data();
+    I.tri_rec = tri_rec_.gpu_data();
+    I.dih_rec = dih_rec_.gpu_data();
+    I.contrib_offsets = contrib_offsets_.gpu_data();
+    I.contribs = contribs_.gpu_data();
+    I.rhs = rhs_.gpu_data();
+    I.x = s.x;
+    I.r = r_.gpu_data();
+    I.z = z_.gpu_data();
+    I.p0 = p0_.gpu_data();
+    I.p1 = p1_.gpu_data();
+    I.q = q_.gpu_data();
+    I.minv = minv_.gpu_data();
+    I.diag16 = s.bf16_blocks ? diag16_.gpu_data() : nullptr;
+    I.values16 = s.bf16_blocks ? values16_.gpu_data() : nullptr;
+    I.n_rows = n_;
+    I.nnz = nnz_;
+    I.n_pins = static_cast<int>(pins_.cpu_size());
+    I.n_tris = static_cast<int>(tris_.cpu_size());
+    I.n_dihs = static_cast<int>(dihs_.cpu_size());
+    I.tile_begin = 0;
+    I.tile_end = item_count(n_);
+    I.dt = s.dt;
+    I.damping = s.damping;
+    I.gx = s.gx;
+    I.gy = s.gy;
+    I.gz = s.gz;
+    I.max_iter = s.max_iter;
+    I.tol = s.tol;
+    return I;
+}
+
+// ---------------------------------------------------------------------------
+// BatchRunner
+// ---------------------------------------------------------------------------
+
+void BatchRunner::set_lanes_per_row(int L) {
+    if (L != 4 && L != 8 && L != 16) throw std::invalid_argument("chysx::engine: lanes_per_row must be 4, 8 or 16");
+    lanes_ = L;
+    desc_.lanes_per_row = L;
+}
+
+void BatchRunner::size_outputs(int n_instances, int n_tiles) {
+    const std::size_t I = static_cast<std::size_t>(std::max(n_instances, 1));
+    if (inst_iters_.gpu_size() < I) {
+        inst_iters_.ensure_device(I);
+        inst_done_.ensure_device(I);
+        inst_rr_.ensure_device(I);
+        inst_bb_.ensure_device(I);
+        inst_iters_.zero_device();
+        inst_rr_.zero_device();
+        inst_bb_.zero_device();
+    }
+    const std::size_t P = static_cast<std::size_t>(CHX_TILE_PARTIALS) * std::max(n_tiles, 1);
+    if (partials_.gpu_size() < P) partials_.ensure_device(P);
+    if (sync_.gpu_size() == 0) {
+        sync_.ensure_device(CHX_SYNC_WORDS);
+        sync_.zero_device();
+    }
+    desc_.partials = partials_.gpu_data();
+    desc_.inst_iters = inst_iters_.gpu_data();
+    desc_.inst_rr = inst_rr_.gpu_data();
+    desc_.inst_bb = inst_bb_.gpu_data();
+    desc_.inst_done = inst_done_.gpu_data();
+    desc_.sync = sync_.gpu_data();
+}
+
+void BatchRunner::bind_single(const chx_instance& d) {
+    desc_.n_instances = 1;
+    desc_.n_tiles = item_count(d.n_rows);
+    desc_.n_elem_items = 0;
+    desc_.n_slot_items = 0;
+    desc_.lanes_per_row = lanes_;
+    desc_.grid_blocks = grid_;
+    desc_.bf16_blocks = bf16_ ? 1 : 0;
+    desc_.ir_period = ir_period_;
+    desc_.single = d;
+    desc_.single.tile_begin = 0;
+    desc_.single.tile_end = desc_.n_tiles;
+    desc_.inst = nullptr;
+    desc_.tiles = nullptr;
+    desc_.elem_items = nullptr;
+    desc_.slot_items = nullptr;
+    size_outputs(1, desc_.n_tiles);
+    check(chx_device_synchronize(), "engine", "bind");
+}
+
+void BatchRunner::bind_many(const std::vector<chx_instance>& ds) {
+    if (ds.empty()) throw std::invalid_argument("chysx::engine: empty batch");
+    table_host_ = ds;
+    auto& ti = tiles_.host();
+    auto& ei = elem_items_.host();
+    auto& si = slot_items_.host();
+    ti.clear();
+    ei.clear();
+    si.clear();
+    for (std::size_t i = 0; i < ds.size(); ++i) {
+        chx_instance& I = table_host_[i];
+        const int inst = static_cast<int>(i);
+        I.tile_begin = static_cast<int>(ti.size() / 4);
+        for (int r0 = 0; r0 < I.n_rows; r0 += CHX_TILE_ROWS)
+            ti.insert(ti.end(), {inst, r0, std::min(I.n_rows, r0 + CHX_TILE_ROWS), 0});
+        I.tile_end = static_cast<int>(ti.size() / 4);
+        const int counts[3] = {I.n_pins, I.n_tris, I.n_dihs};
+        for (int kind = 0; kind < 3; ++kind)
+            for (int b = 0; b < counts[kind]; b += CHX_ITEM_SIZE)
+                ei.insert(ei.end(), {inst, kind, b, std::min(counts[kind], b + CHX_ITEM_SIZE)});
+        const int slots = I.n_rows + I.nnz;
+        for (int b = 0; b < slots; b += CHX_ITEM_SIZE) si.insert(si.end(), {inst, b, std::min(slots, b + CHX_ITEM_SIZE), 0});
+    }
+    if (ti.size() / 4 > static_cast<std::size_t>(std::numeric_limits<int>::max()))
+        throw std::runtime_error("chysx::engine: too many row tiles");
+    tiles_.copy_to_device();
+    elem_items_.copy_to_device();
+    slot_items_.copy_to_device();
+    table_.host() = table_host_;
+    table_.copy_to_device();
+    desc_.n_instances = static_cast<int>(ds.size());
+    desc_.n_tiles = static_cast<int>(ti.size() / 4);
+    desc_.n_elem_items = static_cast<int>(ei.size() / 4);
+    desc_.n_slot_items = static_cast<int>(si.size() / 4);
+    desc_.lanes_per_row = lanes_;
+    desc_.grid_blocks = grid_;
+    desc_.bf16_blocks = bf16_ ? 1 : 0;
+    desc_.ir_period = ir_period_;
+    desc_.single = chx_instance{};
+    desc_.inst = table_.gpu_data();
+    desc_.tiles = tiles_.gpu_data();
+    desc_.elem_items = elem_items_.gpu_data();
+    desc_.slot_items = slot_items_.gpu_data();
+    size_outputs(desc_.n_instances, desc_.n_tiles);
+    check(chx_device_synchronize(), "engine", "bind");
+}
+
+void BatchRunner::update_many(const std::vector<chx_instance>& ds, std::uintptr_t stream) {
+    if (ds.size() != table_host_.size()) throw std::invalid_argument("chysx::engine: instance count changed");
+    bool changed = false;
+    for (std::size_t i = 0; i < ds.size(); ++i) {
+        chx_instance d = ds[i];
+        d.tile_begin = table_host_[i].tile_begin;
+        d.tile_end = table_host_[i].tile_end;
+        if (std::memcmp(&d, &table_host_[i], sizeof(chx_instance)) != 0) {
+            table_host_[i] = d;
+            changed = true;
+        }
+    }
+    if (changed) {
+        // The previous table may still be read by queued launches.
+        check(chx_stream_synchronize(stream), "engine", "instance table update");
+        table_.host() = table_host_;
+        table_.copy_to_device(stream);
+        desc_.inst = table_.gpu_data();
+    }
+}
+
+void BatchRunner::step(std::uintptr_t stream) { check(chx_cloth_step(&desc_, stream), "cloth::ClothSimulator", "step"); }
+void BatchRunner::assemble(std::uintptr_t stream) {
+    check(chx_cloth_assemble(&desc_, stream), "cloth::ClothSimulator", "assemble");
+}
+void BatchRunner::solve(bool finalize, std::uintptr_t stream) {
+    check(chx_cloth_solve(&desc_, finalize ? 1 : 0, stream), "solver::PCGSolver", "solve");
+}
+double BatchRunner::energy(int kind_mask, std::uintptr_t stream) {
+    double e = 0.0;
+    check(chx_batch_energy(&desc_, kind_mask, &e, stream), "cloth", "energy");
+    return e;
+}
+
+void BatchRunner::read_stats(std::vector<int>& iters, std::vector<double>& rr, std::vector<double>& bb,
+                             std::uintptr_t stream) {
+    check_status(stream);
+    const int I = desc_.n_instances;
+    iters.assign(I, 0);
+    rr.assign(I, 0.0);
+    bb.assign(I, 0.0);
+    if (I == 0) return;
+    check(chx_memcpy_d2h(iters.data(), desc_.inst_iters, sizeof(int) * I, stream), "engine", "stats copy");
+    check(chx_memcpy_d2h(rr.data(), desc_.inst_rr, sizeof(double) * I, stream), "engine", "stats copy");
+    check(chx_memcpy_d2h(bb.data(), desc_.inst_bb, sizeof(double) * I, stream), "engine", "stats copy");
+}
+
+void BatchRunner::check_status(std::uintptr_t stream) {
+    if (desc_.sync) check(chx_batch_check(&desc_, stream), "solver::PCGSolver", "solve");
+}
+
+}  // namespace engine
+}  // namespace chysx

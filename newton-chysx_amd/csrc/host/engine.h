@@ -1,0 +1,139 @@
+// Device-side state of the cloth step, one cloth instance at a time.
+//
+// InstanceData owns every device array of one cloth: its BSR pattern (local
+// indices, exactly the reference's), element arrays, per-step element
+// records, the gather map that inverts the reference's per-element slot
+// tables (constraint_n.h:115-153) into per-slot contribution lists, and the
+// PCG work vectors.  BatchRunner owns what a launch needs besides: the
+// single-instance descriptor or a device table of instance descriptors, the
+// work lists, per-tile partial sums, per-instance solver outputs and the
+// grid-barrier area.  A batch of cloths is therefore a table over instances
+// that already live in HBM; admitting or retiring one rebuilds only the table.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "chysx_common.h"
+#include "constraints.h"
+#include "topology.h"
+
+namespace chysx {
+namespace engine {
+
+struct InstanceSource {
+    int n = 0;
+    float* pos = nullptr;
+    float* vel = nullptr;
+    const float* inv_mass = nullptr;
+    float* x = nullptr;  // warm start / solution (dx), owned by the caller
+    const topology::Pattern* pattern = nullptr;
+    const constraint::PinConstraint* pins = nullptr;
+    const constraint::TriangleStretchConstraint* stretch = nullptr;
+    const constraint::TriangleStretchConstraint* shear = nullptr;
+    const constraint::BendingConstraint* bending = nullptr;
+    float dt = 1.0f / 60.0f;
+    float damping = 0.0f;
+    float gx = 0.0f, gy = 0.0f, gz = -9.81f;
+    int max_iter = 50;
+    double tol = 0.0;
+    bool bf16_blocks = false;  // keep BF16 copies of the Hessian blocks
+};
+
+class InstanceData {
+public:
+    InstanceData() = default;
+    InstanceData(const InstanceData&) = delete;
+    InstanceData& operator=(const InstanceData&) = delete;
+
+    // Builds every device array (slot tables must be bound to s.pattern).
+    void build(const InstanceSource& s);
+    // Re-uploads the element arrays (stiffness or pin targets changed).
+    void update_elements(const InstanceSource& s);
+    // Descriptor for a launch: device arrays + the step scalars of `s`.
+    chx_instance descriptor(const InstanceSource& s) const;
+    bool built() const noexcept { return built_; }
+    int n_rows() const noexcept { return n_; }
+    int nnz() const noexcept { return nnz_; }
+    std::size_t device_bytes() const noexcept { return bytes_; }
+    std::size_t contributions() const noexcept { return contribs_.cpu_size(); }
+    const float* diag() const noexcept { return diag_.gpu_data(); }
+    const float* values() const noexcept { return values_.gpu_data(); }
+    const float* rhs() const noexcept { return rhs_.gpu_data(); }
+
+private:
+    void pack_elements(const InstanceSource& s);
+
+    bool built_ = false;
+    int n_ = 0, nnz_ = 0;
+    std::size_t bytes_ = 0;
+    DeviceArray<int> row_offsets_, col_indices_;
+    DeviceArray<chx_pin> pins_;
+    DeviceArray<chx_tri> tris_;
+    DeviceArray<chx_dih> dihs_;
+    DeviceArray<float> pin_rec_, tri_rec_, dih_rec_, diag_, values_;
+    DeviceArray<unsigned> contrib_offsets_, contribs_;
+    DeviceArray<float> rhs_, r_, z_, p0_, p1_, q_, minv_;
+    DeviceArray<uint16_t> diag16_, values16_;
+};
+
+class BatchRunner {
+public:
+    BatchRunner() = default;
+    BatchRunner(const BatchRunner&) = delete;
+    BatchRunner& operator=(const BatchRunner&) = delete;
+
+    // One instance: its descriptor travels in the kernel arguments.
+    void bind_single(const chx_instance& d);
+    // Many instances: builds the work lists and uploads the table.
+    void bind_many(const std::vector<chx_instance>& ds);
+    // Per-step scalars / external pointers of the bound instances.
+    void update_single(const chx_instance& d) { desc_.single = d; }
+    void update_many(const std::vector<chx_instance>& ds, std::uintptr_t stream);
+
+    void step(std::uintptr_t stream);
+    void assemble(std::uintptr_t stream);
+    void solve(bool finalize, std::uintptr_t stream);
+    double energy(int kind_mask, std::uintptr_t stream);
+
+    void set_lanes_per_row(int L);
+    int lanes_per_row() const noexcept { return lanes_; }
+    void set_grid_blocks(int g) {
+        grid_ = g < 0 ? 0 : g;
+        desc_.grid_blocks = grid_;
+    }
+    // Precision mode of the solve: FP32 blocks, or BF16 blocks in the inner
+    // SpMV with a true-residual refinement every `ir_period` iterations.
+    void set_bf16(bool on, int ir_period) {
+        bf16_ = on;
+        ir_period_ = ir_period < 1 ? 1 : ir_period;
+        desc_.bf16_blocks = on ? 1 : 0;
+        desc_.ir_period = ir_period_;
+    }
+    int n_instances() const noexcept { return desc_.n_instances; }
+    int n_tiles() const noexcept { return desc_.n_tiles; }
+    bool bound() const noexcept { return desc_.n_instances > 0; }
+    const chx_cloth_batch& desc() const noexcept { return desc_; }
+
+    // Synchronous read-backs.
+    void read_stats(std::vector<int>& iters, std::vector<double>& rr, std::vector<double>& bb, std::uintptr_t stream);
+    void check_status(std::uintptr_t stream);
+
+private:
+    void size_outputs(int n_instances, int n_tiles);
+
+    chx_cloth_batch desc_{};
+    int lanes_ = 8;
+    int grid_ = 0;
+    bool bf16_ = false;
+    int ir_period_ = 10;
+    std::vector<chx_instance> table_host_;
+    DeviceArray<chx_instance> table_;
+    DeviceArray<int> tiles_, elem_items_, slot_items_;
+    DeviceArray<double> partials_, inst_rr_, inst_bb_;
+    DeviceArray<int> inst_iters_, inst_done_;
+    DeviceArray<unsigned> sync_;
+};
+
+}  // namespace engine
+}  // namespace chysx

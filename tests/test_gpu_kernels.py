@@ -107,7 +107,14 @@ def test_pcg_matches_oracle(gpu, tol, max_iter):
     else:
         assert it == max_iter == it_o
     assert abs(bb - bb_o) <= 1e-12 * bb_o
-    assert helpers.rel_maxnorm(x, xo) <= 1e-4
+    # x0 is ~40x larger than the solution, so r0 = b - A x0 cancels and its
+    # FP32 rounding (which depends on the SpMV summation order) stays in x.
+    # The FP32 oracle sits ~7e-5 (relative max-norm) from the FP64 one on this
+    # system; the GPU solve must be as close to the FP64 result as that.
+    x64, it64, _, _ = pyoracle.pcg_solve(ro, ci, diag, vals, rhs, x0, max_iter, tol, precision=64)
+    d32 = helpers.rel_maxnorm(xo, x64)
+    assert helpers.rel_maxnorm(x, x64) <= 2.0 * d32 + 1e-6, (helpers.rel_maxnorm(x, x64), d32)
+    assert helpers.rel_maxnorm(x, xo) <= 3.0 * d32 + 1e-6
     # True residual ||b - A x|| / ||b|| (FP64): an FP32 solve cannot go
     # below ~kappa * eps_32 on this system (pins at k = 1e6), so it is
     # compared with the FP32 oracle's own true residual.
