@@ -96,8 +96,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--autotune", action="store_true", help="sweep lanes per row x grid size first")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the lanes-per-row x grid-size sweep and use --lanes / --grid")
+    ap.add_argument("--grid", type=int, default=0, help="persistent PCG workgroups (0: all co-resident)")
     ap.add_argument("--no-plain-large", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run an elementwise copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -118,6 +122,8 @@ def main():
     cloths = [scenes.DeviceCloth(s, lanes_per_row=args.lanes) for s in specs]
     batch = chysx.ClothBatch()
     batch.set_lanes_per_row(args.lanes)
+    if args.grid > 0:
+        batch.set_grid_blocks(args.grid)
     for c in cloths:
         batch.add_instance(c.sim)
     batch.finalize(specs[0].dt)
@@ -128,11 +134,14 @@ def main():
     n_rows = int(sizes[:, 0].sum())
     dt = specs[0].dt
     tuned = None
-    if args.autotune:
+    if not args.no_autotune:
+        # Lanes per block row x persistent grid size, picked on this batch
+        # (untimed; a few dozen substeps).
         from chysx import autotune
 
-        tuned = autotune.tune_batch(batch, dt, stream, log=log)
+        tuned = autotune.tune_batch(batch, dt, stream, cache_path=None, log=log)
         args.lanes = tuned["lanes_per_row"]
+        args.grid = tuned["grid_blocks"]
         log(f"autotune picked lanes={tuned['lanes_per_row']} grid={tuned['grid_blocks']}")
 
     with torch.cuda.stream(stream):
@@ -166,13 +175,36 @@ def main():
     mean_ms = float(np.mean(pcg_ms))
     achieved = float(np.mean(pcg_bytes)) / (mean_ms * 1e-3) / 1e9
     traffic = load_traffic()
+    key = f"lanes{args.lanes}_grid{args.grid}_inst{len(specs)}"
+    traffic_bytes = None
+    if traffic and key in traffic.get("configs", {}):
+        traffic_bytes = traffic["configs"][key]["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                "traffic": traffic_bytes,
                 "kernel": "pcg_kernel<%d> (persistent block-Jacobi PCG + finalize)" % args.lanes,
                 "mean_launch_ms": round(mean_ms, 3), "algorithmic_bytes_per_launch": float(np.mean(pcg_bytes))}
 
     log(f"PCG launch {mean_ms:.3f} ms, {achieved:.0f} GB/s algorithmic")
+    copy_probe = None
+    if args.copy_probe_mib > 0:
+        # A plain streaming kernel of known size (read B, write B bytes):
+        # torch's elementwise multiply by one, timed with events on `stream`.
+        nbytes = args.copy_probe_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        with torch.cuda.stream(stream):
+            torch.mul(src, 1.0, out=dst)
+            ev0.record(stream)
+            for _ in range(5):
+                torch.mul(src, 1.0, out=dst)
+            ev1.record(stream)
+            ev1.synchronize()
+        ms = ev0.elapsed_time(ev1) / 5
+        copy_probe = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(ms, 4),
+                      "GB_per_s": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+        log(f"copy probe {args.copy_probe_mib} MiB: {copy_probe['GB_per_s']} GB/s")
+        del src, dst
     plain = None
     if not args.no_plain:
         # The same instances, one ClothSimulator.step at a time (each packs
@@ -231,7 +263,8 @@ def main():
         "config": {"workload": "batched: 512 cloth grids, side logU[16,128] vertices (seed 2), BW98 + bending, "
                                "PCG tol 1e-6 cap 100, one fused substep per step",
                    "instances": len(specs), "vertices": n_rows, "off_diag_blocks": int(sizes[:, 1].sum()),
-                   "pcg_tol": 1e-6, "pcg_cap": 100, "lanes_per_row": args.lanes, "parallelism": "single GPU"},
+                   "pcg_tol": 1e-6, "pcg_cap": 100, "lanes_per_row": args.lanes, "grid_blocks": args.grid,
+                   "parallelism": "single GPU"},
         "pcg_iterations": {"mean": float(iters.mean()), "max": int(iters.max()), "min": int(iters.min())},
         "max_relative_residual": float(res.max()),
         "plain": plain,
@@ -240,6 +273,7 @@ def main():
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "copy_probe": copy_probe,
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(out))

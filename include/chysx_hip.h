@@ -18,9 +18,17 @@
  *
  * Memory layout shared with the kernels:
  *   positions / velocities / vectors : AoS float3, 12 bytes per vertex
- *   3x3 blocks                        : 9 floats, row-major, 36 bytes
+ *   3x3 blocks passed by the caller   : 9 floats, row-major, 36 bytes
+ *     (chx_spmv_bsr3, chx_pcg_solve_bsr3)
  * which is the reference's Vec3f / Mat3f layout (ChysX/src/math/vec.cuh,
  * ChysX/src/math/matrix.cuh:130-133), so caller buffers are used in place.
+ * The kernels index arrays with 32-bit byte offsets: every block array (and
+ * every per-instance array of a cloth) must stay below 4 GiB, i.e. at most
+ * 119,304,647 block rows or off-diagonal blocks per matrix.
+ * The matrices a cloth instance assembles for itself (chx_instance diag,
+ * values, minv) are stored planar instead: nine planes, one per block
+ * component, so that a wavefront reads one component of consecutive blocks
+ * as consecutive words.
  */
 #ifndef CHYSX_HIP_H_
 #define CHYSX_HIP_H_
@@ -80,8 +88,9 @@ typedef struct chx_instance {
     const float* inv_mass;           /* external inverse masses, or NULL (mass 1 kg)         */
     const int* row_offsets;          /* [n_rows+1]                                           */
     const int* col_indices;          /* [nnz]                                                */
-    float* diag;                     /* [n_rows*9] row-major 3x3                              */
-    float* values;                   /* [nnz*9]                                              */
+    float* diag;                     /* [9][n_rows] planar: component c (row-major r*3+col)  */
+                                     /* of block row i at diag[c*n_rows + i]                 */
+    float* values;                   /* [9][nnz] planar, same convention                      */
     const chx_pin* pins;
     const chx_tri* tris;             /* stretch elements, then shear elements                */
     const chx_dih* dihs;
@@ -97,9 +106,9 @@ typedef struct chx_instance {
     float* p0;
     float* p1;
     float* q;
-    float* minv;                     /* [n_rows*9] block-Jacobi preconditioner               */
-    uint16_t* diag16;                /* BF16 copies of diag / values for the inner SpMV of   */
-    uint16_t* values16;              /* the BF16 precision mode, or NULL                     */
+    float* minv;                     /* [9][n_rows] planar block-Jacobi preconditioner       */
+    uint16_t* diag16;                /* planar BF16 copies of diag / values for the inner    */
+    uint16_t* values16;              /* SpMV of the BF16 precision mode, or NULL             */
     int n_rows, nnz, n_pins, n_tris, n_dihs;
     int tile_begin;                  /* this instance's row tiles in the batch tile list     */
     int tile_end;

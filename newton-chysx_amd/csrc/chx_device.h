@@ -12,6 +12,19 @@ namespace chx {
 
 constexpr int kWave = 64;
 
+// Global-memory pointers.  Array pointers that the kernels read out of an
+// instance descriptor are generic pointers to the compiler, which then emits
+// FLAT loads and stores (64-bit VGPR addresses, counted against both the
+// vector-memory and the LDS/scalar wait counters).  Retyping them into
+// address space 1 gives GLOBAL instructions with an SGPR base and a 32-bit
+// VGPR offset.  All arrays of this library live in device (global) memory.
+template <class T>
+using gp = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gp<T> glb(T* p) {
+    return (gp<T>)p;
+}
+
 // ---- float3 --------------------------------------------------------------
 struct f3 {
     float x, y, z;
@@ -26,19 +39,65 @@ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
 }
 __device__ __forceinline__ float len(f3 a) { return sqrtf(dot(a, a)); }
 
-__device__ __forceinline__ f3 ld3(const float* p, long i) {
-    const float* q = p + 3 * i;
+// Element i / record i of a global array, addressed with a 32-bit byte
+// offset: the access then compiles to a GLOBAL instruction with the array
+// base in SGPRs and one 32-bit VGPR offset (64-bit index arithmetic would
+// put a full 64-bit address in VGPRs per access).  The host keeps every
+// array that the hot kernels index this way below 4 GiB per instance.
+template <class T>
+__device__ __forceinline__ gp<const T> rec_at(gp<const T> p, unsigned i, unsigned fields) {
+    return (gp<const T>)((gp<const char>)p + i * (fields * unsigned(sizeof(T))));
+}
+template <class T>
+__device__ __forceinline__ gp<T> rec_at(gp<T> p, unsigned i, unsigned fields) {
+    return (gp<T>)((gp<char>)p + i * (fields * unsigned(sizeof(T))));
+}
+template <class T>
+__device__ __forceinline__ T ldx(gp<const T> p, unsigned i) {
+    return *rec_at(p, i, 1u);
+}
+template <class T>
+__device__ __forceinline__ void stx(gp<T> p, unsigned i, T v) {
+    *rec_at(p, i, 1u) = v;
+}
+__device__ __forceinline__ f3 ld3g(gp<const float> p, unsigned i) {
+    const gp<const float> q = rec_at(p, i, 3u);
     return {q[0], q[1], q[2]};
 }
-__device__ __forceinline__ void st3(float* p, long i, f3 v) {
-    float* q = p + 3 * i;
+__device__ __forceinline__ void st3g(gp<float> p, unsigned i, f3 v) {
+    const gp<float> q = rec_at(p, i, 3u);
     q[0] = v.x;
     q[1] = v.y;
     q[2] = v.z;
 }
 
+// Loads record i of 4-byte fields through a global pointer (a C++ copy of an
+// address-space-1 lvalue would bind it to a generic reference).
+template <class T>
+__device__ __forceinline__ T ldg(gp<const T> base, unsigned i) {
+    static_assert(sizeof(T) % 4 == 0, "record of 4-byte fields");
+    T v;
+    const gp<const unsigned> src = (gp<const unsigned>)rec_at(base, i, 1u);
+    unsigned* dst = reinterpret_cast<unsigned*>(&v);
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(T) / 4; ++k) dst[k] = src[k];
+    return v;
+}
+
+template <class P>
+__device__ __forceinline__ f3 ld3(P p, long i) {
+    return {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+}
+template <class P>
+__device__ __forceinline__ void st3(P p, long i, f3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+
 // 3x3 row-major block times vector: m[r*3+c].
-__device__ __forceinline__ f3 mv9(const float* m, f3 v) {
+template <class P>
+__device__ __forceinline__ f3 mv9(P m, f3 v) {
     return {m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
             m[6] * v.x + m[7] * v.y + m[8] * v.z};
 }
@@ -70,45 +129,136 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<unsigned>(h) << 16); }
 
-// 3x3 block stored as BF16, applied in FP32.
-__device__ __forceinline__ f3 mv9_bf16(const uint16_t* m, f3 v) {
+__device__ __forceinline__ float to_f32(float v) { return v; }
+__device__ __forceinline__ float to_f32(uint16_t v) { return bf16_to_f32(v); }
+
+// Matrix views for the SpMV and the preconditioner set-up.  diag(row, x) and
+// off(k, x) apply a diagonal / off-diagonal 3x3 block in FP32; diag_block
+// loads a diagonal block as nine row-major floats.
+//
+// Mat32: row-major 3x3 blocks, 36 bytes each (the reference's BlockCSR3
+// layout, used by the standalone SpMV / PCG entry points).
+struct Mat32G {
+    gp<const float> d;
+    gp<const float> v;
+    __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv9(rec_at(d, row, 9u), x); }
+    __device__ __forceinline__ f3 off(unsigned k, f3 x) const { return mv9(rec_at(v, k, 9u), x); }
+    __device__ __forceinline__ void diag_block(unsigned row, float (&m)[9]) const {
+        const gp<const float> q = rec_at(d, row, 9u);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) m[c] = q[c];
+    }
+};
+
+// Planar blocks: component c (row-major r*3+col) of block i at p[c*stride + i],
+// nine planes per array.  One load instruction of a wavefront then reads one
+// component of consecutive blocks, i.e. consecutive words, instead of a
+// 36-byte stride that spreads each instruction over ~9x more cache lines.
+template <class T>
+__device__ __forceinline__ f3 mv9p(gp<const T> m, unsigned stride, unsigned i, f3 v) {
     float a[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) a[i] = bf16_to_f32(m[i]);
+    for (unsigned c = 0; c < 9; ++c) a[c] = to_f32(ldx(m, c * stride + i));
     return mv9(a, v);
 }
-
-// Matrix views for the SpMV: FP32 blocks, or BF16 copies with FP32 math.
-struct Mat32 {
-    const float* d;
-    const float* v;
-    __device__ __forceinline__ f3 diag(long row, f3 x) const { return mv9(d + 9 * row, x); }
-    __device__ __forceinline__ f3 off(long k, f3 x) const { return mv9(v + 9 * k, x); }
+template <class T>
+struct MatP {
+    gp<const T> d;
+    gp<const T> v;
+    unsigned n;    // plane stride of d (block rows)
+    unsigned nnz;  // plane stride of v (off-diagonal blocks)
+    __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv9p(d, n, row, x); }
+    __device__ __forceinline__ f3 off(unsigned k, f3 x) const { return mv9p(v, nnz, k, x); }
+    __device__ __forceinline__ void diag_block(unsigned row, float (&m)[9]) const {
+#pragma unroll
+        for (unsigned c = 0; c < 9; ++c) m[c] = to_f32(ldx(d, c * n + row));
+    }
 };
-struct Mat16 {
-    const uint16_t* d;
-    const uint16_t* v;
-    __device__ __forceinline__ f3 diag(long row, f3 x) const { return mv9_bf16(d + 9 * row, x); }
-    __device__ __forceinline__ f3 off(long k, f3 x) const { return mv9_bf16(v + 9 * k, x); }
-};
+using MatP32 = MatP<float>;
+using MatP16 = MatP<uint16_t>;  // BF16 copies, FP32 arithmetic
 
 __device__ __forceinline__ double dot64(f3 a, f3 b) {
     return double(a.x) * double(b.x) + double(a.y) * double(b.y) + double(a.z) * double(b.z);
 }
 
-// Instance descriptor i of a batch (the inline `single` one when the batch
-// has no device instance table).  Both cases go through one pointer so the
-// code that reads the descriptor is the same instruction sequence.
-__device__ __forceinline__ const chx_instance& inst_at(const chx_cloth_batch& B, int i) {
-    const chx_instance* p = (B.inst != nullptr) ? B.inst + i : &B.single;
-    return *p;
+// Device view of one chx_instance: the same fields with global-memory
+// pointer types.  Built from the device instance table, or from the inline
+// `single` descriptor when the batch has none (both with scalar loads).
+struct Inst {
+    gp<float> pos, vel;
+    gp<const float> inv_mass;
+    gp<const int> row_offsets, col_indices;
+    gp<float> diag, values;
+    gp<const chx_pin> pins;
+    gp<const chx_tri> tris;
+    gp<const chx_dih> dihs;
+    gp<float> pin_rec, tri_rec, dih_rec;
+    gp<const unsigned> contrib_offsets, contribs;
+    gp<float> rhs, x, r, z, p0, p1, q, minv;
+    gp<uint16_t> diag16, values16;
+    int n_rows, nnz, n_pins, n_tris, n_dihs, tile_begin, tile_end;
+    float dt, damping, gx, gy, gz;
+    int max_iter;
+    double tol;
+};
+
+template <class P>
+__device__ __forceinline__ Inst make_inst(P d) {
+    Inst I;
+    I.pos = glb(d->pos);
+    I.vel = glb(d->vel);
+    I.inv_mass = glb(d->inv_mass);
+    I.row_offsets = glb(d->row_offsets);
+    I.col_indices = glb(d->col_indices);
+    I.diag = glb(d->diag);
+    I.values = glb(d->values);
+    I.pins = glb(d->pins);
+    I.tris = glb(d->tris);
+    I.dihs = glb(d->dihs);
+    I.pin_rec = glb(d->pin_rec);
+    I.tri_rec = glb(d->tri_rec);
+    I.dih_rec = glb(d->dih_rec);
+    I.contrib_offsets = glb(d->contrib_offsets);
+    I.contribs = glb(d->contribs);
+    I.rhs = glb(d->rhs);
+    I.x = glb(d->x);
+    I.r = glb(d->r);
+    I.z = glb(d->z);
+    I.p0 = glb(d->p0);
+    I.p1 = glb(d->p1);
+    I.q = glb(d->q);
+    I.minv = glb(d->minv);
+    I.diag16 = glb(d->diag16);
+    I.values16 = glb(d->values16);
+    I.n_rows = d->n_rows;
+    I.nnz = d->nnz;
+    I.n_pins = d->n_pins;
+    I.n_tris = d->n_tris;
+    I.n_dihs = d->n_dihs;
+    I.tile_begin = d->tile_begin;
+    I.tile_end = d->tile_end;
+    I.dt = d->dt;
+    I.damping = d->damping;
+    I.gx = d->gx;
+    I.gy = d->gy;
+    I.gz = d->gz;
+    I.max_iter = d->max_iter;
+    I.tol = d->tol;
+    return I;
+}
+
+// Instance i of a batch (the inline `single` one when the batch has no
+// device instance table).
+__device__ __forceinline__ Inst inst_at(const chx_cloth_batch& B, int i) {
+    return (B.inst != nullptr) ? make_inst(rec_at(glb((const chx_instance*)B.inst), unsigned(i), 1u))
+                               : make_inst(&B.single);
 }
 
 // Work-list entries (int4 arrays of the batch, or implied by the counts of
 // the single instance).
 __device__ __forceinline__ void get_tile(const chx_cloth_batch& B, int t, int& inst, int& rb, int& re) {
     if (B.tiles != nullptr) {
-        const int* p = B.tiles + 4 * t;
+        const gp<const int> p = rec_at(glb((const int*)B.tiles), unsigned(t), 4u);
         inst = p[0];
         rb = p[1];
         re = p[2];
@@ -123,7 +273,7 @@ __device__ __forceinline__ int blocks_of(int n) { return (n + CHX_ITEM_SIZE - 1)
 
 __device__ __forceinline__ void get_elem_item(const chx_cloth_batch& B, int w, int& inst, int& kind, int& b, int& e) {
     if (B.elem_items != nullptr) {
-        const int* p = B.elem_items + 4 * w;
+        const gp<const int> p = rec_at(glb((const int*)B.elem_items), unsigned(w), 4u);
         inst = p[0];
         kind = p[1];
         b = p[2];
@@ -152,7 +302,7 @@ __device__ __forceinline__ void get_elem_item(const chx_cloth_batch& B, int w, i
 
 __device__ __forceinline__ void get_slot_item(const chx_cloth_batch& B, int w, int& inst, int& b, int& e) {
     if (B.slot_items != nullptr) {
-        const int* p = B.slot_items + 4 * w;
+        const gp<const int> p = rec_at(glb((const int*)B.slot_items), unsigned(w), 4u);
         inst = p[0];
         b = p[1];
         e = p[2];

@@ -61,20 +61,20 @@ __global__ __launch_bounds__(kBlock) void energy_kernel(chx_cloth_batch B, int m
     __shared__ double lds[16];
     int inst, kind, ib, ie;
     get_elem_item(B, blockIdx.x, inst, kind, ib, ie);
-    const chx_instance& I = inst_at(B, inst);
+    const Inst I = inst_at(B, inst);
     const int i = ib + threadIdx.x;
     double E[1] = {0.0};
     if (i < ie) {
         if (kind == 0 && (mask & 1)) {
-            const chx_pin p = I.pins[i];
+            const chx_pin p = ldg(I.pins, unsigned(i));
             const f3 d = sub(ld3(I.pos, p.v), mk(p.tx, p.ty, p.tz));
             E[0] = double(0.5f * p.k * dot(d, d));  // pin_constraint.cu:40-53
         } else if (kind == 1 && (mask & 2)) {
-            const chx_tri t = I.tris[i];
+            const chx_tri t = ldg(I.tris, unsigned(i));
             E[0] = tri_energy(ld3(I.pos, t.v0), ld3(I.pos, t.v1), ld3(I.pos, t.v2), t.dm00, t.dm01, t.dm10, t.dm11,
                               t.kA);
         } else if (kind == 2 && (mask & 4)) {
-            const chx_dih d = I.dihs[i];
+            const chx_dih d = ldg(I.dihs, unsigned(i));
             const DihFrame F =
                 dihedral_frame(ld3(I.pos, d.v0), ld3(I.pos, d.v1), ld3(I.pos, d.v2), ld3(I.pos, d.v3));
             if (F.valid) {  // bending_constraint.cu:154-172

@@ -49,30 +49,33 @@ __device__ __forceinline__ float guarded_div(double a, double b) {
 __global__ __launch_bounds__(kBlock) void element_kernel(chx_cloth_batch B) {
     int inst, kind, ib, ie;
     get_elem_item(B, blockIdx.x, inst, kind, ib, ie);
-    const int i = ib + threadIdx.x;
-    if (i >= ie) return;
-    const chx_instance& I = inst_at(B, inst);
-    const float* pos = I.pos;
+    const int ii = ib + int(threadIdx.x);
+    if (ii >= ie) return;
+    const unsigned i = unsigned(ii);
+    const Inst I = inst_at(B, inst);
+    const gp<const float> pos = I.pos;
     if (kind == 0) {
-        const chx_pin p = I.pins[i];
-        const f3 x = ld3(pos, p.v);
+        const chx_pin p = ldg(I.pins, i);
+        const f3 x = ld3g(pos, unsigned(p.v));
         const f3 g = mul(sub(x, mk(p.tx, p.ty, p.tz)), p.k);  // pin_constraint.cu:61-75
-        float* rec = I.pin_rec + long(CHX_PIN_REC) * i;
+        const gp<float> rec = rec_at(I.pin_rec, i, unsigned(CHX_PIN_REC));
         rec[0] = g.x;
         rec[1] = g.y;
         rec[2] = g.z;
         rec[3] = p.k;
     } else if (kind == 1) {
-        const chx_tri t = I.tris[i];
+        const chx_tri t = ldg(I.tris, i);
         float rec[CHX_TRI_REC];
-        tri_record(ld3(pos, t.v0), ld3(pos, t.v1), ld3(pos, t.v2), t.dm00, t.dm01, t.dm10, t.dm11, t.kA, rec);
-        float* out = I.tri_rec + long(CHX_TRI_REC) * i;
+        tri_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00, t.dm01,
+                   t.dm10, t.dm11, t.kA, rec);
+        const gp<float> out = rec_at(I.tri_rec, i, unsigned(CHX_TRI_REC));
 #pragma unroll
         for (int k = 0; k < CHX_TRI_REC; ++k) out[k] = rec[k];
     } else {
-        const chx_dih d = I.dihs[i];
-        const DihFrame F = dihedral_frame(ld3(pos, d.v0), ld3(pos, d.v1), ld3(pos, d.v2), ld3(pos, d.v3));
-        float* out = I.dih_rec + long(CHX_DIH_REC) * i;
+        const chx_dih d = ldg(I.dihs, i);
+        const DihFrame F = dihedral_frame(ld3g(pos, unsigned(d.v0)), ld3g(pos, unsigned(d.v1)),
+                                          ld3g(pos, unsigned(d.v2)), ld3g(pos, unsigned(d.v3)));
+        const gp<float> out = rec_at(I.dih_rec, i, unsigned(CHX_DIH_REC));
         // An invalid frame contributes nothing (bending_constraint.cu:190-191,227-228).
         const float c = F.valid ? d.k * (F.theta - d.rest) : 0.0f;
         const float k = F.valid ? d.k : 0.0f;
@@ -101,25 +104,26 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
     }
     int inst, sb, se;
     get_slot_item(B, blockIdx.x, inst, sb, se);
-    const int s = sb + threadIdx.x;
-    if (s >= se) return;
-    const chx_instance& I = inst_at(B, inst);
-    const int n = I.n_rows;
+    const int ss = sb + int(threadIdx.x);
+    if (ss >= se) return;
+    const unsigned s = unsigned(ss);
+    const Inst I = inst_at(B, inst);
+    const unsigned n = unsigned(I.n_rows);
     const bool is_diag = s < n;
-    const unsigned c0 = I.contrib_offsets[s], c1 = I.contrib_offsets[s + 1];
+    const unsigned c0 = ldx(I.contrib_offsets, s), c1 = ldx(I.contrib_offsets, s + 1u);
     float h[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     f3 grad = mk(0.f, 0.f, 0.f);
     for (unsigned c = c0; c < c1; ++c) {
-        const unsigned w = I.contribs[c];
+        const unsigned w = ldx(I.contribs, c);
         const unsigned kind = w >> 30, a = (w >> 28) & 3u, b = (w >> 26) & 3u, id = w & CHX_CONTRIB_MAX_ELEMENT;
         if (kind == CHX_CONTRIB_PIN) {
-            const float* r = I.pin_rec + long(CHX_PIN_REC) * id;
+            const gp<const float> r = rec_at(gp<const float>(I.pin_rec), id, unsigned(CHX_PIN_REC));
             h[0] += r[3];
             h[4] += r[3];
             h[8] += r[3];
             grad = add(grad, mk(r[0], r[1], r[2]));
         } else if (kind == CHX_CONTRIB_TRI) {
-            const float* r = I.tri_rec + long(CHX_TRI_REC) * id;
+            const gp<const float> r = rec_at(gp<const float>(I.tri_rec), id, unsigned(CHX_TRI_REC));
             const float su = r[18 + a] * r[18 + b];
             const float sv = r[21 + a] * r[21 + b];
             const float m0 = su * r[6] + sv * r[12], m1 = su * r[7] + sv * r[13], m2 = su * r[8] + sv * r[14];
@@ -132,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
                 grad = add(grad, mk(ua * r[0] + va * r[3], ua * r[1] + va * r[4], ua * r[2] + va * r[5]));
             }
         } else {
-            const float* r = I.dih_rec + long(CHX_DIH_REC) * id;
+            const gp<const float> r = rec_at(gp<const float>(I.dih_rec), id, unsigned(CHX_DIH_REC));
             const float k = r[1];
             const f3 ai = mk(r[2 + 3 * a], r[3 + 3 * a], r[4 + 3 * a]);
             const f3 aj = mk(r[2 + 3 * b], r[3 + 3 * b], r[4 + 3 * b]);
@@ -143,11 +147,11 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
         }
     }
     if (is_diag) {
-        const int row = s;
-        const f3 p = ld3(I.pos, row), v = ld3(I.vel, row);
+        const unsigned row = s;
+        const f3 p = ld3g(I.pos, row), v = ld3g(I.vel, row);
         float m = 1.0f;  // mass_from_inv_mass_kernel, cloth_simulator.cu:66-77
         if (I.inv_mass != nullptr) {
-            const float w = I.inv_mass[row];
+            const float w = ldx(I.inv_mass, row);
             m = (w > 1.0e-12f) ? (1.0f / w) : 1.0e8f;
         }
         const float dt = I.dt;
@@ -156,26 +160,24 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
         const f3 xt = mk(p.x + dt * v.x + dt * dt * I.gx, p.y + dt * v.y + dt * dt * I.gy,
                          p.z + dt * v.z + dt * dt * I.gz);
         const float mi = m * inv_dt2;
-        st3(I.rhs, row, mk(mi * (xt.x - p.x) - grad.x, mi * (xt.y - p.y) - grad.y, mi * (xt.z - p.z) - grad.z));
+        st3g(I.rhs, row, mk(mi * (xt.x - p.x) - grad.x, mi * (xt.y - p.y) - grad.y, mi * (xt.z - p.z) - grad.z));
         h[0] += mi;
         h[4] += mi;
         h[8] += mi;
-        float* d = I.diag + 9L * row;
+        // Planar blocks: component k of block row `row` at diag[k*n + row].
 #pragma unroll
-        for (int k = 0; k < 9; ++k) d[k] = h[k];
+        for (unsigned k = 0; k < 9; ++k) stx(I.diag, k * n + row, h[k]);
         if (I.diag16 != nullptr) {
-            uint16_t* d16 = I.diag16 + 9L * row;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) d16[k] = f32_to_bf16(h[k]);
+            for (unsigned k = 0; k < 9; ++k) stx(I.diag16, k * n + row, f32_to_bf16(h[k]));
         }
     } else {
-        float* d = I.values + 9L * (s - n);
+        const unsigned e = s - n, nnz = unsigned(I.nnz);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) d[k] = h[k];
+        for (unsigned k = 0; k < 9; ++k) stx(I.values, k * nnz + e, h[k]);
         if (I.values16 != nullptr) {
-            uint16_t* d16 = I.values16 + 9L * (s - n);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) d16[k] = f32_to_bf16(h[k]);
+            for (unsigned k = 0; k < 9; ++k) stx(I.values16, k * nnz + e, f32_to_bf16(h[k]));
         }
     }
 }
@@ -188,11 +190,20 @@ __global__ void reset_sync_kernel(unsigned* sync) {
 // SpMV row loop: L lanes per block row, 64/L rows per wave and pass.
 // src(j) returns the source vector entry j; fin(row, Ax) runs on the first
 // lane of each row after the segmented reduction.
+//
+// Each lane takes U = 16/L of its row's off-diagonal blocks per pass (a pass
+// covers 16 blocks of a row), loading them at clamped indices without
+// branches so that all their loads are in flight together; the products are
+// masked and added in ascending block order.  The diagonal block is applied
+// by every lane of the row (same addresses, no branch) and kept on lane 0.
+// Per lane the sum is diag + off[sub] + off[sub+L] + ..., then a segmented
+// butterfly over the L lanes.
 // ---------------------------------------------------------------------------
 template <int L, class Mat, class Src, class Fin>
-__device__ __forceinline__ void spmv_rows(const int* __restrict__ ro, const int* __restrict__ ci, Mat mat, int rb,
-                                          int re, Src src, Fin fin) {
+__device__ __forceinline__ void spmv_rows(gp<const int> ro, gp<const int> ci, Mat mat, int rb, int re, Src src,
+                                          Fin fin) {
     constexpr int RPW = kWave / L;
+    constexpr int U = (16 / L) > 1 ? (16 / L) : 1;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
@@ -200,16 +211,27 @@ __device__ __forceinline__ void spmv_rows(const int* __restrict__ ro, const int*
     for (int base = rb + wid * RPW; base < re; base += nw * RPW) {
         const int row = base + g;
         const bool valid = row < re;
-        f3 acc = mk(0.f, 0.f, 0.f);
-        if (valid) {
-            const int beg = ro[row], end = ro[row + 1];
-            if (sub_ == 0) acc = mat.diag(row, src(row));
-            for (int k = beg + sub_; k < end; k += L) acc = add(acc, mat.off(k, src(ci[k])));
+        const unsigned rowc = unsigned(valid ? row : rb);
+        const unsigned beg = unsigned(ldx(ro, rowc));
+        const unsigned end = valid ? unsigned(ldx(ro, rowc + 1u)) : beg;
+        const f3 dprod = mat.diag(rowc, src(rowc));
+        f3 acc = (valid && sub_ == 0) ? dprod : mk(0.f, 0.f, 0.f);
+        for (unsigned k0 = beg + unsigned(sub_); k0 < end; k0 += unsigned(U * L)) {
+            f3 part[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned k = k0 + unsigned(u * L);
+                const unsigned kc = (k < end) ? k : k0;
+                part[u] = mat.off(kc, src(unsigned(ldx(ci, kc))));
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k0 + unsigned(u * L) < end) acc = add(acc, part[u]);
         }
         acc.x = seg_sum<L>(acc.x);
         acc.y = seg_sum<L>(acc.y);
         acc.z = seg_sum<L>(acc.z);
-        if (valid && sub_ == 0) fin(row, acc);
+        if (valid && sub_ == 0) fin(rowc, acc);
     }
 }
 
@@ -217,7 +239,7 @@ __device__ __forceinline__ void spmv_rows(const int* __restrict__ ro, const int*
 // in a fixed order (lane-strided then butterfly), identical in every
 // workgroup.  Result in lds[0..3] for all threads; lds[4] is 1.0 when the
 // instance has already finished its solve (*done set in an earlier phase).
-__device__ __forceinline__ void instance_sums(const double* __restrict__ part, int tb, int te, int f0, int f1, int f2,
+__device__ __forceinline__ void instance_sums(gp<const double> part, int tb, int te, int f0, int f1, int f2,
                                               int f3_, double* lds, int* done) {
     __syncthreads();
     if (threadIdx.x == 0)
@@ -225,7 +247,7 @@ __device__ __forceinline__ void instance_sums(const double* __restrict__ part, i
     if (threadIdx.x < kWave) {
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
         for (int t = tb + int(threadIdx.x); t < te; t += kWave) {
-            const double* p = part + long(CHX_TILE_PARTIALS) * t;
+            const gp<const double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
             s0 += p[f0];
             s1 += p[f1];
             s2 += p[f2];
@@ -250,7 +272,7 @@ __device__ __forceinline__ void instance_sums(const double* __restrict__ part, i
 // refinement step).  Finishing is final: once an instance stops, its tiles no
 // longer refresh the parity-indexed partials, so the test is never re-run on
 // them (`finished`).
-__device__ __forceinline__ bool instance_active(const chx_instance& I, int k, double rr, double bb, bool checkable,
+__device__ __forceinline__ bool instance_active(const Inst& I, int k, double rr, double bb, bool checkable,
                                                 bool finished) {
     if (finished) return false;
     if (k >= I.max_iter) return false;
@@ -269,61 +291,92 @@ __device__ __forceinline__ bool instance_active(const chx_instance& I, int k, do
 // ---------------------------------------------------------------------------
 // Setup / refinement step of one tile: r = b - A32 x (true residual with
 // the FP32 matrix), M^-1 = inverse(D) (setup only), z = M^-1 r, p = z; tile
-// partials <r,z>, <r,r> (and <b,b> at setup).
-template <int L>
-__device__ __forceinline__ void residual_tile(const chx_instance& I, int rb, int re, bool setup, float* pdst,
-                                              double (&acc)[3]) {
-    const float* xv = I.x;
+// partials <r,z>, <r,r> (and <b,b> at setup).  M^-1 is stored planar.
+template <int L, class MatR>
+__device__ __forceinline__ void residual_tile(const Inst& I, MatR mat, int rb, int re, bool setup,
+                                              gp<float> pdst, double (&acc)[3]) {
+    const gp<const float> xv = I.x;
+    const unsigned n = unsigned(I.n_rows);
     spmv_rows<L>(
-        I.row_offsets, I.col_indices, Mat32{I.diag, I.values}, rb, re, [&](int j) { return ld3(xv, j); },
-        [&](int row, f3 Ax) {
-            const f3 b = ld3(I.rhs, row);
+        I.row_offsets, I.col_indices, mat, rb, re, [&](unsigned j) { return ld3g(xv, j); },
+        [&](unsigned row, f3 Ax) {
+            const f3 b = ld3g(I.rhs, row);
             const f3 r = sub(b, Ax);
             float mi[9];
             if (setup) {
-                inverse9(I.diag + 9L * row, mi);
-                float* mo = I.minv + 9L * row;
+                float d9[9];
+                mat.diag_block(row, d9);
+                inverse9(d9, mi);
 #pragma unroll
-                for (int k = 0; k < 9; ++k) mo[k] = mi[k];
+                for (unsigned k = 0; k < 9; ++k) stx(I.minv, k * n + row, mi[k]);
             } else {
-                const float* mo = I.minv + 9L * row;
 #pragma unroll
-                for (int k = 0; k < 9; ++k) mi[k] = mo[k];
+                for (unsigned k = 0; k < 9; ++k) mi[k] = ldx(gp<const float>(I.minv), k * n + row);
             }
             const f3 z = mv9(mi, r);
-            st3(I.r, row, r);
-            st3(I.z, row, z);
-            st3(pdst, row, z);
+            st3g(I.r, row, r);
+            st3g(I.z, row, z);
+            st3g(pdst, row, z);
             acc[0] += dot64(r, z);
             acc[1] += dot64(r, r);
             acc[2] += dot64(b, b);
         });
 }
 
-template <int L, bool BF16>
+// Matrix variants of the PCG kernel: the view of the inner SpMV (A) and of
+// the exact residual / preconditioner set-up (R).
+//   V = 0  row-major 3x3 blocks (standalone chx_pcg_solve_bsr3)
+//   V = 1  planar FP32 blocks (cloth instances)
+//   V = 2  planar BF16 copies for the inner SpMV, planar FP32 for refinement
+template <int V>
+struct PcgMats;
+template <>
+struct PcgMats<0> {
+    static __device__ __forceinline__ Mat32G a(const Inst& I) { return Mat32G{I.diag, I.values}; }
+    static __device__ __forceinline__ Mat32G r(const Inst& I) { return a(I); }
+};
+template <>
+struct PcgMats<1> {
+    static __device__ __forceinline__ MatP32 a(const Inst& I) {
+        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.nnz)};
+    }
+    static __device__ __forceinline__ MatP32 r(const Inst& I) { return a(I); }
+};
+template <>
+struct PcgMats<2> {
+    static __device__ __forceinline__ MatP16 a(const Inst& I) {
+        return MatP16{I.diag16, I.values16, unsigned(I.n_rows), unsigned(I.nnz)};
+    }
+    static __device__ __forceinline__ MatP32 r(const Inst& I) {
+        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.nnz)};
+    }
+};
+
+template <int L, int V>
 __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int finalize) {
+    constexpr bool BF16 = V == 2;
     __shared__ double s_red[16];
     __shared__ double s_sc[5];
     const unsigned G = gridDim.x;
     unsigned* sync = B.sync;
-    double* part = B.partials;
+    const gp<double> part = glb(B.partials);
     // BF16 mode: every `period` iterations the next iteration starts from the
     // true residual (FP32 matrix) and restarts the search directions.
-    const int period = BF16 ? max(B.ir_period, 1) : 0;
+    const int period = BF16 ? max(B.ir_period, 1) : 1;
 
     // ---- setup: r = b - A x, M^-1, z = M^-1 r, p = z; rho, ||r||^2, ||b||^2 ----
     if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&sync[SYNC_ACTIVE], unsigned(B.n_instances));
     for (int t = blockIdx.x; t < B.n_tiles; t += G) {
         int inst, rb, re;
         get_tile(B, t, inst, rb, re);
-        const chx_instance& I = inst_at(B, inst);
+        const Inst I = inst_at(B, inst);
         if (threadIdx.x == 0 && t == I.tile_begin)
             __hip_atomic_store(&B.inst_done[inst], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         double acc[3] = {0.0, 0.0, 0.0};
-        residual_tile<L>(I, rb, re, true, I.p0, acc);
+        residual_tile<L>(I, PcgMats<V>::r(I), rb, re, true, I.p0, acc);
         block_sum(acc, s_red);
         if (threadIdx.x == 0) {
-            double* p = part + long(CHX_TILE_PARTIALS) * t;
+            const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
             p[P_RHO0] = acc[0];
             p[P_RR0] = acc[1];
             p[P_BB] = acc[2];
@@ -341,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         for (int t = blockIdx.x; t < B.n_tiles; t += G) {
             int inst, rb, re;
             get_tile(B, t, inst, rb, re);
-            const chx_instance& I = inst_at(B, inst);
+            const Inst I = inst_at(B, inst);
             if (inst != cached) {
                 // A refinement step rewrites the rho slot of parity cur^1, so
                 // it does not read it (its beta is not needed).
@@ -361,50 +414,49 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 }
                 continue;
             }
-            const float* pprev = cur ? I.p0 : I.p1;  // p_{k-1}
-            float* pk = cur ? I.p1 : I.p0;           // p_k
+            const gp<float> pprev = cur ? I.p0 : I.p1;  // p_{k-1}
+            const gp<float> pk = cur ? I.p1 : I.p0;     // p_k
             if (refine) {
                 // True residual of the current x; p_{k+1} = z goes to the
                 // buffer of parity k+1 (p_{k-1}'s, unused this iteration).
                 double acc[3] = {0.0, 0.0, 0.0};
-                residual_tile<L>(I, rb, re, false, const_cast<float*>(pprev), acc);
+                residual_tile<L>(I, PcgMats<V>::r(I), rb, re, false, pprev, acc);
                 block_sum(acc, s_red);
                 if (threadIdx.x == 0) {
-                    double* p = part + long(CHX_TILE_PARTIALS) * t;
+                    const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
                     p[P_RHO0 + (cur ^ 1)] = acc[0];
                     p[P_RR0 + (cur ^ 1)] = acc[1];
                 }
                 continue;
             }
-            const float* z = I.z;
-            float* q = I.q;
+            const gp<const float> z = I.z;
+            const gp<float> q = I.q;
             double acc[1] = {0.0};
             auto run = [&](auto mat) {
                 if (j == 0) {
                     spmv_rows<L>(
-                        I.row_offsets, I.col_indices, mat, rb, re, [&](int jj) { return ld3(pk, jj); },
-                        [&](int row, f3 Ap) {
-                            st3(q, row, Ap);
-                            acc[0] += dot64(ld3(pk, row), Ap);
+                        I.row_offsets, I.col_indices, mat, rb, re, [&](unsigned jj) { return ld3g(pk, jj); },
+                        [&](unsigned row, f3 Ap) {
+                            st3g(q, row, Ap);
+                            acc[0] += dot64(ld3g(pk, row), Ap);
                         });
                 } else {
                     const float beta = guarded_div(rho, rho_prev);
-                    auto pj = [&](int jj) {
-                        const f3 zj = ld3(z, jj), pp = ld3(pprev, jj);
+                    auto pj = [&](unsigned jj) {
+                        const f3 zj = ld3g(z, jj), pp = ld3g(pprev, jj);
                         return mk(fmaf(beta, pp.x, zj.x), fmaf(beta, pp.y, zj.y), fmaf(beta, pp.z, zj.z));
                     };
-                    spmv_rows<L>(I.row_offsets, I.col_indices, mat, rb, re, pj, [&](int row, f3 Ap) {
+                    spmv_rows<L>(I.row_offsets, I.col_indices, mat, rb, re, pj, [&](unsigned row, f3 Ap) {
                         const f3 pr = pj(row);
-                        st3(pk, row, pr);
-                        st3(q, row, Ap);
+                        st3g(pk, row, pr);
+                        st3g(q, row, Ap);
                         acc[0] += dot64(pr, Ap);
                     });
                 }
             };
-            if (BF16) run(Mat16{I.diag16, I.values16});
-            else run(Mat32{I.diag, I.values});
+            run(PcgMats<V>::a(I));
             block_sum(acc, s_red);
-            if (threadIdx.x == 0) part[long(CHX_TILE_PARTIALS) * t + P_SIG] = acc[0];
+            if (threadIdx.x == 0) rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS))[P_SIG] = acc[0];
         }
         unsigned active = 0;
         if (!grid_sync(sync, G, cur, &active)) return;
@@ -416,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         for (int t = blockIdx.x; t < B.n_tiles; t += G) {
             int inst, rb, re;
             get_tile(B, t, inst, rb, re);
-            const chx_instance& I = inst_at(B, inst);
+            const Inst I = inst_at(B, inst);
             if (inst != cached) {
                 instance_sums(part, I.tile_begin, I.tile_end, P_RHO0 + cur, P_SIG, P_RR0 + cur, P_BB, s_sc,
                               &B.inst_done[inst]);
@@ -425,23 +477,24 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
             const double rho = s_sc[0], sigma = s_sc[1], rr = s_sc[2], bb = s_sc[3];
             if (refine || !instance_active(I, k, rr, bb, checkable, s_sc[4] != 0.0)) continue;
             const float alpha = guarded_div(rho, sigma);
-            const float* pk = cur ? I.p1 : I.p0;
+            const gp<const float> pk = cur ? I.p1 : I.p0;
+            const unsigned n = unsigned(I.n_rows);
             double acc[2] = {0.0, 0.0};
-            for (int row = rb + int(threadIdx.x); row < re; row += blockDim.x) {
-                const f3 p = ld3(pk, row), qq = ld3(I.q, row);
-                f3 x = ld3(I.x, row), r = ld3(I.r, row);
+            for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
+                const f3 p = ld3g(pk, row), qq = ld3g(I.q, row);
+                f3 x = ld3g(I.x, row), r = ld3g(I.r, row);
                 x = mk(fmaf(alpha, p.x, x.x), fmaf(alpha, p.y, x.y), fmaf(alpha, p.z, x.z));
                 r = mk(fmaf(-alpha, qq.x, r.x), fmaf(-alpha, qq.y, r.y), fmaf(-alpha, qq.z, r.z));
-                const f3 zz = mv9(I.minv + 9L * row, r);
-                st3(I.x, row, x);
-                st3(I.r, row, r);
-                st3(I.z, row, zz);
+                const f3 zz = mv9p(gp<const float>(I.minv), n, row, r);
+                st3g(I.x, row, x);
+                st3g(I.r, row, r);
+                st3g(I.z, row, zz);
                 acc[0] += dot64(r, zz);
                 acc[1] += dot64(r, r);
             }
             block_sum(acc, s_red);
             if (threadIdx.x == 0) {
-                double* p = part + long(CHX_TILE_PARTIALS) * t;
+                const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
                 p[P_RHO0 + nxt] = acc[0];
                 p[P_RR0 + nxt] = acc[1];
             }
@@ -454,16 +507,16 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         for (int t = blockIdx.x; t < B.n_tiles; t += G) {
             int inst, rb, re;
             get_tile(B, t, inst, rb, re);
-            const chx_instance& I = inst_at(B, inst);
+            const Inst I = inst_at(B, inst);
             const float inv_dt = 1.0f / I.dt;
             const float decay = (I.damping > 0.0f) ? expf(-I.damping * I.dt) : 1.0f;
-            for (int row = rb + int(threadIdx.x); row < re; row += blockDim.x) {
-                const f3 d = ld3(I.x, row);
-                const f3 p = ld3(I.pos, row);
-                st3(I.pos, row, add(p, d));
+            for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
+                const f3 d = ld3g(I.x, row);
+                const f3 p = ld3g(I.pos, row);
+                st3g(I.pos, row, add(p, d));
                 f3 v = mul(d, inv_dt);
                 if (I.damping > 0.0f) v = mul(v, decay);
-                st3(I.vel, row, v);
+                st3g(I.vel, row, v);
             }
         }
     }
@@ -480,11 +533,13 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const float* __rest
     constexpr int RPB = (kBlock / kWave) * (kWave / L);  // rows per block
     const int rb = blockIdx.x * RPB;
     const int re = min(n, rb + RPB);
+    const gp<const float> xg = glb(x);
+    const gp<float> yg = glb(y);
     spmv_rows<L>(
-        ro, ci, Mat32{diag, val}, rb, re, [&](int j) { return ld3(x, j); },
-        [&](int row, f3 Ax) {
-            if (beta == 0.0f) st3(y, row, mul(Ax, alpha));
-            else st3(y, row, add(mul(ld3(y, row), beta), mul(Ax, alpha)));
+        glb(ro), glb(ci), Mat32G{glb(diag), glb(val)}, rb, re, [&](unsigned j) { return ld3g(xg, j); },
+        [&](unsigned row, f3 Ax) {
+            if (beta == 0.0f) st3g(yg, row, mul(Ax, alpha));
+            else st3g(yg, row, add(mul(ld3g(yg, row), beta), mul(Ax, alpha)));
         });
 }
 
@@ -493,11 +548,21 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const float* __rest
 // ---------------------------------------------------------------------------
 using PcgFn = void (*)(chx_cloth_batch, int);
 
-PcgFn pcg_fn(int L, bool bf16 = false) {
+template <int L>
+PcgFn pcg_fn_l(int v) {
+    switch (v) {
+        case 0: return pcg_kernel<L, 0>;
+        case 1: return pcg_kernel<L, 1>;
+        default: return pcg_kernel<L, 2>;
+    }
+}
+
+// v: matrix variant (0 row-major, 1 planar FP32, 2 planar BF16 + FP32).
+PcgFn pcg_fn(int L, int v = 1) {
     switch (L) {
-        case 4: return bf16 ? pcg_kernel<4, true> : pcg_kernel<4, false>;
-        case 8: return bf16 ? pcg_kernel<8, true> : pcg_kernel<8, false>;
-        case 16: return bf16 ? pcg_kernel<16, true> : pcg_kernel<16, false>;
+        case 4: return pcg_fn_l<4>(v);
+        case 8: return pcg_fn_l<8>(v);
+        case 16: return pcg_fn_l<16>(v);
         default: throw Error(CHX_ERR_INVALID_ARGUMENT, "chysx: lanes_per_row must be 4, 8 or 16");
     }
 }
@@ -505,8 +570,8 @@ PcgFn pcg_fn(int L, bool bf16 = false) {
 struct DeviceLimits {
     int device = -1;
     int cus = 0;
-    int max_blocks[2][17] = {{0}};
-    int api_blocks[2][17] = {{0}};
+    int max_blocks[3][17] = {{0}};
+    int api_blocks[3][17] = {{0}};
 };
 
 std::mutex g_lim_mu;
@@ -518,7 +583,7 @@ DeviceLimits g_lim[16];
 // gfx950), so every workgroup of the cooperative launch is resident.
 constexpr int kBlocksPerCu = 4;
 
-int max_coresident(int L, bool bf16 = false) {
+int max_coresident(int L, int v = 1) {
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     std::lock_guard<std::mutex> lk(g_lim_mu);
@@ -530,11 +595,11 @@ int max_coresident(int L, bool bf16 = false) {
         hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
         d.cus = prop.multiProcessorCount;
     }
-    const int m = bf16 ? 1 : 0;
+    const int m = v;
     if (d.max_blocks[m][L] == 0) {
         int per_cu = 0;
         hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                      &per_cu, reinterpret_cast<const void*>(pcg_fn(L, bf16)), kBlock, 0),
+                      &per_cu, reinterpret_cast<const void*>(pcg_fn(L, v)), kBlock, 0),
                   "hipOccupancyMaxActiveBlocksPerMultiprocessor");
         d.api_blocks[m][L] = per_cu;
         if (per_cu < 1) per_cu = 1;
@@ -543,11 +608,14 @@ int max_coresident(int L, bool bf16 = false) {
     return d.max_blocks[m][L];
 }
 
-void launch_pcg(const chx_cloth_batch& B, int finalize, hipStream_t stream) {
+// row_major: the caller's blocks are row-major 3x3 (standalone solve);
+// cloth instances store planar blocks.
+void launch_pcg(const chx_cloth_batch& B, int finalize, hipStream_t stream, bool row_major = false) {
     if (B.n_tiles <= 0) return;
     const int L = B.lanes_per_row;
-    PcgFn fn = pcg_fn(L, B.bf16_blocks != 0);
-    int grid = max_coresident(L, B.bf16_blocks != 0);
+    const int v = row_major ? 0 : (B.bf16_blocks != 0 ? 2 : 1);
+    PcgFn fn = pcg_fn(L, v);
+    int grid = max_coresident(L, v);
     if (B.grid_blocks > 0) grid = std::min(grid, B.grid_blocks);
     grid = std::min(grid, B.n_tiles);
     chx_cloth_batch arg = B;
@@ -637,6 +705,7 @@ int chx_spmv_bsr3(int n_rows, const float* diag, const int* row_offsets, const i
     return guarded([&] {
         require(n_rows >= 0, "spmv: negative row count");
         if (n_rows == 0) return;
+        require(size_t(n_rows) * 36u < (size_t(1) << 32), "spmv: block arrays must stay below 4 GiB");
         require(diag && row_offsets && x && y, "spmv: null pointer");
         const hipStream_t s = as_stream(stream);
         auto go = [&](auto kern, int L) {
@@ -679,6 +748,7 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
             return;
         }
         require(diag && row_offsets && b && x && workspace, "pcg: null pointer");
+        require(size_t(n_rows) * 36u < (size_t(1) << 32), "pcg: block arrays must stay below 4 GiB");
         size_t need = 0;
         if (chx_pcg_workspace_size(n_rows, &need) != CHX_OK) throw Error(CHX_ERR_RUNTIME, chx_last_error_message());
         require(workspace_bytes >= need, "pcg: workspace too small (see chx_pcg_workspace_size)");
@@ -721,10 +791,10 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
         B.inst_done = reinterpret_cast<int*>(take(sizeof(int)));
         B.sync = reinterpret_cast<unsigned*>(take(SYNC_WORDS * sizeof(unsigned)));
         const hipStream_t s = as_stream(stream);
-        pcg_fn(lanes_per_row);  // validates L
+        pcg_fn(lanes_per_row, 0);  // validates L
         reset_sync_kernel<<<1, kBlock, 0, s>>>(B.sync);
         hip_check(hipGetLastError(), "reset_sync_kernel launch");
-        launch_pcg(B, 0, s);
+        launch_pcg(B, 0, s, true);
         if (iterations || rr || bb) {
             int it = 0;
             double r2 = 0.0, b2 = 0.0;

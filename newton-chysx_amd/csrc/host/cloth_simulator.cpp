@@ -1,5 +1,7 @@
 #include "cloth_simulator.h"
 
+#include <vector>
+
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -299,12 +301,24 @@ double ClothSimulator::last_relative_residual(std::uintptr_t cuda_stream) {
     return bb[0] > 0.0 ? std::sqrt(rr[0] / bb[0]) : std::sqrt(rr[0]);
 }
 
+namespace {
+// Device blocks are planar (component c of block i at p[c*count + i]); the
+// reference's debug taps return row-major 3x3 blocks.
+void copy_planar_blocks(float* out, const float* dev, int count, const char* what) {
+    std::vector<float> planes(static_cast<std::size_t>(9) * count);
+    check(chx_memcpy_d2h(planes.data(), dev, sizeof(float) * planes.size(), 0), "cloth", what);
+    for (int i = 0; i < count; ++i)
+        for (int c = 0; c < 9; ++c)
+            out[static_cast<std::size_t>(9) * i + c] = planes[static_cast<std::size_t>(c) * count + i];
+}
+}  // namespace
+
 void ClothSimulator::debug_copy_hessian_diag(float* out) const {
     const int n = H_num_block_rows_;
     if (n == 0) return;
     if (out == nullptr) throw std::invalid_argument("ClothSimulator::debug_copy_hessian_diag: out must be non-null");
     check(chx_device_synchronize(), "cloth", "device synchronize(debug_copy_hessian_diag)");
-    check(chx_memcpy_d2h(out, inst_.diag(), sizeof(float) * 9 * n, 0), "cloth", "diag copy");
+    copy_planar_blocks(out, inst_.diag(), n, "diag copy");
 }
 
 void ClothSimulator::debug_copy_hessian_csr(int* out_row_offsets, int* out_col_indices, float* out_values) const {
@@ -320,7 +334,7 @@ void ClothSimulator::debug_copy_hessian_csr(int* out_row_offsets, int* out_col_i
             throw std::invalid_argument(
                 "ClothSimulator::debug_copy_hessian_csr: col_indices/values null but nnz_off > 0");
         std::memcpy(out_col_indices, pattern_.col_indices.data(), sizeof(int) * nnz);
-        check(chx_memcpy_d2h(out_values, inst_.values(), sizeof(float) * 9 * nnz, 0), "cloth", "values copy");
+        copy_planar_blocks(out_values, inst_.values(), nnz, "values copy");
     }
 }
 

@@ -121,6 +121,16 @@ void InstanceData::build(const InstanceSource& s) {
     std::vector<unsigned> fill(off.begin(), off.end() - 1);
     walk([&](std::size_t g, unsigned kind, int a, int b, std::size_t id) { ent[fill[g]++] = encode(kind, a, b, id); });
 
+    // The kernels address every per-instance array with 32-bit byte offsets.
+    const auto fits = [](std::size_t count, std::size_t bytes_each) {
+        return count * bytes_each < (std::size_t(1) << 32);
+    };
+    if (!fits(static_cast<std::size_t>(n_), 36) || !fits(static_cast<std::size_t>(nnz_), 36) ||
+        !fits(tris_.cpu_size(), sizeof(float) * CHX_TRI_REC) || !fits(dihs_.cpu_size(), sizeof(float) * CHX_DIH_REC) ||
+        !fits(pins_.cpu_size(), sizeof(chx_pin)) || !fits(ent.size(), sizeof(unsigned)))
+        throw std::invalid_argument(
+            "chysx::engine: a per-instance array would reach 4 GiB (at most 119,304,647 vertices or off-diagonal "
+            "blocks per cloth); split the cloth into several instances");
     upload(row_offsets_, bytes_);
     upload(col_indices_, bytes_);
     upload(contrib_offsets_, bytes_);
