@@ -99,6 +99,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the lanes-per-row x grid-size sweep and use --lanes / --grid")
     ap.add_argument("--grid", type=int, default=0, help="persistent PCG workgroups (0: all co-resident)")
+    ap.add_argument("--tile-rows", type=int, default=256, help="rows per PCG tile (multiple of 256)")
     ap.add_argument("--no-plain-large", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run an elementwise copy of this many MiB (calibrates HBM byte counters)")
@@ -124,6 +125,7 @@ def main():
     batch.set_lanes_per_row(args.lanes)
     if args.grid > 0:
         batch.set_grid_blocks(args.grid)
+    batch.set_tile_rows(args.tile_rows)
     for c in cloths:
         batch.add_instance(c.sim)
     batch.finalize(specs[0].dt)
@@ -142,7 +144,9 @@ def main():
         tuned = autotune.tune_batch(batch, dt, stream, cache_path=None, log=log)
         args.lanes = tuned["lanes_per_row"]
         args.grid = tuned["grid_blocks"]
-        log(f"autotune picked lanes={tuned['lanes_per_row']} grid={tuned['grid_blocks']}")
+        args.tile_rows = tuned["tile_rows"]
+        log(f"autotune picked lanes={tuned['lanes_per_row']} grid={tuned['grid_blocks']} "
+            f"tile_rows={tuned['tile_rows']}")
 
     with torch.cuda.stream(stream):
         for _ in range(args.warmup):
@@ -175,7 +179,7 @@ def main():
     mean_ms = float(np.mean(pcg_ms))
     achieved = float(np.mean(pcg_bytes)) / (mean_ms * 1e-3) / 1e9
     traffic = load_traffic()
-    key = f"lanes{args.lanes}_grid{args.grid}_inst{len(specs)}"
+    key = f"lanes{args.lanes}_grid{args.grid}_tile{args.tile_rows}_inst{len(specs)}"
     traffic_bytes = None
     if traffic and key in traffic.get("configs", {}):
         traffic_bytes = traffic["configs"][key]["bytes_per_launch"]
@@ -263,7 +267,7 @@ def main():
         "config": {"workload": "batched: 512 cloth grids, side logU[16,128] vertices (seed 2), BW98 + bending, "
                                "PCG tol 1e-6 cap 100, one fused substep per step",
                    "instances": len(specs), "vertices": n_rows, "off_diag_blocks": int(sizes[:, 1].sum()),
-                   "pcg_tol": 1e-6, "pcg_cap": 100, "lanes_per_row": args.lanes, "grid_blocks": args.grid,
+                   "pcg_tol": 1e-6, "pcg_cap": 100, "lanes_per_row": args.lanes, "grid_blocks": args.grid, "tile_rows": args.tile_rows,
                    "parallelism": "single GPU"},
         "pcg_iterations": {"mean": float(iters.mean()), "max": int(iters.max()), "min": int(iters.min())},
         "max_relative_residual": float(res.max()),

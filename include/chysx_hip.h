@@ -73,9 +73,13 @@ typedef struct chx_dih { int v0, v1, v2, v3; float rest; float k; int pad0, pad1
 /* 32-bit words of the grid-barrier / control area `sync` of a batch. */
 #define CHX_SYNC_WORDS 448
 
-/* Rows per PCG row tile (one workgroup pass) and elements / BSR slots per
+/* Default rows per PCG row tile (the unit of workgroup assignment and of the
+ * FP64 partial sums; the batch may use any multiple of CHX_STAGE_ROWS),
+ * rows per staging step of the PCG's SpMV (the rows whose pattern and source
+ * window the kernel loads into LDS at once), and elements / BSR slots per
  * element-pass and gather-pass work item. */
 #define CHX_TILE_ROWS 256
+#define CHX_STAGE_ROWS 256
 #define CHX_ITEM_SIZE 256
 
 /* One cloth instance: every array is the instance's own device memory, with
@@ -109,6 +113,9 @@ typedef struct chx_instance {
     float* minv;                     /* [9][n_rows] planar block-Jacobi preconditioner       */
     uint16_t* diag16;                /* planar BF16 copies of diag / values for the inner    */
     uint16_t* values16;              /* SpMV of the BF16 precision mode, or NULL             */
+    const int* windows;              /* [ceil(n_rows/CHX_STAGE_ROWS)][2]: smallest and       */
+                                     /* largest column (own rows included) of each stage of  */
+                                     /* CHX_STAGE_ROWS rows, or NULL (no LDS staging)        */
     int n_rows, nnz, n_pins, n_tris, n_dihs;
     int tile_begin;                  /* this instance's row tiles in the batch tile list     */
     int tile_end;
@@ -121,12 +128,13 @@ typedef struct chx_instance {
 } chx_instance;
 
 /* A batch of instances advanced together.  Work lists are device int4
- * arrays: row tiles {instance, row_begin, row_end, 0} (local rows, at most
- * CHX_TILE_ROWS each, grouped by instance), element items {instance, kind
+ * arrays: row tiles {instance, row_begin, row_end, 0} (local rows, row_begin
+ * a multiple of CHX_STAGE_ROWS, grouped by instance), element items {instance, kind
  * (0 pin, 1 triangle, 2 dihedral), begin, end} and slot items {instance,
  * begin, end, 0} (at most CHX_ITEM_SIZE each).  A batch with inst == NULL
- * holds one instance in `single` (passed by value in the kernel arguments)
- * and its work lists are implied by the counts. */
+ * holds one instance in `single` (passed by value in the kernel arguments);
+ * its element and slot lists are implied by the counts, and so is its tile
+ * list when `tiles` is NULL (CHX_TILE_ROWS rows per tile). */
 typedef struct chx_cloth_batch {
     int n_instances;
     int n_tiles;
@@ -136,6 +144,8 @@ typedef struct chx_cloth_batch {
     int grid_blocks;                 /* persistent PCG workgroups (0 = auto)          */
     int bf16_blocks;                 /* 1: inner SpMV on BF16 blocks (FP32 accumulate) */
     int ir_period;                   /* BF16 mode: true-residual refinement period    */
+    int tile_rows;                   /* rows per tile of the tile list (informational) */
+    int pad0;
     chx_instance single;
     const chx_instance* inst;        /* device instance table, or NULL                */
     const int* tiles;
@@ -147,6 +157,13 @@ typedef struct chx_cloth_batch {
     double* inst_bb;                 /* [n_instances] ||b||^2                         */
     int* inst_done;                  /* [n_instances] scratch                         */
     unsigned* sync;                  /* [CHX_SYNC_WORDS] grid barrier + control words */
+    unsigned long long* trace;       /* optional solver trace, or NULL: entry e =      */
+                                     /* {100 MHz real-time stamp, active instances} at */
+                                     /* barrier e (0: after set-up, 2k+1 / 2k+2: after */
+                                     /* phase A / B of iteration k), written by        */
+                                     /* workgroup 0                                    */
+    int trace_entries;               /* capacity of `trace` in entries (2 words each) */
+    int pad1;
 } chx_cloth_batch;
 
 /* ---- errors and runtime ------------------------------------------------- */

@@ -1,8 +1,8 @@
 """Autotuner of the batched cloth substep.
 
 Sweeps the SpMV lanes per block row (4, 8, 16 — i.e. 16, 8 or 4 block rows
-per wavefront) and the persistent PCG grid size (workgroups per compute
-unit) on the actual batch, timing whole substeps with HIP events on the
+per wavefront), the persistent PCG grid size (workgroups per compute unit)
+and the rows per PCG tile (256, 512, 1024) on the actual batch, timing whole substeps with HIP events on the
 caller's stream, and keeps the fastest.  Results are cached in a JSON file
 keyed by device name and a workload signature (instance count, vertices,
 off-diagonal blocks) so a later run of the same workload skips the sweep.
@@ -32,10 +32,11 @@ def _device_name() -> str:
     return torch.cuda.get_device_name(0)
 
 
-def tune_batch(batch, dt: float, stream=None, lanes=(4, 8, 16), blocks_per_cu=(2, 3, 4), warmup: int = 1,
-               steps: int = 3, cache_path: str | None = DEFAULT_CACHE, log=None) -> dict:
-    """Returns {"lanes_per_row", "grid_blocks", "ms_per_substep", "table"} and
-    applies the best setting to `batch`."""
+def tune_batch(batch, dt: float, stream=None, lanes=(4, 8, 16), blocks_per_cu=(1, 2, 3), tile_rows=(256, 512, 1024),
+               warmup: int = 1, steps: int = 3, cache_path: str | None = DEFAULT_CACHE, log=None) -> dict:
+    """Returns {"lanes_per_row", "grid_blocks", "tile_rows", "ms_per_substep",
+    "table"} and applies the best setting to `batch`.  Lanes and grid size
+    are swept at the first tile size, then the tile size at the best pair."""
     import torch
 
     from . import abi
@@ -52,32 +53,44 @@ def tune_batch(batch, dt: float, stream=None, lanes=(4, 8, 16), blocks_per_cu=(2
         if cached:
             batch.set_lanes_per_row(int(cached["lanes_per_row"]))
             batch.set_grid_blocks(int(cached["grid_blocks"]))
+            batch.set_tile_rows(int(cached.get("tile_rows", 256)))
             return dict(cached, cached=True)
     cus, cap, _ = abi.device_info()
     s = stream if stream is not None else torch.cuda.current_stream()
     sh = s.cuda_stream
     table = []
+
+    def trial(L, g, T):
+        batch.set_lanes_per_row(L)
+        batch.set_grid_blocks(g)
+        batch.set_tile_rows(T)
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                batch.step(dt, sh)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(steps):
+                batch.step(dt, sh)
+            e1.record(s)
+            e1.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        table.append({"lanes_per_row": L, "grid_blocks": g, "tile_rows": T, "ms_per_substep": ms})
+        if log:
+            log(f"autotune lanes={L} grid={g} tile_rows={T}: {ms:.3f} ms/substep")
+
     for L in lanes:
         for b in blocks_per_cu:
             g = min(cap, b * cus)
-            batch.set_lanes_per_row(L)
-            batch.set_grid_blocks(g)
-            with torch.cuda.stream(s):
-                for _ in range(warmup):
-                    batch.step(dt, sh)
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(s)
-                for _ in range(steps):
-                    batch.step(dt, sh)
-                e1.record(s)
-                e1.synchronize()
-            ms = e0.elapsed_time(e1) / steps
-            table.append({"lanes_per_row": L, "grid_blocks": g, "ms_per_substep": ms})
-            if log:
-                log(f"autotune lanes={L} grid={g}: {ms:.3f} ms/substep")
+            if any(r["lanes_per_row"] == L and r["grid_blocks"] == g for r in table):
+                continue
+            trial(L, g, tile_rows[0])
+    best = min(table, key=lambda r: r["ms_per_substep"])
+    for T in tile_rows[1:]:
+        trial(best["lanes_per_row"], best["grid_blocks"], T)
     best = min(table, key=lambda r: r["ms_per_substep"])
     batch.set_lanes_per_row(best["lanes_per_row"])
     batch.set_grid_blocks(best["grid_blocks"])
+    batch.set_tile_rows(best["tile_rows"])
     result = dict(best, table=table, device=dev, signature=sig, tuned_at=time.time())
     if cache_path:
         try:
@@ -86,7 +99,7 @@ def tune_batch(batch, dt: float, stream=None, lanes=(4, 8, 16), blocks_per_cu=(2
             if os.path.exists(cache_path):
                 with open(cache_path) as f:
                     data = json.load(f)
-            data[key] = {k: result[k] for k in ("lanes_per_row", "grid_blocks", "ms_per_substep")}
+            data[key] = {k: result[k] for k in ("lanes_per_row", "grid_blocks", "tile_rows", "ms_per_substep")}
             with open(cache_path, "w") as f:
                 json.dump(data, f, indent=1)
         except (OSError, ValueError):

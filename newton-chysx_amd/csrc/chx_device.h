@@ -196,6 +196,7 @@ struct Inst {
     gp<const unsigned> contrib_offsets, contribs;
     gp<float> rhs, x, r, z, p0, p1, q, minv;
     gp<uint16_t> diag16, values16;
+    gp<const int> windows;
     int n_rows, nnz, n_pins, n_tris, n_dihs, tile_begin, tile_end;
     float dt, damping, gx, gy, gz;
     int max_iter;
@@ -230,6 +231,7 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.minv = glb(d->minv);
     I.diag16 = glb(d->diag16);
     I.values16 = glb(d->values16);
+    I.windows = glb(d->windows);
     I.n_rows = d->n_rows;
     I.nnz = d->nnz;
     I.n_pins = d->n_pins;
@@ -321,9 +323,9 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// Sum over a segment of `width` consecutive lanes (width a power of two <= 64).
-template <int W>
-__device__ __forceinline__ float seg_sum(float v) {
+// Sum over a segment of W consecutive lanes (W a power of two <= 64).
+template <int W, class T>
+__device__ __forceinline__ T seg_sum(T v) {
 #pragma unroll
     for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;

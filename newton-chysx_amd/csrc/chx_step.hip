@@ -188,8 +188,9 @@ __global__ void reset_sync_kernel(unsigned* sync) {
 
 // ---------------------------------------------------------------------------
 // SpMV row loop: L lanes per block row, 64/L rows per wave and pass.
-// src(j) returns the source vector entry j; fin(row, Ax) runs on the first
-// lane of each row after the segmented reduction.
+// ro_at(r) / ci_at(k) give row offsets and column indices, src(j) the source
+// vector entry j; fin(row, Ax, x_row) runs on the first lane of each row
+// after the segmented reduction (x_row = src(row)).
 //
 // Each lane takes U = 16/L of its row's off-diagonal blocks per pass (a pass
 // covers 16 blocks of a row), loading them at clamped indices without
@@ -199,9 +200,8 @@ __global__ void reset_sync_kernel(unsigned* sync) {
 // Per lane the sum is diag + off[sub] + off[sub+L] + ..., then a segmented
 // butterfly over the L lanes.
 // ---------------------------------------------------------------------------
-template <int L, class Mat, class Src, class Fin>
-__device__ __forceinline__ void spmv_rows(gp<const int> ro, gp<const int> ci, Mat mat, int rb, int re, Src src,
-                                          Fin fin) {
+template <int L, class Mat, class RoAt, class CiAt, class Src, class Fin>
+__device__ __forceinline__ void spmv_core(Mat mat, int rb, int re, RoAt ro_at, CiAt ci_at, Src src, Fin fin) {
     constexpr int RPW = kWave / L;
     constexpr int U = (16 / L) > 1 ? (16 / L) : 1;
     const int lane = threadIdx.x & (kWave - 1);
@@ -212,9 +212,10 @@ __device__ __forceinline__ void spmv_rows(gp<const int> ro, gp<const int> ci, Ma
         const int row = base + g;
         const bool valid = row < re;
         const unsigned rowc = unsigned(valid ? row : rb);
-        const unsigned beg = unsigned(ldx(ro, rowc));
-        const unsigned end = valid ? unsigned(ldx(ro, rowc + 1u)) : beg;
-        const f3 dprod = mat.diag(rowc, src(rowc));
+        const unsigned beg = ro_at(rowc);
+        const unsigned end = valid ? ro_at(rowc + 1u) : beg;
+        const f3 xr = src(rowc);
+        const f3 dprod = mat.diag(rowc, xr);
         f3 acc = (valid && sub_ == 0) ? dprod : mk(0.f, 0.f, 0.f);
         for (unsigned k0 = beg + unsigned(sub_); k0 < end; k0 += unsigned(U * L)) {
             f3 part[U];
@@ -222,7 +223,7 @@ __device__ __forceinline__ void spmv_rows(gp<const int> ro, gp<const int> ci, Ma
             for (int u = 0; u < U; ++u) {
                 const unsigned k = k0 + unsigned(u * L);
                 const unsigned kc = (k < end) ? k : k0;
-                part[u] = mat.off(kc, src(unsigned(ldx(ci, kc))));
+                part[u] = mat.off(kc, src(ci_at(kc)));
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
@@ -231,38 +232,147 @@ __device__ __forceinline__ void spmv_rows(gp<const int> ro, gp<const int> ci, Ma
         acc.x = seg_sum<L>(acc.x);
         acc.y = seg_sum<L>(acc.y);
         acc.z = seg_sum<L>(acc.z);
-        if (valid && sub_ == 0) fin(rowc, acc);
+        if (valid && sub_ == 0) fin(rowc, acc, xr);
     }
 }
 
-// Per-instance sums of up to four partial fields over the instance's tiles,
-// in a fixed order (lane-strided then butterfly), identical in every
-// workgroup.  Result in lds[0..3] for all threads; lds[4] is 1.0 when the
-// instance has already finished its solve (*done set in an earlier phase).
-__device__ __forceinline__ void instance_sums(gp<const double> part, int tb, int te, int f0, int f1, int f2,
-                                              int f3_, double* lds, int* done) {
+// ---------------------------------------------------------------------------
+// LDS of the persistent PCG kernel.  A stage is CHX_STAGE_ROWS rows of a
+// tile: before its SpMV the workgroup loads the stage's row offsets, column
+// indices and source-vector window (every row its blocks reference, from
+// chx_instance::windows) into LDS with independent coalesced loads, so the
+// SpMV's dependent chain row offset -> column -> gathered row runs in LDS and
+// only the block values come from HBM.  Stages beyond the LDS capacity (or
+// without a window) run the same SpMV from global memory.
+// ---------------------------------------------------------------------------
+constexpr int kStage = CHX_STAGE_ROWS;
+static_assert(kStage == kBlock, "one thread per stage row for the row offsets");
+constexpr unsigned kCiCap = 4096;   // column indices per staged stage
+constexpr unsigned kWinCap = 2816;  // source rows per staged stage
+constexpr int kChunk = 16;          // tiles whose instance scalars are summed together
+static_assert(kChunk * 16 == kBlock, "16 lanes per tile in chunk_sums");
+
+struct PcgLds {
+    int ro[kStage + 1];
+    int ci[kCiCap];
+    float win[3 * kWinCap];
+    double red[16];
+    double sc[kChunk][6];
+};
+
+__device__ __forceinline__ f3 lds3(const float* w, unsigned i) { return {w[3 * i], w[3 * i + 1], w[3 * i + 2]}; }
+
+// Loads stage [sb, se) of instance I into S (true), or leaves it to the
+// global path (false: no window, or over capacity).  srcg(j) is the source
+// vector entry j.  Uniform across the workgroup.
+template <class SrcG>
+__device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG srcg, PcgLds& S, unsigned& beg0,
+                                           unsigned& lo) {
+    if (I.windows == nullptr) return false;
+    const unsigned b0 = unsigned(ldx(I.row_offsets, unsigned(sb)));
+    const unsigned nnz_s = unsigned(ldx(I.row_offsets, unsigned(se))) - b0;
+    const unsigned st = unsigned(sb) / unsigned(kStage);
+    const unsigned wlo = unsigned(ldx(I.windows, 2u * st)), whi = unsigned(ldx(I.windows, 2u * st + 1u));
+    const unsigned W = whi - wlo + 1u;
+    if (nnz_s > kCiCap || W > kWinCap) return false;
+    const unsigned R = unsigned(se - sb);
+    const unsigned tid = threadIdx.x;
+    __syncthreads();  // the previous stage's readers are done
+    const unsigned iters = max(max((nnz_s + 2047u) / 2048u, (W + 1023u) / 1024u), 1u);
+    for (unsigned it = 0; it < iters; ++it) {
+        int cv[8];
+        f3 wv[4];
+        if (nnz_s > 0) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const unsigned k = it * 2048u + unsigned(u) * 256u + tid;
+                cv[u] = ldx(I.col_indices, b0 + (k < nnz_s ? k : 0u));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const unsigned w = it * 1024u + unsigned(u) * 256u + tid;
+            wv[u] = srcg(wlo + (w < W ? w : 0u));
+        }
+        if (it == 0 && tid < R) S.ro[tid] = ldx(I.row_offsets, unsigned(sb) + tid) - int(b0);
+        if (nnz_s > 0) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const unsigned k = it * 2048u + unsigned(u) * 256u + tid;
+                if (k < nnz_s) S.ci[k] = cv[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const unsigned w = it * 1024u + unsigned(u) * 256u + tid;
+            if (w < W) {
+                S.win[3 * w] = wv[u].x;
+                S.win[3 * w + 1] = wv[u].y;
+                S.win[3 * w + 2] = wv[u].z;
+            }
+        }
+    }
+    if (tid == 0) S.ro[R] = int(nnz_s);
     __syncthreads();
-    if (threadIdx.x == 0)
-        lds[4] = __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ? 1.0 : 0.0;
-    if (threadIdx.x < kWave) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        for (int t = tb + int(threadIdx.x); t < te; t += kWave) {
-            const gp<const double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
+    beg0 = b0;
+    lo = wlo;
+    return true;
+}
+
+// SpMV over rows [rb, re) of instance I (a tile), one stage at a time.
+template <int L, class Mat, class SrcG, class Fin>
+__device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re, PcgLds& S, SrcG srcg, Fin fin) {
+    for (int sb = rb; sb < re; sb += kStage) {
+        const int se = min(re, sb + kStage);
+        unsigned beg0 = 0, lo = 0;
+        if (stage_load(I, sb, se, srcg, S, beg0, lo)) {
+            spmv_core<L>(
+                mat, sb, se, [&](unsigned r) { return unsigned(S.ro[r - unsigned(sb)]) + beg0; },
+                [&](unsigned k) { return unsigned(S.ci[k - beg0]); }, [&](unsigned j) { return lds3(S.win, j - lo); },
+                fin);
+        } else {
+            spmv_core<L>(
+                mat, sb, se, [&](unsigned r) { return unsigned(ldx(I.row_offsets, r)); },
+                [&](unsigned k) { return unsigned(ldx(I.col_indices, k)); }, srcg, fin);
+        }
+    }
+}
+
+// Scalars of the instances of up to kChunk tiles t = t0 + m*G (m < kChunk):
+// S.sc[m][0..3] = sums over the instance's tiles of partial fields f0..f3,
+// S.sc[m][4] = 1.0 when the instance has finished its solve.  16 lanes per
+// tile sum lane-strided over the instance's tiles, then a 16-lane
+// butterfly, so an instance's sums are the same bits in every workgroup.
+__device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const double> part, int t0, int G, int f0,
+                                           int f1, int f2, int f3_, PcgLds& S) {
+    __syncthreads();
+    const int m = int(threadIdx.x) >> 4, l = int(threadIdx.x) & 15;
+    const int t = t0 + m * G;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    double done = 0.0;
+    if (t < B.n_tiles) {
+        int inst, rb, re;
+        get_tile(B, t, inst, rb, re);
+        const Inst I = inst_at(B, inst);
+        for (int u = I.tile_begin + l; u < I.tile_end; u += 16) {
+            const gp<const double> p = rec_at(part, unsigned(u), unsigned(CHX_TILE_PARTIALS));
             s0 += p[f0];
             s1 += p[f1];
             s2 += p[f2];
             s3 += p[f3_];
         }
-        s0 = wave_sum(s0);
-        s1 = wave_sum(s1);
-        s2 = wave_sum(s2);
-        s3 = wave_sum(s3);
-        if (threadIdx.x == 0) {
-            lds[0] = s0;
-            lds[1] = s1;
-            lds[2] = s2;
-            lds[3] = s3;
-        }
+        if (l == 0) done = __hip_atomic_load(&B.inst_done[inst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1.0 : 0.0;
+    }
+    s0 = seg_sum<16>(s0);
+    s1 = seg_sum<16>(s1);
+    s2 = seg_sum<16>(s2);
+    s3 = seg_sum<16>(s3);
+    if (l == 0) {
+        S.sc[m][0] = s0;
+        S.sc[m][1] = s1;
+        S.sc[m][2] = s2;
+        S.sc[m][3] = s3;
+        S.sc[m][4] = done;
     }
     __syncthreads();
 }
@@ -293,34 +403,32 @@ __device__ __forceinline__ bool instance_active(const Inst& I, int k, double rr,
 // the FP32 matrix), M^-1 = inverse(D) (setup only), z = M^-1 r, p = z; tile
 // partials <r,z>, <r,r> (and <b,b> at setup).  M^-1 is stored planar.
 template <int L, class MatR>
-__device__ __forceinline__ void residual_tile(const Inst& I, MatR mat, int rb, int re, bool setup,
-                                              gp<float> pdst, double (&acc)[3]) {
+__device__ __forceinline__ void residual_tile(const Inst& I, MatR mat, int rb, int re, bool setup, gp<float> pdst,
+                                              PcgLds& S, double (&acc)[3]) {
     const gp<const float> xv = I.x;
     const unsigned n = unsigned(I.n_rows);
-    spmv_rows<L>(
-        I.row_offsets, I.col_indices, mat, rb, re, [&](unsigned j) { return ld3g(xv, j); },
-        [&](unsigned row, f3 Ax) {
-            const f3 b = ld3g(I.rhs, row);
-            const f3 r = sub(b, Ax);
-            float mi[9];
-            if (setup) {
-                float d9[9];
-                mat.diag_block(row, d9);
-                inverse9(d9, mi);
+    tile_spmv<L>(I, mat, rb, re, S, [&](unsigned j) { return ld3g(xv, j); }, [&](unsigned row, f3 Ax, f3) {
+        const f3 b = ld3g(I.rhs, row);
+        const f3 r = sub(b, Ax);
+        float mi[9];
+        if (setup) {
+            float d9[9];
+            mat.diag_block(row, d9);
+            inverse9(d9, mi);
 #pragma unroll
-                for (unsigned k = 0; k < 9; ++k) stx(I.minv, k * n + row, mi[k]);
-            } else {
+            for (unsigned k = 0; k < 9; ++k) stx(I.minv, k * n + row, mi[k]);
+        } else {
 #pragma unroll
-                for (unsigned k = 0; k < 9; ++k) mi[k] = ldx(gp<const float>(I.minv), k * n + row);
-            }
-            const f3 z = mv9(mi, r);
-            st3g(I.r, row, r);
-            st3g(I.z, row, z);
-            st3g(pdst, row, z);
-            acc[0] += dot64(r, z);
-            acc[1] += dot64(r, r);
-            acc[2] += dot64(b, b);
-        });
+            for (unsigned k = 0; k < 9; ++k) mi[k] = ldx(gp<const float>(I.minv), k * n + row);
+        }
+        const f3 z = mv9(mi, r);
+        st3g(I.r, row, r);
+        st3g(I.z, row, z);
+        st3g(pdst, row, z);
+        acc[0] += dot64(r, z);
+        acc[1] += dot64(r, r);
+        acc[2] += dot64(b, b);
+    });
 }
 
 // Matrix variants of the PCG kernel: the view of the inner SpMV (A) and of
@@ -352,12 +460,21 @@ struct PcgMats<2> {
     }
 };
 
+// Optional solver trace (workgroup 0, thread 0): real-time stamp and
+// active-instance count at barrier e.
+__device__ __forceinline__ void trace_point(const chx_cloth_batch& B, int e, unsigned active) {
+    if (B.trace != nullptr && blockIdx.x == 0 && threadIdx.x == 0 && e < B.trace_entries) {
+        const gp<unsigned long long> t = glb(B.trace);
+        t[2 * e] = __builtin_amdgcn_s_memrealtime();
+        t[2 * e + 1] = active;
+    }
+}
+
 template <int L, int V>
 __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int finalize) {
     constexpr bool BF16 = V == 2;
-    __shared__ double s_red[16];
-    __shared__ double s_sc[5];
-    const unsigned G = gridDim.x;
+    __shared__ PcgLds S;
+    const int G = int(gridDim.x);
     unsigned* sync = B.sync;
     const gp<double> part = glb(B.partials);
     // BF16 mode: every `period` iterations the next iteration starts from the
@@ -366,6 +483,8 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
 
     // ---- setup: r = b - A x, M^-1, z = M^-1 r, p = z; rho, ||r||^2, ||b||^2 ----
     if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&sync[SYNC_ACTIVE], unsigned(B.n_instances));
+    if (blockIdx.x == 0 && B.trace != nullptr)
+        for (int e = threadIdx.x; e < 2 * B.trace_entries; e += blockDim.x) glb(B.trace)[e] = 0ull;
     for (int t = blockIdx.x; t < B.n_tiles; t += G) {
         int inst, rb, re;
         get_tile(B, t, inst, rb, re);
@@ -373,8 +492,8 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         if (threadIdx.x == 0 && t == I.tile_begin)
             __hip_atomic_store(&B.inst_done[inst], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         double acc[3] = {0.0, 0.0, 0.0};
-        residual_tile<L>(I, PcgMats<V>::r(I), rb, re, true, I.p0, acc);
-        block_sum(acc, s_red);
+        residual_tile<L>(I, PcgMats<V>::r(I), rb, re, true, I.p0, S, acc);
+        block_sum(acc, S.red);
         if (threadIdx.x == 0) {
             const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
             p[P_RHO0] = acc[0];
@@ -383,6 +502,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         }
     }
     if (!grid_sync(sync, G)) return;
+    trace_point(B, 0, unsigned(B.n_instances));
 
     for (int k = 0;; ++k) {
         const int cur = k & 1;
@@ -390,116 +510,109 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         const bool refine = BF16 && (j == period - 1);     // this iteration computes the true residual
         const bool checkable = !BF16 || j == 0;
         // ---- phase A: p_k = z + beta p_{k-1}; q = A p_k; sigma = <p_k, q> ----
-        int cached = -1;
-        for (int t = blockIdx.x; t < B.n_tiles; t += G) {
-            int inst, rb, re;
-            get_tile(B, t, inst, rb, re);
-            const Inst I = inst_at(B, inst);
-            if (inst != cached) {
-                // A refinement step rewrites the rho slot of parity cur^1, so
-                // it does not read it (its beta is not needed).
-                instance_sums(part, I.tile_begin, I.tile_end, P_RHO0 + cur, refine ? P_RHO0 + cur : P_RHO0 + (cur ^ 1),
-                              P_RR0 + cur, P_BB, s_sc, &B.inst_done[inst]);
-                cached = inst;
-            }
-            const double rho = s_sc[0], rho_prev = s_sc[1], rr = s_sc[2], bb = s_sc[3];
-            const bool finished = s_sc[4] != 0.0;
-            if (!instance_active(I, k, rr, bb, checkable, finished)) {
-                if (threadIdx.x == 0 && t == I.tile_begin && !finished) {
-                    __hip_atomic_store(&B.inst_done[inst], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    B.inst_iters[inst] = k;
-                    B.inst_rr[inst] = rr;
-                    B.inst_bb[inst] = bb;
-                    __hip_atomic_fetch_sub(&sync[SYNC_ACTIVE], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // A refinement step rewrites the rho slot of parity cur^1, so it
+        // does not read it (its beta is not needed).
+        const int f_prev = refine ? P_RHO0 + cur : P_RHO0 + (cur ^ 1);
+        for (int t0 = blockIdx.x; t0 < B.n_tiles; t0 += G * kChunk) {
+            chunk_sums(B, part, t0, G, P_RHO0 + cur, f_prev, P_RR0 + cur, P_BB, S);
+            for (int m = 0; m < kChunk; ++m) {
+                const int t = t0 + m * G;
+                if (t >= B.n_tiles) break;
+                int inst, rb, re;
+                get_tile(B, t, inst, rb, re);
+                const Inst I = inst_at(B, inst);
+                const double rho = S.sc[m][0], rho_prev = S.sc[m][1], rr = S.sc[m][2], bb = S.sc[m][3];
+                const bool finished = S.sc[m][4] != 0.0;
+                if (!instance_active(I, k, rr, bb, checkable, finished)) {
+                    if (threadIdx.x == 0 && t == I.tile_begin && !finished) {
+                        __hip_atomic_store(&B.inst_done[inst], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        B.inst_iters[inst] = k;
+                        B.inst_rr[inst] = rr;
+                        B.inst_bb[inst] = bb;
+                        __hip_atomic_fetch_sub(&sync[SYNC_ACTIVE], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    continue;
                 }
-                continue;
-            }
-            const gp<float> pprev = cur ? I.p0 : I.p1;  // p_{k-1}
-            const gp<float> pk = cur ? I.p1 : I.p0;     // p_k
-            if (refine) {
-                // True residual of the current x; p_{k+1} = z goes to the
-                // buffer of parity k+1 (p_{k-1}'s, unused this iteration).
-                double acc[3] = {0.0, 0.0, 0.0};
-                residual_tile<L>(I, PcgMats<V>::r(I), rb, re, false, pprev, acc);
-                block_sum(acc, s_red);
-                if (threadIdx.x == 0) {
-                    const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
-                    p[P_RHO0 + (cur ^ 1)] = acc[0];
-                    p[P_RR0 + (cur ^ 1)] = acc[1];
+                const gp<float> pprev = cur ? I.p0 : I.p1;  // p_{k-1}
+                const gp<float> pk = cur ? I.p1 : I.p0;     // p_k
+                if (refine) {
+                    // True residual of the current x; p_{k+1} = z goes to the
+                    // buffer of parity k+1 (p_{k-1}'s, unused this iteration).
+                    double acc[3] = {0.0, 0.0, 0.0};
+                    residual_tile<L>(I, PcgMats<V>::r(I), rb, re, false, pprev, S, acc);
+                    block_sum(acc, S.red);
+                    if (threadIdx.x == 0) {
+                        const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
+                        p[P_RHO0 + (cur ^ 1)] = acc[0];
+                        p[P_RR0 + (cur ^ 1)] = acc[1];
+                    }
+                    continue;
                 }
-                continue;
-            }
-            const gp<const float> z = I.z;
-            const gp<float> q = I.q;
-            double acc[1] = {0.0};
-            auto run = [&](auto mat) {
+                const gp<const float> z = I.z;
+                const gp<float> q = I.q;
+                double acc[1] = {0.0};
+                auto fin = [&](unsigned row, f3 Ap, f3 pr) {
+                    if (j != 0) st3g(pk, row, pr);
+                    st3g(q, row, Ap);
+                    acc[0] += dot64(pr, Ap);
+                };
                 if (j == 0) {
-                    spmv_rows<L>(
-                        I.row_offsets, I.col_indices, mat, rb, re, [&](unsigned jj) { return ld3g(pk, jj); },
-                        [&](unsigned row, f3 Ap) {
-                            st3g(q, row, Ap);
-                            acc[0] += dot64(ld3g(pk, row), Ap);
-                        });
+                    tile_spmv<L>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(pk, jj); }, fin);
                 } else {
                     const float beta = guarded_div(rho, rho_prev);
                     auto pj = [&](unsigned jj) {
                         const f3 zj = ld3g(z, jj), pp = ld3g(pprev, jj);
                         return mk(fmaf(beta, pp.x, zj.x), fmaf(beta, pp.y, zj.y), fmaf(beta, pp.z, zj.z));
                     };
-                    spmv_rows<L>(I.row_offsets, I.col_indices, mat, rb, re, pj, [&](unsigned row, f3 Ap) {
-                        const f3 pr = pj(row);
-                        st3g(pk, row, pr);
-                        st3g(q, row, Ap);
-                        acc[0] += dot64(pr, Ap);
-                    });
+                    tile_spmv<L>(I, PcgMats<V>::a(I), rb, re, S, pj, fin);
                 }
-            };
-            run(PcgMats<V>::a(I));
-            block_sum(acc, s_red);
-            if (threadIdx.x == 0) rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS))[P_SIG] = acc[0];
+                block_sum(acc, S.red);
+                if (threadIdx.x == 0) rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS))[P_SIG] = acc[0];
+            }
         }
         unsigned active = 0;
         if (!grid_sync(sync, G, cur, &active)) return;
+        trace_point(B, 2 * k + 1, active);
         if (active == 0u) break;
 
         // ---- phase B: alpha; x += alpha p; r -= alpha q; z = M^-1 r; rho', rr' --
-        cached = -1;
         const int nxt = cur ^ 1;
-        for (int t = blockIdx.x; t < B.n_tiles; t += G) {
-            int inst, rb, re;
-            get_tile(B, t, inst, rb, re);
-            const Inst I = inst_at(B, inst);
-            if (inst != cached) {
-                instance_sums(part, I.tile_begin, I.tile_end, P_RHO0 + cur, P_SIG, P_RR0 + cur, P_BB, s_sc,
-                              &B.inst_done[inst]);
-                cached = inst;
-            }
-            const double rho = s_sc[0], sigma = s_sc[1], rr = s_sc[2], bb = s_sc[3];
-            if (refine || !instance_active(I, k, rr, bb, checkable, s_sc[4] != 0.0)) continue;
-            const float alpha = guarded_div(rho, sigma);
-            const gp<const float> pk = cur ? I.p1 : I.p0;
-            const unsigned n = unsigned(I.n_rows);
-            double acc[2] = {0.0, 0.0};
-            for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
-                const f3 p = ld3g(pk, row), qq = ld3g(I.q, row);
-                f3 x = ld3g(I.x, row), r = ld3g(I.r, row);
-                x = mk(fmaf(alpha, p.x, x.x), fmaf(alpha, p.y, x.y), fmaf(alpha, p.z, x.z));
-                r = mk(fmaf(-alpha, qq.x, r.x), fmaf(-alpha, qq.y, r.y), fmaf(-alpha, qq.z, r.z));
-                const f3 zz = mv9p(gp<const float>(I.minv), n, row, r);
-                st3g(I.x, row, x);
-                st3g(I.r, row, r);
-                st3g(I.z, row, zz);
-                acc[0] += dot64(r, zz);
-                acc[1] += dot64(r, r);
-            }
-            block_sum(acc, s_red);
-            if (threadIdx.x == 0) {
-                const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
-                p[P_RHO0 + nxt] = acc[0];
-                p[P_RR0 + nxt] = acc[1];
+        for (int t0 = blockIdx.x; t0 < B.n_tiles; t0 += G * kChunk) {
+            chunk_sums(B, part, t0, G, P_RHO0 + cur, P_SIG, P_RR0 + cur, P_BB, S);
+            for (int m = 0; m < kChunk; ++m) {
+                const int t = t0 + m * G;
+                if (t >= B.n_tiles) break;
+                int inst, rb, re;
+                get_tile(B, t, inst, rb, re);
+                const Inst I = inst_at(B, inst);
+                const double rho = S.sc[m][0], sigma = S.sc[m][1], rr = S.sc[m][2], bb = S.sc[m][3];
+                if (refine || !instance_active(I, k, rr, bb, checkable, S.sc[m][4] != 0.0)) continue;
+                const float alpha = guarded_div(rho, sigma);
+                const gp<const float> pk = cur ? I.p1 : I.p0;
+                const unsigned n = unsigned(I.n_rows);
+                double acc[2] = {0.0, 0.0};
+                for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
+                    const f3 p = ld3g(pk, row), qq = ld3g(I.q, row);
+                    f3 x = ld3g(I.x, row), r = ld3g(I.r, row);
+                    x = mk(fmaf(alpha, p.x, x.x), fmaf(alpha, p.y, x.y), fmaf(alpha, p.z, x.z));
+                    r = mk(fmaf(-alpha, qq.x, r.x), fmaf(-alpha, qq.y, r.y), fmaf(-alpha, qq.z, r.z));
+                    const f3 zz = mv9p(gp<const float>(I.minv), n, row, r);
+                    st3g(I.x, row, x);
+                    st3g(I.r, row, r);
+                    st3g(I.z, row, zz);
+                    acc[0] += dot64(r, zz);
+                    acc[1] += dot64(r, r);
+                }
+                block_sum(acc, S.red);
+                if (threadIdx.x == 0) {
+                    const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
+                    p[P_RHO0 + nxt] = acc[0];
+                    p[P_RR0 + nxt] = acc[1];
+                }
             }
         }
         if (!grid_sync(sync, G)) return;
+        trace_point(B, 2 * k + 2, active);
     }
 
     // ---- finalize: x_{n+1} = x_n + dx, v = dx/dt (cloth_simulator.cu:222-244) --
@@ -535,9 +648,11 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const float* __rest
     const int re = min(n, rb + RPB);
     const gp<const float> xg = glb(x);
     const gp<float> yg = glb(y);
-    spmv_rows<L>(
-        glb(ro), glb(ci), Mat32G{glb(diag), glb(val)}, rb, re, [&](unsigned j) { return ld3g(xg, j); },
-        [&](unsigned row, f3 Ax) {
+    const gp<const int> rog = glb(ro), cig = glb(ci);
+    spmv_core<L>(
+        Mat32G{glb(diag), glb(val)}, rb, re, [&](unsigned r) { return unsigned(ldx(rog, r)); },
+        [&](unsigned k) { return unsigned(ldx(cig, k)); }, [&](unsigned j) { return ld3g(xg, j); },
+        [&](unsigned row, f3 Ax, f3) {
             if (beta == 0.0f) st3g(yg, row, mul(Ax, alpha));
             else st3g(yg, row, add(mul(ld3g(yg, row), beta), mul(Ax, alpha)));
         });
@@ -635,8 +750,8 @@ void validate_batch(const chx_cloth_batch* b) {
     require(b->bf16_blocks == 0 || b->ir_period >= 1, "chysx: BF16 mode needs ir_period >= 1");
     if (b->inst == nullptr) {
         require(b->n_instances == 1, "chysx: a batch without an instance table holds exactly one instance");
-        require(b->tiles == nullptr && b->elem_items == nullptr && b->slot_items == nullptr,
-                "chysx: single-instance work lists are implied");
+        require(b->elem_items == nullptr && b->slot_items == nullptr,
+                "chysx: single-instance element and slot lists are implied");
         const chx_instance& I = b->single;
         require(I.n_rows >= 0 && I.nnz >= 0 && I.n_pins >= 0 && I.n_tris >= 0 && I.n_dihs >= 0,
                 "chysx: negative size in instance");

@@ -37,6 +37,17 @@ py::array_t<int> to_array(const std::vector<int>& v) {
 
 }  // namespace
 
+
+// Solver trace as an (E, 2) uint64 array {stamp, active}, cut after the last
+// written entry.
+py::array_t<unsigned long long> trace_array(const std::vector<unsigned long long>& t) {
+    std::size_t e = 0;
+    while (2 * e + 1 < t.size() && t[2 * e] != 0) ++e;
+    py::array_t<unsigned long long> a({static_cast<py::ssize_t>(e), static_cast<py::ssize_t>(2)});
+    if (e) std::memcpy(a.mutable_data(), t.data(), 2 * e * sizeof(unsigned long long));
+    return a;
+}
+
 PYBIND11_MODULE(_chysx_native, m) {
     m.doc() = "ChysX cloth step on AMD Instinct MI355X (gfx950 HIP kernels behind a C ABI)";
 
@@ -114,6 +125,13 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("set_pcg_tolerance", &ClothSimulator::set_pcg_tolerance, py::arg("tolerance"))
         .def("pcg_tolerance", &ClothSimulator::pcg_tolerance)
         .def("set_lanes_per_row", &ClothSimulator::set_lanes_per_row, py::arg("lanes"))
+        .def("set_tile_rows", &ClothSimulator::set_tile_rows, py::arg("rows"))
+        .def("set_trace", &ClothSimulator::set_trace, py::arg("entries"))
+        .def(
+            "last_trace", [](ClothSimulator& s, std::uintptr_t stream) { return trace_array(s.last_trace(stream)); },
+            py::arg("cuda_stream") = 0)
+        .def("tile_rows", &ClothSimulator::tile_rows)
+        .def("set_grid_blocks", &ClothSimulator::set_grid_blocks, py::arg("blocks"))
         .def("lanes_per_row", &ClothSimulator::lanes_per_row)
         .def("set_bf16_blocks", &ClothSimulator::set_bf16_blocks, py::arg("enabled"), py::arg("ir_period") = 10)
         .def("bf16_blocks", &ClothSimulator::bf16_blocks)
@@ -168,6 +186,13 @@ PYBIND11_MODULE(_chysx_native, m) {
                  std::copy(s.bending().rest_angles().begin(), s.bending().rest_angles().end(), r.mutable_data());
                  return py::make_tuple(a, r);
              })
+        .def("debug_stage_windows",
+             [](const ClothSimulator& s) {
+                 const auto& w = s.debug_stage_windows();
+                 py::array_t<int> a({static_cast<py::ssize_t>(w.size() / 2), static_cast<py::ssize_t>(2)});
+                 if (!w.empty()) std::memcpy(a.mutable_data(), w.data(), w.size() * sizeof(int));
+                 return a;
+             })
         .def("debug_edges",
              [](const ClothSimulator& s) {
                  const auto& e = s.buffers().edges;
@@ -191,6 +216,13 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("device_bytes", &chysx::cloth::ClothBatch::device_bytes)
         .def("set_lanes_per_row", &chysx::cloth::ClothBatch::set_lanes_per_row, py::arg("lanes"))
         .def("set_grid_blocks", &chysx::cloth::ClothBatch::set_grid_blocks, py::arg("blocks"))
+        .def("set_tile_rows", &chysx::cloth::ClothBatch::set_tile_rows, py::arg("rows"))
+        .def("set_trace", &chysx::cloth::ClothBatch::set_trace, py::arg("entries"))
+        .def(
+            "last_trace",
+            [](chysx::cloth::ClothBatch& b, std::uintptr_t stream) { return trace_array(b.last_trace(stream)); },
+            py::arg("cuda_stream") = 0)
+        .def("tile_rows", &chysx::cloth::ClothBatch::tile_rows)
         .def("set_bf16_blocks", &chysx::cloth::ClothBatch::set_bf16_blocks, py::arg("enabled"),
              py::arg("ir_period") = 10)
         .def("finalize", &chysx::cloth::ClothBatch::finalize, py::arg("dt") = 1.0f / 60.0f)

@@ -53,6 +53,14 @@ public:
     void set_lanes_per_row(int lanes) { runner_.set_lanes_per_row(lanes); }
     int lanes_per_row() const noexcept { return runner_.lanes_per_row(); }
     void set_grid_blocks(int blocks) { runner_.set_grid_blocks(blocks); }
+    // Rows per PCG tile: a multiple of CHX_STAGE_ROWS (256).
+    void set_tile_rows(int rows) { runner_.set_tile_rows(rows); }
+    // Solver trace of the persistent PCG launch: capacity in barrier entries
+    // (0: off); last_trace() returns {stamp in 100 MHz ticks, active
+    // instances} per barrier of the last solve.
+    void set_trace(int entries) { runner_.set_trace(entries); }
+    std::vector<unsigned long long> last_trace(std::uintptr_t cuda_stream = 0) { return runner_.read_trace(cuda_stream); }
+    int tile_rows() const noexcept { return runner_.tile_rows(); }
     // BF16 precision mode for the whole batch; every instance must have been
     // switched with ClothSimulator::set_bf16_blocks(true) as well.
     void set_bf16_blocks(bool on, int ir_period = 10) {

@@ -103,6 +103,19 @@ public:
     void set_pcg_tolerance(double tol) noexcept { pcg_tolerance_ = tol; }
     double pcg_tolerance() const noexcept { return pcg_tolerance_; }
     void set_lanes_per_row(int lanes);
+    // Rows per PCG tile (a multiple of 256) and persistent PCG workgroups
+    // (0: every co-resident workgroup).
+    void set_tile_rows(int rows) { runner_.set_tile_rows(rows); }
+    // Solver trace of the persistent PCG launch: capacity in barrier entries
+    // (0: off); last_trace() returns {stamp in 100 MHz ticks, active
+    // instances} per barrier of the last solve.
+    void set_trace(int entries) { runner_.set_trace(entries); }
+    std::vector<unsigned long long> last_trace(std::uintptr_t cuda_stream = 0) { return runner_.read_trace(cuda_stream); }
+    int tile_rows() const noexcept { return runner_.tile_rows(); }
+    void set_grid_blocks(int blocks) { runner_.set_grid_blocks(blocks); }
+    // {smallest, largest} column of each 256-row stage of the assembled
+    // pattern (after the first step): the source rows its SpMV reads.
+    const std::vector<int>& debug_stage_windows() const noexcept { return inst_.stage_windows(); }
     int lanes_per_row() const noexcept { return lanes_per_row_; }
     // Precision of the solve: false = FP32 Hessian blocks (default); true =
     // BF16 copies of the blocks in the inner SpMV (FP32 accumulation) with

@@ -151,6 +151,26 @@ void InstanceData::build(const InstanceSource& s) {
     alloc(p1_, 3 * n, bytes_);
     alloc(q_, 3 * n, bytes_);
     alloc(minv_, 9 * n, bytes_);
+    // Column window of every stage of CHX_STAGE_ROWS rows (its own rows
+    // included): the rows of the source vector its SpMV reads.
+    {
+        const auto& ro = row_offsets_.host();
+        const auto& ci = col_indices_.host();
+        auto& w = windows_.host();
+        const int stages = (n_ + CHX_STAGE_ROWS - 1) / CHX_STAGE_ROWS;
+        w.assign(2 * static_cast<std::size_t>(stages), 0);
+        for (int st = 0; st < stages; ++st) {
+            const int r0 = st * CHX_STAGE_ROWS, r1 = std::min(n_, r0 + CHX_STAGE_ROWS);
+            int lo = r0, hi = r1 - 1;
+            for (int k = ro[r0]; k < ro[r1]; ++k) {
+                lo = std::min(lo, ci[k]);
+                hi = std::max(hi, ci[k]);
+            }
+            w[2 * st] = lo;
+            w[2 * st + 1] = hi;
+        }
+        upload(windows_, bytes_);
+    }
     if (s.bf16_blocks) {
         alloc(diag16_, 9 * n, bytes_);
         alloc(values16_, 9 * static_cast<std::size_t>(nnz_), bytes_);
@@ -199,6 +219,7 @@ chx_instance InstanceData::descriptor(const InstanceSource& s) const {
     I.minv = minv_.gpu_data();
     I.diag16 = s.bf16_blocks ? diag16_.gpu_data() : nullptr;
     I.values16 = s.bf16_blocks ? values16_.gpu_data() : nullptr;
+    I.windows = windows_.gpu_data();
     I.n_rows = n_;
     I.nnz = nnz_;
     I.n_pins = static_cast<int>(pins_.cpu_size());
@@ -251,20 +272,80 @@ void BatchRunner::size_outputs(int n_instances, int n_tiles) {
     desc_.sync = sync_.gpu_data();
 }
 
+void BatchRunner::set_trace(int entries) {
+    if (entries <= 0) {
+        desc_.trace = nullptr;
+        desc_.trace_entries = 0;
+        return;
+    }
+    check(chx_device_synchronize(), "engine", "trace buffer");
+    trace_.ensure_device(2 * static_cast<std::size_t>(entries));
+    desc_.trace = trace_.gpu_data();
+    desc_.trace_entries = entries;
+}
+
+std::vector<unsigned long long> BatchRunner::read_trace(std::uintptr_t stream) {
+    std::vector<unsigned long long> out;
+    if (desc_.trace == nullptr) return out;
+    out.resize(2 * static_cast<std::size_t>(desc_.trace_entries));
+    check(chx_memcpy_d2h(out.data(), desc_.trace, out.size() * sizeof(unsigned long long), stream), "engine",
+          "trace read");
+    check(chx_stream_synchronize(stream), "engine", "trace read");
+    return out;
+}
+
+void BatchRunner::set_tile_rows(int rows) {
+    if (rows < CHX_STAGE_ROWS || rows % CHX_STAGE_ROWS != 0 || rows > 64 * CHX_STAGE_ROWS)
+        throw std::invalid_argument("chysx::engine: tile rows must be a multiple of CHX_STAGE_ROWS, at most 64 stages");
+    if (rows == tile_rows_) return;
+    tile_rows_ = rows;
+    if (!bound()) return;
+    // Queued launches may still read the old list.
+    check(chx_device_synchronize(), "engine", "tile list rebuild");
+    if (desc_.inst == nullptr) {
+        std::vector<chx_instance> one{desc_.single};
+        build_tiles(one);
+        desc_.single = one[0];
+    } else {
+        build_tiles(table_host_);
+        table_.host() = table_host_;
+        table_.copy_to_device();
+        desc_.inst = table_.gpu_data();
+    }
+    size_outputs(desc_.n_instances, desc_.n_tiles);
+    check(chx_device_synchronize(), "engine", "tile list rebuild");
+}
+
+void BatchRunner::build_tiles(std::vector<chx_instance>& ds) {
+    auto& ti = tiles_.host();
+    ti.clear();
+    for (std::size_t i = 0; i < ds.size(); ++i) {
+        chx_instance& I = ds[i];
+        I.tile_begin = static_cast<int>(ti.size() / 4);
+        for (int r0 = 0; r0 < I.n_rows; r0 += tile_rows_)
+            ti.insert(ti.end(), {static_cast<int>(i), r0, std::min(I.n_rows, r0 + tile_rows_), 0});
+        I.tile_end = static_cast<int>(ti.size() / 4);
+    }
+    if (ti.size() / 4 > static_cast<std::size_t>(std::numeric_limits<int>::max()))
+        throw std::runtime_error("chysx::engine: too many row tiles");
+    tiles_.copy_to_device();
+    desc_.n_tiles = static_cast<int>(ti.size() / 4);
+    desc_.tiles = tiles_.gpu_data();
+    desc_.tile_rows = tile_rows_;
+}
+
 void BatchRunner::bind_single(const chx_instance& d) {
+    std::vector<chx_instance> one{d};
+    build_tiles(one);
     desc_.n_instances = 1;
-    desc_.n_tiles = item_count(d.n_rows);
     desc_.n_elem_items = 0;
     desc_.n_slot_items = 0;
     desc_.lanes_per_row = lanes_;
     desc_.grid_blocks = grid_;
     desc_.bf16_blocks = bf16_ ? 1 : 0;
     desc_.ir_period = ir_period_;
-    desc_.single = d;
-    desc_.single.tile_begin = 0;
-    desc_.single.tile_end = desc_.n_tiles;
+    desc_.single = one[0];
     desc_.inst = nullptr;
-    desc_.tiles = nullptr;
     desc_.elem_items = nullptr;
     desc_.slot_items = nullptr;
     size_outputs(1, desc_.n_tiles);
@@ -274,19 +355,14 @@ void BatchRunner::bind_single(const chx_instance& d) {
 void BatchRunner::bind_many(const std::vector<chx_instance>& ds) {
     if (ds.empty()) throw std::invalid_argument("chysx::engine: empty batch");
     table_host_ = ds;
-    auto& ti = tiles_.host();
+    build_tiles(table_host_);
     auto& ei = elem_items_.host();
     auto& si = slot_items_.host();
-    ti.clear();
     ei.clear();
     si.clear();
-    for (std::size_t i = 0; i < ds.size(); ++i) {
-        chx_instance& I = table_host_[i];
+    for (std::size_t i = 0; i < table_host_.size(); ++i) {
+        const chx_instance& I = table_host_[i];
         const int inst = static_cast<int>(i);
-        I.tile_begin = static_cast<int>(ti.size() / 4);
-        for (int r0 = 0; r0 < I.n_rows; r0 += CHX_TILE_ROWS)
-            ti.insert(ti.end(), {inst, r0, std::min(I.n_rows, r0 + CHX_TILE_ROWS), 0});
-        I.tile_end = static_cast<int>(ti.size() / 4);
         const int counts[3] = {I.n_pins, I.n_tris, I.n_dihs};
         for (int kind = 0; kind < 3; ++kind)
             for (int b = 0; b < counts[kind]; b += CHX_ITEM_SIZE)
@@ -294,15 +370,11 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds) {
         const int slots = I.n_rows + I.nnz;
         for (int b = 0; b < slots; b += CHX_ITEM_SIZE) si.insert(si.end(), {inst, b, std::min(slots, b + CHX_ITEM_SIZE), 0});
     }
-    if (ti.size() / 4 > static_cast<std::size_t>(std::numeric_limits<int>::max()))
-        throw std::runtime_error("chysx::engine: too many row tiles");
-    tiles_.copy_to_device();
     elem_items_.copy_to_device();
     slot_items_.copy_to_device();
     table_.host() = table_host_;
     table_.copy_to_device();
     desc_.n_instances = static_cast<int>(ds.size());
-    desc_.n_tiles = static_cast<int>(ti.size() / 4);
     desc_.n_elem_items = static_cast<int>(ei.size() / 4);
     desc_.n_slot_items = static_cast<int>(si.size() / 4);
     desc_.lanes_per_row = lanes_;
@@ -311,7 +383,6 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds) {
     desc_.ir_period = ir_period_;
     desc_.single = chx_instance{};
     desc_.inst = table_.gpu_data();
-    desc_.tiles = tiles_.gpu_data();
     desc_.elem_items = elem_items_.gpu_data();
     desc_.slot_items = slot_items_.gpu_data();
     size_outputs(desc_.n_instances, desc_.n_tiles);

@@ -60,6 +60,7 @@ public:
     const float* diag() const noexcept { return diag_.gpu_data(); }
     const float* values() const noexcept { return values_.gpu_data(); }
     const float* rhs() const noexcept { return rhs_.gpu_data(); }
+    const std::vector<int>& stage_windows() const noexcept { return windows_.host(); }
 
 private:
     void pack_elements(const InstanceSource& s);
@@ -75,6 +76,7 @@ private:
     DeviceArray<unsigned> contrib_offsets_, contribs_;
     DeviceArray<float> rhs_, r_, z_, p0_, p1_, q_, minv_;
     DeviceArray<uint16_t> diag16_, values16_;
+    DeviceArray<int> windows_;  // per CHX_STAGE_ROWS-row stage: {min column, max column}
 };
 
 class BatchRunner {
@@ -88,7 +90,12 @@ public:
     // Many instances: builds the work lists and uploads the table.
     void bind_many(const std::vector<chx_instance>& ds);
     // Per-step scalars / external pointers of the bound instances.
-    void update_single(const chx_instance& d) { desc_.single = d; }
+    void update_single(const chx_instance& d) {
+        const int tb = desc_.single.tile_begin, te = desc_.single.tile_end;
+        desc_.single = d;
+        desc_.single.tile_begin = tb;
+        desc_.single.tile_end = te;
+    }
     void update_many(const std::vector<chx_instance>& ds, std::uintptr_t stream);
 
     void step(std::uintptr_t stream);
@@ -98,6 +105,10 @@ public:
 
     void set_lanes_per_row(int L);
     int lanes_per_row() const noexcept { return lanes_; }
+    // Rows per PCG tile (a multiple of CHX_STAGE_ROWS); rebuilds the tile
+    // list of a bound batch.
+    void set_tile_rows(int rows);
+    int tile_rows() const noexcept { return tile_rows_; }
     void set_grid_blocks(int g) {
         grid_ = g < 0 ? 0 : g;
         desc_.grid_blocks = grid_;
@@ -115,16 +126,25 @@ public:
     bool bound() const noexcept { return desc_.n_instances > 0; }
     const chx_cloth_batch& desc() const noexcept { return desc_; }
 
+    // Solver trace: capacity in barrier entries (0 switches it off).
+    void set_trace(int entries);
+    // {stamp (100 MHz ticks), active instances} per barrier of the last
+    // solve, up to the last one written.
+    std::vector<unsigned long long> read_trace(std::uintptr_t stream);
+
     // Synchronous read-backs.
     void read_stats(std::vector<int>& iters, std::vector<double>& rr, std::vector<double>& bb, std::uintptr_t stream);
     void check_status(std::uintptr_t stream);
 
 private:
     void size_outputs(int n_instances, int n_tiles);
+    // Tile list of the bound instances; sets their tile_begin / tile_end.
+    void build_tiles(std::vector<chx_instance>& ds);
 
     chx_cloth_batch desc_{};
     int lanes_ = 8;
     int grid_ = 0;
+    int tile_rows_ = CHX_TILE_ROWS;
     bool bf16_ = false;
     int ir_period_ = 10;
     std::vector<chx_instance> table_host_;
@@ -133,6 +153,7 @@ private:
     DeviceArray<double> partials_, inst_rr_, inst_bb_;
     DeviceArray<int> inst_iters_, inst_done_;
     DeviceArray<unsigned> sync_;
+    DeviceArray<unsigned long long> trace_;
 };
 
 }  // namespace engine

@@ -125,6 +125,57 @@ def test_lanes_per_row_variants(gpu, lanes):
     assert helpers.rel_maxnorm(dc.state()[0], o.state()[0]) <= 1e-5
 
 
+@pytest.mark.parametrize("tile_rows", [512, 1024])
+def test_tile_rows_variants(gpu, tile_rows):
+    """Tiles of several 256-row stages (one workgroup and one FP64 partial
+    per tile) give the oracle's substeps and iteration counts."""
+    from chysx import scenes
+
+    spec = scenes.plain_small(seed=4)
+    dc = _make(spec)
+    dc.sim.set_tile_rows(tile_rows)
+    o = helpers.oracle_from_spec(spec)
+    for _ in range(5):
+        dc.step()
+        it_o = o.step(spec.dt)
+        assert abs(dc.sim.last_pcg_iterations() - it_o) <= 1
+    assert helpers.rel_maxnorm(dc.state()[0], o.state()[0]) <= 1e-5
+
+
+def test_scattered_numbering_uses_global_path(gpu):
+    """A randomly renumbered 64x64 grid: each 256-row stage references
+    columns across the whole mesh, beyond the LDS window, so the PCG's SpMV
+    runs from global memory.  Same oracle bar, substeps started from the
+    oracle's state."""
+    import torch
+
+    from chysx import scenes
+
+    base = scenes.plain_small(seed=6, n=64)
+    rng = np.random.default_rng(6)
+    perm = rng.permutation(base.n)          # new index of old vertex i: inv[i]
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(base.n)
+    spec = scenes.ClothSpec(pos=base.pos[perm].copy(), tris=inv[base.tris].astype(np.int32),
+                            pins=np.sort(inv[base.pins]).astype(np.int32), k_stretch=base.k_stretch,
+                            k_shear=base.k_shear, k_bend=base.k_bend, density=base.density, name="scattered64")
+    dc = _make(spec)
+    o = helpers.oracle_from_spec(spec)
+    for s in range(20):
+        po, vo = o.state()
+        dxo = o.system()[3] if s else np.zeros((spec.n, 3))
+        dc.pos.copy_(torch.from_numpy(po.astype(np.float32)).cuda())
+        dc.vel.copy_(torch.from_numpy(vo.astype(np.float32)).cuda())
+        dc.sim.debug_set_warm_start(dxo.astype(np.float32))
+        dc.step()
+        it_o = o.step(spec.dt)
+        if s == 0:
+            windows = np.asarray(dc.sim.debug_stage_windows())
+            assert np.all(windows[:, 1] - windows[:, 0] + 1 > 2816), windows[:4]
+        assert helpers.rel_maxnorm(dc.state()[0], o.state()[0]) <= 1e-5, s
+        assert abs(dc.sim.last_pcg_iterations() - it_o) <= 1, s
+
+
 def test_freefall_closed_form(gpu):
     """The reference's free-fall example check (example_chysx_freefall.py:
     116-178): no elastic elements, particle 0 pinned; free particles follow
