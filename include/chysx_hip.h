@@ -54,6 +54,7 @@ enum {
 #define CHX_CONTRIB_PIN 0u
 #define CHX_CONTRIB_TRI 1u
 #define CHX_CONTRIB_DIH 2u
+#define CHX_CONTRIB_STVK 3u
 #define CHX_CONTRIB_MAX_ELEMENT ((1u << 26) - 1u)
 
 /* Packed element records (device arrays). Vertex indices are local to the
@@ -61,11 +62,21 @@ enum {
 typedef struct chx_pin { int v; int pad; float tx, ty, tz; float k; } chx_pin;
 typedef struct chx_tri { int v0, v1, v2, pad; float dm00, dm01, dm10, dm11; float kA; } chx_tri;
 typedef struct chx_dih { int v0, v1, v2, v3; float rest; float k; int pad0, pad1; } chx_dih;
+/* StVK membrane triangle (no reference counterpart; ClothMaterial's
+ * lame_mu / lame_lambda hooks): rest shape as chx_tri with theta = 0,
+ * muA = mu * area * thickness, lamA = lambda * area * thickness. */
+typedef struct chx_stvk {
+    int v0, v1, v2, pad;
+    float dm00, dm01, dm10, dm11;
+    float muA, lamA;
+    int pad0, pad1;
+} chx_stvk;
 
 /* Record sizes (floats) written per element by the element pass. */
 #define CHX_PIN_REC 4
 #define CHX_TRI_REC 24
 #define CHX_DIH_REC 16
+#define CHX_STVK_REC 36
 
 /* Number of float64 partial sums kept per row tile by the PCG kernel. */
 #define CHX_TILE_PARTIALS 8
@@ -113,13 +124,15 @@ typedef struct chx_instance {
     float* minv;                     /* [9][n_rows] planar block-Jacobi preconditioner       */
     uint16_t* diag16;                /* planar BF16 copies of diag / values for the inner    */
     uint16_t* values16;              /* SpMV of the BF16 precision mode, or NULL             */
+    const chx_stvk* stvks;           /* StVK membrane elements, or NULL                      */
+    float* stvk_rec;                 /* [n_stvks*CHX_STVK_REC]                               */
     const int* windows;              /* [ceil(n_rows/CHX_STAGE_ROWS)][2]: smallest and       */
                                      /* largest column (own rows included) of each stage of  */
                                      /* CHX_STAGE_ROWS rows, or NULL (no LDS staging)        */
     int n_rows, nnz, n_pins, n_tris, n_dihs;
     int tile_begin;                  /* this instance's row tiles in the batch tile list     */
     int tile_end;
-    int pad0;
+    int n_stvks;
     float dt;                        /* substep                                              */
     float damping;                   /* v *= exp(-damping*dt) when > 0                       */
     float gx, gy, gz;                /* gravity                                              */
@@ -130,7 +143,7 @@ typedef struct chx_instance {
 /* A batch of instances advanced together.  Work lists are device int4
  * arrays: row tiles {instance, row_begin, row_end, 0} (local rows, row_begin
  * a multiple of CHX_STAGE_ROWS, grouped by instance), element items {instance, kind
- * (0 pin, 1 triangle, 2 dihedral), begin, end} and slot items {instance,
+ * (0 pin, 1 triangle, 2 dihedral, 3 StVK triangle), begin, end} and slot items {instance,
  * begin, end, 0} (at most CHX_ITEM_SIZE each).  A batch with inst == NULL
  * holds one instance in `single` (passed by value in the kernel arguments);
  * its element and slot lists are implied by the counts, and so is its tile
@@ -212,7 +225,7 @@ int chx_mass_area_weighted(const int* tris, const float* pos, const int* vertex_
 /* Elastic energy of the batch's elements at the instances' current positions
  * (FP64 sum, synchronous).  Replaces Constraint::compute_energy
  * (pin_constraint.cu:120-147, triangle_stretch_constraint.cu:358-384,
- * bending_constraint.cu:284-308). kind_mask: 1 pins, 2 triangles, 4 dihedrals. */
+ * bending_constraint.cu:284-308). kind_mask: 1 pins, 2 triangles, 4 dihedrals, 8 StVK triangles. */
 int chx_batch_energy(const chx_cloth_batch* batch, int kind_mask, double* energy, uintptr_t stream);
 
 /* ---- sparse operator and solver -------------------------------------------- */

@@ -50,6 +50,7 @@ class ClothSpec:
     tol: float = 1.0e-6
     name: str = "cloth"
     pos_start: np.ndarray | None = None  # positions after set-up (None: = pos)
+    stvk: tuple | None = None       # StVK membrane (mu [Pa], lambda [Pa], thickness [m]), or None
     meta: dict = field(default_factory=dict)
 
     @property
@@ -107,6 +108,25 @@ def plain_small(seed: int = 0, n: int = 32) -> ClothSpec:
     return ClothSpec(pos=pos, tris=tris, pins=top_corners(n), k_stretch=k, k_shear=k, name=f"grid{n}")
 
 
+def stvk_lame(E: float, nu: float) -> tuple:
+    """(mu, lambda) of a membrane: mu = E / (2 (1 + nu)) and the
+    plane-stress lambda = E nu / (1 - nu^2) (= 2 mu lambda3 / (2 mu + lambda3)
+    of the 3-D Lame lambda3)."""
+    return E / (2.0 * (1.0 + nu)), E * nu / (1.0 - nu * nu)
+
+
+def plain_small_stvk(seed: int = 0, n: int = 32, E: float = 1.0e5, nu: float = 0.3) -> ClothSpec:
+    """The 32x32 workload with the StVK membrane (E = 1e5 Pa, nu = 0.3,
+    h = 1 mm) in place of the BW98 stretch / shear pair."""
+    spec = plain_small(seed=seed, n=n)
+    mu, lam = stvk_lame(E, nu)
+    spec.k_stretch = 0.0
+    spec.k_shear = 0.0
+    spec.stvk = (mu, lam, THICKNESS)
+    spec.name = f"grid{n}_stvk"
+    return spec
+
+
 def plain_large(seed: int = 1, n: int = 256) -> ClothSpec:
     rng = np.random.default_rng(seed)
     E = float(rng.uniform(1.0e4, 1.0e6))
@@ -146,15 +166,43 @@ def precision(seed: int = 3, n: int = 512, max_shift: float = 0.05) -> ClothSpec
 
 
 @dataclass
+class JobSpec:
+    """A serving job's cloth, built on demand (`build()`): an n_side x
+    n_side grid with BW98 stiffness E*h, density rho and two pinned corners."""
+    n_side: int
+    E: float
+    rho: float
+    pins: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return self.n_side * self.n_side
+
+    @property
+    def dt(self) -> float:
+        return ClothSpec.dt
+
+    def build(self) -> ClothSpec:
+        pos, tris = cloth_grid(self.n_side)
+        k = self.E * THICKNESS
+        return ClothSpec(pos=pos, tris=tris, pins=self.pins, k_stretch=k, k_shear=k, density=self.rho,
+                         name=f"job_grid{self.n_side}", meta={"E": self.E})
+
+
+@dataclass
 class Job:
     arrival: float
     n_side: int
     substeps: int
-    spec: ClothSpec
+    spec: JobSpec
 
 
-def serving_jobs(seed: int = 4, rate: float = 200.0, duration: float = 60.0, lo: float = 1.0e3, hi: float = 2.5e5,
-                 build_specs: bool = False) -> list[Job]:
+def serving_jobs(seed: int = 4, rate: float = 200.0, duration: float = 60.0, lo: float = 1.0e3,
+                 hi: float = 2.5e5) -> list[Job]:
+    """Poisson arrivals at `rate` per second for `duration` seconds; mesh
+    sizes log-uniform in [lo, hi] vertices (square grids), 30-300 substeps,
+    E ~ U[1e4, 1e6] Pa, density ~ U[0.1, 0.4] kg/m^2, two random corners
+    pinned."""
     rng = np.random.default_rng(seed)
     t, jobs = 0.0, []
     while True:
@@ -168,12 +216,7 @@ def serving_jobs(seed: int = 4, rate: float = 200.0, duration: float = 60.0, lo:
         rho = float(rng.uniform(0.1, 0.4))
         corners = np.array([0, n - 1, (n - 1) * n, n * n - 1], np.int32)
         pins = np.sort(rng.choice(corners, size=2, replace=False)).astype(np.int32)
-        spec = None
-        if build_specs:
-            pos, tris = cloth_grid(n)
-            k = E * THICKNESS
-            spec = ClothSpec(pos=pos, tris=tris, pins=pins, k_stretch=k, k_shear=k, density=rho)
-        jobs.append(Job(arrival=t, n_side=n, substeps=steps, spec=spec))
+        jobs.append(Job(arrival=t, n_side=n, substeps=steps, spec=JobSpec(n_side=n, E=E, rho=rho, pins=pins)))
     return jobs
 
 
@@ -200,6 +243,10 @@ def setup_simulator(sim, spec: ClothSpec, pos_ptr: int, vel_ptr: int, inv_mass_p
         sim.build_fem_stretch_from_current_positions(stiffness=float(spec.k_stretch), cuda_stream=stream)
     if spec.k_shear > 0:
         sim.build_fem_shear_from_current_positions(stiffness=float(spec.k_shear), cuda_stream=stream)
+    if spec.stvk is not None:
+        mu, lam, h = spec.stvk
+        sim.build_fem_stvk_from_current_positions(mu=float(mu), lam=float(lam), thickness=float(h),
+                                                  cuda_stream=stream)
     if spec.k_bend > 0:
         sim.build_bending_from_current_positions(stiffness=float(spec.k_bend), cuda_stream=stream)
     if len(spec.pins):
@@ -210,22 +257,30 @@ def setup_simulator(sim, spec: ClothSpec, pos_ptr: int, vel_ptr: int, inv_mass_p
 class DeviceCloth:
     """A ClothSimulator with its own torch-allocated device buffers."""
 
-    def __init__(self, spec: ClothSpec, device: str = "cuda", lanes_per_row: int | None = None):
+    def __init__(self, spec: ClothSpec, device: str = "cuda", lanes_per_row: int | None = None, stream=None):
+        """With `stream` (a torch.cuda.Stream) every set-up operation runs on
+        that stream and waits only for it, so a cloth can be set up on one
+        host thread while another steps a batch."""
+        import contextlib
+
         import torch
 
         import chysx
 
         self.spec = spec
-        self.pos = torch.from_numpy(np.ascontiguousarray(spec.pos, np.float32)).to(device)
-        self.vel = torch.zeros_like(self.pos)
-        self.inv_mass = torch.ones(spec.n, dtype=torch.float32, device=device)
-        self.sim = chysx.ClothSimulator()
-        torch.cuda.synchronize()
-        setup_simulator(self.sim, spec, self.pos.data_ptr(), self.vel.data_ptr(), self.inv_mass.data_ptr(),
-                        lanes_per_row=lanes_per_row)
-        if spec.pos_start is not None:
-            self.pos.copy_(torch.from_numpy(np.ascontiguousarray(spec.pos_start, np.float32)).to(device))
-            torch.cuda.synchronize()
+        ctx = torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
+        sync = stream.synchronize if stream is not None else torch.cuda.synchronize
+        with ctx:
+            self.pos = torch.from_numpy(np.ascontiguousarray(spec.pos, np.float32)).to(device)
+            self.vel = torch.zeros_like(self.pos)
+            self.inv_mass = torch.ones(spec.n, dtype=torch.float32, device=device)
+            self.sim = chysx.ClothSimulator()
+            sync()
+            setup_simulator(self.sim, spec, self.pos.data_ptr(), self.vel.data_ptr(), self.inv_mass.data_ptr(),
+                            stream=stream.cuda_stream if stream is not None else 0, lanes_per_row=lanes_per_row)
+            if spec.pos_start is not None:
+                self.pos.copy_(torch.from_numpy(np.ascontiguousarray(spec.pos_start, np.float32)).to(device))
+            sync()
 
     def step(self, dt: float | None = None, stream: int = 0):
         self.sim.step(self.spec.dt if dt is None else dt, stream)

@@ -2,19 +2,29 @@
 
 Jobs (a cloth spec and a substep count) arrive over time.  The server keeps
 every admitted job's state resident in HBM (its ClothSimulator arrays and
-torch-owned pos/vel/inv_mass) and advances all admitted jobs together, one
+torch-owned pos/vel/inv_mass) and advances all running jobs together, one
 fused substep per loop iteration (`ClothBatch.step`: one element pass, one
-gather pass, one persistent PCG launch for the whole mix).  Between substeps
-it retires finished jobs and admits waiting ones, first come first served,
-while their estimated HBM footprint fits the budget — so requests that
-arrive while others run join as soon as memory frees up and their work
-overlaps on the device.  Each job's result is identical alone and in any
-mix (the batch runs the same kernels on the job's own arrays with
-per-instance reductions).
+gather pass, one persistent PCG launch for the whole mix).
+
+A job moves through four states:
+
+  waiting    arrived, not admitted yet;
+  preparing  admitted first come first served while its estimated HBM
+             footprint fits the budget; its set-up (topology, Hessian pattern,
+             scatter maps, device arrays) runs on a worker thread while the
+             GPU keeps stepping the running jobs;
+  running    in the batch; it joins at the next substep boundary once its
+             set-up is done;
+  finished   after its last substep; it leaves the batch and frees its memory.
+
+So requests that arrive while others run join as soon as memory and their
+set-up allow, and their substeps overlap on the device with everyone
+else's.  Each job's result is identical alone and in any mix (the batch runs
+the same kernels on the job's own arrays with per-instance reductions).
 
 Reported: throughput (vertex-substeps/s), p50/p99 request latency (arrival
-to the end of the substep that finishes the job), queueing delay, peak
-concurrency and peak HBM in use.
+to the end of the substep that finishes the job), queueing and set-up
+delays, peak concurrency and peak HBM in use.
 """
 from __future__ import annotations
 
@@ -34,14 +44,16 @@ BYTES_PER_VERTEX_ESTIMATE = 1800
 class Request:
     job_id: int
     arrival: float                 # seconds since the start of the stream
-    spec: object                   # chysx.scenes.ClothSpec
+    spec: object                   # chysx.scenes.ClothSpec, or a JobSpec built on demand
     substeps: int
-    admitted: float | None = None
+    admitted: float | None = None  # set-up started
+    joined: float | None = None    # entered the batch
     finished: float | None = None
     done: int = 0
     handle: object = None
     bytes: int = 0
     result: object = None
+    future: object = None
 
 
 @dataclass
@@ -54,9 +66,11 @@ class ServingReport:
     p50_latency_s: float | None
     p99_latency_s: float | None
     p50_queue_s: float | None
+    p50_setup_s: float | None
     max_concurrency: int
     peak_bytes: int
     substep_launches: int
+    unfinished: int = 0
     latencies: list = field(default_factory=list)
 
     def as_dict(self):
@@ -66,9 +80,13 @@ class ServingReport:
 
 
 class GpuExecutor:
-    """Runs admitted jobs on the GPU through one chysx.ClothBatch."""
+    """Runs admitted jobs on the GPU through one chysx.ClothBatch; job set-up
+    runs on `prepare_workers` host threads and its own stream."""
 
-    def __init__(self, lanes_per_row: int = 8, stream=None, keep_results: bool = False):
+    def __init__(self, lanes_per_row: int = 8, stream=None, keep_results: bool = False, prepare_workers: int = 2,
+                 tile_rows: int | None = None, grid_blocks: int | None = None):
+        import concurrent.futures
+
         import torch
 
         import chysx
@@ -76,14 +94,35 @@ class GpuExecutor:
         self.torch = torch
         self.batch = chysx.ClothBatch()
         self.batch.set_lanes_per_row(lanes_per_row)
+        self.lanes = lanes_per_row
+        if tile_rows:
+            self.batch.set_tile_rows(tile_rows)
+        if grid_blocks:
+            self.batch.set_grid_blocks(grid_blocks)
         self.stream = stream if stream is not None else torch.cuda.Stream()
+        self.prep_stream = torch.cuda.Stream()
         self.keep_results = keep_results
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=prepare_workers) if prepare_workers > 0 else None
 
-    def admit(self, req: Request):
+    def _build(self, spec, dt):
         from . import scenes
 
-        with self.torch.cuda.stream(self.stream):
-            cloth = scenes.DeviceCloth(req.spec)
+        if hasattr(spec, "build"):  # a job spec built on demand
+            spec = spec.build()
+        cloth = scenes.DeviceCloth(spec, lanes_per_row=self.lanes, stream=self.prep_stream)
+        cloth.sim.prepare(dt)
+        return cloth
+
+    def begin(self, req: Request, dt: float):
+        if self.pool is not None:
+            req.future = self.pool.submit(self._build, req.spec, dt)
+
+    def ready(self, req: Request) -> bool:
+        return req.future is None or req.future.done()
+
+    def admit(self, req: Request, dt: float):
+        cloth = req.future.result() if req.future is not None else self._build(req.spec, dt)
+        req.future = None
         req.handle = cloth
         req.bytes = int(cloth.sim.device_bytes() + 28 * req.spec.n)
         self.batch.add_instance(cloth.sim)
@@ -102,6 +141,10 @@ class GpuExecutor:
     def sync(self):
         self.stream.synchronize()
 
+    def close(self):
+        if self.pool is not None:
+            self.pool.shutdown(wait=True)
+
 
 class ServingEngine:
     def __init__(self, executor, hbm_budget_bytes: float = 200e9, max_concurrency: int = 1 << 30,
@@ -112,12 +155,25 @@ class ServingEngine:
         self.clock = clock
         self.sleep = sleep
 
+    def _begin(self, req, dt):
+        begin = getattr(self.ex, "begin", None)
+        if begin is not None:
+            begin(req, dt)
+
+    def _ready(self, req):
+        ready = getattr(self.ex, "ready", None)
+        return True if ready is None else ready(req)
+
+    def _admit(self, req, dt):
+        self.ex.admit(req, dt)
+
     def run(self, requests, dt: float, horizon_s: float | None = None, log=None) -> ServingReport:
         """Serves `requests` (sorted by arrival).  Arrivals are replayed in
         real time from the start of the call; with `horizon_s`, the server
-        stops admitting at that time and reports the jobs completed."""
+        stops at that time and reports the jobs completed by then."""
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
+        preparing: list[Request] = []
         active: list[Request] = []
         done: list[Request] = []
         in_use = 0
@@ -126,30 +182,42 @@ class ServingEngine:
         launches = 0
         vs = 0.0
         t0 = self.clock()
-        while pending or waiting or active:
+        while pending or waiting or preparing or active:
             now = self.clock() - t0
             if horizon_s is not None and now > horizon_s:
                 break
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
             # admission: FCFS while the estimated footprint fits
-            while waiting and len(active) < self.max_conc:
+            while waiting and len(active) + len(preparing) < self.max_conc:
                 req = waiting[0]
                 est = BYTES_PER_VERTEX_ESTIMATE * req.spec.n
-                if active and in_use + est > self.budget:
+                if (active or preparing) and in_use + est > self.budget:
                     break
                 waiting.popleft()
-                self.ex.admit(req)
                 req.admitted = self.clock() - t0
-                in_use += req.bytes or est
+                req.bytes = est
+                in_use += est
+                self._begin(req, dt)
+                preparing.append(req)
+            # join: every prepared job enters the batch at this boundary; with
+            # nothing running, wait for the oldest set-up.
+            joined = [r for r in preparing if self._ready(r)]
+            if not joined and not active and preparing:
+                joined = [preparing[0]]
+            for req in joined:
+                preparing.remove(req)
+                est = req.bytes
+                self._admit(req, dt)
+                in_use += (req.bytes or est) - est
+                req.joined = self.clock() - t0
                 active.append(req)
             if not active:
-                if pending:
+                if pending and not preparing:
                     self.sleep(max(0.0, min(pending[0].arrival - (self.clock() - t0), 0.01)))
                 continue
             max_conc = max(max_conc, len(active))
             peak_bytes = max(peak_bytes, in_use)
-            self.max_in_use = peak_bytes
             self.ex.step(dt)
             launches += 1
             self.ex.sync()
@@ -167,15 +235,20 @@ class ServingEngine:
                     still.append(req)
             active = still
             if log and launches % 200 == 0:
-                log(f"serving t={t_end:.1f}s active={len(active)} waiting={len(waiting)} done={len(done)}")
+                log(f"serving t={t_end:.1f}s running={len(active)} preparing={len(preparing)} "
+                    f"waiting={len(waiting)} done={len(done)}")
         wall = self.clock() - t0
         for req in active:  # stopped at the horizon: release unfinished jobs
             self.ex.retire(req)
+        for req in preparing:
+            self._admit(req, dt)
+            self.ex.retire(req)
         lat = [r.finished - r.arrival for r in done]
         qd = [r.admitted - r.arrival for r in done]
+        sd = [r.joined - r.admitted for r in done]
         pct = lambda a, q: float(np.percentile(a, q)) if a else None  # noqa: E731
         return ServingReport(
             completed=len(done), submitted=len(requests), vertex_substeps=vs, wall_s=wall,
             throughput=vs / wall if wall > 0 else 0.0, p50_latency_s=pct(lat, 50), p99_latency_s=pct(lat, 99),
-            p50_queue_s=pct(qd, 50), max_concurrency=max_conc, peak_bytes=int(peak_bytes),
-            substep_launches=launches, latencies=lat)
+            p50_queue_s=pct(qd, 50), p50_setup_s=pct(sd, 50), max_concurrency=max_conc, peak_bytes=int(peak_bytes),
+            substep_launches=launches, unfinished=len(requests) - len(done), latencies=lat)

@@ -197,6 +197,9 @@ struct Inst {
     gp<float> rhs, x, r, z, p0, p1, q, minv;
     gp<uint16_t> diag16, values16;
     gp<const int> windows;
+    gp<const chx_stvk> stvks;
+    gp<float> stvk_rec;
+    int n_stvks;
     int n_rows, nnz, n_pins, n_tris, n_dihs, tile_begin, tile_end;
     float dt, damping, gx, gy, gz;
     int max_iter;
@@ -232,6 +235,9 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.diag16 = glb(d->diag16);
     I.values16 = glb(d->values16);
     I.windows = glb(d->windows);
+    I.stvks = glb(d->stvks);
+    I.stvk_rec = glb(d->stvk_rec);
+    I.n_stvks = d->n_stvks;
     I.n_rows = d->n_rows;
     I.nnz = d->nnz;
     I.n_pins = d->n_pins;
@@ -284,7 +290,7 @@ __device__ __forceinline__ void get_elem_item(const chx_cloth_batch& B, int w, i
     }
     const chx_instance& I = B.single;
     inst = 0;
-    const int np = blocks_of(I.n_pins), nt = blocks_of(I.n_tris);
+    const int np = blocks_of(I.n_pins), nt = blocks_of(I.n_tris), nd = blocks_of(I.n_dihs);
     int n;
     if (w < np) {
         kind = 0;
@@ -293,10 +299,14 @@ __device__ __forceinline__ void get_elem_item(const chx_cloth_batch& B, int w, i
         kind = 1;
         w -= np;
         n = I.n_tris;
-    } else {
+    } else if (w < np + nt + nd) {
         kind = 2;
         w -= np + nt;
         n = I.n_dihs;
+    } else {
+        kind = 3;
+        w -= np + nt + nd;
+        n = I.n_stvks;
     }
     b = w * CHX_ITEM_SIZE;
     e = min(n, b + CHX_ITEM_SIZE);

@@ -85,6 +85,85 @@ __device__ __forceinline__ double tri_energy(f3 x0, f3 x1, f3 x2, float d00, flo
     return double(0.5f * kA * (cu * cu + cv * cv));
 }
 
+// ---------------------------------------------------------------------------
+// StVK membrane (no reference counterpart).  F = [fu fv] from Dm^-1,
+// G = (F^T F - I)/2, E = muA ||G||^2 + lamA/2 (tr G)^2, S = 2 muA G +
+// lamA tr(G) I, dE/dF = F S.  Hessian: the Gauss-Newton part
+// 2 muA ||dG||^2 + lamA (tr dG)^2 (PSD) plus the stress part S+ : dF^T dF
+// with S clamped to its non-negative eigen-part S+ (exact when S is PSD):
+//   Huu = (2 muA + lamA) fu fu^T + muA fv fv^T + S+uu I
+//   Hvv = (2 muA + lamA) fv fv^T + muA fu fu^T + S+vv I
+//   Huv = muA fv fu^T + lamA fu fv^T + S+uv I,   Hvu = Huv^T
+// and H_ab = dwu_a dwu_b Huu + dwu_a dwv_b Huv + dwv_a dwu_b Hvu + dwv_a dwv_b Hvv.
+// Record (CHX_STVK_REC floats): [0..2] Gu = (F S)_u, [3..5] Gv, [6..11] Huu
+// (xx xy xz yy yz zz), [12..17] Hvv, [18..26] Huv row-major, [27..29] dwu,
+// [30..32] dwv, [33..35] 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void stvk_psd2(float a, float b, float c, float& pa, float& pb, float& pc) {
+    const float m = 0.5f * (a + b), d = 0.5f * (a - b);
+    const float r = sqrtf(d * d + c * c);
+    const float l1 = m + r, l2 = m - r;
+    if (l2 >= 0.0f) { pa = a; pb = b; pc = c; return; }
+    if (l1 <= 0.0f) { pa = pb = pc = 0.0f; return; }
+    float vx, vy;
+    if (d >= 0.0f) { vx = d + r; vy = c; } else { vx = c; vy = r - d; }
+    const float nn = vx * vx + vy * vy;
+    if (nn <= 0.0f) { pa = l1; pb = 0.0f; pc = 0.0f; return; }
+    const float s = l1 / nn;
+    pa = s * vx * vx;
+    pb = s * vy * vy;
+    pc = s * vx * vy;
+}
+
+__device__ __forceinline__ void stvk_record(f3 x0, f3 x1, f3 x2, float d00, float d01, float d10, float d11,
+                                            float muA, float lamA, float* rec) {
+    const f3 a = sub(x1, x0), b = sub(x2, x0);
+    const f3 fu = add(mul(a, d00), mul(b, d10));
+    const f3 fv = add(mul(a, d01), mul(b, d11));
+    const float Guu = 0.5f * (dot(fu, fu) - 1.0f), Gvv = 0.5f * (dot(fv, fv) - 1.0f), Guv = 0.5f * dot(fu, fv);
+    const float tr = Guu + Gvv;
+    const float Suu = 2.0f * muA * Guu + lamA * tr, Svv = 2.0f * muA * Gvv + lamA * tr, Suv = 2.0f * muA * Guv;
+    const f3 Pu = add(mul(fu, Suu), mul(fv, Suv)), Pv = add(mul(fu, Suv), mul(fv, Svv));
+    float Puu, Pvv, Puv;
+    stvk_psd2(Suu, Svv, Suv, Puu, Pvv, Puv);
+    const float c1 = 2.0f * muA + lamA;
+    rec[0] = Pu.x; rec[1] = Pu.y; rec[2] = Pu.z;
+    rec[3] = Pv.x; rec[4] = Pv.y; rec[5] = Pv.z;
+    // Huu
+    rec[6] = c1 * fu.x * fu.x + muA * fv.x * fv.x + Puu;
+    rec[7] = c1 * fu.x * fu.y + muA * fv.x * fv.y;
+    rec[8] = c1 * fu.x * fu.z + muA * fv.x * fv.z;
+    rec[9] = c1 * fu.y * fu.y + muA * fv.y * fv.y + Puu;
+    rec[10] = c1 * fu.y * fu.z + muA * fv.y * fv.z;
+    rec[11] = c1 * fu.z * fu.z + muA * fv.z * fv.z + Puu;
+    // Hvv
+    rec[12] = c1 * fv.x * fv.x + muA * fu.x * fu.x + Pvv;
+    rec[13] = c1 * fv.x * fv.y + muA * fu.x * fu.y;
+    rec[14] = c1 * fv.x * fv.z + muA * fu.x * fu.z;
+    rec[15] = c1 * fv.y * fv.y + muA * fu.y * fu.y + Pvv;
+    rec[16] = c1 * fv.y * fv.z + muA * fu.y * fu.z;
+    rec[17] = c1 * fv.z * fv.z + muA * fu.z * fu.z + Pvv;
+    // Huv[r][c] = muA fv_r fu_c + lamA fu_r fv_c + S+uv delta_rc
+    const float fur[3] = {fu.x, fu.y, fu.z}, fvr[3] = {fv.x, fv.y, fv.z};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rec[18 + 3 * r + c] = muA * fvr[r] * fur[c] + lamA * fur[r] * fvr[c] + (r == c ? Puv : 0.0f);
+    rec[27] = -d00 - d10; rec[28] = d00; rec[29] = d10;
+    rec[30] = -d01 - d11; rec[31] = d01; rec[32] = d11;
+    rec[33] = 0.0f; rec[34] = 0.0f; rec[35] = 0.0f;
+}
+
+__device__ __forceinline__ double stvk_energy(f3 x0, f3 x1, f3 x2, float d00, float d01, float d10, float d11,
+                                              float muA, float lamA) {
+    const f3 a = sub(x1, x0), b = sub(x2, x0);
+    const f3 fu = add(mul(a, d00), mul(b, d10));
+    const f3 fv = add(mul(a, d01), mul(b, d11));
+    const float Guu = 0.5f * (dot(fu, fu) - 1.0f), Gvv = 0.5f * (dot(fv, fv) - 1.0f), Guv = 0.5f * dot(fu, fv);
+    const float tr = Guu + Gvv;
+    return double(muA * (Guu * Guu + Gvv * Gvv + 2.0f * Guv * Guv) + 0.5f * lamA * tr * tr);
+}
+
 // Dihedral frame: bending_constraint.cu:54-126.
 struct DihFrame {
     float theta;

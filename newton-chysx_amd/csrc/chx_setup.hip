@@ -73,6 +73,10 @@ __global__ __launch_bounds__(kBlock) void energy_kernel(chx_cloth_batch B, int m
             const chx_tri t = ldg(I.tris, unsigned(i));
             E[0] = tri_energy(ld3(I.pos, t.v0), ld3(I.pos, t.v1), ld3(I.pos, t.v2), t.dm00, t.dm01, t.dm10, t.dm11,
                               t.kA);
+        } else if (kind == 3 && (mask & 8)) {
+            const chx_stvk t = ldg(I.stvks, unsigned(i));
+            E[0] = stvk_energy(ld3(I.pos, t.v0), ld3(I.pos, t.v1), ld3(I.pos, t.v2), t.dm00, t.dm01, t.dm10, t.dm11,
+                               t.muA, t.lamA);
         } else if (kind == 2 && (mask & 4)) {
             const chx_dih d = ldg(I.dihs, unsigned(i));
             const DihFrame F =
@@ -135,7 +139,8 @@ int chx_batch_energy(const chx_cloth_batch* batch, int kind_mask, double* energy
         auto cnt = [](int n) { return (n + CHX_ITEM_SIZE - 1) / CHX_ITEM_SIZE; };
         const long total = B.inst != nullptr
                                ? long(B.n_elem_items)
-                               : long(cnt(B.single.n_pins)) + cnt(B.single.n_tris) + cnt(B.single.n_dihs);
+                               : long(cnt(B.single.n_pins)) + cnt(B.single.n_tris) + cnt(B.single.n_dihs) +
+                                     cnt(B.single.n_stvks);
         *energy = 0.0;
         if (total == 0) return;
         double* dev = nullptr;

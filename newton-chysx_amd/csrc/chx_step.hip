@@ -71,6 +71,14 @@ __global__ __launch_bounds__(kBlock) void element_kernel(chx_cloth_batch B) {
         const gp<float> out = rec_at(I.tri_rec, i, unsigned(CHX_TRI_REC));
 #pragma unroll
         for (int k = 0; k < CHX_TRI_REC; ++k) out[k] = rec[k];
+    } else if (kind == 3) {
+        const chx_stvk t = ldg(I.stvks, i);
+        float rec[CHX_STVK_REC];
+        stvk_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00, t.dm01,
+                    t.dm10, t.dm11, t.muA, t.lamA, rec);
+        const gp<float> out = rec_at(I.stvk_rec, i, unsigned(CHX_STVK_REC));
+#pragma unroll
+        for (int k = 0; k < CHX_STVK_REC; ++k) out[k] = rec[k];
     } else {
         const chx_dih d = ldg(I.dihs, i);
         const DihFrame F = dihedral_frame(ld3g(pos, unsigned(d.v0)), ld3g(pos, unsigned(d.v1)),
@@ -135,6 +143,21 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
                 const float ua = r[18 + a], va = r[21 + a];
                 grad = add(grad, mk(ua * r[0] + va * r[3], ua * r[1] + va * r[4], ua * r[2] + va * r[5]));
             }
+        } else if (kind == CHX_CONTRIB_STVK) {
+            const gp<const float> r = rec_at(gp<const float>(I.stvk_rec), id, unsigned(CHX_STVK_REC));
+            const float wua = r[27 + a], wva = r[30 + a], wub = r[27 + b], wvb = r[30 + b];
+            const float su = wua * wub, sv = wva * wvb, suv = wua * wvb, svu = wva * wub;
+            // Huu / Hvv symmetric (xx xy xz yy yz zz), Huv full; Hvu = Huv^T.
+            const float uu[9] = {r[6], r[7], r[8], r[7], r[9], r[10], r[8], r[10], r[11]};
+            const float vv[9] = {r[12], r[13], r[14], r[13], r[15], r[16], r[14], r[16], r[17]};
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    h[3 * q + c] += su * uu[3 * q + c] + sv * vv[3 * q + c] + suv * r[18 + 3 * q + c] +
+                                    svu * r[18 + 3 * c + q];
+            if (is_diag && a == b)
+                grad = add(grad, mk(wua * r[0] + wva * r[3], wua * r[1] + wva * r[4], wua * r[2] + wva * r[5]));
         } else {
             const gp<const float> r = rec_at(gp<const float>(I.dih_rec), id, unsigned(CHX_DIH_REC));
             const float k = r[1];
@@ -753,7 +776,7 @@ void validate_batch(const chx_cloth_batch* b) {
         require(b->elem_items == nullptr && b->slot_items == nullptr,
                 "chysx: single-instance element and slot lists are implied");
         const chx_instance& I = b->single;
-        require(I.n_rows >= 0 && I.nnz >= 0 && I.n_pins >= 0 && I.n_tris >= 0 && I.n_dihs >= 0,
+        require(I.n_rows >= 0 && I.nnz >= 0 && I.n_pins >= 0 && I.n_tris >= 0 && I.n_dihs >= 0 && I.n_stvks >= 0,
                 "chysx: negative size in instance");
         if (I.n_rows > 0)
             require(I.row_offsets && I.diag && I.rhs && I.x && I.r && I.z && I.p0 && I.p1 && I.q && I.minv,
@@ -774,7 +797,8 @@ int item_count(int n) { return (n + CHX_ITEM_SIZE - 1) / CHX_ITEM_SIZE; }
 // Launch counts: the single instance's lists are implied by its sizes.
 int elem_items(const chx_cloth_batch& B) {
     if (B.inst != nullptr) return B.n_elem_items;
-    return item_count(B.single.n_pins) + item_count(B.single.n_tris) + item_count(B.single.n_dihs);
+    return item_count(B.single.n_pins) + item_count(B.single.n_tris) + item_count(B.single.n_dihs) +
+           item_count(B.single.n_stvks);
 }
 int slot_items(const chx_cloth_batch& B) {
     if (B.inst != nullptr) return B.n_slot_items;

@@ -67,7 +67,9 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("set_material", &ClothSimulator::set_material, py::arg("material"))
         .def("set_external_buffers", &ClothSimulator::set_external_buffers, py::arg("pos_ptr"), py::arg("vel_ptr"),
              py::arg("particle_count"), py::arg("inv_mass_ptr") = 0, py::arg("force_ptr") = 0)
-        .def("step", &ClothSimulator::step, py::arg("dt"), py::arg("cuda_stream") = 0)
+        .def("step", &ClothSimulator::step, py::arg("dt"), py::arg("cuda_stream") = 0, py::call_guard<py::gil_scoped_release>())
+        .def("prepare", &ClothSimulator::prepare, py::arg("dt"), py::call_guard<py::gil_scoped_release>(),
+             "Builds the pattern, scatter maps and device arrays ahead of the first step.")
         .def(
             "set_pins",
             [](ClothSimulator& self, IArr indices, FArr targets, float stiffness) {
@@ -96,7 +98,10 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def(
             "set_mesh",
             [](ClothSimulator& self, IArr tris) {
-                self.set_mesh(as_tris(tris, "ClothSimulator.set_mesh"), static_cast<int>(tris.shape(0)));
+                const auto* t = as_tris(tris, "ClothSimulator.set_mesh");
+                const int count = static_cast<int>(tris.shape(0));
+                py::gil_scoped_release release;
+                self.set_mesh(t, count);
             },
             py::arg("triangles"))
         .def("build_springs_from_current_positions", &ClothSimulator::build_springs_from_current_positions,
@@ -104,15 +109,20 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("num_springs", [](ClothSimulator& s) { return s.springs().size(); })
         .def("redistribute_mass_area_weighted", &ClothSimulator::redistribute_mass_area_weighted,
              py::arg("surface_density"), py::arg("inv_mass_ptr"), py::arg("particle_count"),
-             py::arg("cuda_stream") = 0)
+             py::arg("cuda_stream") = 0, py::call_guard<py::gil_scoped_release>())
         .def("build_fem_stretch_from_current_positions", &ClothSimulator::build_fem_stretch_from_current_positions,
-             py::arg("stiffness"), py::arg("cuda_stream") = 0)
+             py::arg("stiffness"), py::arg("cuda_stream") = 0, py::call_guard<py::gil_scoped_release>())
         .def("num_fem_stretch_triangles", [](const ClothSimulator& s) { return s.fem_stretch().size(); })
         .def("build_fem_shear_from_current_positions", &ClothSimulator::build_fem_shear_from_current_positions,
-             py::arg("stiffness"), py::arg("cuda_stream") = 0)
+             py::arg("stiffness"), py::arg("cuda_stream") = 0, py::call_guard<py::gil_scoped_release>())
         .def("num_fem_shear_triangles", [](const ClothSimulator& s) { return s.fem_shear().size(); })
+        .def("build_fem_stvk_from_current_positions", &ClothSimulator::build_fem_stvk_from_current_positions,
+             py::arg("mu"), py::arg("lam"), py::arg("thickness"), py::arg("cuda_stream") = 0,
+             py::call_guard<py::gil_scoped_release>(),
+             "StVK membrane (no reference counterpart): Lame mu, lambda [Pa], thickness [m].")
+        .def("num_fem_stvk_triangles", [](const ClothSimulator& s) { return s.fem_stvk().size(); })
         .def("build_bending_from_current_positions", &ClothSimulator::build_bending_from_current_positions,
-             py::arg("stiffness"), py::arg("cuda_stream") = 0)
+             py::arg("stiffness"), py::arg("cuda_stream") = 0, py::call_guard<py::gil_scoped_release>())
         .def("num_bending_dihedrals", [](const ClothSimulator& s) { return s.bending().size(); })
         .def(
             "set_solver_type",
@@ -165,7 +175,7 @@ PYBIND11_MODULE(_chysx_native, m) {
                 s.debug_set_warm_start(dx.data());
             },
             py::arg("dx"))
-        .def("compute_energy", &ClothSimulator::compute_energy, py::arg("kind_mask") = 7)
+        .def("compute_energy", &ClothSimulator::compute_energy, py::arg("kind_mask") = 15)
         .def("device_bytes", &ClothSimulator::device_bytes)
         .def(
             "debug_slot_tables",
@@ -174,6 +184,7 @@ PYBIND11_MODULE(_chysx_native, m) {
                 d["pins"] = to_array(s.pins().hessian_slots());
                 d["stretch"] = to_array(s.fem_stretch().hessian_slots());
                 d["shear"] = to_array(s.fem_shear().hessian_slots());
+                d["stvk"] = to_array(s.fem_stvk().hessian_slots());
                 d["bending"] = to_array(s.bending().hessian_slots());
                 return d;
             })
@@ -229,9 +240,9 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("remove_instance", &chysx::cloth::ClothBatch::remove_instance, py::arg("simulator"))
         .def("clear", &chysx::cloth::ClothBatch::clear)
         .def("lanes_per_row", &chysx::cloth::ClothBatch::lanes_per_row)
-        .def("step", &chysx::cloth::ClothBatch::step, py::arg("dt"), py::arg("cuda_stream") = 0)
-        .def("assemble", &chysx::cloth::ClothBatch::assemble, py::arg("dt"), py::arg("cuda_stream") = 0)
-        .def("solve", &chysx::cloth::ClothBatch::solve, py::arg("cuda_stream") = 0)
+        .def("step", &chysx::cloth::ClothBatch::step, py::arg("dt"), py::arg("cuda_stream") = 0, py::call_guard<py::gil_scoped_release>())
+        .def("assemble", &chysx::cloth::ClothBatch::assemble, py::arg("dt"), py::arg("cuda_stream") = 0, py::call_guard<py::gil_scoped_release>())
+        .def("solve", &chysx::cloth::ClothBatch::solve, py::arg("cuda_stream") = 0, py::call_guard<py::gil_scoped_release>())
         .def("instance_sizes", &chysx::cloth::ClothBatch::instance_sizes)
         .def("last_pcg_iterations", &chysx::cloth::ClothBatch::last_pcg_iterations, py::arg("cuda_stream") = 0)
         .def("last_relative_residuals", &chysx::cloth::ClothBatch::last_relative_residuals,

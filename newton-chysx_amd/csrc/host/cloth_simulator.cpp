@@ -103,6 +103,23 @@ void ClothSimulator::build_fem_shear_from_current_positions(float stiffness, std
     topology_dirty_ = true;
 }
 
+void ClothSimulator::build_fem_stvk_from_current_positions(float mu, float lambda, float thickness,
+                                                           std::uintptr_t cuda_stream) {
+    fem_stvk_.set_material(mu, lambda, thickness);
+    const int n_tris = static_cast<int>(buffers_.tris.size());
+    if (n_tris == 0) {
+        fem_stvk_.set_triangles_from_positions(nullptr, 0, buffers_.pos, cuda_stream);
+        topology_dirty_ = true;
+        return;
+    }
+    if (buffers_.pos.data() == nullptr)
+        throw std::runtime_error(
+            "ClothSimulator::build_fem_stvk_from_current_positions: external positions must be set first via "
+            "set_external_buffers().");
+    fem_stvk_.set_triangles_from_positions(buffers_.tris.data(), n_tris, buffers_.pos, cuda_stream);
+    topology_dirty_ = true;
+}
+
 // cloth_simulator.cu:451-514
 void ClothSimulator::build_bending_from_current_positions(float stiffness, std::uintptr_t cuda_stream) {
     bending_.set_stiffness(stiffness);
@@ -189,11 +206,13 @@ void ClothSimulator::prepare_topology() {
     std::vector<int> rows, cols;
     topology::append_triangle_pairs(fem_stretch_.indices(), rows, cols);
     topology::append_triangle_pairs(fem_shear_.indices(), rows, cols);
+    topology::append_triangle_pairs(fem_stvk_.indices(), rows, cols);
     topology::append_dihedral_pairs(bending_.indices(), rows, cols);
     pattern_ = topology::build_pattern(N, rows.data(), cols.data(), static_cast<int>(rows.size()));
     pins_.bind_hessian_layout(pattern_);
     fem_stretch_.bind_hessian_layout(pattern_);
     fem_shear_.bind_hessian_layout(pattern_);
+    fem_stvk_.bind_hessian_layout(pattern_);
     bending_.bind_hessian_layout(pattern_);
 }
 
@@ -209,6 +228,9 @@ void ClothSimulator::ensure_hessian_topology() {
     packed_k_[1] = fem_stretch_.stiffness();
     packed_k_[2] = fem_shear_.stiffness();
     packed_k_[3] = bending_.stiffness();
+    packed_stvk_[0] = fem_stvk_.mu();
+    packed_stvk_[1] = fem_stvk_.lambda();
+    packed_stvk_[2] = fem_stvk_.thickness();
     H_num_block_rows_ = N;
     topology_dirty_ = false;
     pins_dirty_ = false;
@@ -225,6 +247,7 @@ engine::InstanceSource ClothSimulator::source() const {
     s.pins = &pins_;
     s.stretch = &fem_stretch_;
     s.shear = &fem_shear_;
+    s.stvk = &fem_stvk_;
     s.bending = &bending_;
     s.damping = material_.damping;
     s.gx = material_.gx;
@@ -240,13 +263,18 @@ engine::InstanceSource ClothSimulator::source() const {
 // reference; re-pack the element arrays when one changed since packing.
 void ClothSimulator::check_elements_current() {
     const bool k_changed = packed_k_[0] != pins_.stiffness() || packed_k_[1] != fem_stretch_.stiffness() ||
-                           packed_k_[2] != fem_shear_.stiffness() || packed_k_[3] != bending_.stiffness();
+                           packed_k_[2] != fem_shear_.stiffness() || packed_k_[3] != bending_.stiffness() ||
+                           packed_stvk_[0] != fem_stvk_.mu() || packed_stvk_[1] != fem_stvk_.lambda() ||
+                           packed_stvk_[2] != fem_stvk_.thickness();
     if (k_changed || pins_dirty_) {
         inst_.update_elements(source());
         packed_k_[0] = pins_.stiffness();
         packed_k_[1] = fem_stretch_.stiffness();
         packed_k_[2] = fem_shear_.stiffness();
         packed_k_[3] = bending_.stiffness();
+        packed_stvk_[0] = fem_stvk_.mu();
+        packed_stvk_[1] = fem_stvk_.lambda();
+        packed_stvk_[2] = fem_stvk_.thickness();
         pins_dirty_ = false;
     }
 }

@@ -83,6 +83,15 @@ public:
     const constraint::TriangleStretchConstraint& fem_stretch() const noexcept { return fem_stretch_; }
     constraint::TriangleStretchConstraint& fem_shear() noexcept { return fem_shear_; }
     const constraint::TriangleStretchConstraint& fem_shear() const noexcept { return fem_shear_; }
+    // StVK membrane (no reference counterpart; the material hooks are
+    // ClothMaterial::lame_mu / lame_lambda): Lame parameters mu, lambda
+    // [Pa] and thickness [m], rest shape from the current positions.
+    // Replaces nothing: combine it with or use it instead of the BW98
+    // stretch / shear sets.
+    void build_fem_stvk_from_current_positions(float mu, float lambda, float thickness,
+                                               std::uintptr_t cuda_stream = 0);
+    constraint::StvkConstraint& fem_stvk() noexcept { return fem_stvk_; }
+    const constraint::StvkConstraint& fem_stvk() const noexcept { return fem_stvk_; }
     constraint::BendingConstraint& bending() noexcept { return bending_; }
     const constraint::BendingConstraint& bending() const noexcept { return bending_; }
 
@@ -138,7 +147,7 @@ public:
     void debug_set_warm_start(const float* in);
     // Elastic energy at the current positions (FP64, synchronous);
     // kind_mask 1 pins, 2 triangles, 4 dihedrals.
-    double compute_energy(int kind_mask = 7);
+    double compute_energy(int kind_mask = 15);
     const topology::Pattern& pattern() const noexcept { return pattern_; }
 
     // Host part of the topology build (pattern + slot tables).
@@ -147,6 +156,10 @@ public:
     // returns the launch descriptor for substep `dt`; ClothBatch uses it to
     // advance this cloth as one instance of a batch.
     chx_instance prepare_instance(float dt);
+    // Builds the Hessian pattern, scatter maps and device arrays now (the
+    // first step would otherwise do it), so the set-up of a cloth can run
+    // ahead of its first step, e.g. on another host thread.
+    void prepare(float dt) { (void)prepare_instance(dt); }
     std::size_t device_bytes() const noexcept { return inst_.device_bytes() + dx_.gpu_size() * sizeof(math::Vec3f); }
 
 private:
@@ -161,6 +174,8 @@ private:
     constraint::SpringConstraint springs_;
     constraint::TriangleStretchConstraint fem_stretch_;
     constraint::TriangleStretchConstraint fem_shear_;
+    constraint::StvkConstraint fem_stvk_;
+    float packed_stvk_[3] = {0.f, 0.f, 0.f};
     constraint::BendingConstraint bending_;
 
     DeviceArray<math::Vec3f> dx_;

@@ -68,6 +68,14 @@ void BendingConstraint::set_dihedrals_from_positions(const math::Vec4i* host_dih
     rest_ = d_rest.host();
 }
 
+void StvkConstraint::set_material(float mu, float lambda, float thickness) {
+    if (!(mu >= 0.0f) || !(lambda >= 0.0f) || !(thickness >= 0.0f))
+        throw std::invalid_argument("StvkConstraint::set_material: mu, lambda and thickness must be non-negative");
+    mu_ = mu;
+    lambda_ = lambda;
+    thickness_ = thickness;
+}
+
 void SpringConstraint::set_springs(const math::Vec2i* edges, int n) {
     if (n < 0) throw std::invalid_argument("SpringConstraint::set_springs: negative count");
     edges_.assign(edges ? edges : nullptr, edges ? edges + n : nullptr);

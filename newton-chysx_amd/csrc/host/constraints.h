@@ -65,6 +65,34 @@ private:
     float rotation_ = 0.0f;
 };
 
+// StVK membrane: no reference counterpart (the reference keeps
+// ClothMaterial::lame_mu / lame_lambda as unused placeholders,
+// cloth_material.h:93-116).  Rest shape as the BW98 stretch triangles
+// (theta = 0); per element muA = mu * area * h, lamA = lambda * area * h.
+class StvkConstraint {
+public:
+    void set_triangles_from_positions(const math::Vec3i* host_triangles, int n, DeviceSpan<math::Vec3f> positions,
+                                      std::uintptr_t stream = 0) {
+        shape_.set_triangles_from_positions(host_triangles, n, positions, stream, 0.0f);
+        slots_.clear();
+    }
+    void set_material(float mu, float lambda, float thickness);
+    float mu() const noexcept { return mu_; }
+    float lambda() const noexcept { return lambda_; }
+    float thickness() const noexcept { return thickness_; }
+    int size() const noexcept { return shape_.size(); }
+    const std::vector<math::Vec3i>& indices() const noexcept { return shape_.indices(); }
+    const std::vector<math::Mat2f>& Dm_inv() const noexcept { return shape_.Dm_inv(); }
+    const std::vector<float>& areas() const noexcept { return shape_.areas(); }
+    void bind_hessian_layout(const topology::Pattern& p) { slots_ = topology::slot_table3(p, indices()); }
+    const std::vector<int>& hessian_slots() const noexcept { return slots_; }
+
+private:
+    TriangleStretchConstraint shape_;
+    std::vector<int> slots_;
+    float mu_ = 0.0f, lambda_ = 0.0f, thickness_ = 0.0f;
+};
+
 class BendingConstraint {
 public:
     void set_dihedrals_from_positions(const math::Vec4i* host_dihedrals, int n, DeviceSpan<math::Vec3f> positions,

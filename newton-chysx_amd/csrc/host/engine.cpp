@@ -37,9 +37,11 @@ void InstanceData::pack_elements(const InstanceSource& s) {
     auto& P = pins_.host();
     auto& T = tris_.host();
     auto& D = dihs_.host();
+    auto& V = stvks_.host();
     P.clear();
     T.clear();
     D.clear();
+    V.clear();
     if (s.pins) {
         const float k = s.pins->stiffness();
         for (int c = 0; c < s.pins->size(); ++c) {
@@ -55,6 +57,15 @@ void InstanceData::pack_elements(const InstanceSource& s) {
             const auto& dm = ts->Dm_inv()[t].data;
             // k_eff = areas[t] * k (triangle_stretch_constraint.cu:193).
             T.push_back(chx_tri{v.x, v.y, v.z, 0, dm[0], dm[1], dm[2], dm[3], ts->areas()[t] * k});
+        }
+    }
+    if (s.stvk) {
+        const float mu = s.stvk->mu(), lam = s.stvk->lambda(), h = s.stvk->thickness();
+        for (int t = 0; t < s.stvk->size(); ++t) {
+            const auto& v = s.stvk->indices()[t];
+            const auto& dm = s.stvk->Dm_inv()[t].data;
+            const float ah = s.stvk->areas()[t] * h;
+            V.push_back(chx_stvk{v.x, v.y, v.z, 0, dm[0], dm[1], dm[2], dm[3], mu * ah, lam * ah, 0, 0});
         }
     }
     if (s.bending) {
@@ -77,8 +88,9 @@ void InstanceData::build(const InstanceSource& s) {
 
     // Gather map.  Slots: diagonal block of row r -> r, off-diagonal block k
     // -> n + k.  Entries are appended in element order (pins, stretch,
-    // shear, bending; element; a; b) — the order of the reference's launches
-    // — so every slot sums its terms in that order.
+    // shear, StVK, bending; element; a; b) — the order of the reference's
+    // launches, StVK (no reference counterpart) after the BW98 triangles —
+    // so every slot sums its terms in that order.
     const std::size_t S = static_cast<std::size_t>(n_) + nnz_;
     auto slot_of = [&](int slot) -> std::size_t {
         return slot < 0 ? static_cast<std::size_t>(-slot - 1) : static_cast<std::size_t>(n_) + slot;
@@ -98,6 +110,14 @@ void InstanceData::build(const InstanceSource& s) {
                 for (int a = 0; a < 3; ++a)
                     for (int b = 0; b < 3; ++b) visit(slot_of(sl[9 * t + 3 * a + b]), CHX_CONTRIB_TRI, a, b, tri_base + t);
             tri_base += ts->size();
+        }
+        if (s.stvk) {
+            const auto& sl = s.stvk->hessian_slots();
+            if (sl.size() != 9u * s.stvk->size()) throw std::runtime_error("chysx::engine: StVK slots not bound");
+            for (int t = 0; t < s.stvk->size(); ++t)
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        visit(slot_of(sl[9 * t + 3 * a + b]), CHX_CONTRIB_STVK, a, b, std::size_t(t));
         }
         if (s.bending) {
             const auto& sl = s.bending->hessian_slots();
@@ -127,7 +147,8 @@ void InstanceData::build(const InstanceSource& s) {
     };
     if (!fits(static_cast<std::size_t>(n_), 36) || !fits(static_cast<std::size_t>(nnz_), 36) ||
         !fits(tris_.cpu_size(), sizeof(float) * CHX_TRI_REC) || !fits(dihs_.cpu_size(), sizeof(float) * CHX_DIH_REC) ||
-        !fits(pins_.cpu_size(), sizeof(chx_pin)) || !fits(ent.size(), sizeof(unsigned)))
+        !fits(pins_.cpu_size(), sizeof(chx_pin)) || !fits(ent.size(), sizeof(unsigned)) ||
+        !fits(stvks_.cpu_size(), sizeof(float) * CHX_STVK_REC))
         throw std::invalid_argument(
             "chysx::engine: a per-instance array would reach 4 GiB (at most 119,304,647 vertices or off-diagonal "
             "blocks per cloth); split the cloth into several instances");
@@ -138,10 +159,12 @@ void InstanceData::build(const InstanceSource& s) {
     upload(pins_, bytes_);
     upload(tris_, bytes_);
     upload(dihs_, bytes_);
+    upload(stvks_, bytes_);
     const std::size_t n = static_cast<std::size_t>(n_);
     alloc(pin_rec_, pins_.cpu_size() * CHX_PIN_REC, bytes_);
     alloc(tri_rec_, tris_.cpu_size() * CHX_TRI_REC, bytes_);
     alloc(dih_rec_, dihs_.cpu_size() * CHX_DIH_REC, bytes_);
+    alloc(stvk_rec_, stvks_.cpu_size() * CHX_STVK_REC, bytes_);
     alloc(diag_, 9 * n, bytes_);
     alloc(values_, 9 * static_cast<std::size_t>(nnz_), bytes_);
     alloc(rhs_, 3 * n, bytes_);
@@ -182,13 +205,14 @@ void InstanceData::build(const InstanceSource& s) {
 }
 
 void InstanceData::update_elements(const InstanceSource& s) {
-    const std::size_t nP = pins_.cpu_size(), nT = tris_.cpu_size(), nD = dihs_.cpu_size();
+    const std::size_t nP = pins_.cpu_size(), nT = tris_.cpu_size(), nD = dihs_.cpu_size(), nV = stvks_.cpu_size();
     pack_elements(s);
-    if (pins_.cpu_size() != nP || tris_.cpu_size() != nT || dihs_.cpu_size() != nD)
+    if (pins_.cpu_size() != nP || tris_.cpu_size() != nT || dihs_.cpu_size() != nD || stvks_.cpu_size() != nV)
         throw std::runtime_error("chysx::engine: element counts changed; rebuild the instance");
     pins_.copy_to_device();
     tris_.copy_to_device();
     dihs_.copy_to_device();
+    stvks_.copy_to_device();
 }
 
 chx_instance InstanceData::descriptor(const InstanceSource& s) const {
@@ -207,6 +231,8 @@ chx_instance InstanceData::descriptor(const InstanceSource& s) const {
     I.pin_rec = pin_rec_.gpu_data();
     I.tri_rec = tri_rec_.gpu_data();
     I.dih_rec = dih_rec_.gpu_data();
+    I.stvks = stvks_.gpu_data();
+    I.stvk_rec = stvk_rec_.gpu_data();
     I.contrib_offsets = contrib_offsets_.gpu_data();
     I.contribs = contribs_.gpu_data();
     I.rhs = rhs_.gpu_data();
@@ -225,6 +251,7 @@ chx_instance InstanceData::descriptor(const InstanceSource& s) const {
     I.n_pins = static_cast<int>(pins_.cpu_size());
     I.n_tris = static_cast<int>(tris_.cpu_size());
     I.n_dihs = static_cast<int>(dihs_.cpu_size());
+    I.n_stvks = static_cast<int>(stvks_.cpu_size());
     I.tile_begin = 0;
     I.tile_end = item_count(n_);
     I.dt = s.dt;
@@ -363,8 +390,8 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds) {
     for (std::size_t i = 0; i < table_host_.size(); ++i) {
         const chx_instance& I = table_host_[i];
         const int inst = static_cast<int>(i);
-        const int counts[3] = {I.n_pins, I.n_tris, I.n_dihs};
-        for (int kind = 0; kind < 3; ++kind)
+        const int counts[4] = {I.n_pins, I.n_tris, I.n_dihs, I.n_stvks};
+        for (int kind = 0; kind < 4; ++kind)
             for (int b = 0; b < counts[kind]; b += CHX_ITEM_SIZE)
                 ei.insert(ei.end(), {inst, kind, b, std::min(counts[kind], b + CHX_ITEM_SIZE)});
         const int slots = I.n_rows + I.nnz;

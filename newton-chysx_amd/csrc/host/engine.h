@@ -31,6 +31,7 @@ struct InstanceSource {
     const constraint::PinConstraint* pins = nullptr;
     const constraint::TriangleStretchConstraint* stretch = nullptr;
     const constraint::TriangleStretchConstraint* shear = nullptr;
+    const constraint::StvkConstraint* stvk = nullptr;
     const constraint::BendingConstraint* bending = nullptr;
     float dt = 1.0f / 60.0f;
     float damping = 0.0f;
@@ -72,6 +73,8 @@ private:
     DeviceArray<chx_pin> pins_;
     DeviceArray<chx_tri> tris_;
     DeviceArray<chx_dih> dihs_;
+    DeviceArray<chx_stvk> stvks_;
+    DeviceArray<float> stvk_rec_;
     DeviceArray<float> pin_rec_, tri_rec_, dih_rec_, diag_, values_;
     DeviceArray<unsigned> contrib_offsets_, contribs_;
     DeviceArray<float> rhs_, r_, z_, p0_, p1_, q_, minv_;

@@ -72,6 +72,12 @@ template <class T> inline M3<T> outer(const V3<T>& a, const V3<T>& b) {
     return r;
 }
 template <class T> inline M3<T> identity3() { M3<T> r; r.m[0] = r.m[4] = r.m[8] = T(1); return r; }
+template <class T> inline M3<T> transpose(const M3<T>& a) {
+    M3<T> r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r.m[i * 3 + j] = a.m[j * 3 + i];
+    return r;
+}
 template <class T> inline M3<T> operator+(M3<T> a, const M3<T>& b) { for (int i = 0; i < 9; ++i) a.m[i] += b.m[i]; return a; }
 template <class T> inline M3<T> operator-(M3<T> a, const M3<T>& b) { for (int i = 0; i < 9; ++i) a.m[i] -= b.m[i]; return a; }
 template <class T> inline M3<T> operator*(M3<T> a, T s) { for (int i = 0; i < 9; ++i) a.m[i] *= s; return a; }
@@ -305,6 +311,103 @@ inline void triangle_hessian(const V3<T>& x0, const V3<T>& x1, const V3<T>& x2, 
         }
 }
 
+// ---------------------------------------------------------------------------
+// StVK membrane.  No reference counterpart: the reference keeps
+// ClothMaterial::lame_mu / lame_lambda as unused placeholders
+// (ChysX/src/cloth/cloth_material.h:93-116), so this part is pinned by
+// closed forms, finite differences and invariants only.
+//   F = [wu wv] (3x2) from the stretch rest shape (theta = 0) of
+//   triangle_rest_shape, G = (F^T F - I)/2,
+//   E = muA ||G||_F^2 + lamA/2 (tr G)^2, muA = mu*area*h, lamA = lam*area*h.
+//   dE/dF = F S, S = 2 muA G + lamA tr(G) I (2x2, symmetric).
+//   Second variation: 2 muA ||dG||^2 + lamA (tr dG)^2 + S : (dF^T dF).  The
+//   first two terms are PSD for muA > 0, lamA >= 0; the projected Hessian
+//   clamps S to its non-negative eigen-part S+ (exact when S is PSD).
+// ---------------------------------------------------------------------------
+template <class T>
+inline void stvk_deformation(const V3<T>& x0, const V3<T>& x1, const V3<T>& x2, const M2<T>& D, V3<T>& fu,
+                             V3<T>& fv) {
+    const V3<T> a = x1 - x0, b = x2 - x0;
+    fu = a * D.m[0] + b * D.m[2];
+    fv = a * D.m[1] + b * D.m[3];
+}
+
+// Stress S = {Suu, Svv, Suv} and Green strain {Guu, Gvv, Guv}.
+template <class T>
+inline void stvk_stress(const V3<T>& fu, const V3<T>& fv, T muA, T lamA, T S[3], T G[3]) {
+    G[0] = T(0.5) * (dot(fu, fu) - T(1));
+    G[1] = T(0.5) * (dot(fv, fv) - T(1));
+    G[2] = T(0.5) * dot(fu, fv);
+    const T tr = G[0] + G[1];
+    S[0] = T(2) * muA * G[0] + lamA * tr;
+    S[1] = T(2) * muA * G[1] + lamA * tr;
+    S[2] = T(2) * muA * G[2];
+}
+
+template <class T>
+inline T stvk_energy(const V3<T>& x0, const V3<T>& x1, const V3<T>& x2, const M2<T>& D, T muA, T lamA) {
+    V3<T> fu, fv;
+    stvk_deformation(x0, x1, x2, D, fu, fv);
+    T S[3], G[3];
+    stvk_stress(fu, fv, muA, lamA, S, G);
+    const T tr = G[0] + G[1];
+    return muA * (G[0] * G[0] + G[1] * G[1] + T(2) * G[2] * G[2]) + T(0.5) * lamA * tr * tr;
+}
+
+template <class T>
+inline void stvk_gradient(const V3<T>& x0, const V3<T>& x1, const V3<T>& x2, const M2<T>& D, T muA, T lamA,
+                          V3<T> g[3]) {
+    V3<T> fu, fv;
+    stvk_deformation(x0, x1, x2, D, fu, fv);
+    T S[3], G[3];
+    stvk_stress(fu, fv, muA, lamA, S, G);
+    const V3<T> Pu = fu * S[0] + fv * S[2], Pv = fu * S[2] + fv * S[1];
+    const T dwu[3] = {-D.m[0] - D.m[2], D.m[0], D.m[2]};
+    const T dwv[3] = {-D.m[1] - D.m[3], D.m[1], D.m[3]};
+    for (int i = 0; i < 3; ++i) g[i] = Pu * dwu[i] + Pv * dwv[i];
+}
+
+// Non-negative eigen-part of the symmetric 2x2 {a, c; c, b}.
+template <class T>
+inline void psd2(T a, T b, T c, T out[3]) {
+    const T m = T(0.5) * (a + b), d = T(0.5) * (a - b);
+    const T r = std::sqrt(d * d + c * c);
+    const T l1 = m + r, l2 = m - r;  // l1 >= l2
+    if (l2 >= T(0)) { out[0] = a; out[1] = b; out[2] = c; return; }
+    if (l1 <= T(0)) { out[0] = out[1] = out[2] = T(0); return; }
+    // Only l1 > 0: l1 v v^T, v the unit eigenvector of l1.
+    T vx, vy;
+    if (d >= T(0)) { vx = d + r; vy = c; } else { vx = c; vy = r - d; }
+    const T nn = vx * vx + vy * vy;
+    if (nn <= T(0)) { out[0] = l1; out[1] = T(0); out[2] = T(0); return; }
+    out[0] = l1 * vx * vx / nn;
+    out[1] = l1 * vy * vy / nn;
+    out[2] = l1 * vx * vy / nn;
+}
+
+// Hessian blocks H[a*3+b] (projected unless `exact`).
+template <class T>
+inline void stvk_hessian(const V3<T>& x0, const V3<T>& x1, const V3<T>& x2, const M2<T>& D, T muA, T lamA,
+                         M3<T> H[9], bool exact = false) {
+    V3<T> fu, fv;
+    stvk_deformation(x0, x1, x2, D, fu, fv);
+    T S[3], G[3];
+    stvk_stress(fu, fv, muA, lamA, S, G);
+    T Sp[3] = {S[0], S[1], S[2]};
+    if (!exact) psd2(S[0], S[1], S[2], Sp);
+    const M3<T> I3 = identity3<T>();
+    const M3<T> Huu = outer(fu, fu) * (T(2) * muA + lamA) + outer(fv, fv) * muA + I3 * Sp[0];
+    const M3<T> Hvv = outer(fv, fv) * (T(2) * muA + lamA) + outer(fu, fu) * muA + I3 * Sp[1];
+    const M3<T> Huv = outer(fv, fu) * muA + outer(fu, fv) * lamA + I3 * Sp[2];
+    const M3<T> Hvu = transpose(Huv);
+    const T dwu[3] = {-D.m[0] - D.m[2], D.m[0], D.m[2]};
+    const T dwv[3] = {-D.m[1] - D.m[3], D.m[1], D.m[3]};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            H[i * 3 + j] = Huu * (dwu[i] * dwu[j]) + Huv * (dwu[i] * dwv[j]) + Hvu * (dwv[i] * dwu[j]) +
+                           Hvv * (dwv[i] * dwv[j]);
+}
+
 // Dihedral frame (Bridson shape vectors): ChysX/src/constraint/cloth/
 // bending_constraint.cu:48-126 (kMinNorm = 1e-12 at :34).
 template <class T> struct DihedralFrame {
@@ -355,6 +458,13 @@ template <class T> struct TriSet {
     std::vector<int> slots;
 };
 
+template <class T> struct StvkSet {
+    std::vector<std::array<int, 3>> idx;
+    std::vector<M2<T>> Dm_inv;
+    std::vector<T> muA, lamA;
+    std::vector<int> slots;
+};
+
 template <class T> struct Sim {
     int n = 0;
     std::vector<V3<T>> pos, vel;
@@ -365,6 +475,7 @@ template <class T> struct Sim {
     std::vector<std::array<int, 3>> tris;
     std::vector<std::array<int, 2>> edges;
     TriSet<T> stretch, shear;
+    StvkSet<T> stvk;
     std::vector<std::array<int, 4>> dih;
     std::vector<T> rest_angle;
     T k_bend = T(0);
@@ -426,6 +537,23 @@ template <class T> struct Sim {
     // kQuarterPi: ChysX/src/cloth/cloth_simulator.cu:428.
     void build_shear(T k) { build_tri_set(shear, k, 0.7853981633974483f); }
 
+    // StVK membrane with Lame parameters (mu, lam) and thickness h, rest
+    // shape from the current positions (theta = 0).
+    void build_stvk(T mu, T lam, T h) {
+        for (const auto& t : tris) check_indices(t.data(), 3);
+        stvk.idx = tris;
+        stvk.Dm_inv.assign(tris.size(), M2<T>{});
+        stvk.muA.assign(tris.size(), T(0));
+        stvk.lamA.assign(tris.size(), T(0));
+        for (size_t i = 0; i < tris.size(); ++i) {
+            T area = T(0);
+            triangle_rest_shape(pos[tris[i][0]], pos[tris[i][1]], pos[tris[i][2]], T(1), T(0), stvk.Dm_inv[i], area);
+            stvk.muA[i] = mu * area * h;
+            stvk.lamA[i] = lam * area * h;
+        }
+        topology_dirty = true;
+    }
+
     void build_bending(T k) {
         k_bend = k;
         std::vector<int> flat(tris.size() * 3);
@@ -478,6 +606,10 @@ template <class T> struct Sim {
         };
         tri_pairs(stretch);
         tri_pairs(shear);
+        for (const auto& t : stvk.idx)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    if (a != b && t[a] != t[b]) { rows.push_back(t[a]); cols.push_back(t[b]); }
         for (const auto& d : dih)
             for (int a = 0; a < 4; ++a)
                 for (int b = 0; b < 4; ++b)
@@ -487,6 +619,7 @@ template <class T> struct Sim {
         for (size_t c = 0; c < pin_idx.size(); ++c) pin_slots[c] = resolve_slot(pat, pin_idx[c], pin_idx[c]);
         stretch.slots = slot_table<3>(pat, stretch.idx);
         shear.slots = slot_table<3>(pat, shear.idx);
+        stvk.slots = slot_table<3>(pat, stvk.idx);
         dih_slots = slot_table<4>(pat, dih);
         topo_n = n;
         topology_dirty = false;
@@ -515,6 +648,12 @@ template <class T> struct Sim {
         };
         tri_grad(stretch);
         tri_grad(shear);
+        for (size_t t = 0; t < stvk.idx.size(); ++t) {
+            V3<T> ge[3];
+            stvk_gradient(x[stvk.idx[t][0]], x[stvk.idx[t][1]], x[stvk.idx[t][2]], stvk.Dm_inv[t], stvk.muA[t],
+                          stvk.lamA[t], ge);
+            for (int a = 0; a < 3; ++a) g[stvk.idx[t][a]] = g[stvk.idx[t][a]] + ge[a];
+        }
         // bending_constraint.cu:180-206.
         for (size_t e = 0; e < dih.size(); ++e) {
             const auto F = dihedral_frame(x[dih[e][0]], x[dih[e][1]], x[dih[e][2]], x[dih[e][3]]);
@@ -533,6 +672,9 @@ template <class T> struct Sim {
         for (const TriSet<T>* s : {&stretch, &shear})
             for (size_t t = 0; t < s->idx.size(); ++t)
                 E += double(triangle_energy(x[s->idx[t][0]], x[s->idx[t][1]], x[s->idx[t][2]], s->Dm_inv[t], s->area[t], s->k));
+        for (size_t t = 0; t < stvk.idx.size(); ++t)
+            E += double(stvk_energy(x[stvk.idx[t][0]], x[stvk.idx[t][1]], x[stvk.idx[t][2]], stvk.Dm_inv[t],
+                                    stvk.muA[t], stvk.lamA[t]));
         for (size_t e = 0; e < dih.size(); ++e) {
             const auto F = dihedral_frame(x[dih[e][0]], x[dih[e][1]], x[dih[e][2]], x[dih[e][3]]);
             if (!F.valid) continue;
@@ -557,6 +699,12 @@ template <class T> struct Sim {
                 triangle_hessian(x[s->idx[t][0]], x[s->idx[t][1]], x[s->idx[t][2]], s->Dm_inv[t], s->area[t], s->k, H);
                 for (int q = 0; q < 9; ++q) add_block(s->slots[9 * t + q], H[q]);
             }
+        for (size_t t = 0; t < stvk.idx.size(); ++t) {
+            M3<T> H[9];
+            stvk_hessian(x[stvk.idx[t][0]], x[stvk.idx[t][1]], x[stvk.idx[t][2]], stvk.Dm_inv[t], stvk.muA[t],
+                         stvk.lamA[t], H);
+            for (int q = 0; q < 9; ++q) add_block(stvk.slots[9 * t + q], H[q]);
+        }
         for (size_t e = 0; e < dih.size(); ++e) {  // bending_constraint.cu:217-245
             const auto F = dihedral_frame(x[dih[e][0]], x[dih[e][1]], x[dih[e][2]], x[dih[e][3]]);
             if (!F.valid) continue;

@@ -41,6 +41,26 @@ void put3(const std::vector<V3<T>>& v, double* out) {
         out[3 * i + 2] = double(v[i].z);
     }
 }
+template <class T>
+void stvk_eval(const double* x, const double* dinv, double muA, double lamA, int exact, double* E, double* g,
+                      double* H) {
+    V3<T> v[3];
+    for (int a = 0; a < 3; ++a) v[a] = V3<T>(T(x[3 * a]), T(x[3 * a + 1]), T(x[3 * a + 2]));
+    M2<T> D;
+    for (int q = 0; q < 4; ++q) D.m[q] = T(dinv[q]);
+    if (E) *E = double(stvk_energy(v[0], v[1], v[2], D, T(muA), T(lamA)));
+    if (g) {
+        V3<T> ge[3];
+        stvk_gradient(v[0], v[1], v[2], D, T(muA), T(lamA), ge);
+        for (int a = 0; a < 3; ++a) { g[3 * a] = double(ge[a].x); g[3 * a + 1] = double(ge[a].y); g[3 * a + 2] = double(ge[a].z); }
+    }
+    if (H) {
+        M3<T> hb[9];
+        stvk_hessian(v[0], v[1], v[2], D, T(muA), T(lamA), hb, exact != 0);
+        for (int b = 0; b < 9; ++b)
+            for (int q = 0; q < 9; ++q) H[9 * b + q] = double(hb[b].m[q]);
+    }
+}
 }  // namespace
 
 #define DISPATCH(h, expr)                      \
@@ -89,6 +109,17 @@ int orc_detect_dihedrals(const int* tris, int n_tris, int* out, int* out_count) 
     });
 }
 
+// ---- single StVK triangle (pinning tests) ---------------------------------------
+// x: 3 vertices (9 doubles), dinv: Dm^-1 row-major (4), energy / gradient (9)
+// / Hessian blocks H[a*3+b] row-major (81); exact != 0 skips the projection.
+int orc_stvk_element(int precision, const double* x, const double* dinv, double muA, double lamA, int exact,
+                     double* E, double* g, double* H) {
+    return guarded([&] {
+        if (precision == 64) stvk_eval<double>(x, dinv, muA, lamA, exact, E, g, H);
+        else stvk_eval<float>(x, dinv, muA, lamA, exact, E, g, H);
+    });
+}
+
 // ---- simulator ---------------------------------------------------------------
 void* orc_sim_create(int precision) {
     auto* h = new Handle;
@@ -129,6 +160,10 @@ int orc_sim_build_stretch(void* hv, double k) {
 int orc_sim_build_shear(void* hv, double k) {
     auto* h = static_cast<Handle*>(hv);
     return guarded([&] { DISPATCH(h, { using T = decltype(S.damping); S.build_shear(T(float(k))); }); });
+}
+int orc_sim_build_stvk(void* hv, double mu, double lam, double thickness) {
+    auto* h = static_cast<Handle*>(hv);
+    return guarded([&] { DISPATCH(h, { using T = decltype(S.damping); S.build_stvk(T(float(mu)), T(float(lam)), T(float(thickness))); }); });
 }
 int orc_sim_build_bending(void* hv, double k) {
     auto* h = static_cast<Handle*>(hv);
@@ -185,7 +220,8 @@ int orc_sim_get_slots(void* hv, int kind, int* out) {
     auto* h = static_cast<Handle*>(hv);
     return guarded([&] { DISPATCH(h, {
         if (S.topology_dirty || S.topo_n != S.n) S.ensure_topology();
-        const std::vector<int>* v = kind == 0 ? &S.pin_slots : kind == 1 ? &S.stretch.slots : kind == 2 ? &S.shear.slots : &S.dih_slots;
+        const std::vector<int>* v = kind == 0 ? &S.pin_slots : kind == 1 ? &S.stretch.slots : kind == 2 ? &S.shear.slots
+                                  : kind == 4 ? &S.stvk.slots : &S.dih_slots;
         std::copy(v->begin(), v->end(), out); }); });
 }
 int orc_sim_get_dihedrals(void* hv, int* out, double* rest) {

@@ -45,13 +45,16 @@ def lib():
             "orc_sim_build_bending", "orc_sim_set_pins", "orc_sim_set_pcg", "orc_sim_step", "orc_sim_run",
             "orc_sim_last_stats", "orc_sim_sizes", "orc_sim_get_pattern", "orc_sim_get_slots",
             "orc_sim_get_dihedrals", "orc_sim_get_edges", "orc_sim_get_rest_shape", "orc_sim_get_system",
-            "orc_sim_set_dx", "orc_sim_energy", "orc_sim_gradient", "orc_sim_hessian",
+            "orc_sim_set_dx", "orc_sim_energy", "orc_sim_gradient", "orc_sim_hessian", "orc_sim_build_stvk",
+            "orc_stvk_element",
         ):
             getattr(L, name).restype = _I
         L.orc_sim_step.argtypes = [_P, _D]
         L.orc_sim_build_stretch.argtypes = [_P, _D]
         L.orc_sim_build_shear.argtypes = [_P, _D]
         L.orc_sim_build_bending.argtypes = [_P, _D]
+        L.orc_sim_build_stvk.argtypes = [_P, _D, _D, _D]
+        L.orc_stvk_element.argtypes = [_I, _P, _P, _D, _D, _I, _P, _P, _P]
         L.orc_sim_redistribute_mass.argtypes = [_P, _D]
         L.orc_sim_set_material.argtypes = [_P, _D, _D, _D, _D]
         L.orc_sim_set_pcg.argtypes = [_P, _I, _D]
@@ -134,6 +137,9 @@ class OracleSim:
     def build_shear(self, k):
         _check(self.L.orc_sim_build_shear(self.h, k))
 
+    def build_stvk(self, mu, lam, thickness):
+        _check(self.L.orc_sim_build_stvk(self.h, mu, lam, thickness))
+
     def build_bending(self, k):
         _check(self.L.orc_sim_build_bending(self.h, k))
 
@@ -188,7 +194,7 @@ class OracleSim:
         return ro, ci[: s["nnz"]].copy()
 
     def slots(self, kind: str, count: int) -> np.ndarray:
-        k = {"pin": 0, "stretch": 1, "shear": 2, "bending": 3}[kind]
+        k = {"pin": 0, "stretch": 1, "shear": 2, "bending": 3, "stvk": 4}[kind]
         out = np.zeros(max(1, count), np.int32)
         _check(self.L.orc_sim_get_slots(self.h, k, _ptr(out)))
         return out[:count].copy()
@@ -277,3 +283,16 @@ def set_num_threads(n: int):
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+def stvk_element(x, dinv, muA, lamA, exact=False, precision=64):
+    """Energy, gradient (3, 3) and Hessian blocks (3, 3, 3, 3) [a, b, i, j] of
+    one StVK triangle with vertices x (3, 3) and rest shape Dm^-1 (2, 2)."""
+    x = _f64(x).reshape(9)
+    dinv = _f64(dinv).reshape(4)
+    E = ctypes.c_double()
+    g = np.zeros(9)
+    H = np.zeros(81)
+    _check(lib().orc_stvk_element(int(precision), _ptr(x), _ptr(dinv), float(muA), float(lamA), int(bool(exact)),
+                                  ctypes.byref(E), _ptr(g), _ptr(H)))
+    return E.value, g.reshape(3, 3), H.reshape(3, 3, 3, 3)

@@ -19,6 +19,8 @@ def oracle_from_spec(spec, precision=32, pcg=None):
         o.build_stretch(spec.k_stretch)
     if spec.k_shear > 0:
         o.build_shear(spec.k_shear)
+    if getattr(spec, "stvk", None) is not None:
+        o.build_stvk(*spec.stvk)
     if spec.k_bend > 0:
         o.build_bending(spec.k_bend)
     if len(spec.pins):

@@ -81,3 +81,69 @@ def test_bf16_batch_needs_bf16_instances(gpu):
     batch.add_instance(plain.sim)
     with pytest.raises(ValueError):
         batch.step(spec.dt)
+
+
+def test_precision_report(gpu):
+    """The precision workload: one n x n grid (512 in the workload; default
+    128 here, CHYSX_PRECISION_N to change), E = 1e6 Pa, seeded initial shear
+    of up to 5 cm, 100 substeps (CHYSX_PRECISION_STEPS).  Two variants -- FP32
+    blocks / SpMV with FP64 reductions, and BF16 blocks + iterative
+    refinement -- against a full-FP64 oracle run of the same substeps.
+    Reports the relative max-norm position error after the last substep, the
+    GPU time per substep and the PCG iterations.  Both variants must stay
+    within the drift a free-running FP32 trajectory shows."""
+    import json
+    import os
+    import time
+
+    import torch
+
+    from chysx import scenes
+
+    n = int(os.environ.get("CHYSX_PRECISION_N", "128"))
+    steps = int(os.environ.get("CHYSX_PRECISION_STEPS", "100"))
+    spec = scenes.precision(n=n)
+    report = {"workload": f"precision grid {n}x{n}, E=1e6 Pa, shear start, {steps} substeps", "vertices": spec.n}
+    finals = {}
+    for name, bf16 in (("fp32", False), ("bf16_ir10", True)):
+        dc = scenes.DeviceCloth(spec)
+        if bf16:
+            dc.sim.set_bf16_blocks(True, 10)
+        its = []
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ms = 0.0
+        for s in range(steps):
+            e0.record()
+            dc.step()
+            e1.record()
+            e1.synchronize()
+            ms += e0.elapsed_time(e1)
+            its.append(dc.sim.last_pcg_iterations())
+        finals[name] = dc.state()[0]
+        report[name] = {"ms_per_substep": round(ms / steps, 3), "pcg_iterations_mean": float(np.mean(its)),
+                        "pcg_iterations_max": int(np.max(its)),
+                        "last_relative_residual": float(dc.sim.last_relative_residual())}
+        print(f"{name}: {ms / steps:.3f} ms/substep, iterations mean {np.mean(its):.1f}", flush=True)
+        del dc
+    o = helpers.oracle_from_spec(spec, precision=64)
+    t0 = time.perf_counter()
+    its64 = []
+    for s in range(steps):
+        its64.append(o.step(spec.dt))
+        if (s + 1) % 10 == 0:
+            print(f"FP64 oracle substep {s + 1}/{steps} ({time.perf_counter() - t0:.0f} s)", flush=True)
+    p64 = o.state()[0]
+    report["fp64_oracle"] = {"pcg_iterations_mean": float(np.mean(its64)), "seconds": round(time.perf_counter() - t0, 1),
+                             "threads": pyoracle_threads()}
+    for name in finals:
+        report[name]["rel_maxnorm_error_vs_fp64"] = helpers.rel_maxnorm(finals[name], p64)
+        report[name]["max_abs_error_m"] = float(np.max(np.abs(finals[name] - p64)))
+    print("PRECISION " + json.dumps(report), flush=True)
+    for name in finals:
+        assert report[name]["rel_maxnorm_error_vs_fp64"] <= 1e-3, (name, report[name])
+
+
+def pyoracle_threads():
+    import pyoracle
+
+    return pyoracle.num_threads()

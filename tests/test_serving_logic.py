@@ -24,7 +24,7 @@ class FakeExecutor:
         self.admitted = []
         self.max_bytes = 0
 
-    def admit(self, req):
+    def admit(self, req, dt=None):
         req.bytes = req.spec.n * 1000
         self.active.append(req)
         self.admitted.append(req.job_id)
