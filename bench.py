@@ -54,31 +54,46 @@ def load_traffic():
         return None
 
 
-def cpu_baseline(specs, budget_s=12.0):
+def cpu_baseline(specs, budget_s=12.0, budget_1core_s=8.0):
+    """The CPU oracle (FP32 restatement of the reference step, OpenMP over
+    rows and elements) on a bounded sample of the same instances: one timed
+    substep per instance after one warm-up substep, first on every host
+    thread, then on one thread for a smaller sample."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import helpers
     import pyoracle
 
+    def sample(budget, threads):
+        pyoracle.set_num_threads(threads)
+        done_vs, secs, used, iters = 0, 0.0, 0, []
+        t_all = time.perf_counter()
+        for spec in specs:
+            o = helpers.oracle_from_spec(spec, precision=32)
+            o.step(spec.dt)  # warm-up substep (untimed)
+            s, _ = o.run(spec.dt, 1)
+            iters.append(o.last_stats()[0])
+            secs += s
+            done_vs += spec.n
+            used += 1
+            if secs >= budget or time.perf_counter() - t_all > 3 * budget:
+                break
+        return done_vs / secs if secs > 0 else None, used, secs, float(sum(iters) / max(len(iters), 1))
+
     threads = pyoracle.num_threads()
-    done_vs, secs, used = 0, 0.0, 0
-    t_all = time.perf_counter()
-    for spec in specs:
-        o = helpers.oracle_from_spec(spec, precision=32)
-        o.step(spec.dt)  # warm-up substep (untimed)
-        s, _ = o.run(spec.dt, 1)
-        secs += s
-        done_vs += spec.n
-        used += 1
-        if secs >= budget_s or time.perf_counter() - t_all > 3 * budget_s:
-            break
+    v_all, used, secs, it_all = sample(budget_s, threads)
+    v_one, used1, secs1, _ = sample(budget_1core_s, 1)
+    pyoracle.set_num_threads(threads)
     return {
-        "value": done_vs / secs if secs > 0 else None,
+        "value": v_all,
         "unit": "vertex-substeps/s",
         "cores": threads,
         "kind": "port",
+        "value_1core": v_one,
+        "pcg_iterations_mean": it_all,
         "sample": f"{used} of {len(specs)} batched instances (seeded order), 1 timed substep each after 1 warm-up, "
-                  f"FP32 oracle, {secs:.1f} s of timed CPU work",
+                  f"FP32 oracle on {threads} threads, {secs:.1f} s of timed CPU work; 1-thread figure over the "
+                  f"first {used1} instances ({secs1:.1f} s)",
     }
 
 
