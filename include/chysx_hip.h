@@ -122,8 +122,10 @@ typedef struct chx_instance {
     float* p1;
     float* q;
     float* minv;                     /* [9][n_rows] planar block-Jacobi preconditioner       */
-    uint16_t* diag16;                /* planar BF16 copies of diag / values for the inner    */
-    uint16_t* values16;              /* SpMV of the BF16 precision mode, or NULL             */
+    float* diag_c;                   /* BF16 precision mode, else NULL: the inner SpMV's     */
+    uint16_t* values16;              /* matrix, planar FP32 compensated diagonal             */
+                                     /* D + sum_j (V_ij - bf16(V_ij)) [9][n_rows] and planar */
+                                     /* BF16 off-diagonal copies [9][nnz]                     */
     const chx_stvk* stvks;           /* StVK membrane elements, or NULL                      */
     float* stvk_rec;                 /* [n_stvks*CHX_STVK_REC]                               */
     const int* windows;              /* [ceil(n_rows/CHX_STAGE_ROWS)][2]: smallest and       */
@@ -155,8 +157,11 @@ typedef struct chx_cloth_batch {
     int n_slot_items;
     int lanes_per_row;               /* SpMV lanes per block row: 4, 8 or 16          */
     int grid_blocks;                 /* persistent PCG workgroups (0 = auto)          */
-    int bf16_blocks;                 /* 1: inner SpMV on BF16 blocks (FP32 accumulate) */
-    int ir_period;                   /* BF16 mode: true-residual refinement period    */
+    int bf16_blocks;                 /* 1: inner CG on the BF16 matrix (FP32          */
+                                     /* accumulate), outer refinement on the FP32     */
+                                     /* true residual when the inner residual meets   */
+                                     /* the tolerance                                 */
+    int pad_bf16;
     int tile_rows;                   /* rows per tile of the tile list (informational) */
     int pad0;
     chx_instance single;

@@ -175,7 +175,17 @@ struct MatP {
     }
 };
 using MatP32 = MatP<float>;
-using MatP16 = MatP<uint16_t>;  // BF16 copies, FP32 arithmetic
+
+// BF16 precision mode's inner matrix: FP32 compensated diagonal blocks and
+// BF16 off-diagonal blocks, FP32 arithmetic.
+struct MatP16 {
+    gp<const float> d;
+    gp<const uint16_t> v;
+    unsigned n;
+    unsigned nnz;
+    __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv9p(d, n, row, x); }
+    __device__ __forceinline__ f3 off(unsigned k, f3 x) const { return mv9p(v, nnz, k, x); }
+};
 
 __device__ __forceinline__ double dot64(f3 a, f3 b) {
     return double(a.x) * double(b.x) + double(a.y) * double(b.y) + double(a.z) * double(b.z);
@@ -195,7 +205,8 @@ struct Inst {
     gp<float> pin_rec, tri_rec, dih_rec;
     gp<const unsigned> contrib_offsets, contribs;
     gp<float> rhs, x, r, z, p0, p1, q, minv;
-    gp<uint16_t> diag16, values16;
+    gp<float> diag_c;
+    gp<uint16_t> values16;
     gp<const int> windows;
     gp<const chx_stvk> stvks;
     gp<float> stvk_rec;
@@ -232,7 +243,7 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.p1 = glb(d->p1);
     I.q = glb(d->q);
     I.minv = glb(d->minv);
-    I.diag16 = glb(d->diag16);
+    I.diag_c = glb(d->diag_c);
     I.values16 = glb(d->values16);
     I.windows = glb(d->windows);
     I.stvks = glb(d->stvks);

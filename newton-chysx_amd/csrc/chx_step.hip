@@ -35,7 +35,9 @@ namespace {
 constexpr int kBlock = 256;
 static_assert(CHX_ITEM_SIZE == kBlock, "one thread per work-item entry");
 
-enum { P_RHO0 = 0, P_RHO1 = 1, P_RR0 = 2, P_RR1 = 3, P_SIG = 4, P_BB = 5 };
+// Tile partials: rho = <r,z> and ||r||^2 per parity, sigma = <p,q>, ||b||^2,
+// and per parity whether that residual is the true one (BF16 mode).
+enum { P_RHO0 = 0, P_RHO1 = 1, P_RR0 = 2, P_RR1 = 3, P_SIG = 4, P_BB = 5, P_TRUE0 = 6, P_TRUE1 = 7 };
 
 // Divide guard |den| > 1e-37 (ChysX/src/solver/pcg_solver.cu:122-127), with
 // the FP64 inner products rounded to FP32 once.
@@ -190,10 +192,6 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
         // Planar blocks: component k of block row `row` at diag[k*n + row].
 #pragma unroll
         for (unsigned k = 0; k < 9; ++k) stx(I.diag, k * n + row, h[k]);
-        if (I.diag16 != nullptr) {
-#pragma unroll
-            for (unsigned k = 0; k < 9; ++k) stx(I.diag16, k * n + row, f32_to_bf16(h[k]));
-        }
     } else {
         const unsigned e = s - n, nnz = unsigned(I.nnz);
 #pragma unroll
@@ -203,6 +201,38 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
             for (unsigned k = 0; k < 9; ++k) stx(I.values16, k * nnz + e, f32_to_bf16(h[k]));
         }
     }
+}
+
+// BF16 mode: the diagonal of the inner matrix absorbs the rounding of its
+// row's off-diagonal blocks, D_c = D + sum_j (V_ij - bf16(V_ij)), so the
+// BF16 matrix maps block-constant vectors exactly as the FP32 one does
+// (A_c (1 (x) c) = A (1 (x) c)).  The stiffness rows sum to zero, so without
+// it the rounding of the off-diagonal blocks perturbs the smooth modes --
+// the slowest ones for CG -- by O(eps * stiffness) against eigenvalues of
+// O(stiffness / n^2); with it the perturbation acts on differences
+// x_j - x_i only.  One thread per block row, over the gather's slot items.
+__global__ __launch_bounds__(kBlock) void compensate_kernel(chx_cloth_batch B) {
+    int inst, sb, se;
+    get_slot_item(B, blockIdx.x, inst, sb, se);
+    const int ss = sb + int(threadIdx.x);
+    if (ss >= se) return;
+    const Inst I = inst_at(B, inst);
+    const unsigned n = unsigned(I.n_rows), row = unsigned(ss);
+    if (row >= n) return;
+    const unsigned nnz = unsigned(I.nnz);
+    const unsigned k0 = ldx(I.row_offsets, row), k1 = ldx(I.row_offsets, row + 1u);
+    float d[9];
+#pragma unroll
+    for (unsigned c = 0; c < 9; ++c) d[c] = ldx(gp<const float>(I.diag), c * n + row);
+    for (unsigned k = k0; k < k1; ++k) {
+#pragma unroll
+        for (unsigned c = 0; c < 9; ++c) {
+            const float v = ldx(gp<const float>(I.values), c * nnz + k);
+            d[c] += v - bf16_to_f32(f32_to_bf16(v));
+        }
+    }
+#pragma unroll
+    for (unsigned c = 0; c < 9; ++c) stx(I.diag_c, c * n + row, d[c]);
 }
 
 __global__ void reset_sync_kernel(unsigned* sync) {
@@ -366,12 +396,13 @@ __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re
 // S.sc[m][4] = 1.0 when the instance has finished its solve.  16 lanes per
 // tile sum lane-strided over the instance's tiles, then a 16-lane
 // butterfly, so an instance's sums are the same bits in every workgroup.
+template <bool F4 = false>
 __device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const double> part, int t0, int G, int f0,
-                                           int f1, int f2, int f3_, PcgLds& S) {
+                                           int f1, int f2, int f3_, PcgLds& S, int f4 = 0) {
     __syncthreads();
     const int m = int(threadIdx.x) >> 4, l = int(threadIdx.x) & 15;
     const int t = t0 + m * G;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
     double done = 0.0;
     if (t < B.n_tiles) {
         int inst, rb, re;
@@ -383,6 +414,7 @@ __device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const do
             s1 += p[f1];
             s2 += p[f2];
             s3 += p[f3_];
+            if (F4) s4 += p[f4];
         }
         if (l == 0) done = __hip_atomic_load(&B.inst_done[inst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1.0 : 0.0;
     }
@@ -390,27 +422,31 @@ __device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const do
     s1 = seg_sum<16>(s1);
     s2 = seg_sum<16>(s2);
     s3 = seg_sum<16>(s3);
+    if (F4) s4 = seg_sum<16>(s4);
     if (l == 0) {
         S.sc[m][0] = s0;
         S.sc[m][1] = s1;
         S.sc[m][2] = s2;
         S.sc[m][3] = s3;
         S.sc[m][4] = done;
+        S.sc[m][5] = s4;
     }
     __syncthreads();
 }
 
-// `checkable`: the residual norm at hand is one the convergence test may use
-// (always in FP32 mode; in BF16 mode only the true residual after a
-// refinement step).  Finishing is final: once an instance stops, its tiles no
-// longer refresh the parity-indexed partials, so the test is never re-run on
-// them (`finished`).
-__device__ __forceinline__ bool instance_active(const Inst& I, int k, double rr, double bb, bool checkable,
-                                                bool finished) {
-    if (finished) return false;
-    if (k >= I.max_iter) return false;
-    if (checkable && I.tol > 0.0 && rr <= I.tol * I.tol * bb) return false;
-    return true;
+// What an instance does in iteration slot k.  `true_res`: the residual at
+// hand is b - A x with the FP32 matrix (always in FP32 mode; in BF16 mode
+// after the set-up and after a refinement slot).  A BF16 instance whose
+// recursive residual meets the tolerance spends the slot on the true
+// residual instead of stopping (outer iterative refinement).  Finishing is
+// final: once an instance stops, its tiles no longer refresh the
+// parity-indexed partials, so the test is never re-run on them (`finished`).
+enum Slot { STOP, ITERATE, REFINE };
+__device__ __forceinline__ Slot instance_slot(const Inst& I, int k, double rr, double bb, bool true_res,
+                                              bool finished) {
+    if (finished || k >= I.max_iter) return STOP;
+    if (I.tol > 0.0 && rr <= I.tol * I.tol * bb) return true_res ? STOP : REFINE;
+    return ITERATE;
 }
 
 // ---------------------------------------------------------------------------
@@ -423,8 +459,9 @@ __device__ __forceinline__ bool instance_active(const Inst& I, int k, double rr,
 // grid barrier and one pass over p per iteration.
 // ---------------------------------------------------------------------------
 // Setup / refinement step of one tile: r = b - A32 x (true residual with
-// the FP32 matrix), M^-1 = inverse(D) (setup only), z = M^-1 r, p = z; tile
-// partials <r,z>, <r,r> (and <b,b> at setup).  M^-1 is stored planar.
+// the FP32 matrix), M^-1 = inverse(D) (setup only), z = M^-1 r, and p = z
+// when pdst is given (setup); tile partials <r,z>, <r,r> (and <b,b> at
+// setup).  M^-1 is stored planar.
 template <int L, class MatR>
 __device__ __forceinline__ void residual_tile(const Inst& I, MatR mat, int rb, int re, bool setup, gp<float> pdst,
                                               PcgLds& S, double (&acc)[3]) {
@@ -447,7 +484,7 @@ __device__ __forceinline__ void residual_tile(const Inst& I, MatR mat, int rb, i
         const f3 z = mv9(mi, r);
         st3g(I.r, row, r);
         st3g(I.z, row, z);
-        st3g(pdst, row, z);
+        if (pdst != nullptr) st3g(pdst, row, z);
         acc[0] += dot64(r, z);
         acc[1] += dot64(r, r);
         acc[2] += dot64(b, b);
@@ -476,7 +513,7 @@ struct PcgMats<1> {
 template <>
 struct PcgMats<2> {
     static __device__ __forceinline__ MatP16 a(const Inst& I) {
-        return MatP16{I.diag16, I.values16, unsigned(I.n_rows), unsigned(I.nnz)};
+        return MatP16{I.diag_c, I.values16, unsigned(I.n_rows), unsigned(I.nnz)};
     }
     static __device__ __forceinline__ MatP32 r(const Inst& I) {
         return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.nnz)};
@@ -500,9 +537,11 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
     const int G = int(gridDim.x);
     unsigned* sync = B.sync;
     const gp<double> part = glb(B.partials);
-    // BF16 mode: every `period` iterations the next iteration starts from the
-    // true residual (FP32 matrix) and restarts the search directions.
-    const int period = BF16 ? max(B.ir_period, 1) : 1;
+    // BF16 mode: the inner iterations are CG on the BF16 matrix (compensated
+    // FP32 diagonal) and test the recursively updated residual; when it meets
+    // the tolerance, the next slot recomputes the true residual with the FP32
+    // matrix.  The solve stops only on a true residual; otherwise CG restarts
+    // from it (p = z): outer iterative refinement against the FP32 residual.
 
     // ---- setup: r = b - A x, M^-1, z = M^-1 r, p = z; rho, ||r||^2, ||b||^2 ----
     if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&sync[SYNC_ACTIVE], unsigned(B.n_instances));
@@ -522,6 +561,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
             p[P_RHO0] = acc[0];
             p[P_RR0] = acc[1];
             p[P_BB] = acc[2];
+            p[P_TRUE0] = 1.0;
         }
     }
     if (!grid_sync(sync, G)) return;
@@ -529,15 +569,12 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
 
     for (int k = 0;; ++k) {
         const int cur = k & 1;
-        const int j = BF16 ? k % period : k;                // iterations since the last (re)start
-        const bool refine = BF16 && (j == period - 1);     // this iteration computes the true residual
-        const bool checkable = !BF16 || j == 0;
+        const bool first = (k == 0);  // p_0 = z_0, written by the set-up
         // ---- phase A: p_k = z + beta p_{k-1}; q = A p_k; sigma = <p_k, q> ----
-        // A refinement step rewrites the rho slot of parity cur^1, so it
-        // does not read it (its beta is not needed).
-        const int f_prev = refine ? P_RHO0 + cur : P_RHO0 + (cur ^ 1);
+        // A refining instance rewrites its parity cur^1 partials during this
+        // phase; its own tiles read them here but do not use them (no beta).
         for (int t0 = blockIdx.x; t0 < B.n_tiles; t0 += G * kChunk) {
-            chunk_sums(B, part, t0, G, P_RHO0 + cur, f_prev, P_RR0 + cur, P_BB, S);
+            chunk_sums<BF16>(B, part, t0, G, P_RHO0 + cur, P_RHO0 + (cur ^ 1), P_RR0 + cur, P_BB, S, P_TRUE0 + cur);
             for (int m = 0; m < kChunk; ++m) {
                 const int t = t0 + m * G;
                 if (t >= B.n_tiles) break;
@@ -546,7 +583,9 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 const Inst I = inst_at(B, inst);
                 const double rho = S.sc[m][0], rho_prev = S.sc[m][1], rr = S.sc[m][2], bb = S.sc[m][3];
                 const bool finished = S.sc[m][4] != 0.0;
-                if (!instance_active(I, k, rr, bb, checkable, finished)) {
+                const bool true_res = !BF16 || S.sc[m][5] != 0.0;
+                const Slot slot = instance_slot(I, k, rr, bb, true_res, finished);
+                if (slot == STOP) {
                     if (threadIdx.x == 0 && t == I.tile_begin && !finished) {
                         __hip_atomic_store(&B.inst_done[inst], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         B.inst_iters[inst] = k;
@@ -558,16 +597,17 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 }
                 const gp<float> pprev = cur ? I.p0 : I.p1;  // p_{k-1}
                 const gp<float> pk = cur ? I.p1 : I.p0;     // p_k
-                if (refine) {
-                    // True residual of the current x; p_{k+1} = z goes to the
-                    // buffer of parity k+1 (p_{k-1}'s, unused this iteration).
+                if (BF16 && slot == REFINE) {
+                    // True residual r = b - A32 x, z = M^-1 r and their sums
+                    // become the parity cur^1 partials, marked true.
                     double acc[3] = {0.0, 0.0, 0.0};
-                    residual_tile<L>(I, PcgMats<V>::r(I), rb, re, false, pprev, S, acc);
+                    residual_tile<L>(I, PcgMats<V>::r(I), rb, re, false, gp<float>(nullptr), S, acc);
                     block_sum(acc, S.red);
                     if (threadIdx.x == 0) {
                         const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
                         p[P_RHO0 + (cur ^ 1)] = acc[0];
                         p[P_RR0 + (cur ^ 1)] = acc[1];
+                        p[P_TRUE0 + (cur ^ 1)] = 1.0;
                     }
                     continue;
                 }
@@ -575,12 +615,15 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 const gp<float> q = I.q;
                 double acc[1] = {0.0};
                 auto fin = [&](unsigned row, f3 Ap, f3 pr) {
-                    if (j != 0) st3g(pk, row, pr);
+                    if (!first) st3g(pk, row, pr);
                     st3g(q, row, Ap);
                     acc[0] += dot64(pr, Ap);
                 };
-                if (j == 0) {
+                if (first) {
                     tile_spmv<L>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(pk, jj); }, fin);
+                } else if (BF16 && true_res) {
+                    // restart after a refinement slot: p_k = z_k
+                    tile_spmv<L>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(z, jj); }, fin);
                 } else {
                     const float beta = guarded_div(rho, rho_prev);
                     auto pj = [&](unsigned jj) {
@@ -601,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         // ---- phase B: alpha; x += alpha p; r -= alpha q; z = M^-1 r; rho', rr' --
         const int nxt = cur ^ 1;
         for (int t0 = blockIdx.x; t0 < B.n_tiles; t0 += G * kChunk) {
-            chunk_sums(B, part, t0, G, P_RHO0 + cur, P_SIG, P_RR0 + cur, P_BB, S);
+            chunk_sums<BF16>(B, part, t0, G, P_RHO0 + cur, P_SIG, P_RR0 + cur, P_BB, S, P_TRUE0 + cur);
             for (int m = 0; m < kChunk; ++m) {
                 const int t = t0 + m * G;
                 if (t >= B.n_tiles) break;
@@ -609,7 +652,8 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 get_tile(B, t, inst, rb, re);
                 const Inst I = inst_at(B, inst);
                 const double rho = S.sc[m][0], sigma = S.sc[m][1], rr = S.sc[m][2], bb = S.sc[m][3];
-                if (refine || !instance_active(I, k, rr, bb, checkable, S.sc[m][4] != 0.0)) continue;
+                const bool true_res = !BF16 || S.sc[m][5] != 0.0;
+                if (instance_slot(I, k, rr, bb, true_res, S.sc[m][4] != 0.0) != ITERATE) continue;
                 const float alpha = guarded_div(rho, sigma);
                 const gp<const float> pk = cur ? I.p1 : I.p0;
                 const unsigned n = unsigned(I.n_rows);
@@ -631,6 +675,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                     const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
                     p[P_RHO0 + nxt] = acc[0];
                     p[P_RR0 + nxt] = acc[1];
+                    p[P_TRUE0 + nxt] = 0.0;
                 }
             }
         }
@@ -770,7 +815,6 @@ void validate_batch(const chx_cloth_batch* b) {
     require(b->lanes_per_row == 4 || b->lanes_per_row == 8 || b->lanes_per_row == 16,
             "chysx: lanes_per_row must be 4, 8 or 16");
     require(b->sync != nullptr, "chysx: batch descriptor without a sync area");
-    require(b->bf16_blocks == 0 || b->ir_period >= 1, "chysx: BF16 mode needs ir_period >= 1");
     if (b->inst == nullptr) {
         require(b->n_instances == 1, "chysx: a batch without an instance table holds exactly one instance");
         require(b->elem_items == nullptr && b->slot_items == nullptr,
@@ -782,7 +826,7 @@ void validate_batch(const chx_cloth_batch* b) {
             require(I.row_offsets && I.diag && I.rhs && I.x && I.r && I.z && I.p0 && I.p1 && I.q && I.minv,
                     "chysx: null array in instance");
         if (b->bf16_blocks && I.n_rows > 0)
-            require(I.diag16 && (I.values16 || I.nnz == 0), "chysx: BF16 mode needs the BF16 block copies");
+            require(I.diag_c && (I.values16 || I.nnz == 0), "chysx: BF16 mode needs the BF16 block copies");
     } else {
         require(b->tiles != nullptr || b->n_tiles == 0, "chysx: batch without a tile list");
         require(b->elem_items != nullptr || b->n_elem_items == 0, "chysx: batch without an element list");
@@ -815,6 +859,10 @@ void assemble(const chx_cloth_batch& B, hipStream_t stream) {
     if (ns > 0) {
         gather_kernel<<<ns, kBlock, 0, stream>>>(B);
         hip_check(hipGetLastError(), "gather_kernel launch");
+        if (B.bf16_blocks) {
+            compensate_kernel<<<ns, kBlock, 0, stream>>>(B);
+            hip_check(hipGetLastError(), "compensate_kernel launch");
+        }
     } else {
         reset_sync_kernel<<<1, kBlock, 0, stream>>>(B.sync);
         hip_check(hipGetLastError(), "reset_sync_kernel launch");

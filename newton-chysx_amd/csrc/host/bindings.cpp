@@ -143,7 +143,7 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("tile_rows", &ClothSimulator::tile_rows)
         .def("set_grid_blocks", &ClothSimulator::set_grid_blocks, py::arg("blocks"))
         .def("lanes_per_row", &ClothSimulator::lanes_per_row)
-        .def("set_bf16_blocks", &ClothSimulator::set_bf16_blocks, py::arg("enabled"), py::arg("ir_period") = 10)
+        .def("set_bf16_blocks", &ClothSimulator::set_bf16_blocks, py::arg("enabled"))
         .def("bf16_blocks", &ClothSimulator::bf16_blocks)
         .def("last_pcg_iterations", &ClothSimulator::last_pcg_iterations, py::arg("cuda_stream") = 0)
         .def("last_relative_residual", &ClothSimulator::last_relative_residual, py::arg("cuda_stream") = 0)
@@ -166,6 +166,15 @@ PYBIND11_MODULE(_chysx_native, m) {
                  s.debug_copy_last_rhs(rhs.mutable_data());
                  s.debug_copy_last_dx(dx.mutable_data());
                  return py::make_tuple(diag, row_offsets, col_indices, values, rhs, dx);
+             })
+        .def("debug_bf16_matrix",
+             [](const ClothSimulator& s) {
+                 const int n = s.num_particles();
+                 const int nnz = s.num_off_diag_blocks();
+                 py::array_t<float> diag_c({n, 3, 3});
+                 py::array_t<uint16_t> values16({nnz, 3, 3});
+                 s.debug_copy_bf16_matrix(diag_c.mutable_data(), nnz > 0 ? values16.mutable_data() : nullptr);
+                 return py::make_tuple(diag_c, values16);
              })
         .def(
             "debug_set_warm_start",
@@ -234,8 +243,7 @@ PYBIND11_MODULE(_chysx_native, m) {
             [](chysx::cloth::ClothBatch& b, std::uintptr_t stream) { return trace_array(b.last_trace(stream)); },
             py::arg("cuda_stream") = 0)
         .def("tile_rows", &chysx::cloth::ClothBatch::tile_rows)
-        .def("set_bf16_blocks", &chysx::cloth::ClothBatch::set_bf16_blocks, py::arg("enabled"),
-             py::arg("ir_period") = 10)
+        .def("set_bf16_blocks", &chysx::cloth::ClothBatch::set_bf16_blocks, py::arg("enabled"))
         .def("finalize", &chysx::cloth::ClothBatch::finalize, py::arg("dt") = 1.0f / 60.0f)
         .def("remove_instance", &chysx::cloth::ClothBatch::remove_instance, py::arg("simulator"))
         .def("clear", &chysx::cloth::ClothBatch::clear)

@@ -33,7 +33,7 @@ std::vector<chx_instance> ClothBatch::descriptors(float dt) {
     ds.reserve(sims_.size());
     for (ClothSimulator* s : sims_) {
         ds.push_back(s->prepare_instance(dt));
-        if (bf16_ && ds.back().n_rows > 0 && ds.back().diag16 == nullptr)
+        if (bf16_ && ds.back().n_rows > 0 && ds.back().diag_c == nullptr)
             throw std::invalid_argument(
                 "ClothBatch: BF16 mode needs every instance switched with set_bf16_blocks(true)");
     }

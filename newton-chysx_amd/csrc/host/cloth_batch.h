@@ -63,9 +63,9 @@ public:
     int tile_rows() const noexcept { return runner_.tile_rows(); }
     // BF16 precision mode for the whole batch; every instance must have been
     // switched with ClothSimulator::set_bf16_blocks(true) as well.
-    void set_bf16_blocks(bool on, int ir_period = 10) {
+    void set_bf16_blocks(bool on) {
         bf16_ = on;
-        runner_.set_bf16(on, ir_period);
+        runner_.set_bf16(on);
     }
     std::vector<int> last_pcg_iterations(std::uintptr_t cuda_stream = 0);
     std::vector<double> last_relative_residuals(std::uintptr_t cuda_stream = 0);

@@ -181,12 +181,10 @@ void ClothSimulator::set_lanes_per_row(int lanes) {
     runner_.set_lanes_per_row(lanes);
 }
 
-void ClothSimulator::set_bf16_blocks(bool on, int ir_period) {
-    if (ir_period < 1) throw std::invalid_argument("ClothSimulator::set_bf16_blocks: ir_period must be >= 1");
+void ClothSimulator::set_bf16_blocks(bool on) {
     if (on != bf16_) topology_dirty_ = true;  // the block copies are (de)allocated with the instance
     bf16_ = on;
-    ir_period_ = ir_period;
-    runner_.set_bf16(on, ir_period);
+    runner_.set_bf16(on);
 }
 
 // cloth_simulator.cu:795-812: dx (the PCG warm start) is zeroed only when
@@ -222,7 +220,7 @@ void ClothSimulator::ensure_hessian_topology() {
     prepare_topology();
     inst_.build(source());
     runner_.set_lanes_per_row(lanes_per_row_);
-    runner_.set_bf16(bf16_, ir_period_);
+    runner_.set_bf16(bf16_);
     runner_.bind_single(inst_.descriptor(source()));
     packed_k_[0] = pins_.stiffness();
     packed_k_[1] = fem_stretch_.stiffness();
@@ -332,9 +330,10 @@ double ClothSimulator::last_relative_residual(std::uintptr_t cuda_stream) {
 namespace {
 // Device blocks are planar (component c of block i at p[c*count + i]); the
 // reference's debug taps return row-major 3x3 blocks.
-void copy_planar_blocks(float* out, const float* dev, int count, const char* what) {
-    std::vector<float> planes(static_cast<std::size_t>(9) * count);
-    check(chx_memcpy_d2h(planes.data(), dev, sizeof(float) * planes.size(), 0), "cloth", what);
+template <class T>
+void copy_planar_blocks(T* out, const T* dev, int count, const char* what) {
+    std::vector<T> planes(static_cast<std::size_t>(9) * count);
+    check(chx_memcpy_d2h(planes.data(), dev, sizeof(T) * planes.size(), 0), "cloth", what);
     for (int i = 0; i < count; ++i)
         for (int c = 0; c < 9; ++c)
             out[static_cast<std::size_t>(9) * i + c] = planes[static_cast<std::size_t>(c) * count + i];
@@ -347,6 +346,16 @@ void ClothSimulator::debug_copy_hessian_diag(float* out) const {
     if (out == nullptr) throw std::invalid_argument("ClothSimulator::debug_copy_hessian_diag: out must be non-null");
     check(chx_device_synchronize(), "cloth", "device synchronize(debug_copy_hessian_diag)");
     copy_planar_blocks(out, inst_.diag(), n, "diag copy");
+}
+
+void ClothSimulator::debug_copy_bf16_matrix(float* out_diag_c, uint16_t* out_values16) const {
+    const int n = H_num_block_rows_;
+    if (n == 0) return;
+    if (!bf16_ || inst_.diag_c() == nullptr)
+        throw std::logic_error("ClothSimulator::debug_copy_bf16_matrix: BF16 mode is off or not built yet");
+    check(chx_device_synchronize(), "cloth", "device synchronize(debug_copy_bf16_matrix)");
+    copy_planar_blocks(out_diag_c, inst_.diag_c(), n, "diag_c copy");
+    if (pattern_.nnz() > 0) copy_planar_blocks(out_values16, inst_.values16(), pattern_.nnz(), "values16 copy");
 }
 
 void ClothSimulator::debug_copy_hessian_csr(int* out_row_offsets, int* out_col_indices, float* out_values) const {

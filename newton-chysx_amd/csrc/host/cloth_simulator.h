@@ -127,12 +127,12 @@ public:
     const std::vector<int>& debug_stage_windows() const noexcept { return inst_.stage_windows(); }
     int lanes_per_row() const noexcept { return lanes_per_row_; }
     // Precision of the solve: false = FP32 Hessian blocks (default); true =
-    // BF16 copies of the blocks in the inner SpMV (FP32 accumulation) with
-    // the true residual (FP32 blocks, FP64 norms) recomputed and the
-    // iteration restarted every `ir_period` iterations.
-    void set_bf16_blocks(bool on, int ir_period = 10);
+    // the inner CG runs on BF16 copies of the off-diagonal blocks and an FP32
+    // diagonal that absorbs their rounding (FP32 accumulation); when its
+    // residual meets the tolerance, the true residual (FP32 blocks, FP64
+    // norms) decides: stop, or restart the inner CG from it.
+    void set_bf16_blocks(bool on);
     bool bf16_blocks() const noexcept { return bf16_; }
-    int ir_period() const noexcept { return ir_period_; }
     int last_pcg_iterations(std::uintptr_t cuda_stream = 0);
     double last_relative_residual(std::uintptr_t cuda_stream = 0);
 
@@ -141,6 +141,9 @@ public:
     int num_off_diag_blocks() const noexcept { return pattern_.nnz(); }
     void debug_copy_hessian_diag(float* out) const;
     void debug_copy_hessian_csr(int* out_row_offsets, int* out_col_indices, float* out_values) const;
+    // BF16 mode: the inner matrix of the last step, row-major blocks
+    // (compensated FP32 diagonal [n][3][3], BF16 bit patterns [nnz][3][3]).
+    void debug_copy_bf16_matrix(float* out_diag_c, uint16_t* out_values16) const;
     void debug_copy_last_rhs(float* out) const;
     void debug_copy_last_dx(float* out) const;
     // Overwrites the PCG warm start (the previous solve's dx); n*3 floats.
@@ -190,7 +193,6 @@ private:
     double pcg_tolerance_ = 0.0;
     int lanes_per_row_ = 8;
     bool bf16_ = false;
-    int ir_period_ = 10;
     SolverType solver_type_ = SolverType::PCG;
 };
 

@@ -195,10 +195,10 @@ void InstanceData::build(const InstanceSource& s) {
         upload(windows_, bytes_);
     }
     if (s.bf16_blocks) {
-        alloc(diag16_, 9 * n, bytes_);
+        alloc(diag_c_, 9 * n, bytes_);
         alloc(values16_, 9 * static_cast<std::size_t>(nnz_), bytes_);
     } else {
-        diag16_.release();
+        diag_c_.release();
         values16_.release();
     }
     built_ = true;
@@ -243,7 +243,7 @@ chx_instance InstanceData::descriptor(const InstanceSource& s) const {
     I.p1 = p1_.gpu_data();
     I.q = q_.gpu_data();
     I.minv = minv_.gpu_data();
-    I.diag16 = s.bf16_blocks ? diag16_.gpu_data() : nullptr;
+    I.diag_c = s.bf16_blocks ? diag_c_.gpu_data() : nullptr;
     I.values16 = s.bf16_blocks ? values16_.gpu_data() : nullptr;
     I.windows = windows_.gpu_data();
     I.n_rows = n_;
@@ -370,7 +370,6 @@ void BatchRunner::bind_single(const chx_instance& d) {
     desc_.lanes_per_row = lanes_;
     desc_.grid_blocks = grid_;
     desc_.bf16_blocks = bf16_ ? 1 : 0;
-    desc_.ir_period = ir_period_;
     desc_.single = one[0];
     desc_.inst = nullptr;
     desc_.elem_items = nullptr;
@@ -407,7 +406,6 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds) {
     desc_.lanes_per_row = lanes_;
     desc_.grid_blocks = grid_;
     desc_.bf16_blocks = bf16_ ? 1 : 0;
-    desc_.ir_period = ir_period_;
     desc_.single = chx_instance{};
     desc_.inst = table_.gpu_data();
     desc_.elem_items = elem_items_.gpu_data();

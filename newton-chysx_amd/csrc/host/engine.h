@@ -60,6 +60,8 @@ public:
     std::size_t contributions() const noexcept { return contribs_.cpu_size(); }
     const float* diag() const noexcept { return diag_.gpu_data(); }
     const float* values() const noexcept { return values_.gpu_data(); }
+    const float* diag_c() const noexcept { return diag_c_.gpu_data(); }
+    const uint16_t* values16() const noexcept { return values16_.gpu_data(); }
     const float* rhs() const noexcept { return rhs_.gpu_data(); }
     const std::vector<int>& stage_windows() const noexcept { return windows_.host(); }
 
@@ -78,7 +80,8 @@ private:
     DeviceArray<float> pin_rec_, tri_rec_, dih_rec_, diag_, values_;
     DeviceArray<unsigned> contrib_offsets_, contribs_;
     DeviceArray<float> rhs_, r_, z_, p0_, p1_, q_, minv_;
-    DeviceArray<uint16_t> diag16_, values16_;
+    DeviceArray<float> diag_c_;
+    DeviceArray<uint16_t> values16_;
     DeviceArray<int> windows_;  // per CHX_STAGE_ROWS-row stage: {min column, max column}
 };
 
@@ -116,13 +119,11 @@ public:
         grid_ = g < 0 ? 0 : g;
         desc_.grid_blocks = grid_;
     }
-    // Precision mode of the solve: FP32 blocks, or BF16 blocks in the inner
-    // SpMV with a true-residual refinement every `ir_period` iterations.
-    void set_bf16(bool on, int ir_period) {
+    // Precision mode of the solve: FP32 blocks, or inner CG on the BF16
+    // matrix with outer refinement on the FP32 true residual.
+    void set_bf16(bool on) {
         bf16_ = on;
-        ir_period_ = ir_period < 1 ? 1 : ir_period;
         desc_.bf16_blocks = on ? 1 : 0;
-        desc_.ir_period = ir_period_;
     }
     int n_instances() const noexcept { return desc_.n_instances; }
     int n_tiles() const noexcept { return desc_.n_tiles; }
@@ -149,7 +150,6 @@ private:
     int grid_ = 0;
     int tile_rows_ = CHX_TILE_ROWS;
     bool bf16_ = false;
-    int ir_period_ = 10;
     std::vector<chx_instance> table_host_;
     DeviceArray<chx_instance> table_;
     DeviceArray<int> tiles_, elem_items_, slot_items_;

@@ -1,6 +1,7 @@
 """BF16 precision mode of the cloth step: Hessian blocks stored in BF16 for
-the inner PCG iterations (FP32 arithmetic), iterative refinement against the
-FP32 true residual.  Checked against the full-FP64 oracle."""
+the inner PCG iterations (FP32 arithmetic) with a diagonal that absorbs
+their rounding, outer iterative refinement against the FP32 true residual.
+Checked against the full-FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -9,27 +10,64 @@ import helpers
 pytestmark = pytest.mark.gpu
 
 
-def _bf16_cloth(spec, ir_period=10):
+def _bf16_cloth(spec):
     from chysx import scenes
 
     dc = scenes.DeviceCloth(spec)
-    dc.sim.set_bf16_blocks(True, ir_period)
+    dc.sim.set_bf16_blocks(True)
     return dc
 
 
-@pytest.mark.parametrize("ir_period", [5, 10])
-def test_bf16_substeps_match_fp64_oracle(gpu, ir_period):
+def _bf16_round(v):
+    """float32 -> BF16 bit patterns, round to nearest even (finite inputs)."""
+    u = np.ascontiguousarray(v, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def test_bf16_inner_matrix_bits(gpu):
+    """The inner matrix of the BF16 mode, bit for bit: off-diagonal blocks
+    are the round-to-nearest-even BF16 of the FP32 blocks, and the diagonal
+    is D + sum_j (V_ij - bf16(V_ij)) accumulated in FP32 in column order.
+    Hence the BF16 matrix maps block-constant vectors like the FP32 one."""
+    from chysx import scenes
+
+    spec = scenes.plain_small()
+    dc = _bf16_cloth(spec)
+    for _ in range(3):
+        dc.step()
+    D, ro, ci, V, _, _ = dc.sim.debug_dump_last_solve()
+    Dc, V16 = dc.sim.debug_bf16_matrix()
+    assert np.array_equal(V16, _bf16_round(V))
+    back = (V16.astype(np.uint32) << 16).view(np.float32)
+    delta = (V - back).astype(np.float32)  # exact in FP32
+    expect = D.astype(np.float32).copy()
+    deg = np.diff(ro)
+    for pos in range(int(deg.max())):
+        rows = np.nonzero(deg > pos)[0]
+        expect[rows] = (expect[rows] + delta[ro[rows] + pos]).astype(np.float32)
+    assert np.array_equal(Dc, expect)
+    rows = np.repeat(np.arange(len(D)), deg)
+    c = np.array([0.3, -1.2, 0.7])
+    a32 = D.astype(np.float64) @ c
+    np.add.at(a32, rows, V.astype(np.float64) @ c)
+    a16 = Dc.astype(np.float64) @ c
+    np.add.at(a16, rows, back.astype(np.float64) @ c)
+    assert np.abs(a16 - a32).max() <= 1e-6 * np.abs(D @ c).max()
+
+
+def test_bf16_substeps_match_fp64_oracle(gpu):
     """60 substeps of the 32x32 workload, each started from the FP64
-    oracle's state: relative max-norm position error <= 1e-5 per substep,
-    and the solve stops on the true FP32 residual (<= tol) or at the cap."""
+    oracle's state: relative max-norm position error <= 1e-5 per substep;
+    the solve stops on the true FP32 residual (<= tol) or at the cap."""
     import torch
 
     from chysx import scenes
 
     spec = scenes.plain_small()
-    dc = _bf16_cloth(spec, ir_period)
+    dc = _bf16_cloth(spec)
     o = helpers.oracle_from_spec(spec, precision=64)
     worst = 0.0
+    its, its64 = [], []
     for s in range(60):
         po, vo = o.state()
         dxo = o.system()[3] if s else np.zeros((spec.n, 3))
@@ -37,17 +75,17 @@ def test_bf16_substeps_match_fp64_oracle(gpu, ir_period):
         dc.vel.copy_(torch.from_numpy(vo.astype(np.float32)).cuda())
         dc.sim.debug_set_warm_start(dxo.astype(np.float32))
         dc.step()
-        o.step(spec.dt)
+        its64.append(o.step(spec.dt))
         p, _ = dc.state()
         err = helpers.rel_maxnorm(p, o.state()[0])
         worst = max(worst, err)
         it = dc.sim.last_pcg_iterations()
+        its.append(it)
         res = dc.sim.last_relative_residual()
         assert err <= 1e-5, (s, err)
         assert it == spec.max_iter or res <= spec.tol, (s, it, res)
-        # Convergence is tested only on true residuals (restart points).
-        assert it == spec.max_iter or it % ir_period == 0, (s, it)
-    print(f"BF16 ir_period={ir_period}: worst one-substep rel err vs FP64 {worst:.3e}")
+    print(f"BF16: worst one-substep rel err vs FP64 {worst:.3e}; PCG iterations mean {np.mean(its):.1f} "
+          f"(FP64 oracle {np.mean(its64):.1f}), max excess {max(a - b for a, b in zip(its, its64))}")
 
 
 def test_bf16_batch_equals_alone_bitwise(gpu):
@@ -58,7 +96,7 @@ def test_bf16_batch_equals_alone_bitwise(gpu):
     alone = [_bf16_cloth(s) for s in specs]
     mixed = [_bf16_cloth(s) for s in specs]
     batch = chysx.ClothBatch()
-    batch.set_bf16_blocks(True, 10)
+    batch.set_bf16_blocks(True)
     for c in mixed:
         batch.add_instance(c.sim)
     for step in range(6):
@@ -77,38 +115,35 @@ def test_bf16_batch_needs_bf16_instances(gpu):
     spec = scenes.batched(seed=22, count=1, lo=8, hi=9)[0]
     plain = scenes.DeviceCloth(spec)
     batch = chysx.ClothBatch()
-    batch.set_bf16_blocks(True, 10)
+    batch.set_bf16_blocks(True)
     batch.add_instance(plain.sim)
     with pytest.raises(ValueError):
         batch.step(spec.dt)
 
 
-def test_precision_report(gpu):
-    """The precision workload: one n x n grid (512 in the workload; default
-    128 here, CHYSX_PRECISION_N to change), E = 1e6 Pa, seeded initial shear
-    of up to 5 cm, 100 substeps (CHYSX_PRECISION_STEPS).  Two variants -- FP32
-    blocks / SpMV with FP64 reductions, and BF16 blocks + iterative
-    refinement -- against a full-FP64 oracle run of the same substeps.
-    Reports the relative max-norm position error after the last substep, the
-    GPU time per substep and the PCG iterations.  Both variants must stay
-    within the drift a free-running FP32 trajectory shows."""
+def _precision_run(n, steps, max_iter):
+    """The precision workload: one n x n grid, E = 1e6 Pa, seeded initial
+    shear of up to 5 cm, `steps` substeps, PCG tol 1e-6 and `max_iter`.  Two
+    forms -- FP32 blocks / SpMV with FP64 reductions, and BF16 blocks + outer
+    iterative refinement -- against a full-FP64 oracle run of the same
+    substeps: relative max-norm position error after the last substep, GPU
+    time per substep, PCG iterations."""
+    import dataclasses
     import json
-    import os
     import time
 
     import torch
 
     from chysx import scenes
 
-    n = int(os.environ.get("CHYSX_PRECISION_N", "128"))
-    steps = int(os.environ.get("CHYSX_PRECISION_STEPS", "100"))
-    spec = scenes.precision(n=n)
-    report = {"workload": f"precision grid {n}x{n}, E=1e6 Pa, shear start, {steps} substeps", "vertices": spec.n}
+    spec = dataclasses.replace(scenes.precision(n=n), max_iter=max_iter)
+    report = {"workload": f"precision grid {n}x{n}, E=1e6 Pa, shear start, {steps} substeps, "
+                          f"tol {spec.tol:g}, max_iter {max_iter}", "vertices": spec.n}
     finals = {}
-    for name, bf16 in (("fp32", False), ("bf16_ir10", True)):
+    for name, bf16 in (("fp32", False), ("bf16_ir", True)):
         dc = scenes.DeviceCloth(spec)
         if bf16:
-            dc.sim.set_bf16_blocks(True, 10)
+            dc.sim.set_bf16_blocks(True)
         its = []
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ms = 0.0
@@ -139,8 +174,34 @@ def test_precision_report(gpu):
         report[name]["rel_maxnorm_error_vs_fp64"] = helpers.rel_maxnorm(finals[name], p64)
         report[name]["max_abs_error_m"] = float(np.max(np.abs(finals[name] - p64)))
     print("PRECISION " + json.dumps(report), flush=True)
-    for name in finals:
-        assert report[name]["rel_maxnorm_error_vs_fp64"] <= 1e-3, (name, report[name])
+    return report
+
+
+def _precision_size():
+    import os
+
+    return int(os.environ.get("CHYSX_PRECISION_N", "128")), int(os.environ.get("CHYSX_PRECISION_STEPS", "100"))
+
+
+def test_precision_report_converged(gpu):
+    """Every substep solved to the tolerance (iteration cap far above what
+    the solves need): both forms end within 1e-4 of the FP64 run.  Default
+    128 x 128 (CHYSX_PRECISION_N / CHYSX_PRECISION_STEPS to change)."""
+    n, steps = _precision_size()
+    rep = _precision_run(n, steps, max_iter=40 * n)
+    for name in ("fp32", "bf16_ir"):
+        assert rep[name]["pcg_iterations_max"] < 40 * n, (name, rep[name])
+        assert rep[name]["rel_maxnorm_error_vs_fp64"] <= 1e-4, (name, rep[name])
+
+
+def test_precision_report_capped(gpu):
+    """The reference's iteration cap (100): no form converges on this stiff
+    grid, so each follows its own truncated Krylov path and the error
+    against the FP64 run measures that path's sensitivity.  BF16 stays
+    within 10x of FP32's."""
+    n, steps = _precision_size()
+    rep = _precision_run(n, steps, max_iter=100)
+    assert rep["bf16_ir"]["rel_maxnorm_error_vs_fp64"] <= 10 * rep["fp32"]["rel_maxnorm_error_vs_fp64"] + 1e-5, rep
 
 
 def pyoracle_threads():

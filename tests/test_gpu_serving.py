@@ -12,7 +12,10 @@ def test_served_results_equal_alone(gpu):
 
     specs = scenes.batched(seed=31, count=6, lo=8, hi=40)
     dt = specs[0].dt
-    reqs = [serving.Request(job_id=i, arrival=0.03 * i, spec=s, substeps=6 + 4 * i) for i, s in enumerate(specs)]
+    # Job 0 runs 400 substeps (each one is at least a launch and a
+    # synchronisation, > 50 us); the others arrive 5 ms apart while it runs.
+    reqs = [serving.Request(job_id=i, arrival=0.005 * i, spec=s, substeps=400 if i == 0 else 10 + 4 * i)
+            for i, s in enumerate(specs)]
     ex = serving.GpuExecutor(lanes_per_row=8, keep_results=True, prepare_workers=2)
     try:
         rep = serving.ServingEngine(ex).run(reqs, dt)
