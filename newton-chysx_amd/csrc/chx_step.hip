@@ -36,7 +36,9 @@ constexpr int kBlock = 256;
 static_assert(CHX_ITEM_SIZE == kBlock, "one thread per work-item entry");
 
 // Tile partials: rho = <r,z> and ||r||^2 per parity, sigma = <p,q>, ||b||^2,
-// and per parity whether that residual is the true one (BF16 mode).
+// and per parity the refinement state of that residual (BF16 mode): 2 *
+// (refinements so far) + 1 if it is the true residual.  Every tile of an
+// instance holds the same state.
 enum { P_RHO0 = 0, P_RHO1 = 1, P_RR0 = 2, P_RR1 = 3, P_SIG = 4, P_BB = 5, P_TRUE0 = 6, P_TRUE1 = 7 };
 
 // Divide guard |den| > 1e-37 (ChysX/src/solver/pcg_solver.cu:122-127), with
@@ -304,6 +306,7 @@ constexpr unsigned kCiCap = 4096;   // column indices per staged stage
 constexpr unsigned kWinCap = 2816;  // source rows per staged stage
 constexpr int kChunk = 16;          // tiles whose instance scalars are summed together
 static_assert(kChunk * 16 == kBlock, "16 lanes per tile in chunk_sums");
+constexpr int kCompactMax = 4 * kBlock;  // instances up to which the active tiles are compacted
 
 struct PcgLds {
     int ro[kStage + 1];
@@ -311,6 +314,14 @@ struct PcgLds {
     float win[3 * kWinCap];
     double red[16];
     double sc[kChunk][6];
+    // Active tile list (batches of up to kCompactMax instances): instance i
+    // owns tiles [tb[i], tb[i] + tc[i]); pre[i] = active tiles before it.
+    int tb[kCompactMax];
+    int tc[kCompactMax];
+    int pre[kCompactMax];
+    int wsum[kBlock / kWave];
+    int nact;
+    int ctile[kChunk];  // the tiles of the chunk at hand, -1 past the end
 };
 
 __device__ __forceinline__ f3 lds3(const float* w, unsigned i) { return {w[3 * i], w[3 * i + 1], w[3 * i + 2]}; }
@@ -391,20 +402,22 @@ __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re
     }
 }
 
-// Scalars of the instances of up to kChunk tiles t = t0 + m*G (m < kChunk):
-// S.sc[m][0..3] = sums over the instance's tiles of partial fields f0..f3,
-// S.sc[m][4] = 1.0 when the instance has finished its solve.  16 lanes per
-// tile sum lane-strided over the instance's tiles, then a 16-lane
-// butterfly, so an instance's sums are the same bits in every workgroup.
+// Scalars of the instances of the chunk's tiles t = S.ctile[m] (m < kChunk):
+// S.sc[m][0..3] = sums over the instance's tiles of partial fields f0..f3
+// (and the tile mean of f4 into S.sc[m][5] when F4, a per-instance value
+// every tile holds), S.sc[m][4] = 1.0 when the instance has
+// finished its solve.  16 lanes per tile sum lane-strided over the
+// instance's tiles, then a 16-lane butterfly, so an instance's sums are the
+// same bits in every workgroup.
 template <bool F4 = false>
-__device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const double> part, int t0, int G, int f0,
-                                           int f1, int f2, int f3_, PcgLds& S, int f4 = 0) {
+__device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const double> part, int f0, int f1, int f2,
+                                           int f3_, PcgLds& S, int f4 = 0) {
     __syncthreads();
     const int m = int(threadIdx.x) >> 4, l = int(threadIdx.x) & 15;
-    const int t = t0 + m * G;
+    const int t = S.ctile[m];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
-    double done = 0.0;
-    if (t < B.n_tiles) {
+    double done = 0.0, ntl = 1.0;
+    if (t >= 0) {
         int inst, rb, re;
         get_tile(B, t, inst, rb, re);
         const Inst I = inst_at(B, inst);
@@ -417,6 +430,7 @@ __device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const do
             if (F4) s4 += p[f4];
         }
         if (l == 0) done = __hip_atomic_load(&B.inst_done[inst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1.0 : 0.0;
+        ntl = double(I.tile_end - I.tile_begin);
     }
     s0 = seg_sum<16>(s0);
     s1 = seg_sum<16>(s1);
@@ -429,23 +443,127 @@ __device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const do
         S.sc[m][2] = s2;
         S.sc[m][3] = s3;
         S.sc[m][4] = done;
-        S.sc[m][5] = s4;
+        S.sc[m][5] = F4 ? s4 / ntl : 0.0;
     }
     __syncthreads();
 }
 
+// ---- work distribution ----------------------------------------------------
+// A phase's tiles form a list of T entries: every tile (static), or the tiles
+// of the instances still solving (compacted, in instance order).  With G a
+// multiple of 8 the list is cut into 8 contiguous segments and segment x
+// goes to the workgroups g = x (mod 8), which the dispatcher places on one
+// XCD, so neighbouring tiles (overlapping source windows) share an L2.
+__device__ __forceinline__ int list_entry(int T, int i) {
+    const int G = int(gridDim.x), g = int(blockIdx.x);
+    int a;
+    if ((G & 7) == 0) {
+        const int gx = G >> 3, seg = (T + 7) >> 3;
+        const int pos = (g >> 3) + i * gx;
+        if (pos >= seg) return -1;
+        a = (g & 7) * seg + pos;
+    } else {
+        a = g + i * G;
+    }
+    return a < T ? a : -1;
+}
+
+// Tile of list entry a of the compacted list: the instance j with
+// pre[j] <= a < pre[j] + (active tiles of j), found by binary search.
+__device__ __forceinline__ int compact_tile(const PcgLds& S, int n, int a) {
+    int lo = 0, hi = n;  // pre[lo] <= a, answer in [lo, hi)
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (S.pre[mid] <= a) lo = mid;
+        else hi = mid;
+    }
+    return S.tb[lo] + (a - S.pre[lo]);
+}
+
+// Static per-launch table of the instances' tile ranges (compacted mode).
+__device__ __forceinline__ void load_tile_ranges(const chx_cloth_batch& B, PcgLds& S) {
+    const gp<const int> tab = glb(reinterpret_cast<const int*>(B.inst));
+    constexpr unsigned words = unsigned(sizeof(chx_instance) / 4);
+    constexpr unsigned ob = unsigned(offsetof(chx_instance, tile_begin) / 4);
+    constexpr unsigned oe = unsigned(offsetof(chx_instance, tile_end) / 4);
+    for (int i = threadIdx.x; i < B.n_instances; i += blockDim.x) {
+        const gp<const int> d = rec_at(tab, unsigned(i), words);
+        const int b = d[ob], e = d[oe];
+        S.tb[i] = b;
+        S.tc[i] = e - b;
+    }
+}
+
+// Builds pre[] / nact from the done flags: instance i counts tc[i] tiles
+// while it is solving, 0 once finished.  The flags must not change while
+// any workgroup builds (every workgroup has to derive the same list).
+__device__ __forceinline__ void build_active(const chx_cloth_batch& B, PcgLds& S) {
+    const int n = B.n_instances;
+    const int base = int(threadIdx.x) * 4;
+    int c[4];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = base + q;
+        c[q] = 0;
+        if (i < n && __hip_atomic_load(&B.inst_done[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            c[q] = S.tc[i];
+        sum += c[q];
+    }
+    // inclusive scan of `sum` over the workgroup: wave scan, then wave totals
+    const int lane = int(threadIdx.x) & (kWave - 1), wave = int(threadIdx.x) / kWave;
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int v = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += v;
+    }
+    __syncthreads();  // the previous list's readers are done
+    if (lane == kWave - 1) S.wsum[wave] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += S.wsum[w];
+    int run = before + incl - sum;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = base + q;
+        if (i < n) S.pre[i] = run;
+        run += c[q];
+    }
+    if (threadIdx.x == blockDim.x - 1) S.nact = before + incl;
+    __syncthreads();
+}
+
+// Fills S.ctile with the tiles of this workgroup's chunk c of the phase
+// list; returns false when the workgroup has no tile left (uniform).
+__device__ __forceinline__ bool next_chunk(const chx_cloth_batch& B, PcgLds& S, bool compact, int c) {
+    __syncthreads();  // the previous chunk's readers are done
+    const int T = compact ? S.nact : B.n_tiles;
+    if (threadIdx.x < kChunk) {
+        const int a = list_entry(T, c * kChunk + int(threadIdx.x));
+        S.ctile[threadIdx.x] = (a < 0) ? -1 : (compact ? compact_tile(S, B.n_instances, a) : a);
+    }
+    __syncthreads();
+    return S.ctile[0] >= 0;
+}
+
 // What an instance does in iteration slot k.  `true_res`: the residual at
 // hand is b - A x with the FP32 matrix (always in FP32 mode; in BF16 mode
-// after the set-up and after a refinement slot).  A BF16 instance whose
-// recursive residual meets the tolerance spends the slot on the true
-// residual instead of stopping (outer iterative refinement).  Finishing is
-// final: once an instance stops, its tiles no longer refresh the
-// parity-indexed partials, so the test is never re-run on them (`finished`).
+// after the set-up and after a refinement slot); `refines`: refinement slots
+// spent so far.  A BF16 instance whose recursive residual meets the
+// tolerance spends the slot on the true residual instead of stopping (outer
+// iterative refinement), up to kMaxRefine times; after that the recursive
+// residual decides, as in FP32 mode.  (With x stored in FP32 the true
+// residual of a stiff system can sit above the tolerance for good; the
+// recursive one keeps falling.)  Finishing is final: once an instance
+// stops, its tiles no longer refresh the parity-indexed partials, so the
+// test is never re-run on them (`finished`).
+constexpr int kMaxRefine = 1;
 enum Slot { STOP, ITERATE, REFINE };
-__device__ __forceinline__ Slot instance_slot(const Inst& I, int k, double rr, double bb, bool true_res,
+__device__ __forceinline__ Slot instance_slot(const Inst& I, int k, double rr, double bb, bool true_res, int refines,
                                               bool finished) {
     if (finished || k >= I.max_iter) return STOP;
-    if (I.tol > 0.0 && rr <= I.tol * I.tol * bb) return true_res ? STOP : REFINE;
+    if (I.tol > 0.0 && rr <= I.tol * I.tol * bb) return (true_res || refines >= kMaxRefine) ? STOP : REFINE;
     return ITERATE;
 }
 
@@ -564,6 +682,13 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
             p[P_TRUE0] = 1.0;
         }
     }
+    // Compacted work lists for batches of up to kCompactMax instances: built
+    // at the start of phase B, when no done flag changes (instances stop in
+    // phase A only), and reused by the next phase A.  Phase A of iteration 0
+    // runs the static list (every instance is solving).
+    const bool compactable = B.inst != nullptr && B.n_instances <= kCompactMax;
+    if (compactable) load_tile_ranges(B, S);
+    bool compact = false;
     if (!grid_sync(sync, G)) return;
     trace_point(B, 0, unsigned(B.n_instances));
 
@@ -573,18 +698,19 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         // ---- phase A: p_k = z + beta p_{k-1}; q = A p_k; sigma = <p_k, q> ----
         // A refining instance rewrites its parity cur^1 partials during this
         // phase; its own tiles read them here but do not use them (no beta).
-        for (int t0 = blockIdx.x; t0 < B.n_tiles; t0 += G * kChunk) {
-            chunk_sums<BF16>(B, part, t0, G, P_RHO0 + cur, P_RHO0 + (cur ^ 1), P_RR0 + cur, P_BB, S, P_TRUE0 + cur);
+        for (int c = 0; next_chunk(B, S, compact, c); ++c) {
+            chunk_sums<BF16>(B, part, P_RHO0 + cur, P_RHO0 + (cur ^ 1), P_RR0 + cur, P_BB, S, P_TRUE0 + cur);
             for (int m = 0; m < kChunk; ++m) {
-                const int t = t0 + m * G;
-                if (t >= B.n_tiles) break;
+                const int t = S.ctile[m];
+                if (t < 0) break;
                 int inst, rb, re;
                 get_tile(B, t, inst, rb, re);
                 const Inst I = inst_at(B, inst);
                 const double rho = S.sc[m][0], rho_prev = S.sc[m][1], rr = S.sc[m][2], bb = S.sc[m][3];
                 const bool finished = S.sc[m][4] != 0.0;
-                const bool true_res = !BF16 || S.sc[m][5] != 0.0;
-                const Slot slot = instance_slot(I, k, rr, bb, true_res, finished);
+                const int rstate = BF16 ? int(S.sc[m][5] + 0.5) : 1;
+                const bool true_res = (rstate & 1) != 0;
+                const Slot slot = instance_slot(I, k, rr, bb, true_res, rstate >> 1, finished);
                 if (slot == STOP) {
                     if (threadIdx.x == 0 && t == I.tile_begin && !finished) {
                         __hip_atomic_store(&B.inst_done[inst], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -607,7 +733,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                         const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
                         p[P_RHO0 + (cur ^ 1)] = acc[0];
                         p[P_RR0 + (cur ^ 1)] = acc[1];
-                        p[P_TRUE0 + (cur ^ 1)] = 1.0;
+                        p[P_TRUE0 + (cur ^ 1)] = double(rstate + 3);  // one more refinement, true
                     }
                     continue;
                 }
@@ -643,17 +769,22 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
 
         // ---- phase B: alpha; x += alpha p; r -= alpha q; z = M^-1 r; rho', rr' --
         const int nxt = cur ^ 1;
-        for (int t0 = blockIdx.x; t0 < B.n_tiles; t0 += G * kChunk) {
-            chunk_sums<BF16>(B, part, t0, G, P_RHO0 + cur, P_SIG, P_RR0 + cur, P_BB, S, P_TRUE0 + cur);
+        if (compactable) {
+            build_active(B, S);
+            compact = true;
+        }
+        for (int c = 0; next_chunk(B, S, compact, c); ++c) {
+            chunk_sums<BF16>(B, part, P_RHO0 + cur, P_SIG, P_RR0 + cur, P_BB, S, P_TRUE0 + cur);
             for (int m = 0; m < kChunk; ++m) {
-                const int t = t0 + m * G;
-                if (t >= B.n_tiles) break;
+                const int t = S.ctile[m];
+                if (t < 0) break;
                 int inst, rb, re;
                 get_tile(B, t, inst, rb, re);
                 const Inst I = inst_at(B, inst);
                 const double rho = S.sc[m][0], sigma = S.sc[m][1], rr = S.sc[m][2], bb = S.sc[m][3];
-                const bool true_res = !BF16 || S.sc[m][5] != 0.0;
-                if (instance_slot(I, k, rr, bb, true_res, S.sc[m][4] != 0.0) != ITERATE) continue;
+                const int rstate = BF16 ? int(S.sc[m][5] + 0.5) : 1;
+                if (instance_slot(I, k, rr, bb, (rstate & 1) != 0, rstate >> 1, S.sc[m][4] != 0.0) != ITERATE)
+                    continue;
                 const float alpha = guarded_div(rho, sigma);
                 const gp<const float> pk = cur ? I.p1 : I.p0;
                 const unsigned n = unsigned(I.n_rows);
@@ -675,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                     const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
                     p[P_RHO0 + nxt] = acc[0];
                     p[P_RR0 + nxt] = acc[1];
-                    p[P_TRUE0 + nxt] = 0.0;
+                    p[P_TRUE0 + nxt] = double(rstate & ~1);  // recursive residual
                 }
             }
         }
