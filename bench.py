@@ -101,7 +101,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_serving(rate, duration, horizon, lanes, grid, tile_rows, workers=4, seed=4):
+def run_serving(rate, duration, horizon, lanes, grid, tile_rows, workers=4, seed=4, lds_staging=True):
     """Continuous-batching server on a Poisson stream of cloth-drape jobs
     (mesh sizes log-uniform in [1e3, 2.5e5] vertices, 30-300 substeps,
     materials as in the batched workload), arrivals for `duration` s, the
@@ -112,7 +112,7 @@ def run_serving(rate, duration, horizon, lanes, grid, tile_rows, workers=4, seed
     reqs = [serving.Request(job_id=i, arrival=j.arrival, spec=j.spec, substeps=j.substeps) for i, j in enumerate(jobs)]
     offered = sum(j.spec.n * j.substeps for j in jobs) / duration
     ex = serving.GpuExecutor(lanes_per_row=lanes, grid_blocks=grid or None, tile_rows=tile_rows,
-                             prepare_workers=workers)
+                             prepare_workers=workers, lds_staging=lds_staging)
     dt = scenes.ClothSpec.dt
     try:
         rep = serving.ServingEngine(ex, hbm_budget_bytes=200e9).run(reqs, dt, horizon_s=horizon, log=log)
@@ -141,6 +141,8 @@ def main():
                     help="skip the lanes-per-row x grid-size sweep and use --lanes / --grid")
     ap.add_argument("--grid", type=int, default=0, help="persistent PCG workgroups (0: all co-resident)")
     ap.add_argument("--tile-rows", type=int, default=256, help="rows per PCG tile (multiple of 256)")
+    ap.add_argument("--no-lds-staging", action="store_true",
+                    help="PCG SpMV gathers from global memory instead of LDS-staged stages (with --no-autotune)")
     ap.add_argument("--no-plain-large", action="store_true")
     ap.add_argument("--workload", choices=["batched", "serving"], default="batched",
                     help="batched (default; includes a short serving leg) or the full serving stream")
@@ -165,7 +167,7 @@ def main():
         # The serving workload: 200 req/s for 60 s, the server stopped at 60 s.
         rate = args.serving_rate or 200.0
         dur = args.serving_duration or 60.0
-        sv = run_serving(rate, dur, dur, args.lanes, args.grid, args.tile_rows)
+        sv = run_serving(rate, dur, dur, args.lanes, args.grid, args.tile_rows, lds_staging=not args.no_lds_staging)
         print(json.dumps({
             "metric": "served cloth vertex-substeps/s and p99 request latency (continuous batching)",
             "value": sv["throughput"], "unit": "vertex-substeps/s", "n_gpus": 1, "higher_is_better": True,
@@ -187,6 +189,7 @@ def main():
     if args.grid > 0:
         batch.set_grid_blocks(args.grid)
     batch.set_tile_rows(args.tile_rows)
+    batch.set_lds_staging(not args.no_lds_staging)
     for c in cloths:
         batch.add_instance(c.sim)
     batch.finalize(specs[0].dt)
@@ -206,8 +209,9 @@ def main():
         args.lanes = tuned["lanes_per_row"]
         args.grid = tuned["grid_blocks"]
         args.tile_rows = tuned["tile_rows"]
-        log(f"autotune picked lanes={tuned['lanes_per_row']} grid={tuned['grid_blocks']} "
-            f"tile_rows={tuned['tile_rows']}")
+        args.no_lds_staging = not tuned["lds_staging"]
+        log(f"autotune picked staging={int(tuned['lds_staging'])} lanes={tuned['lanes_per_row']} "
+            f"grid={tuned['grid_blocks']} tile_rows={tuned['tile_rows']}")
 
     with torch.cuda.stream(stream):
         for _ in range(args.warmup):
@@ -240,14 +244,16 @@ def main():
     mean_ms = float(np.mean(pcg_ms))
     achieved = float(np.mean(pcg_bytes)) / (mean_ms * 1e-3) / 1e9
     traffic = load_traffic()
-    key = f"lanes{args.lanes}_grid{args.grid}_tile{args.tile_rows}_inst{len(specs)}"
+    key = (f"{'global_' if args.no_lds_staging else ''}lanes{args.lanes}_grid{args.grid}_tile{args.tile_rows}"
+           f"_inst{len(specs)}")
     traffic_bytes = None
     if traffic and key in traffic.get("configs", {}):
         traffic_bytes = traffic["configs"][key]["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_bytes,
-                "kernel": "pcg_kernel<%d> (persistent block-Jacobi PCG + finalize)" % args.lanes,
+                "kernel": "pcg_kernel<%d, %s> (persistent block-Jacobi PCG + finalize)"
+                          % (args.lanes, "global SpMV" if args.no_lds_staging else "LDS-staged SpMV"),
                 "mean_launch_ms": round(mean_ms, 3), "algorithmic_bytes_per_launch": float(np.mean(pcg_bytes))}
 
     log(f"PCG launch {mean_ms:.3f} ms, {achieved:.0f} GB/s algorithmic")
@@ -314,7 +320,7 @@ def main():
         # serving workload's distribution: arrivals for 15 s, drained within
         # 60 s of wall time.
         serving_leg = run_serving(args.serving_rate or 4.0, args.serving_duration or 15.0, 60.0, args.lanes,
-                                  args.grid, args.tile_rows)
+                                  args.grid, args.tile_rows, lds_staging=not args.no_lds_staging)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(specs, args.cpu_budget)
@@ -336,6 +342,7 @@ def main():
                                "PCG tol 1e-6 cap 100, one fused substep per step",
                    "instances": len(specs), "vertices": n_rows, "off_diag_blocks": int(sizes[:, 1].sum()),
                    "pcg_tol": 1e-6, "pcg_cap": 100, "lanes_per_row": args.lanes, "grid_blocks": args.grid, "tile_rows": args.tile_rows,
+                   "lds_staging": not args.no_lds_staging,
                    "parallelism": "single GPU"},
         "pcg_iterations": {"mean": float(iters.mean()), "max": int(iters.max()), "min": int(iters.min())},
         "max_relative_residual": float(res.max()),

@@ -93,10 +93,21 @@ typedef struct chx_stvk {
 #define CHX_STAGE_ROWS 256
 #define CHX_ITEM_SIZE 256
 
+/* Rows per slice of the sliced layout of the off-diagonal blocks. */
+#define CHX_SLICE_ROWS 64
+
 /* One cloth instance: every array is the instance's own device memory, with
  * indices local to the instance, so the off-diagonal pattern is exactly the
  * reference's (ChysX/src/sparse/block_csr.h:59-135).  Slots of the gather
- * map: [0, n_rows) diagonal blocks, [n_rows, n_rows + nnz) off-diagonal. */
+ * map: [0, n_rows) diagonal blocks, [n_rows, n_rows + nnz) off-diagonal.
+ *
+ * The off-diagonal block values are stored sliced: rows 64s..64s+63 form
+ * slice s, which owns 64 * width_s slots [slice_offsets[s],
+ * slice_offsets[s+1]) (width_s = the longest of its rows); block j of row
+ * 64s+l sits in slot slice_offsets[s] + 64 j + l, so the 64 rows' j-th
+ * blocks are adjacent.  Slots past a row's length are padding (column =
+ * the row itself, values unused).  block_slots maps the reference's CSR
+ * index k (row_offsets / col_indices order) to its slot. */
 typedef struct chx_instance {
     float* pos;                      /* external AoS float3 positions (x_n in, x_{n+1} out) */
     float* vel;                      /* external AoS float3 velocities                      */
@@ -105,7 +116,7 @@ typedef struct chx_instance {
     const int* col_indices;          /* [nnz]                                                */
     float* diag;                     /* [9][n_rows] planar: component c (row-major r*3+col)  */
                                      /* of block row i at diag[c*n_rows + i]                 */
-    float* values;                   /* [9][nnz] planar, same convention                      */
+    float* values;                   /* [9][n_padded] planar over the sliced slots           */
     const chx_pin* pins;
     const chx_tri* tris;             /* stretch elements, then shear elements                */
     const chx_dih* dihs;
@@ -125,16 +136,21 @@ typedef struct chx_instance {
     float* diag_c;                   /* BF16 precision mode, else NULL: the inner SpMV's     */
     uint16_t* values16;              /* matrix, planar FP32 compensated diagonal             */
                                      /* D + sum_j (V_ij - bf16(V_ij)) [9][n_rows] and planar */
-                                     /* BF16 off-diagonal copies [9][nnz]                     */
+                                     /* BF16 off-diagonal copies [9][n_padded]               */
     const chx_stvk* stvks;           /* StVK membrane elements, or NULL                      */
     float* stvk_rec;                 /* [n_stvks*CHX_STVK_REC]                               */
     const int* windows;              /* [ceil(n_rows/CHX_STAGE_ROWS)][2]: smallest and       */
                                      /* largest column (own rows included) of each stage of  */
                                      /* CHX_STAGE_ROWS rows, or NULL (no LDS staging)        */
+    const int* slice_offsets;        /* [ceil(n_rows/CHX_SLICE_ROWS)+1] first slot of slice  */
+    const int* slice_cols;           /* [n_padded] column of every slot                      */
+    const int* block_slots;          /* [nnz] slot of CSR block k                            */
     int n_rows, nnz, n_pins, n_tris, n_dihs;
     int tile_begin;                  /* this instance's row tiles in the batch tile list     */
     int tile_end;
     int n_stvks;
+    int n_padded;                    /* slots of the sliced layout (>= nnz)                  */
+    int pad_slices;
     float dt;                        /* substep                                              */
     float damping;                   /* v *= exp(-damping*dt) when > 0                       */
     float gx, gy, gz;                /* gravity                                              */
@@ -163,7 +179,9 @@ typedef struct chx_cloth_batch {
                                      /* the tolerance                                 */
     int pad_bf16;
     int tile_rows;                   /* rows per tile of the tile list (informational) */
-    int pad0;
+    int no_lds_staging;              /* 1: the PCG SpMV gathers from global memory;   */
+                                     /* 0: stages of CHX_STAGE_ROWS rows go through   */
+                                     /* LDS (instances with windows)                  */
     chx_instance single;
     const chx_instance* inst;        /* device instance table, or NULL                */
     const int* tiles;
@@ -191,6 +209,9 @@ int chx_device_count(int* count);
 /* Compute units, the persistent PCG grid cap, and the runtime's occupancy
  * figure (workgroups per CU) for the PCG kernel on the current device. */
 int chx_device_info(int* cu_count, int* max_pcg_blocks, int* occupancy_blocks_per_cu);
+/* Co-resident workgroups of one form of the persistent PCG kernel on the
+ * current device (the grid cap of a launch with those settings). */
+int chx_pcg_grid_limit(int lanes_per_row, int bf16_blocks, int no_lds_staging, int* max_blocks);
 
 /* Device memory: replaces CudaArray<T> (ChysX/src/memory/cuda_array.h:64-284,
  * cuda_array.cu:41-90). */

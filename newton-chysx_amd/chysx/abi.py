@@ -25,6 +25,7 @@ _SIGS = {
     "chx_last_error_message": (ctypes.c_char_p, []),
     "chx_device_count": (_I, [_P]),
     "chx_device_info": (_I, [_P, _P, _P]),
+    "chx_pcg_grid_limit": (_I, [_I, _I, _I, _P]),
     "chx_malloc": (_I, [_P, _S]),
     "chx_free": (_I, [_P]),
     "chx_memcpy_h2d": (_I, [_P, _P, _S, _U]),
@@ -114,3 +115,11 @@ def device_info():
     cu, mb, occ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     check(lib().chx_device_info(ctypes.byref(cu), ctypes.byref(mb), ctypes.byref(occ)), "device_info")
     return cu.value, mb.value, occ.value
+
+
+def pcg_grid_limit(lanes_per_row: int, bf16: bool = False, lds_staging: bool = True) -> int:
+    """Co-resident workgroups of that form of the persistent PCG kernel."""
+    mb = ctypes.c_int()
+    check(lib().chx_pcg_grid_limit(int(lanes_per_row), int(bool(bf16)), int(not lds_staging), ctypes.byref(mb)),
+          "pcg_grid_limit")
+    return mb.value

@@ -1,8 +1,10 @@
 """Autotuner of the batched cloth substep.
 
-Sweeps the SpMV lanes per block row (4, 8, 16 — i.e. 16, 8 or 4 block rows
-per wavefront), the persistent PCG grid size (workgroups per compute unit)
-and the rows per PCG tile (256, 512, 1024) on the actual batch, timing whole substeps with HIP events on the
+Sweeps the SpMV form (LDS-staged stages or global gathers), the SpMV lanes
+per block row (4, 8, 16 — i.e. 16, 8 or 4 block rows per wavefront), the
+persistent PCG grid size (workgroups per compute unit, up to what the form
+keeps co-resident) and the rows per PCG tile (256, 512, 1024) on the actual
+batch, timing whole substeps with HIP events on the
 caller's stream, and keeps the fastest.  Results are cached in a JSON file
 keyed by device name and a workload signature (instance count, vertices,
 off-diagonal blocks) so a later run of the same workload skips the sweep.
@@ -32,11 +34,14 @@ def _device_name() -> str:
     return torch.cuda.get_device_name(0)
 
 
-def tune_batch(batch, dt: float, stream=None, lanes=(4, 8, 16), blocks_per_cu=(1, 2, 3), tile_rows=(256, 512, 1024),
-               warmup: int = 1, steps: int = 3, cache_path: str | None = DEFAULT_CACHE, log=None) -> dict:
-    """Returns {"lanes_per_row", "grid_blocks", "tile_rows", "ms_per_substep",
-    "table"} and applies the best setting to `batch`.  Lanes and grid size
-    are swept at the first tile size, then the tile size at the best pair."""
+def tune_batch(batch, dt: float, stream=None, lanes=(4, 8, 16), blocks_per_cu=(1, 2, 3, 4),
+               tile_rows=(256, 512, 1024), staging=(True, False), warmup: int = 1, steps: int = 3,
+               cache_path: str | None = DEFAULT_CACHE, log=None) -> dict:
+    """Returns {"lds_staging", "lanes_per_row", "grid_blocks", "tile_rows",
+    "ms_per_substep", "table"} and applies the best setting to `batch`.
+    SpMV form, lanes and grid size (up to each form's co-residency limit)
+    are swept at the first tile size, then the tile size at the best
+    setting."""
     import torch
 
     from . import abi
@@ -51,19 +56,25 @@ def tune_batch(batch, dt: float, stream=None, lanes=(4, 8, 16), blocks_per_cu=(1
         except (OSError, ValueError):
             cached = None
         if cached:
+            batch.set_lds_staging(bool(cached.get("lds_staging", True)))
             batch.set_lanes_per_row(int(cached["lanes_per_row"]))
             batch.set_grid_blocks(int(cached["grid_blocks"]))
             batch.set_tile_rows(int(cached.get("tile_rows", 256)))
             return dict(cached, cached=True)
-    cus, cap, _ = abi.device_info()
+    cus, _, _ = abi.device_info()
     s = stream if stream is not None else torch.cuda.current_stream()
     sh = s.cuda_stream
     table = []
 
-    def trial(L, g, T):
-        batch.set_lanes_per_row(L)
-        batch.set_grid_blocks(g)
-        batch.set_tile_rows(T)
+    def apply(r):
+        batch.set_lds_staging(r["lds_staging"])
+        batch.set_lanes_per_row(r["lanes_per_row"])
+        batch.set_grid_blocks(r["grid_blocks"])
+        batch.set_tile_rows(r["tile_rows"])
+
+    def trial(st, L, g, T):
+        r = {"lds_staging": st, "lanes_per_row": L, "grid_blocks": g, "tile_rows": T}
+        apply(r)
         with torch.cuda.stream(s):
             for _ in range(warmup):
                 batch.step(dt, sh)
@@ -73,24 +84,25 @@ def tune_batch(batch, dt: float, stream=None, lanes=(4, 8, 16), blocks_per_cu=(1
                 batch.step(dt, sh)
             e1.record(s)
             e1.synchronize()
-        ms = e0.elapsed_time(e1) / steps
-        table.append({"lanes_per_row": L, "grid_blocks": g, "tile_rows": T, "ms_per_substep": ms})
+        r["ms_per_substep"] = e0.elapsed_time(e1) / steps
+        table.append(r)
         if log:
-            log(f"autotune lanes={L} grid={g} tile_rows={T}: {ms:.3f} ms/substep")
+            log(f"autotune staging={int(st)} lanes={L} grid={g} tile_rows={T}: {r['ms_per_substep']:.3f} ms/substep")
 
-    for L in lanes:
-        for b in blocks_per_cu:
-            g = min(cap, b * cus)
-            if any(r["lanes_per_row"] == L and r["grid_blocks"] == g for r in table):
-                continue
-            trial(L, g, tile_rows[0])
+    bf16 = False
+    for st in staging:
+        for L in lanes:
+            limit = abi.pcg_grid_limit(L, bf16, st)
+            for b in blocks_per_cu:
+                g = min(limit, b * cus)
+                if any(r["lds_staging"] == st and r["lanes_per_row"] == L and r["grid_blocks"] == g for r in table):
+                    continue
+                trial(st, L, g, tile_rows[0])
     best = min(table, key=lambda r: r["ms_per_substep"])
     for T in tile_rows[1:]:
-        trial(best["lanes_per_row"], best["grid_blocks"], T)
+        trial(best["lds_staging"], best["lanes_per_row"], best["grid_blocks"], T)
     best = min(table, key=lambda r: r["ms_per_substep"])
-    batch.set_lanes_per_row(best["lanes_per_row"])
-    batch.set_grid_blocks(best["grid_blocks"])
-    batch.set_tile_rows(best["tile_rows"])
+    apply(best)
     result = dict(best, table=table, device=dev, signature=sig, tuned_at=time.time())
     if cache_path:
         try:
@@ -99,7 +111,8 @@ def tune_batch(batch, dt: float, stream=None, lanes=(4, 8, 16), blocks_per_cu=(1
             if os.path.exists(cache_path):
                 with open(cache_path) as f:
                     data = json.load(f)
-            data[key] = {k: result[k] for k in ("lanes_per_row", "grid_blocks", "tile_rows", "ms_per_substep")}
+            data[key] = {k: result[k]
+                         for k in ("lds_staging", "lanes_per_row", "grid_blocks", "tile_rows", "ms_per_substep")}
             with open(cache_path, "w") as f:
                 json.dump(data, f, indent=1)
         except (OSError, ValueError):

@@ -84,7 +84,7 @@ class GpuExecutor:
     runs on `prepare_workers` host threads and its own stream."""
 
     def __init__(self, lanes_per_row: int = 8, stream=None, keep_results: bool = False, prepare_workers: int = 2,
-                 tile_rows: int | None = None, grid_blocks: int | None = None):
+                 tile_rows: int | None = None, grid_blocks: int | None = None, lds_staging: bool = True):
         import concurrent.futures
 
         import torch
@@ -99,6 +99,7 @@ class GpuExecutor:
             self.batch.set_tile_rows(tile_rows)
         if grid_blocks:
             self.batch.set_grid_blocks(grid_blocks)
+        self.batch.set_lds_staging(lds_staging)
         self.stream = stream if stream is not None else torch.cuda.Stream()
         self.prep_stream = torch.cuda.Stream()
         self.keep_results = keep_results

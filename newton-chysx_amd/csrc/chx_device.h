@@ -208,9 +208,10 @@ struct Inst {
     gp<float> diag_c;
     gp<uint16_t> values16;
     gp<const int> windows;
+    gp<const int> slice_offsets, slice_cols, block_slots;
     gp<const chx_stvk> stvks;
     gp<float> stvk_rec;
-    int n_stvks;
+    int n_stvks, n_padded;
     int n_rows, nnz, n_pins, n_tris, n_dihs, tile_begin, tile_end;
     float dt, damping, gx, gy, gz;
     int max_iter;
@@ -246,6 +247,10 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.diag_c = glb(d->diag_c);
     I.values16 = glb(d->values16);
     I.windows = glb(d->windows);
+    I.slice_offsets = glb(d->slice_offsets);
+    I.slice_cols = glb(d->slice_cols);
+    I.block_slots = glb(d->block_slots);
+    I.n_padded = d->n_padded;
     I.stvks = glb(d->stvks);
     I.stvk_rec = glb(d->stvk_rec);
     I.n_stvks = d->n_stvks;

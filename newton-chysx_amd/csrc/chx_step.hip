@@ -195,12 +195,13 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
 #pragma unroll
         for (unsigned k = 0; k < 9; ++k) stx(I.diag, k * n + row, h[k]);
     } else {
-        const unsigned e = s - n, nnz = unsigned(I.nnz);
+        // Sliced layout: CSR block e lives in slot block_slots[e].
+        const unsigned slot = unsigned(ldx(I.block_slots, s - n)), np = unsigned(I.n_padded);
 #pragma unroll
-        for (unsigned k = 0; k < 9; ++k) stx(I.values, k * nnz + e, h[k]);
+        for (unsigned k = 0; k < 9; ++k) stx(I.values, k * np + slot, h[k]);
         if (I.values16 != nullptr) {
 #pragma unroll
-            for (unsigned k = 0; k < 9; ++k) stx(I.values16, k * nnz + e, f32_to_bf16(h[k]));
+            for (unsigned k = 0; k < 9; ++k) stx(I.values16, k * np + slot, f32_to_bf16(h[k]));
         }
     }
 }
@@ -221,15 +222,16 @@ __global__ __launch_bounds__(kBlock) void compensate_kernel(chx_cloth_batch B) {
     const Inst I = inst_at(B, inst);
     const unsigned n = unsigned(I.n_rows), row = unsigned(ss);
     if (row >= n) return;
-    const unsigned nnz = unsigned(I.nnz);
+    const unsigned np = unsigned(I.n_padded);
     const unsigned k0 = ldx(I.row_offsets, row), k1 = ldx(I.row_offsets, row + 1u);
     float d[9];
 #pragma unroll
     for (unsigned c = 0; c < 9; ++c) d[c] = ldx(gp<const float>(I.diag), c * n + row);
     for (unsigned k = k0; k < k1; ++k) {
+        const unsigned slot = unsigned(ldx(I.block_slots, k));
 #pragma unroll
         for (unsigned c = 0; c < 9; ++c) {
-            const float v = ldx(gp<const float>(I.values), c * nnz + k);
+            const float v = ldx(gp<const float>(I.values), c * np + slot);
             d[c] += v - bf16_to_f32(f32_to_bf16(v));
         }
     }
@@ -242,10 +244,12 @@ __global__ void reset_sync_kernel(unsigned* sync) {
 }
 
 // ---------------------------------------------------------------------------
-// SpMV row loop: L lanes per block row, 64/L rows per wave and pass.
-// ro_at(r) / ci_at(k) give row offsets and column indices, src(j) the source
-// vector entry j; fin(row, Ax, x_row) runs on the first lane of each row
-// after the segmented reduction (x_row = src(row)).
+// SpMV row loop: L lanes per block row, 64/L rows per wave and row group; R
+// row groups per trip (R * 64 / L rows per wave), whose loads are all in
+// flight together.  ro_at(r) / ci_at(k) give row offsets and column
+// indices, src(j) the source vector entry j; fin(row, Ax, x_row) runs on the
+// first lane of each row after the segmented reduction (x_row = src(row)),
+// rows in ascending order per lane.
 //
 // Each lane takes U = 16/L of its row's off-diagonal blocks per pass (a pass
 // covers 16 blocks of a row), loading them at clamped indices without
@@ -255,7 +259,7 @@ __global__ void reset_sync_kernel(unsigned* sync) {
 // Per lane the sum is diag + off[sub] + off[sub+L] + ..., then a segmented
 // butterfly over the L lanes.
 // ---------------------------------------------------------------------------
-template <int L, class Mat, class RoAt, class CiAt, class Src, class Fin>
+template <int L, int R = 1, class Mat, class RoAt, class CiAt, class Src, class Fin>
 __device__ __forceinline__ void spmv_core(Mat mat, int rb, int re, RoAt ro_at, CiAt ci_at, Src src, Fin fin) {
     constexpr int RPW = kWave / L;
     constexpr int U = (16 / L) > 1 ? (16 / L) : 1;
@@ -263,31 +267,109 @@ __device__ __forceinline__ void spmv_core(Mat mat, int rb, int re, RoAt ro_at, C
     const int wid = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
     const int g = lane / L, sub_ = lane % L;
-    for (int base = rb + wid * RPW; base < re; base += nw * RPW) {
-        const int row = base + g;
-        const bool valid = row < re;
-        const unsigned rowc = unsigned(valid ? row : rb);
-        const unsigned beg = ro_at(rowc);
-        const unsigned end = valid ? ro_at(rowc + 1u) : beg;
-        const f3 xr = src(rowc);
-        const f3 dprod = mat.diag(rowc, xr);
-        f3 acc = (valid && sub_ == 0) ? dprod : mk(0.f, 0.f, 0.f);
-        for (unsigned k0 = beg + unsigned(sub_); k0 < end; k0 += unsigned(U * L)) {
-            f3 part[U];
+    const int step = nw * RPW;  // rows between a wave's row groups
+    for (int base = rb + wid * RPW; base < re; base += step * R) {
+        unsigned rowc[R], beg[R], end[R];
+        bool valid[R];
+        f3 xr[R], acc[R];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const unsigned k = k0 + unsigned(u * L);
-                const unsigned kc = (k < end) ? k : k0;
-                part[u] = mat.off(kc, src(ci_at(kc)));
+        for (int q = 0; q < R; ++q) {
+            const int row = base + q * step + g;
+            valid[q] = row < re;
+            rowc[q] = unsigned(valid[q] ? row : rb);
+            beg[q] = ro_at(rowc[q]);
+            end[q] = valid[q] ? ro_at(rowc[q] + 1u) : beg[q];
+            xr[q] = src(rowc[q]);
+            const f3 dprod = mat.diag(rowc[q], xr[q]);
+            acc[q] = (valid[q] && sub_ == 0) ? dprod : mk(0.f, 0.f, 0.f);
+        }
+        for (unsigned p = 0;; p += unsigned(U * L)) {
+            unsigned k0[R];
+            bool live[R];
+            bool any = false;
+            unsigned kv = 0;  // an index inside some live row, for clamping
+#pragma unroll
+            for (int q = R - 1; q >= 0; --q) {
+                k0[q] = beg[q] + unsigned(sub_) + p;
+                live[q] = k0[q] < end[q];
+                if (live[q]) {
+                    any = true;
+                    kv = k0[q];
+                }
+            }
+            if (!any) break;
+            f3 part[R][U];
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const unsigned k = k0[q] + unsigned(u * L);
+                    const unsigned kc = (k < end[q]) ? k : kv;
+                    part[q][u] = mat.off(kc, src(ci_at(kc)));
+                }
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (k0[q] + unsigned(u * L) < end[q]) acc[q] = add(acc[q], part[q][u]);
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            acc[q].x = seg_sum<L>(acc[q].x);
+            acc[q].y = seg_sum<L>(acc[q].y);
+            acc[q].z = seg_sum<L>(acc[q].z);
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (valid[q] && sub_ == 0) fin(rowc[q], acc[q], xr[q]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sliced SpMV: one lane per block row, one wave per slice of 64 rows (rb a
+// multiple of 64).  slot_at(s) gives slice s's first slot, ci_at(slot) the
+// column of a slot, len_at(row) the row's block count, src(j) the source
+// entry j; fin(row, Ax, x_row) on every valid row's lane.  A wave walks the
+// slice's width in batches of KS slots whose loads are all in flight
+// together (slots past the width clamp to the batch's first, padding slots
+// load and are masked); each j-th slot load of the wave reads 64 adjacent
+// words.  Per row the sum is diag + off[0] + off[1] + ... in slot order.
+// ---------------------------------------------------------------------------
+constexpr int kSliceBatch = 8;
+constexpr unsigned kSlice = CHX_SLICE_ROWS;
+static_assert(CHX_SLICE_ROWS == kWave, "one lane per row of a slice");
+static_assert(CHX_STAGE_ROWS % CHX_SLICE_ROWS == 0, "stages hold whole slices");
+
+template <class Mat, class SlotAt, class CiAt, class LenAt, class Src, class Fin>
+__device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot_at, CiAt ci_at, LenAt len_at,
+                                            Src src, Fin fin) {
+    constexpr int KS = kSliceBatch;
+    const unsigned lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    for (int s0 = rb + wid * kWave; s0 < re; s0 += nw * kWave) {
+        const unsigned sl = unsigned(s0) / kSlice;
+        const unsigned base = slot_at(sl);
+        const unsigned width = (slot_at(sl + 1u) - base) / kSlice;
+        const unsigned row = unsigned(s0) + lane;
+        const bool valid = row < unsigned(re);
+        const unsigned rowc = valid ? row : unsigned(s0);
+        const unsigned len = valid ? len_at(rowc) : 0u;
+        const f3 xr = src(rowc);
+        f3 acc = mat.diag(rowc, xr);
+        for (unsigned j0 = 0; j0 < width; j0 += KS) {
+            f3 part[KS];
+#pragma unroll
+            for (int u = 0; u < KS; ++u) {
+                const unsigned j = j0 + unsigned(u);
+                const unsigned slot = base + (j < width ? j : j0) * kSlice + lane;
+                part[u] = mat.off(slot, src(ci_at(slot)));
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (k0 + unsigned(u * L) < end) acc = add(acc, part[u]);
+            for (int u = 0; u < KS; ++u)
+                if (j0 + unsigned(u) < len) acc = add(acc, part[u]);
         }
-        acc.x = seg_sum<L>(acc.x);
-        acc.y = seg_sum<L>(acc.y);
-        acc.z = seg_sum<L>(acc.z);
-        if (valid && sub_ == 0) fin(rowc, acc, xr);
+        if (valid) fin(rowc, acc, xr);
     }
 }
 
@@ -308,10 +390,8 @@ constexpr int kChunk = 16;          // tiles whose instance scalars are summed t
 static_assert(kChunk * 16 == kBlock, "16 lanes per tile in chunk_sums");
 constexpr int kCompactMax = 4 * kBlock;  // instances up to which the active tiles are compacted
 
-struct PcgLds {
-    int ro[kStage + 1];
-    int ci[kCiCap];
-    float win[3 * kWinCap];
+// Shared by both SpMV forms.
+struct PcgShared {
     double red[16];
     double sc[kChunk][6];
     // Active tile list (batches of up to kCompactMax instances): instance i
@@ -324,17 +404,39 @@ struct PcgLds {
     int ctile[kChunk];  // the tiles of the chunk at hand, -1 past the end
 };
 
+// Stage arrays of the LDS-staged SpMV.
+struct PcgStageLds {
+    int ro[kStage + 1];
+    int ci[kCiCap];
+    float win[3 * kWinCap];
+};
+
+// ST: LDS-staged SpMV (stage arrays, 64 KB per workgroup) or the global
+// form (gathers through L1/L2, ~13 KB, more workgroups per CU).
+template <bool ST>
+struct PcgLds : PcgShared {
+    PcgStageLds st;
+};
+template <>
+struct PcgLds<false> : PcgShared {};
+
 __device__ __forceinline__ f3 lds3(const float* w, unsigned i) { return {w[3 * i], w[3 * i + 1], w[3 * i + 2]}; }
 
 // Loads stage [sb, se) of instance I into S (true), or leaves it to the
 // global path (false: no window, or over capacity).  srcg(j) is the source
 // vector entry j.  Uniform across the workgroup.
-template <class SrcG>
-__device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG srcg, PcgLds& S, unsigned& beg0,
-                                           unsigned& lo) {
+// SL: the column indices come from the sliced layout (slots of the stage's
+// slices) instead of the CSR order.
+template <bool SL, class SrcG>
+__device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG srcg, PcgStageLds& S,
+                                           unsigned& beg0, unsigned& lo) {
     if (I.windows == nullptr) return false;
-    const unsigned b0 = unsigned(ldx(I.row_offsets, unsigned(sb)));
-    const unsigned nnz_s = unsigned(ldx(I.row_offsets, unsigned(se))) - b0;
+    const unsigned rb0 = unsigned(ldx(I.row_offsets, unsigned(sb)));
+    const unsigned b0 = SL ? unsigned(ldx(I.slice_offsets, unsigned(sb) / kSlice)) : rb0;
+    const unsigned nnz_s =
+        (SL ? unsigned(ldx(I.slice_offsets, (unsigned(se) + kSlice - 1u) / kSlice))
+            : unsigned(ldx(I.row_offsets, unsigned(se)))) - b0;
+    const gp<const int> cols = SL ? I.slice_cols : I.col_indices;
     const unsigned st = unsigned(sb) / unsigned(kStage);
     const unsigned wlo = unsigned(ldx(I.windows, 2u * st)), whi = unsigned(ldx(I.windows, 2u * st + 1u));
     const unsigned W = whi - wlo + 1u;
@@ -350,7 +452,7 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const unsigned k = it * 2048u + unsigned(u) * 256u + tid;
-                cv[u] = ldx(I.col_indices, b0 + (k < nnz_s ? k : 0u));
+                cv[u] = ldx(cols, b0 + (k < nnz_s ? k : 0u));
             }
         }
 #pragma unroll
@@ -358,7 +460,7 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
             const unsigned w = it * 1024u + unsigned(u) * 256u + tid;
             wv[u] = srcg(wlo + (w < W ? w : 0u));
         }
-        if (it == 0 && tid < R) S.ro[tid] = ldx(I.row_offsets, unsigned(sb) + tid) - int(b0);
+        if (it == 0 && tid < R) S.ro[tid] = ldx(I.row_offsets, unsigned(sb) + tid) - int(rb0);
         if (nnz_s > 0) {
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -376,28 +478,56 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
             }
         }
     }
-    if (tid == 0) S.ro[R] = int(nnz_s);
+    if (tid == 0) S.ro[R] = ldx(I.row_offsets, unsigned(se)) - int(rb0);
     __syncthreads();
     beg0 = b0;
     lo = wlo;
     return true;
 }
 
-// SpMV over rows [rb, re) of instance I (a tile), one stage at a time.
-template <int L, class Mat, class SrcG, class Fin>
-__device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re, PcgLds& S, SrcG srcg, Fin fin) {
-    for (int sb = rb; sb < re; sb += kStage) {
-        const int se = min(re, sb + kStage);
-        unsigned beg0 = 0, lo = 0;
-        if (stage_load(I, sb, se, srcg, S, beg0, lo)) {
-            spmv_core<L>(
-                mat, sb, se, [&](unsigned r) { return unsigned(S.ro[r - unsigned(sb)]) + beg0; },
-                [&](unsigned k) { return unsigned(S.ci[k - beg0]); }, [&](unsigned j) { return lds3(S.win, j - lo); },
+// SpMV over rows [rb, re) of instance I (a tile): staged, one stage at a
+// time, or straight from global memory; SL: sliced layout (cloth
+// instances), else CSR with L lanes per row (caller-provided row-major
+// blocks).
+template <int L, bool ST, bool SL, class Mat, class SrcG, class Fin>
+__device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re, PcgLds<ST>& S, SrcG srcg,
+                                          Fin fin) {
+    auto global_rows = [&](int b, int e) {
+        if constexpr (SL) {
+            spmv_sliced(
+                mat, b, e, [&](unsigned sl) { return unsigned(ldx(I.slice_offsets, sl)); },
+                [&](unsigned k) { return unsigned(ldx(I.slice_cols, k)); },
+                [&](unsigned r) { return unsigned(ldx(I.row_offsets, r + 1u) - ldx(I.row_offsets, r)); }, srcg,
                 fin);
         } else {
             spmv_core<L>(
-                mat, sb, se, [&](unsigned r) { return unsigned(ldx(I.row_offsets, r)); },
+                mat, b, e, [&](unsigned r) { return unsigned(ldx(I.row_offsets, r)); },
                 [&](unsigned k) { return unsigned(ldx(I.col_indices, k)); }, srcg, fin);
+        }
+    };
+    if constexpr (!ST) {
+        global_rows(rb, re);
+    } else {
+        PcgStageLds& T = S.st;
+        for (int sb = rb; sb < re; sb += kStage) {
+            const int se = min(re, sb + kStage);
+            unsigned beg0 = 0, lo = 0;
+            if (stage_load<SL>(I, sb, se, srcg, T, beg0, lo)) {
+                if constexpr (SL) {
+                    spmv_sliced(
+                        mat, sb, se, [&](unsigned sl) { return unsigned(ldx(I.slice_offsets, sl)); },
+                        [&](unsigned k) { return unsigned(T.ci[k - beg0]); },
+                        [&](unsigned r) { return unsigned(T.ro[r - unsigned(sb) + 1u] - T.ro[r - unsigned(sb)]); },
+                        [&](unsigned j) { return lds3(T.win, j - lo); }, fin);
+                } else {
+                    spmv_core<L>(
+                        mat, sb, se, [&](unsigned r) { return unsigned(T.ro[r - unsigned(sb)]) + beg0; },
+                        [&](unsigned k) { return unsigned(T.ci[k - beg0]); },
+                        [&](unsigned j) { return lds3(T.win, j - lo); }, fin);
+                }
+            } else {
+                global_rows(sb, se);
+            }
         }
     }
 }
@@ -411,7 +541,7 @@ __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re
 // same bits in every workgroup.
 template <bool F4 = false>
 __device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const double> part, int f0, int f1, int f2,
-                                           int f3_, PcgLds& S, int f4 = 0) {
+                                           int f3_, PcgShared& S, int f4 = 0) {
     __syncthreads();
     const int m = int(threadIdx.x) >> 4, l = int(threadIdx.x) & 15;
     const int t = S.ctile[m];
@@ -470,7 +600,7 @@ __device__ __forceinline__ int list_entry(int T, int i) {
 
 // Tile of list entry a of the compacted list: the instance j with
 // pre[j] <= a < pre[j] + (active tiles of j), found by binary search.
-__device__ __forceinline__ int compact_tile(const PcgLds& S, int n, int a) {
+__device__ __forceinline__ int compact_tile(const PcgShared& S, int n, int a) {
     int lo = 0, hi = n;  // pre[lo] <= a, answer in [lo, hi)
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -481,7 +611,7 @@ __device__ __forceinline__ int compact_tile(const PcgLds& S, int n, int a) {
 }
 
 // Static per-launch table of the instances' tile ranges (compacted mode).
-__device__ __forceinline__ void load_tile_ranges(const chx_cloth_batch& B, PcgLds& S) {
+__device__ __forceinline__ void load_tile_ranges(const chx_cloth_batch& B, PcgShared& S) {
     const gp<const int> tab = glb(reinterpret_cast<const int*>(B.inst));
     constexpr unsigned words = unsigned(sizeof(chx_instance) / 4);
     constexpr unsigned ob = unsigned(offsetof(chx_instance, tile_begin) / 4);
@@ -497,7 +627,7 @@ __device__ __forceinline__ void load_tile_ranges(const chx_cloth_batch& B, PcgLd
 // Builds pre[] / nact from the done flags: instance i counts tc[i] tiles
 // while it is solving, 0 once finished.  The flags must not change while
 // any workgroup builds (every workgroup has to derive the same list).
-__device__ __forceinline__ void build_active(const chx_cloth_batch& B, PcgLds& S) {
+__device__ __forceinline__ void build_active(const chx_cloth_batch& B, PcgShared& S) {
     const int n = B.n_instances;
     const int base = int(threadIdx.x) * 4;
     int c[4];
@@ -536,7 +666,7 @@ __device__ __forceinline__ void build_active(const chx_cloth_batch& B, PcgLds& S
 
 // Fills S.ctile with the tiles of this workgroup's chunk c of the phase
 // list; returns false when the workgroup has no tile left (uniform).
-__device__ __forceinline__ bool next_chunk(const chx_cloth_batch& B, PcgLds& S, bool compact, int c) {
+__device__ __forceinline__ bool next_chunk(const chx_cloth_batch& B, PcgShared& S, bool compact, int c) {
     __syncthreads();  // the previous chunk's readers are done
     const int T = compact ? S.nact : B.n_tiles;
     if (threadIdx.x < kChunk) {
@@ -580,12 +710,12 @@ __device__ __forceinline__ Slot instance_slot(const Inst& I, int k, double rr, d
 // the FP32 matrix), M^-1 = inverse(D) (setup only), z = M^-1 r, and p = z
 // when pdst is given (setup); tile partials <r,z>, <r,r> (and <b,b> at
 // setup).  M^-1 is stored planar.
-template <int L, class MatR>
+template <int L, bool ST, bool SL, class MatR>
 __device__ __forceinline__ void residual_tile(const Inst& I, MatR mat, int rb, int re, bool setup, gp<float> pdst,
-                                              PcgLds& S, double (&acc)[3]) {
+                                              PcgLds<ST>& S, double (&acc)[3]) {
     const gp<const float> xv = I.x;
     const unsigned n = unsigned(I.n_rows);
-    tile_spmv<L>(I, mat, rb, re, S, [&](unsigned j) { return ld3g(xv, j); }, [&](unsigned row, f3 Ax, f3) {
+    tile_spmv<L, ST, SL>(I, mat, rb, re, S, [&](unsigned j) { return ld3g(xv, j); }, [&](unsigned row, f3 Ax, f3) {
         const f3 b = ld3g(I.rhs, row);
         const f3 r = sub(b, Ax);
         float mi[9];
@@ -624,17 +754,17 @@ struct PcgMats<0> {
 template <>
 struct PcgMats<1> {
     static __device__ __forceinline__ MatP32 a(const Inst& I) {
-        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.nnz)};
+        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.n_padded)};
     }
     static __device__ __forceinline__ MatP32 r(const Inst& I) { return a(I); }
 };
 template <>
 struct PcgMats<2> {
     static __device__ __forceinline__ MatP16 a(const Inst& I) {
-        return MatP16{I.diag_c, I.values16, unsigned(I.n_rows), unsigned(I.nnz)};
+        return MatP16{I.diag_c, I.values16, unsigned(I.n_rows), unsigned(I.n_padded)};
     }
     static __device__ __forceinline__ MatP32 r(const Inst& I) {
-        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.nnz)};
+        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.n_padded)};
     }
 };
 
@@ -648,10 +778,11 @@ __device__ __forceinline__ void trace_point(const chx_cloth_batch& B, int e, uns
     }
 }
 
-template <int L, int V>
+template <int L, int V, bool ST>
 __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int finalize) {
     constexpr bool BF16 = V == 2;
-    __shared__ PcgLds S;
+    constexpr bool SL = V != 0;  // cloth instances: sliced blocks
+    __shared__ PcgLds<ST> S;
     const int G = int(gridDim.x);
     unsigned* sync = B.sync;
     const gp<double> part = glb(B.partials);
@@ -672,7 +803,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         if (threadIdx.x == 0 && t == I.tile_begin)
             __hip_atomic_store(&B.inst_done[inst], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         double acc[3] = {0.0, 0.0, 0.0};
-        residual_tile<L>(I, PcgMats<V>::r(I), rb, re, true, I.p0, S, acc);
+        residual_tile<L, ST, SL>(I, PcgMats<V>::r(I), rb, re, true, I.p0, S, acc);
         block_sum(acc, S.red);
         if (threadIdx.x == 0) {
             const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
@@ -727,7 +858,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                     // True residual r = b - A32 x, z = M^-1 r and their sums
                     // become the parity cur^1 partials, marked true.
                     double acc[3] = {0.0, 0.0, 0.0};
-                    residual_tile<L>(I, PcgMats<V>::r(I), rb, re, false, gp<float>(nullptr), S, acc);
+                    residual_tile<L, ST, SL>(I, PcgMats<V>::r(I), rb, re, false, gp<float>(nullptr), S, acc);
                     block_sum(acc, S.red);
                     if (threadIdx.x == 0) {
                         const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
@@ -746,17 +877,17 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                     acc[0] += dot64(pr, Ap);
                 };
                 if (first) {
-                    tile_spmv<L>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(pk, jj); }, fin);
+                    tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(pk, jj); }, fin);
                 } else if (BF16 && true_res) {
                     // restart after a refinement slot: p_k = z_k
-                    tile_spmv<L>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(z, jj); }, fin);
+                    tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(z, jj); }, fin);
                 } else {
                     const float beta = guarded_div(rho, rho_prev);
                     auto pj = [&](unsigned jj) {
                         const f3 zj = ld3g(z, jj), pp = ld3g(pprev, jj);
                         return mk(fmaf(beta, pp.x, zj.x), fmaf(beta, pp.y, zj.y), fmaf(beta, pp.z, zj.z));
                     };
-                    tile_spmv<L>(I, PcgMats<V>::a(I), rb, re, S, pj, fin);
+                    tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, pj, fin);
                 }
                 block_sum(acc, S.red);
                 if (threadIdx.x == 0) rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS))[P_SIG] = acc[0];
@@ -862,30 +993,33 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const float* __rest
 // ---------------------------------------------------------------------------
 using PcgFn = void (*)(chx_cloth_batch, int);
 
-template <int L>
-PcgFn pcg_fn_l(int v) {
-    switch (v) {
-        case 0: return pcg_kernel<L, 0>;
-        case 1: return pcg_kernel<L, 1>;
-        default: return pcg_kernel<L, 2>;
+template <bool ST>
+PcgFn pcg_fn_s(int L, int v) {
+    if (v == 0) {
+        switch (L) {
+            case 4: return pcg_kernel<4, 0, ST>;
+            case 8: return pcg_kernel<8, 0, ST>;
+            default: return pcg_kernel<16, 0, ST>;
+        }
     }
+    // Sliced layout (cloth instances): one lane per row, lanes_per_row
+    // does not apply.
+    return v == 1 ? pcg_kernel<4, 1, ST> : pcg_kernel<4, 2, ST>;
 }
 
-// v: matrix variant (0 row-major, 1 planar FP32, 2 planar BF16 + FP32).
-PcgFn pcg_fn(int L, int v = 1) {
-    switch (L) {
-        case 4: return pcg_fn_l<4>(v);
-        case 8: return pcg_fn_l<8>(v);
-        case 16: return pcg_fn_l<16>(v);
-        default: throw Error(CHX_ERR_INVALID_ARGUMENT, "chysx: lanes_per_row must be 4, 8 or 16");
-    }
+// v: matrix variant (0 row-major CSR, 1 sliced FP32, 2 sliced BF16 + FP32);
+// staged: LDS-staged SpMV or the global form.
+PcgFn pcg_fn(int L, int v = 1, bool staged = true) {
+    if (L != 4 && L != 8 && L != 16)
+        throw Error(CHX_ERR_INVALID_ARGUMENT, "chysx: lanes_per_row must be 4, 8 or 16");
+    return staged ? pcg_fn_s<true>(L, v) : pcg_fn_s<false>(L, v);
 }
 
 struct DeviceLimits {
     int device = -1;
     int cus = 0;
-    int max_blocks[3][17] = {{0}};
-    int api_blocks[3][17] = {{0}};
+    int max_blocks[6][17] = {{0}};  // [2 * v + staged][L]
+    int api_blocks[6][17] = {{0}};
 };
 
 std::mutex g_lim_mu;
@@ -897,7 +1031,7 @@ DeviceLimits g_lim[16];
 // gfx950), so every workgroup of the cooperative launch is resident.
 constexpr int kBlocksPerCu = 4;
 
-int max_coresident(int L, int v = 1) {
+int max_coresident(int L, int v = 1, bool staged = true) {
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     std::lock_guard<std::mutex> lk(g_lim_mu);
@@ -909,11 +1043,11 @@ int max_coresident(int L, int v = 1) {
         hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
         d.cus = prop.multiProcessorCount;
     }
-    const int m = v;
+    const int m = 2 * v + (staged ? 1 : 0);
     if (d.max_blocks[m][L] == 0) {
         int per_cu = 0;
         hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                      &per_cu, reinterpret_cast<const void*>(pcg_fn(L, v)), kBlock, 0),
+                      &per_cu, reinterpret_cast<const void*>(pcg_fn(L, v, staged)), kBlock, 0),
                   "hipOccupancyMaxActiveBlocksPerMultiprocessor");
         d.api_blocks[m][L] = per_cu;
         if (per_cu < 1) per_cu = 1;
@@ -928,8 +1062,9 @@ void launch_pcg(const chx_cloth_batch& B, int finalize, hipStream_t stream, bool
     if (B.n_tiles <= 0) return;
     const int L = B.lanes_per_row;
     const int v = row_major ? 0 : (B.bf16_blocks != 0 ? 2 : 1);
-    PcgFn fn = pcg_fn(L, v);
-    int grid = max_coresident(L, v);
+    const bool staged = B.no_lds_staging == 0;
+    PcgFn fn = pcg_fn(L, v, staged);
+    int grid = max_coresident(L, v, staged);
     if (B.grid_blocks > 0) grid = std::min(grid, B.grid_blocks);
     grid = std::min(grid, B.n_tiles);
     chx_cloth_batch arg = B;
@@ -954,8 +1089,12 @@ void validate_batch(const chx_cloth_batch* b) {
         require(I.n_rows >= 0 && I.nnz >= 0 && I.n_pins >= 0 && I.n_tris >= 0 && I.n_dihs >= 0 && I.n_stvks >= 0,
                 "chysx: negative size in instance");
         if (I.n_rows > 0)
-            require(I.row_offsets && I.diag && I.rhs && I.x && I.r && I.z && I.p0 && I.p1 && I.q && I.minv,
+            require(I.row_offsets && I.diag && I.rhs && I.x && I.r && I.z && I.p0 && I.p1 && I.q && I.minv &&
+                        I.slice_offsets && I.slice_cols && (I.block_slots || I.nnz == 0),
                     "chysx: null array in instance");
+        require(I.n_padded >= I.nnz && (I.n_padded % CHX_SLICE_ROWS) == 0 &&
+                    size_t(I.n_padded) * 36u < (size_t(1) << 32),
+                "chysx: sliced block layout smaller than the pattern or over 4 GiB");
         if (b->bf16_blocks && I.n_rows > 0)
             require(I.diag_c && (I.values16 || I.nnz == 0), "chysx: BF16 mode needs the BF16 block copies");
     } else {
@@ -1007,6 +1146,14 @@ using namespace chx;
 
 extern "C" {
 
+int chx_pcg_grid_limit(int lanes_per_row, int bf16_blocks, int no_lds_staging, int* max_blocks) {
+    return guarded([&] {
+        require(max_blocks != nullptr, "chx_pcg_grid_limit: null output");
+        pcg_fn(lanes_per_row, 1);  // validates L
+        *max_blocks = max_coresident(lanes_per_row, bf16_blocks ? 2 : 1, no_lds_staging == 0);
+    });
+}
+
 int chx_device_info(int* cu_count, int* max_pcg_blocks, int* occupancy_blocks_per_cu) {
     return guarded([&] {
         const int mb = max_coresident(8);
@@ -1014,7 +1161,7 @@ int chx_device_info(int* cu_count, int* max_pcg_blocks, int* occupancy_blocks_pe
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         if (cu_count) *cu_count = g_lim[dev & 15].cus;
         if (max_pcg_blocks) *max_pcg_blocks = mb;
-        if (occupancy_blocks_per_cu) *occupancy_blocks_per_cu = g_lim[dev & 15].api_blocks[0][8];
+        if (occupancy_blocks_per_cu) *occupancy_blocks_per_cu = g_lim[dev & 15].api_blocks[3][8];
     });
 }
 

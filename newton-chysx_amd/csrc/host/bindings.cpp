@@ -136,6 +136,7 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("pcg_tolerance", &ClothSimulator::pcg_tolerance)
         .def("set_lanes_per_row", &ClothSimulator::set_lanes_per_row, py::arg("lanes"))
         .def("set_tile_rows", &ClothSimulator::set_tile_rows, py::arg("rows"))
+        .def("set_lds_staging", &ClothSimulator::set_lds_staging, py::arg("enabled"))
         .def("set_trace", &ClothSimulator::set_trace, py::arg("entries"))
         .def(
             "last_trace", [](ClothSimulator& s, std::uintptr_t stream) { return trace_array(s.last_trace(stream)); },
@@ -237,6 +238,8 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("set_lanes_per_row", &chysx::cloth::ClothBatch::set_lanes_per_row, py::arg("lanes"))
         .def("set_grid_blocks", &chysx::cloth::ClothBatch::set_grid_blocks, py::arg("blocks"))
         .def("set_tile_rows", &chysx::cloth::ClothBatch::set_tile_rows, py::arg("rows"))
+        .def("set_lds_staging", &chysx::cloth::ClothBatch::set_lds_staging, py::arg("enabled"))
+        .def("lds_staging", &chysx::cloth::ClothBatch::lds_staging)
         .def("set_trace", &chysx::cloth::ClothBatch::set_trace, py::arg("entries"))
         .def(
             "last_trace",

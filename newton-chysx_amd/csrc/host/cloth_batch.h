@@ -55,6 +55,9 @@ public:
     void set_grid_blocks(int blocks) { runner_.set_grid_blocks(blocks); }
     // Rows per PCG tile: a multiple of CHX_STAGE_ROWS (256).
     void set_tile_rows(int rows) { runner_.set_tile_rows(rows); }
+    // SpMV form of the PCG launch: LDS-staged (default) or global gathers.
+    void set_lds_staging(bool on) { runner_.set_lds_staging(on); }
+    bool lds_staging() const noexcept { return runner_.lds_staging(); }
     // Solver trace of the persistent PCG launch: capacity in barrier entries
     // (0: off); last_trace() returns {stamp in 100 MHz ticks, active
     // instances} per barrier of the last solve.

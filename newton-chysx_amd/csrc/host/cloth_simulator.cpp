@@ -338,6 +338,17 @@ void copy_planar_blocks(T* out, const T* dev, int count, const char* what) {
         for (int c = 0; c < 9; ++c)
             out[static_cast<std::size_t>(9) * i + c] = planes[static_cast<std::size_t>(c) * count + i];
 }
+// Off-diagonal blocks live in the sliced layout (planes of n_padded slots);
+// out[k] = the block of CSR index k, row-major.
+template <class T>
+void copy_sliced_blocks(T* out, const T* dev, int n_padded, const std::vector<int>& slots, int nnz, const char* what) {
+    std::vector<T> planes(static_cast<std::size_t>(9) * n_padded);
+    check(chx_memcpy_d2h(planes.data(), dev, sizeof(T) * planes.size(), 0), "cloth", what);
+    for (int k = 0; k < nnz; ++k)
+        for (int c = 0; c < 9; ++c)
+            out[static_cast<std::size_t>(9) * k + c] =
+                planes[static_cast<std::size_t>(c) * n_padded + static_cast<std::size_t>(slots[k])];
+}
 }  // namespace
 
 void ClothSimulator::debug_copy_hessian_diag(float* out) const {
@@ -355,7 +366,9 @@ void ClothSimulator::debug_copy_bf16_matrix(float* out_diag_c, uint16_t* out_val
         throw std::logic_error("ClothSimulator::debug_copy_bf16_matrix: BF16 mode is off or not built yet");
     check(chx_device_synchronize(), "cloth", "device synchronize(debug_copy_bf16_matrix)");
     copy_planar_blocks(out_diag_c, inst_.diag_c(), n, "diag_c copy");
-    if (pattern_.nnz() > 0) copy_planar_blocks(out_values16, inst_.values16(), pattern_.nnz(), "values16 copy");
+    if (pattern_.nnz() > 0)
+        copy_sliced_blocks(out_values16, inst_.values16(), inst_.n_padded(), inst_.block_slots(), pattern_.nnz(),
+                           "values16 copy");
 }
 
 void ClothSimulator::debug_copy_hessian_csr(int* out_row_offsets, int* out_col_indices, float* out_values) const {
@@ -371,7 +384,7 @@ void ClothSimulator::debug_copy_hessian_csr(int* out_row_offsets, int* out_col_i
             throw std::invalid_argument(
                 "ClothSimulator::debug_copy_hessian_csr: col_indices/values null but nnz_off > 0");
         std::memcpy(out_col_indices, pattern_.col_indices.data(), sizeof(int) * nnz);
-        copy_planar_blocks(out_values, inst_.values(), nnz, "values copy");
+        copy_sliced_blocks(out_values, inst_.values(), inst_.n_padded(), inst_.block_slots(), nnz, "values copy");
     }
 }
 

@@ -115,6 +115,8 @@ public:
     // Rows per PCG tile (a multiple of 256) and persistent PCG workgroups
     // (0: every co-resident workgroup).
     void set_tile_rows(int rows) { runner_.set_tile_rows(rows); }
+    // SpMV form of the PCG launch: LDS-staged (default) or global gathers.
+    void set_lds_staging(bool on) { runner_.set_lds_staging(on); }
     // Solver trace of the persistent PCG launch: capacity in barrier entries
     // (0: off); last_trace() returns {stamp in 100 MHz ticks, active
     // instances} per barrier of the last solve.

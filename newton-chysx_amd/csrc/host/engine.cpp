@@ -141,11 +141,53 @@ void InstanceData::build(const InstanceSource& s) {
     std::vector<unsigned> fill(off.begin(), off.end() - 1);
     walk([&](std::size_t g, unsigned kind, int a, int b, std::size_t id) { ent[fill[g]++] = encode(kind, a, b, id); });
 
+    // Sliced layout of the off-diagonal blocks: slice s = rows 64s..64s+63
+    // owns 64 * (its longest row) slots; block j of row 64s+l sits in slot
+    // slice_offsets[s] + 64 j + l.  Padding slots point at their own row.
+    {
+        const auto& ro = row_offsets_.host();
+        const auto& ci = col_indices_.host();
+        const int S_ = CHX_SLICE_ROWS;
+        const int slices = (n_ + S_ - 1) / S_;
+        auto& so = slice_offsets_.host();
+        so.assign(static_cast<std::size_t>(slices) + 1, 0);
+        std::size_t total = 0;
+        for (int sl = 0; sl < slices; ++sl) {
+            int width = 0;
+            for (int r = sl * S_; r < std::min(n_, (sl + 1) * S_); ++r) width = std::max(width, ro[r + 1] - ro[r]);
+            total += static_cast<std::size_t>(S_) * width;
+            if (total > static_cast<std::size_t>(std::numeric_limits<int>::max()))
+                throw std::invalid_argument("chysx::engine: sliced block layout exceeds 2^31 slots");
+            so[sl + 1] = static_cast<int>(total);
+        }
+        npad_ = static_cast<int>(total);
+        auto& sc = slice_cols_.host();
+        auto& bs = block_slots_.host();
+        sc.assign(std::max<std::size_t>(total, 1), 0);
+        bs.assign(std::max<std::size_t>(static_cast<std::size_t>(nnz_), 1), 0);
+        for (int sl = 0; sl < slices; ++sl) {
+            const int width = (so[sl + 1] - so[sl]) / S_;
+            for (int l = 0; l < S_; ++l) {
+                const int r = sl * S_ + l;
+                const int len = r < n_ ? ro[r + 1] - ro[r] : 0;
+                for (int j = 0; j < width; ++j) {
+                    const int slot = so[sl] + j * S_ + l;
+                    if (j < len) {
+                        sc[slot] = ci[ro[r] + j];
+                        bs[ro[r] + j] = slot;
+                    } else {
+                        sc[slot] = std::min(r, n_ - 1);
+                    }
+                }
+            }
+        }
+    }
+
     // The kernels address every per-instance array with 32-bit byte offsets.
     const auto fits = [](std::size_t count, std::size_t bytes_each) {
         return count * bytes_each < (std::size_t(1) << 32);
     };
-    if (!fits(static_cast<std::size_t>(n_), 36) || !fits(static_cast<std::size_t>(nnz_), 36) ||
+    if (!fits(static_cast<std::size_t>(n_), 36) || !fits(static_cast<std::size_t>(npad_), 36) ||
         !fits(tris_.cpu_size(), sizeof(float) * CHX_TRI_REC) || !fits(dihs_.cpu_size(), sizeof(float) * CHX_DIH_REC) ||
         !fits(pins_.cpu_size(), sizeof(chx_pin)) || !fits(ent.size(), sizeof(unsigned)) ||
         !fits(stvks_.cpu_size(), sizeof(float) * CHX_STVK_REC))
@@ -154,6 +196,9 @@ void InstanceData::build(const InstanceSource& s) {
             "blocks per cloth); split the cloth into several instances");
     upload(row_offsets_, bytes_);
     upload(col_indices_, bytes_);
+    upload(slice_offsets_, bytes_);
+    upload(slice_cols_, bytes_);
+    upload(block_slots_, bytes_);
     upload(contrib_offsets_, bytes_);
     upload(contribs_, bytes_);
     upload(pins_, bytes_);
@@ -166,7 +211,7 @@ void InstanceData::build(const InstanceSource& s) {
     alloc(dih_rec_, dihs_.cpu_size() * CHX_DIH_REC, bytes_);
     alloc(stvk_rec_, stvks_.cpu_size() * CHX_STVK_REC, bytes_);
     alloc(diag_, 9 * n, bytes_);
-    alloc(values_, 9 * static_cast<std::size_t>(nnz_), bytes_);
+    alloc(values_, 9 * std::max<std::size_t>(static_cast<std::size_t>(npad_), 1), bytes_);
     alloc(rhs_, 3 * n, bytes_);
     alloc(r_, 3 * n, bytes_);
     alloc(z_, 3 * n, bytes_);
@@ -196,7 +241,7 @@ void InstanceData::build(const InstanceSource& s) {
     }
     if (s.bf16_blocks) {
         alloc(diag_c_, 9 * n, bytes_);
-        alloc(values16_, 9 * static_cast<std::size_t>(nnz_), bytes_);
+        alloc(values16_, 9 * std::max<std::size_t>(static_cast<std::size_t>(npad_), 1), bytes_);
     } else {
         diag_c_.release();
         values16_.release();
@@ -246,6 +291,10 @@ chx_instance InstanceData::descriptor(const InstanceSource& s) const {
     I.diag_c = s.bf16_blocks ? diag_c_.gpu_data() : nullptr;
     I.values16 = s.bf16_blocks ? values16_.gpu_data() : nullptr;
     I.windows = windows_.gpu_data();
+    I.slice_offsets = slice_offsets_.gpu_data();
+    I.slice_cols = slice_cols_.gpu_data();
+    I.block_slots = block_slots_.gpu_data();
+    I.n_padded = npad_;
     I.n_rows = n_;
     I.nnz = nnz_;
     I.n_pins = static_cast<int>(pins_.cpu_size());

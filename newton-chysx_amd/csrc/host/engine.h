@@ -56,6 +56,9 @@ public:
     bool built() const noexcept { return built_; }
     int n_rows() const noexcept { return n_; }
     int nnz() const noexcept { return nnz_; }
+    int n_padded() const noexcept { return npad_; }
+    // Slot of CSR block k in the sliced layout (host copy).
+    const std::vector<int>& block_slots() const noexcept { return block_slots_.host(); }
     std::size_t device_bytes() const noexcept { return bytes_; }
     std::size_t contributions() const noexcept { return contribs_.cpu_size(); }
     const float* diag() const noexcept { return diag_.gpu_data(); }
@@ -69,7 +72,7 @@ private:
     void pack_elements(const InstanceSource& s);
 
     bool built_ = false;
-    int n_ = 0, nnz_ = 0;
+    int n_ = 0, nnz_ = 0, npad_ = 0;
     std::size_t bytes_ = 0;
     DeviceArray<int> row_offsets_, col_indices_;
     DeviceArray<chx_pin> pins_;
@@ -83,6 +86,7 @@ private:
     DeviceArray<float> diag_c_;
     DeviceArray<uint16_t> values16_;
     DeviceArray<int> windows_;  // per CHX_STAGE_ROWS-row stage: {min column, max column}
+    DeviceArray<int> slice_offsets_, slice_cols_, block_slots_;  // sliced layout of the blocks
 };
 
 class BatchRunner {
@@ -119,6 +123,10 @@ public:
         grid_ = g < 0 ? 0 : g;
         desc_.grid_blocks = grid_;
     }
+    // SpMV form of the PCG launch: LDS-staged stages (default) or gathers
+    // straight from global memory (smaller LDS, more workgroups per CU).
+    void set_lds_staging(bool on) { desc_.no_lds_staging = on ? 0 : 1; }
+    bool lds_staging() const noexcept { return desc_.no_lds_staging == 0; }
     // Precision mode of the solve: FP32 blocks, or inner CG on the BF16
     // matrix with outer refinement on the FP32 true residual.
     void set_bf16(bool on) {

@@ -88,3 +88,54 @@ def test_batch_grid_smaller_than_tiles(gpu):
         b2.step(specs[0].dt)
     for a, m in zip(ref, test):
         assert np.array_equal(a.state()[0], m.state()[0])
+
+
+@pytest.mark.parametrize("grid", [0, 16, 13])
+def test_global_spmv_form_equals_staged_bitwise(gpu, grid):
+    """The SpMV form without LDS staging (gathers through the caches, more
+    workgroups per CU) computes the same sums in the same order: bitwise the
+    staged form's results, with the full grid, a multiple of 8 (XCD
+    segments of the compacted tile list) and an odd grid."""
+    import chysx
+    from chysx import scenes
+
+    specs = _specs(count=6, seed=14)
+    staged = [scenes.DeviceCloth(s) for s in specs]
+    plain = [scenes.DeviceCloth(s) for s in specs]
+    b1, b2 = chysx.ClothBatch(), chysx.ClothBatch()
+    for c in staged:
+        b1.add_instance(c.sim)
+    for c in plain:
+        b2.add_instance(c.sim)
+    b2.set_lds_staging(False)
+    assert b1.lds_staging() and not b2.lds_staging()
+    if grid:
+        b1.set_grid_blocks(grid)
+        b2.set_grid_blocks(grid)
+    for step in range(6):
+        b1.step(specs[0].dt)
+        b2.step(specs[0].dt)
+        assert b1.last_pcg_iterations() == b2.last_pcg_iterations(), step
+    for a, m in zip(staged, plain):
+        assert np.array_equal(a.state()[0], m.state()[0]) and np.array_equal(a.state()[1], m.state()[1])
+
+
+def test_many_instances_static_tile_list(gpu):
+    """More instances than the compacted tile list holds (1024): the solve
+    walks the static tile list; every instance still equals its solve alone."""
+    import chysx
+    from chysx import scenes
+
+    specs = scenes.batched(seed=15, count=1030, lo=4, hi=7)
+    alone = [scenes.DeviceCloth(s) for s in specs]
+    mixed = [scenes.DeviceCloth(s) for s in specs]
+    batch = chysx.ClothBatch()
+    for c in mixed:
+        batch.add_instance(c.sim)
+    for _ in range(2):
+        for c in alone:
+            c.step()
+        batch.step(specs[0].dt)
+    assert batch.last_pcg_iterations() == [c.sim.last_pcg_iterations() for c in alone]
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a.state()[0], m.state()[0])
