@@ -101,13 +101,21 @@ typedef struct chx_stvk {
  * reference's (ChysX/src/sparse/block_csr.h:59-135).  Slots of the gather
  * map: [0, n_rows) diagonal blocks, [n_rows, n_rows + nnz) off-diagonal.
  *
- * The off-diagonal block values are stored sliced: rows 64s..64s+63 form
- * slice s, which owns 64 * width_s slots [slice_offsets[s],
- * slice_offsets[s+1]) (width_s = the longest of its rows); block j of row
- * 64s+l sits in slot slice_offsets[s] + 64 j + l, so the 64 rows' j-th
- * blocks are adjacent.  Slots past a row's length are padding (column =
- * the row itself, values unused).  block_slots maps the reference's CSR
- * index k (row_offsets / col_indices order) to its slot. */
+ * The off-diagonal blocks are stored once per symmetric pair, as the upper
+ * block A_ij (j > i) of row i, A_ji being its transpose.  Both the upper
+ * blocks and each row's entries (all its off-diagonal columns) are sliced:
+ * rows 64s..64s+63 form slice s.  Slice s owns 64 * width_s entry slots
+ * [slice_offsets[s], slice_offsets[s+1]) (width_s = its longest row); the
+ * j-th off-diagonal column of row 64s+l (CSR order) sits in entry slot
+ * slice_offsets[s] + 64 j + l as (col << 6) | jj, where jj is the index of
+ * the block among the upper blocks of its owner row (i for col > i, col for
+ * col < i).  The upper blocks of slice s own 64 * max(1, upper width_s)
+ * value slots from upper_offsets[s]; upper block jj of row 64s+l is value
+ * slot upper_offsets[s] + 64 jj + l.  Entry slots past a row's length are
+ * padding ((row << 6) | 0, unused).  block_slots maps the reference's CSR
+ * index k (row_offsets / col_indices order) to the value slot of an upper
+ * block, or to -1 - (the partner's value slot) for a lower one.  Limits:
+ * fewer than 2^26 rows and at most 64 upper blocks per row. */
 typedef struct chx_instance {
     float* pos;                      /* external AoS float3 positions (x_n in, x_{n+1} out) */
     float* vel;                      /* external AoS float3 velocities                      */
@@ -116,7 +124,7 @@ typedef struct chx_instance {
     const int* col_indices;          /* [nnz]                                                */
     float* diag;                     /* [9][n_rows] planar: component c (row-major r*3+col)  */
                                      /* of block row i at diag[c*n_rows + i]                 */
-    float* values;                   /* [9][n_padded] planar over the sliced slots           */
+    float* values;                   /* [9][n_upper] planar over the upper value slots       */
     const chx_pin* pins;
     const chx_tri* tris;             /* stretch elements, then shear elements                */
     const chx_dih* dihs;
@@ -136,21 +144,22 @@ typedef struct chx_instance {
     float* diag_c;                   /* BF16 precision mode, else NULL: the inner SpMV's     */
     uint16_t* values16;              /* matrix, planar FP32 compensated diagonal             */
                                      /* D + sum_j (V_ij - bf16(V_ij)) [9][n_rows] and planar */
-                                     /* BF16 off-diagonal copies [9][n_padded]               */
+                                     /* BF16 upper-block copies [9][n_upper]                 */
     const chx_stvk* stvks;           /* StVK membrane elements, or NULL                      */
     float* stvk_rec;                 /* [n_stvks*CHX_STVK_REC]                               */
     const int* windows;              /* [ceil(n_rows/CHX_STAGE_ROWS)][2]: smallest and       */
                                      /* largest column (own rows included) of each stage of  */
                                      /* CHX_STAGE_ROWS rows, or NULL (no LDS staging)        */
-    const int* slice_offsets;        /* [ceil(n_rows/CHX_SLICE_ROWS)+1] first slot of slice  */
-    const int* slice_cols;           /* [n_padded] column of every slot                      */
-    const int* block_slots;          /* [nnz] slot of CSR block k                            */
+    const int* slice_offsets;        /* [ceil(n_rows/CHX_SLICE_ROWS)+1] first entry slot     */
+    const int* slice_entries;        /* [n_padded] (col << 6) | jj of every entry slot       */
+    const int* upper_offsets;        /* [ceil(n_rows/CHX_SLICE_ROWS)+1] first value slot     */
+    const int* block_slots;          /* [nnz] value slot of CSR block k (-1 - slot: lower)   */
     int n_rows, nnz, n_pins, n_tris, n_dihs;
     int tile_begin;                  /* this instance's row tiles in the batch tile list     */
     int tile_end;
     int n_stvks;
-    int n_padded;                    /* slots of the sliced layout (>= nnz)                  */
-    int pad_slices;
+    int n_padded;                    /* entry slots (>= nnz)                                 */
+    int n_upper;                     /* upper value slots (>= nnz / 2)                       */
     float dt;                        /* substep                                              */
     float damping;                   /* v *= exp(-damping*dt) when > 0                       */
     float gx, gy, gz;                /* gravity                                              */

@@ -1,10 +1,11 @@
 """Autotuner of the batched cloth substep.
 
-Sweeps the SpMV form (LDS-staged stages or global gathers), the SpMV lanes
-per block row (4, 8, 16 — i.e. 16, 8 or 4 block rows per wavefront), the
-persistent PCG grid size (workgroups per compute unit, up to what the form
+Sweeps the SpMV form (LDS-staged stages or global gathers), the persistent
+PCG grid size (workgroups per compute unit, up to what the form
 keeps co-resident) and the rows per PCG tile (256, 512, 1024) on the actual
-batch, timing whole substeps with HIP events on the
+batch (lanes per block row only shape the CSR SpMV of caller-provided
+row-major matrices; the cloth instances' sliced SpMV runs one lane per row,
+so `lanes` defaults to a single value), timing whole substeps with HIP events on the
 caller's stream, and keeps the fastest.  Results are cached in a JSON file
 keyed by device name and a workload signature (instance count, vertices,
 off-diagonal blocks) so a later run of the same workload skips the sweep.
@@ -34,7 +35,7 @@ def _device_name() -> str:
     return torch.cuda.get_device_name(0)
 
 
-def tune_batch(batch, dt: float, stream=None, lanes=(4, 8, 16), blocks_per_cu=(1, 2, 3, 4),
+def tune_batch(batch, dt: float, stream=None, lanes=(4,), blocks_per_cu=(1, 2, 3, 4),
                tile_rows=(256, 512, 1024), staging=(True, False), warmup: int = 1, steps: int = 3,
                cache_path: str | None = DEFAULT_CACHE, log=None) -> dict:
     """Returns {"lds_staging", "lanes_per_row", "grid_blocks", "tile_rows",

@@ -161,6 +161,17 @@ __device__ __forceinline__ f3 mv9p(gp<const T> m, unsigned stride, unsigned i, f
     for (unsigned c = 0; c < 9; ++c) a[c] = to_f32(ldx(m, c * stride + i));
     return mv9(a, v);
 }
+// Block i, or its transpose when tr (lower blocks of the symmetric layout
+// are their upper partners' transposes): the same loads, swapped operands.
+template <class T>
+__device__ __forceinline__ f3 mv9p_t(gp<const T> m, unsigned stride, unsigned i, f3 v, bool tr) {
+    float a[9];
+#pragma unroll
+    for (unsigned c = 0; c < 9; ++c) a[c] = to_f32(ldx(m, c * stride + i));
+    const float b[9] = {a[0], tr ? a[3] : a[1], tr ? a[6] : a[2], tr ? a[1] : a[3], a[4],
+                        tr ? a[7] : a[5], tr ? a[2] : a[6], tr ? a[5] : a[7], a[8]};
+    return mv9(b, v);
+}
 template <class T>
 struct MatP {
     gp<const T> d;
@@ -169,6 +180,7 @@ struct MatP {
     unsigned nnz;  // plane stride of v (off-diagonal blocks)
     __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv9p(d, n, row, x); }
     __device__ __forceinline__ f3 off(unsigned k, f3 x) const { return mv9p(v, nnz, k, x); }
+    __device__ __forceinline__ f3 off_t(unsigned k, f3 x, bool tr) const { return mv9p_t(v, nnz, k, x, tr); }
     __device__ __forceinline__ void diag_block(unsigned row, float (&m)[9]) const {
 #pragma unroll
         for (unsigned c = 0; c < 9; ++c) m[c] = to_f32(ldx(d, c * n + row));
@@ -185,6 +197,7 @@ struct MatP16 {
     unsigned nnz;
     __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv9p(d, n, row, x); }
     __device__ __forceinline__ f3 off(unsigned k, f3 x) const { return mv9p(v, nnz, k, x); }
+    __device__ __forceinline__ f3 off_t(unsigned k, f3 x, bool tr) const { return mv9p_t(v, nnz, k, x, tr); }
 };
 
 __device__ __forceinline__ double dot64(f3 a, f3 b) {
@@ -208,10 +221,10 @@ struct Inst {
     gp<float> diag_c;
     gp<uint16_t> values16;
     gp<const int> windows;
-    gp<const int> slice_offsets, slice_cols, block_slots;
+    gp<const int> slice_offsets, slice_entries, upper_offsets, block_slots;
     gp<const chx_stvk> stvks;
     gp<float> stvk_rec;
-    int n_stvks, n_padded;
+    int n_stvks, n_padded, n_upper;
     int n_rows, nnz, n_pins, n_tris, n_dihs, tile_begin, tile_end;
     float dt, damping, gx, gy, gz;
     int max_iter;
@@ -248,9 +261,11 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.values16 = glb(d->values16);
     I.windows = glb(d->windows);
     I.slice_offsets = glb(d->slice_offsets);
-    I.slice_cols = glb(d->slice_cols);
+    I.slice_entries = glb(d->slice_entries);
+    I.upper_offsets = glb(d->upper_offsets);
     I.block_slots = glb(d->block_slots);
     I.n_padded = d->n_padded;
+    I.n_upper = d->n_upper;
     I.stvks = glb(d->stvks);
     I.stvk_rec = glb(d->stvk_rec);
     I.n_stvks = d->n_stvks;

@@ -122,6 +122,11 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
     const Inst I = inst_at(B, inst);
     const unsigned n = unsigned(I.n_rows);
     const bool is_diag = s < n;
+    // Symmetric sliced layout: the upper block of CSR index e lives in value
+    // slot block_slots[e]; lower blocks (their partners' transposes) are not
+    // assembled.
+    const int bslot = is_diag ? 0 : ldx(I.block_slots, s - n);
+    if (bslot < 0) return;
     const unsigned c0 = ldx(I.contrib_offsets, s), c1 = ldx(I.contrib_offsets, s + 1u);
     float h[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     f3 grad = mk(0.f, 0.f, 0.f);
@@ -195,8 +200,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
 #pragma unroll
         for (unsigned k = 0; k < 9; ++k) stx(I.diag, k * n + row, h[k]);
     } else {
-        // Sliced layout: CSR block e lives in slot block_slots[e].
-        const unsigned slot = unsigned(ldx(I.block_slots, s - n)), np = unsigned(I.n_padded);
+        const unsigned slot = unsigned(bslot), np = unsigned(I.n_upper);
 #pragma unroll
         for (unsigned k = 0; k < 9; ++k) stx(I.values, k * np + slot, h[k]);
         if (I.values16 != nullptr) {
@@ -222,16 +226,20 @@ __global__ __launch_bounds__(kBlock) void compensate_kernel(chx_cloth_batch B) {
     const Inst I = inst_at(B, inst);
     const unsigned n = unsigned(I.n_rows), row = unsigned(ss);
     if (row >= n) return;
-    const unsigned np = unsigned(I.n_padded);
+    const unsigned np = unsigned(I.n_upper);
     const unsigned k0 = ldx(I.row_offsets, row), k1 = ldx(I.row_offsets, row + 1u);
     float d[9];
 #pragma unroll
     for (unsigned c = 0; c < 9; ++c) d[c] = ldx(gp<const float>(I.diag), c * n + row);
     for (unsigned k = k0; k < k1; ++k) {
-        const unsigned slot = unsigned(ldx(I.block_slots, k));
+        // lower blocks: the partner's rounding, transposed
+        const int bs = ldx(I.block_slots, k);
+        const bool tr = bs < 0;
+        const unsigned slot = unsigned(tr ? -1 - bs : bs);
 #pragma unroll
         for (unsigned c = 0; c < 9; ++c) {
-            const float v = ldx(gp<const float>(I.values), c * np + slot);
+            const unsigned cs = tr ? (c % 3u) * 3u + c / 3u : c;
+            const float v = ldx(gp<const float>(I.values), cs * np + slot);
             d[c] += v - bf16_to_f32(f32_to_bf16(v));
         }
     }
@@ -326,23 +334,42 @@ __device__ __forceinline__ void spmv_core(Mat mat, int rb, int re, RoAt ro_at, C
 }
 
 // ---------------------------------------------------------------------------
-// Sliced SpMV: one lane per block row, one wave per slice of 64 rows (rb a
-// multiple of 64).  slot_at(s) gives slice s's first slot, ci_at(slot) the
-// column of a slot, len_at(row) the row's block count, src(j) the source
-// entry j; fin(row, Ax, x_row) on every valid row's lane.  A wave walks the
-// slice's width in batches of KS slots whose loads are all in flight
-// together (slots past the width clamp to the batch's first, padding slots
-// load and are masked); each j-th slot load of the wave reads 64 adjacent
-// words.  Per row the sum is diag + off[0] + off[1] + ... in slot order.
+// Sliced symmetric SpMV: one lane per block row, one wave per slice of 64
+// rows (rb a multiple of 64).  slot_at(s) gives slice s's first entry slot,
+// ent_at(slot) an entry (col << 6) | jj, uoff_at(s) slice s's first value
+// slot, len_at(row) the row's block count, src(j) the source entry j;
+// fin(row, Ax, x_row) on every valid row's lane.  Entry j of a row is the
+// upper block jj of the row (col > row) or the transpose of upper block jj
+// of row col (col < row).  A wave walks the slice's width in batches of KS
+// entries whose loads are all in flight together (entries past the width
+// clamp to the batch's first, padding entries load and are masked); the
+// j-th entries of a wave are adjacent words, and so are, on a structured
+// mesh, the value slots they name.  Per row the sum is diag + off[0] +
+// off[1] + ... in column order.
 // ---------------------------------------------------------------------------
-constexpr int kSliceBatch = 8;
+// Compile-time tuning constants of the persistent PCG (LDS and register
+// budget per workgroup): entries per batch of the sliced SpMV, staged
+// column-index and source-row capacities, compacted-list instance limit.
+#ifndef CHX_PCG_SLICE_BATCH
+#define CHX_PCG_SLICE_BATCH 4
+#endif
+#ifndef CHX_PCG_CI_CAP
+#define CHX_PCG_CI_CAP 4096
+#endif
+#ifndef CHX_PCG_WIN_CAP
+#define CHX_PCG_WIN_CAP 2816
+#endif
+#ifndef CHX_PCG_COMPACT_MAX
+#define CHX_PCG_COMPACT_MAX 1024
+#endif
+constexpr int kSliceBatch = CHX_PCG_SLICE_BATCH;
 constexpr unsigned kSlice = CHX_SLICE_ROWS;
 static_assert(CHX_SLICE_ROWS == kWave, "one lane per row of a slice");
 static_assert(CHX_STAGE_ROWS % CHX_SLICE_ROWS == 0, "stages hold whole slices");
 
-template <class Mat, class SlotAt, class CiAt, class LenAt, class Src, class Fin>
-__device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot_at, CiAt ci_at, LenAt len_at,
-                                            Src src, Fin fin) {
+template <class Mat, class SlotAt, class EntAt, class UoffAt, class LenAt, class Src, class Fin>
+__device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot_at, EntAt ent_at, UoffAt uoff_at,
+                                            LenAt len_at, Src src, Fin fin) {
     constexpr int KS = kSliceBatch;
     const unsigned lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x >> 6;
@@ -351,6 +378,7 @@ __device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot
         const unsigned sl = unsigned(s0) / kSlice;
         const unsigned base = slot_at(sl);
         const unsigned width = (slot_at(sl + 1u) - base) / kSlice;
+        const unsigned ubase = uoff_at(sl) + lane;
         const unsigned row = unsigned(s0) + lane;
         const bool valid = row < unsigned(re);
         const unsigned rowc = valid ? row : unsigned(s0);
@@ -362,8 +390,12 @@ __device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot
 #pragma unroll
             for (int u = 0; u < KS; ++u) {
                 const unsigned j = j0 + unsigned(u);
-                const unsigned slot = base + (j < width ? j : j0) * kSlice + lane;
-                part[u] = mat.off(slot, src(ci_at(slot)));
+                const unsigned e = unsigned(ent_at(base + (j < width ? j : j0) * kSlice + lane));
+                const unsigned col = e >> 6, jj = e & 63u;
+                const bool lower = col < rowc;
+                const unsigned vs = lower ? uoff_at(col / kSlice) + (col & (kSlice - 1u)) + jj * kSlice
+                                          : ubase + jj * kSlice;
+                part[u] = mat.off_t(vs, src(col), lower);
             }
 #pragma unroll
             for (int u = 0; u < KS; ++u)
@@ -384,11 +416,12 @@ __device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot
 // ---------------------------------------------------------------------------
 constexpr int kStage = CHX_STAGE_ROWS;
 static_assert(kStage == kBlock, "one thread per stage row for the row offsets");
-constexpr unsigned kCiCap = 4096;   // column indices per staged stage
-constexpr unsigned kWinCap = 2816;  // source rows per staged stage
+constexpr unsigned kCiCap = CHX_PCG_CI_CAP;   // column indices per staged stage
+constexpr unsigned kWinCap = CHX_PCG_WIN_CAP;  // source rows per staged stage
 constexpr int kChunk = 16;          // tiles whose instance scalars are summed together
 static_assert(kChunk * 16 == kBlock, "16 lanes per tile in chunk_sums");
-constexpr int kCompactMax = 4 * kBlock;  // instances up to which the active tiles are compacted
+constexpr int kCompactMax = CHX_PCG_COMPACT_MAX;  // instances up to which the active tiles are compacted
+static_assert(kCompactMax <= 4 * kBlock, "build_active scans 4 instances per thread");
 
 // Shared by both SpMV forms.
 struct PcgShared {
@@ -405,10 +438,12 @@ struct PcgShared {
 };
 
 // Stage arrays of the LDS-staged SpMV.
+constexpr unsigned kWinSlices = kWinCap / CHX_SLICE_ROWS + 2;  // slices a window touches
 struct PcgStageLds {
     int ro[kStage + 1];
     int ci[kCiCap];
     float win[3 * kWinCap];
+    int uo[kWinSlices];  // sliced layout: first value slot of the window's slices
 };
 
 // ST: LDS-staged SpMV (stage arrays, 64 KB per workgroup) or the global
@@ -436,7 +471,7 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
     const unsigned nnz_s =
         (SL ? unsigned(ldx(I.slice_offsets, (unsigned(se) + kSlice - 1u) / kSlice))
             : unsigned(ldx(I.row_offsets, unsigned(se)))) - b0;
-    const gp<const int> cols = SL ? I.slice_cols : I.col_indices;
+    const gp<const int> cols = SL ? I.slice_entries : I.col_indices;
     const unsigned st = unsigned(sb) / unsigned(kStage);
     const unsigned wlo = unsigned(ldx(I.windows, 2u * st)), whi = unsigned(ldx(I.windows, 2u * st + 1u));
     const unsigned W = whi - wlo + 1u;
@@ -479,6 +514,10 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
         }
     }
     if (tid == 0) S.ro[R] = ldx(I.row_offsets, unsigned(se)) - int(rb0);
+    if (SL) {
+        const unsigned s_lo = wlo / kSlice, s_n = whi / kSlice - s_lo + 1u;
+        if (tid < s_n) S.uo[tid] = ldx(I.upper_offsets, s_lo + tid);
+    }
     __syncthreads();
     beg0 = b0;
     lo = wlo;
@@ -496,7 +535,8 @@ __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re
         if constexpr (SL) {
             spmv_sliced(
                 mat, b, e, [&](unsigned sl) { return unsigned(ldx(I.slice_offsets, sl)); },
-                [&](unsigned k) { return unsigned(ldx(I.slice_cols, k)); },
+                [&](unsigned k) { return ldx(I.slice_entries, k); },
+                [&](unsigned sl) { return unsigned(ldx(I.upper_offsets, sl)); },
                 [&](unsigned r) { return unsigned(ldx(I.row_offsets, r + 1u) - ldx(I.row_offsets, r)); }, srcg,
                 fin);
         } else {
@@ -514,9 +554,11 @@ __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re
             unsigned beg0 = 0, lo = 0;
             if (stage_load<SL>(I, sb, se, srcg, T, beg0, lo)) {
                 if constexpr (SL) {
+                    const unsigned s_lo = lo / kSlice;
                     spmv_sliced(
                         mat, sb, se, [&](unsigned sl) { return unsigned(ldx(I.slice_offsets, sl)); },
-                        [&](unsigned k) { return unsigned(T.ci[k - beg0]); },
+                        [&](unsigned k) { return T.ci[k - beg0]; },
+                        [&](unsigned sl) { return unsigned(T.uo[sl - s_lo]); },
                         [&](unsigned r) { return unsigned(T.ro[r - unsigned(sb) + 1u] - T.ro[r - unsigned(sb)]); },
                         [&](unsigned j) { return lds3(T.win, j - lo); }, fin);
                 } else {
@@ -754,17 +796,17 @@ struct PcgMats<0> {
 template <>
 struct PcgMats<1> {
     static __device__ __forceinline__ MatP32 a(const Inst& I) {
-        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.n_padded)};
+        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.n_upper)};
     }
     static __device__ __forceinline__ MatP32 r(const Inst& I) { return a(I); }
 };
 template <>
 struct PcgMats<2> {
     static __device__ __forceinline__ MatP16 a(const Inst& I) {
-        return MatP16{I.diag_c, I.values16, unsigned(I.n_rows), unsigned(I.n_padded)};
+        return MatP16{I.diag_c, I.values16, unsigned(I.n_rows), unsigned(I.n_upper)};
     }
     static __device__ __forceinline__ MatP32 r(const Inst& I) {
-        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.n_padded)};
+        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.n_upper)};
     }
 };
 
@@ -1090,11 +1132,12 @@ void validate_batch(const chx_cloth_batch* b) {
                 "chysx: negative size in instance");
         if (I.n_rows > 0)
             require(I.row_offsets && I.diag && I.rhs && I.x && I.r && I.z && I.p0 && I.p1 && I.q && I.minv &&
-                        I.slice_offsets && I.slice_cols && (I.block_slots || I.nnz == 0),
+                        I.slice_offsets && I.slice_entries && I.upper_offsets && (I.block_slots || I.nnz == 0),
                     "chysx: null array in instance");
-        require(I.n_padded >= I.nnz && (I.n_padded % CHX_SLICE_ROWS) == 0 &&
-                    size_t(I.n_padded) * 36u < (size_t(1) << 32),
-                "chysx: sliced block layout smaller than the pattern or over 4 GiB");
+        require(I.n_padded >= I.nnz && (I.n_padded % CHX_SLICE_ROWS) == 0 && (I.n_upper % CHX_SLICE_ROWS) == 0 &&
+                    2 * size_t(I.n_upper) >= size_t(I.nnz) && size_t(I.n_upper) * 36u < (size_t(1) << 32) &&
+                    I.n_rows < (1 << 26),
+                "chysx: sliced block layout smaller than the pattern, over 4 GiB or over 2^26 rows");
         if (b->bf16_blocks && I.n_rows > 0)
             require(I.diag_c && (I.values16 || I.nnz == 0), "chysx: BF16 mode needs the BF16 block copies");
     } else {

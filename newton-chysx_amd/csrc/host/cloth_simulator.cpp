@@ -338,16 +338,22 @@ void copy_planar_blocks(T* out, const T* dev, int count, const char* what) {
         for (int c = 0; c < 9; ++c)
             out[static_cast<std::size_t>(9) * i + c] = planes[static_cast<std::size_t>(c) * count + i];
 }
-// Off-diagonal blocks live in the sliced layout (planes of n_padded slots);
-// out[k] = the block of CSR index k, row-major.
+// Off-diagonal blocks live once per symmetric pair (upper blocks, planes of
+// n_upper value slots); out[k] = the block of CSR index k, row-major: the
+// stored upper block, or its partner's transpose for a lower one.
 template <class T>
-void copy_sliced_blocks(T* out, const T* dev, int n_padded, const std::vector<int>& slots, int nnz, const char* what) {
-    std::vector<T> planes(static_cast<std::size_t>(9) * n_padded);
+void copy_sliced_blocks(T* out, const T* dev, int n_upper, const std::vector<int>& slots, int nnz, const char* what) {
+    std::vector<T> planes(static_cast<std::size_t>(9) * n_upper);
     check(chx_memcpy_d2h(planes.data(), dev, sizeof(T) * planes.size(), 0), "cloth", what);
-    for (int k = 0; k < nnz; ++k)
-        for (int c = 0; c < 9; ++c)
-            out[static_cast<std::size_t>(9) * k + c] =
-                planes[static_cast<std::size_t>(c) * n_padded + static_cast<std::size_t>(slots[k])];
+    for (int k = 0; k < nnz; ++k) {
+        const bool lower = slots[k] < 0;
+        const std::size_t slot = static_cast<std::size_t>(lower ? -1 - slots[k] : slots[k]);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                const int c = lower ? 3 * b + a : 3 * a + b;
+                out[static_cast<std::size_t>(9) * k + 3 * a + b] = planes[static_cast<std::size_t>(c) * n_upper + slot];
+            }
+    }
 }
 }  // namespace
 
@@ -367,7 +373,7 @@ void ClothSimulator::debug_copy_bf16_matrix(float* out_diag_c, uint16_t* out_val
     check(chx_device_synchronize(), "cloth", "device synchronize(debug_copy_bf16_matrix)");
     copy_planar_blocks(out_diag_c, inst_.diag_c(), n, "diag_c copy");
     if (pattern_.nnz() > 0)
-        copy_sliced_blocks(out_values16, inst_.values16(), inst_.n_padded(), inst_.block_slots(), pattern_.nnz(),
+        copy_sliced_blocks(out_values16, inst_.values16(), inst_.n_upper(), inst_.block_slots(), pattern_.nnz(),
                            "values16 copy");
 }
 
@@ -384,7 +390,7 @@ void ClothSimulator::debug_copy_hessian_csr(int* out_row_offsets, int* out_col_i
             throw std::invalid_argument(
                 "ClothSimulator::debug_copy_hessian_csr: col_indices/values null but nnz_off > 0");
         std::memcpy(out_col_indices, pattern_.col_indices.data(), sizeof(int) * nnz);
-        copy_sliced_blocks(out_values, inst_.values(), inst_.n_padded(), inst_.block_slots(), nnz, "values copy");
+        copy_sliced_blocks(out_values, inst_.values(), inst_.n_upper(), inst_.block_slots(), nnz, "values copy");
     }
 }
 

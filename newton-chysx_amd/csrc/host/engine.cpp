@@ -141,29 +141,59 @@ void InstanceData::build(const InstanceSource& s) {
     std::vector<unsigned> fill(off.begin(), off.end() - 1);
     walk([&](std::size_t g, unsigned kind, int a, int b, std::size_t id) { ent[fill[g]++] = encode(kind, a, b, id); });
 
-    // Sliced layout of the off-diagonal blocks: slice s = rows 64s..64s+63
-    // owns 64 * (its longest row) slots; block j of row 64s+l sits in slot
-    // slice_offsets[s] + 64 j + l.  Padding slots point at their own row.
+    // Symmetric sliced layout (slice s = rows 64s..64s+63).  Entries: the
+    // j-th off-diagonal column of row 64s+l at entry slot
+    // slice_offsets[s] + 64 j + l, encoded (col << 6) | jj with jj the
+    // block's index among its owner row's upper blocks (the owner is the
+    // smaller index: the row for col > row, the column for col < row).
+    // Values: upper block jj of row 64s+l at value slot
+    // upper_offsets[s] + 64 jj + l; lower blocks are their partners'
+    // transposes.
     {
         const auto& ro = row_offsets_.host();
         const auto& ci = col_indices_.host();
         const int S_ = CHX_SLICE_ROWS;
+        if (n_ >= (1 << 26)) throw std::invalid_argument("chysx::engine: a cloth instance holds fewer than 2^26 vertices");
         const int slices = (n_ + S_ - 1) / S_;
+        // first upper block of each row (columns ascending: lower ones first)
+        std::vector<int> up0(static_cast<std::size_t>(n_));
+        for (int r = 0; r < n_; ++r) {
+            int k = ro[r];
+            while (k < ro[r + 1] && ci[k] < r) ++k;
+            up0[r] = k;
+            if (ro[r + 1] - k > 64)
+                throw std::invalid_argument("chysx::engine: a vertex couples to more than 64 higher-numbered vertices");
+        }
         auto& so = slice_offsets_.host();
+        auto& uo = upper_offsets_.host();
         so.assign(static_cast<std::size_t>(slices) + 1, 0);
-        std::size_t total = 0;
+        uo.assign(static_cast<std::size_t>(slices) + 1, 0);
+        std::size_t total = 0, utotal = 0;
         for (int sl = 0; sl < slices; ++sl) {
-            int width = 0;
-            for (int r = sl * S_; r < std::min(n_, (sl + 1) * S_); ++r) width = std::max(width, ro[r + 1] - ro[r]);
+            int width = 0, uwidth = 1;
+            for (int r = sl * S_; r < std::min(n_, (sl + 1) * S_); ++r) {
+                width = std::max(width, ro[r + 1] - ro[r]);
+                uwidth = std::max(uwidth, ro[r + 1] - up0[r]);
+            }
             total += static_cast<std::size_t>(S_) * width;
-            if (total > static_cast<std::size_t>(std::numeric_limits<int>::max()))
+            utotal += static_cast<std::size_t>(S_) * uwidth;
+            if (total > static_cast<std::size_t>(std::numeric_limits<int>::max()) ||
+                utotal > static_cast<std::size_t>(std::numeric_limits<int>::max()))
                 throw std::invalid_argument("chysx::engine: sliced block layout exceeds 2^31 slots");
             so[sl + 1] = static_cast<int>(total);
+            uo[sl + 1] = static_cast<int>(utotal);
         }
         npad_ = static_cast<int>(total);
-        auto& sc = slice_cols_.host();
+        nup_ = static_cast<int>(utotal);
+        auto vslot = [&](int owner, int jj) { return uo[owner / S_] + jj * S_ + owner % S_; };
+        // jj of the block (owner, col) among owner's upper blocks
+        auto upper_index = [&](int owner, int col) {
+            const auto b = ci.begin() + up0[owner], e = ci.begin() + ro[owner + 1];
+            return static_cast<int>(std::lower_bound(b, e, col) - b);
+        };
+        auto& se = slice_entries_.host();
         auto& bs = block_slots_.host();
-        sc.assign(std::max<std::size_t>(total, 1), 0);
+        se.assign(std::max<std::size_t>(total, 1), 0);
         bs.assign(std::max<std::size_t>(static_cast<std::size_t>(nnz_), 1), 0);
         for (int sl = 0; sl < slices; ++sl) {
             const int width = (so[sl + 1] - so[sl]) / S_;
@@ -173,10 +203,13 @@ void InstanceData::build(const InstanceSource& s) {
                 for (int j = 0; j < width; ++j) {
                     const int slot = so[sl] + j * S_ + l;
                     if (j < len) {
-                        sc[slot] = ci[ro[r] + j];
-                        bs[ro[r] + j] = slot;
+                        const int k = ro[r] + j, c = ci[k];
+                        const bool lower = c < r;
+                        const int jj = lower ? upper_index(c, r) : k - up0[r];
+                        se[slot] = (c << 6) | jj;
+                        bs[k] = lower ? -1 - vslot(c, jj) : vslot(r, jj);
                     } else {
-                        sc[slot] = std::min(r, n_ - 1);
+                        se[slot] = std::min(r, n_ - 1) << 6;
                     }
                 }
             }
@@ -187,7 +220,8 @@ void InstanceData::build(const InstanceSource& s) {
     const auto fits = [](std::size_t count, std::size_t bytes_each) {
         return count * bytes_each < (std::size_t(1) << 32);
     };
-    if (!fits(static_cast<std::size_t>(n_), 36) || !fits(static_cast<std::size_t>(npad_), 36) ||
+    if (!fits(static_cast<std::size_t>(n_), 36) || !fits(static_cast<std::size_t>(nup_), 36) ||
+        !fits(static_cast<std::size_t>(npad_), 4) ||
         !fits(tris_.cpu_size(), sizeof(float) * CHX_TRI_REC) || !fits(dihs_.cpu_size(), sizeof(float) * CHX_DIH_REC) ||
         !fits(pins_.cpu_size(), sizeof(chx_pin)) || !fits(ent.size(), sizeof(unsigned)) ||
         !fits(stvks_.cpu_size(), sizeof(float) * CHX_STVK_REC))
@@ -197,7 +231,8 @@ void InstanceData::build(const InstanceSource& s) {
     upload(row_offsets_, bytes_);
     upload(col_indices_, bytes_);
     upload(slice_offsets_, bytes_);
-    upload(slice_cols_, bytes_);
+    upload(slice_entries_, bytes_);
+    upload(upper_offsets_, bytes_);
     upload(block_slots_, bytes_);
     upload(contrib_offsets_, bytes_);
     upload(contribs_, bytes_);
@@ -211,7 +246,7 @@ void InstanceData::build(const InstanceSource& s) {
     alloc(dih_rec_, dihs_.cpu_size() * CHX_DIH_REC, bytes_);
     alloc(stvk_rec_, stvks_.cpu_size() * CHX_STVK_REC, bytes_);
     alloc(diag_, 9 * n, bytes_);
-    alloc(values_, 9 * std::max<std::size_t>(static_cast<std::size_t>(npad_), 1), bytes_);
+    alloc(values_, 9 * std::max<std::size_t>(static_cast<std::size_t>(nup_), 1), bytes_);
     alloc(rhs_, 3 * n, bytes_);
     alloc(r_, 3 * n, bytes_);
     alloc(z_, 3 * n, bytes_);
@@ -241,7 +276,7 @@ void InstanceData::build(const InstanceSource& s) {
     }
     if (s.bf16_blocks) {
         alloc(diag_c_, 9 * n, bytes_);
-        alloc(values16_, 9 * std::max<std::size_t>(static_cast<std::size_t>(npad_), 1), bytes_);
+        alloc(values16_, 9 * std::max<std::size_t>(static_cast<std::size_t>(nup_), 1), bytes_);
     } else {
         diag_c_.release();
         values16_.release();
@@ -292,9 +327,11 @@ chx_instance InstanceData::descriptor(const InstanceSource& s) const {
     I.values16 = s.bf16_blocks ? values16_.gpu_data() : nullptr;
     I.windows = windows_.gpu_data();
     I.slice_offsets = slice_offsets_.gpu_data();
-    I.slice_cols = slice_cols_.gpu_data();
+    I.slice_entries = slice_entries_.gpu_data();
+    I.upper_offsets = upper_offsets_.gpu_data();
     I.block_slots = block_slots_.gpu_data();
     I.n_padded = npad_;
+    I.n_upper = nup_;
     I.n_rows = n_;
     I.nnz = nnz_;
     I.n_pins = static_cast<int>(pins_.cpu_size());

@@ -56,8 +56,9 @@ public:
     bool built() const noexcept { return built_; }
     int n_rows() const noexcept { return n_; }
     int nnz() const noexcept { return nnz_; }
-    int n_padded() const noexcept { return npad_; }
-    // Slot of CSR block k in the sliced layout (host copy).
+    int n_upper() const noexcept { return nup_; }
+    // Value slot of CSR block k, -1 - (partner's slot) for a lower block
+    // (host copy).
     const std::vector<int>& block_slots() const noexcept { return block_slots_.host(); }
     std::size_t device_bytes() const noexcept { return bytes_; }
     std::size_t contributions() const noexcept { return contribs_.cpu_size(); }
@@ -72,7 +73,7 @@ private:
     void pack_elements(const InstanceSource& s);
 
     bool built_ = false;
-    int n_ = 0, nnz_ = 0, npad_ = 0;
+    int n_ = 0, nnz_ = 0, npad_ = 0, nup_ = 0;
     std::size_t bytes_ = 0;
     DeviceArray<int> row_offsets_, col_indices_;
     DeviceArray<chx_pin> pins_;
@@ -86,7 +87,7 @@ private:
     DeviceArray<float> diag_c_;
     DeviceArray<uint16_t> values16_;
     DeviceArray<int> windows_;  // per CHX_STAGE_ROWS-row stage: {min column, max column}
-    DeviceArray<int> slice_offsets_, slice_cols_, block_slots_;  // sliced layout of the blocks
+    DeviceArray<int> slice_offsets_, slice_entries_, upper_offsets_, block_slots_;  // sliced symmetric layout
 };
 
 class BatchRunner {
