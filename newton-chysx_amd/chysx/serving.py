@@ -100,8 +100,14 @@ class GpuExecutor:
         if grid_blocks:
             self.batch.set_grid_blocks(grid_blocks)
         self.batch.set_lds_staging(lds_staging)
+        import queue
+
         self.stream = stream if stream is not None else torch.cuda.Stream()
-        self.prep_stream = torch.cuda.Stream()
+        # One preparation stream per worker, so set-ups wait only for
+        # themselves (never for the batch or for each other).
+        self.prep_streams = queue.SimpleQueue()
+        for _ in range(max(1, prepare_workers)):
+            self.prep_streams.put(torch.cuda.Stream())
         self.keep_results = keep_results
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=prepare_workers) if prepare_workers > 0 else None
 
@@ -110,8 +116,14 @@ class GpuExecutor:
 
         if hasattr(spec, "build"):  # a job spec built on demand
             spec = spec.build()
-        cloth = scenes.DeviceCloth(spec, lanes_per_row=self.lanes, stream=self.prep_stream)
-        cloth.sim.prepare(dt)
+        st = self.prep_streams.get()
+        try:
+            cloth = scenes.DeviceCloth(spec, lanes_per_row=self.lanes, stream=st)
+            # Set-up uploads and waits on the preparation stream only: the
+            # batch keeps stepping on its own stream meanwhile.
+            cloth.sim.prepare(dt, st.cuda_stream)
+        finally:
+            self.prep_streams.put(st)
         return cloth
 
     def begin(self, req: Request, dt: float):

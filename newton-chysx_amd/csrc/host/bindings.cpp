@@ -68,7 +68,8 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("set_external_buffers", &ClothSimulator::set_external_buffers, py::arg("pos_ptr"), py::arg("vel_ptr"),
              py::arg("particle_count"), py::arg("inv_mass_ptr") = 0, py::arg("force_ptr") = 0)
         .def("step", &ClothSimulator::step, py::arg("dt"), py::arg("cuda_stream") = 0, py::call_guard<py::gil_scoped_release>())
-        .def("prepare", &ClothSimulator::prepare, py::arg("dt"), py::call_guard<py::gil_scoped_release>(),
+        .def("prepare", &ClothSimulator::prepare, py::arg("dt"), py::arg("cuda_stream") = 0,
+             py::call_guard<py::gil_scoped_release>(),
              "Builds the pattern, scatter maps and device arrays ahead of the first step.")
         .def(
             "set_pins",
@@ -247,7 +248,8 @@ PYBIND11_MODULE(_chysx_native, m) {
             py::arg("cuda_stream") = 0)
         .def("tile_rows", &chysx::cloth::ClothBatch::tile_rows)
         .def("set_bf16_blocks", &chysx::cloth::ClothBatch::set_bf16_blocks, py::arg("enabled"))
-        .def("finalize", &chysx::cloth::ClothBatch::finalize, py::arg("dt") = 1.0f / 60.0f)
+        .def("finalize", &chysx::cloth::ClothBatch::finalize, py::arg("dt") = 1.0f / 60.0f, py::arg("cuda_stream") = 0,
+             py::call_guard<py::gil_scoped_release>())
         .def("remove_instance", &chysx::cloth::ClothBatch::remove_instance, py::arg("simulator"))
         .def("clear", &chysx::cloth::ClothBatch::clear)
         .def("lanes_per_row", &chysx::cloth::ClothBatch::lanes_per_row)

@@ -28,11 +28,11 @@ void ClothBatch::clear() noexcept {
     bound_ = false;
 }
 
-std::vector<chx_instance> ClothBatch::descriptors(float dt) {
+std::vector<chx_instance> ClothBatch::descriptors(float dt, std::uintptr_t cuda_stream) {
     std::vector<chx_instance> ds;
     ds.reserve(sims_.size());
     for (ClothSimulator* s : sims_) {
-        ds.push_back(s->prepare_instance(dt));
+        ds.push_back(s->prepare_instance(dt, cuda_stream));
         if (bf16_ && ds.back().n_rows > 0 && ds.back().diag_c == nullptr)
             throw std::invalid_argument(
                 "ClothBatch: BF16 mode needs every instance switched with set_bf16_blocks(true)");
@@ -40,23 +40,23 @@ std::vector<chx_instance> ClothBatch::descriptors(float dt) {
     return ds;
 }
 
-void ClothBatch::finalize(float dt) {
+void ClothBatch::finalize(float dt, std::uintptr_t cuda_stream) {
     if (sims_.empty()) throw std::runtime_error("ClothBatch::finalize: no instances");
-    runner_.bind_many(descriptors(dt));
+    runner_.bind_many(descriptors(dt, cuda_stream), cuda_stream);
     bound_ = true;
 }
 
 void ClothBatch::step(float dt, std::uintptr_t cuda_stream) {
     if (!(dt > 0.0f)) throw std::invalid_argument("ClothBatch::step: dt must be positive");
-    if (!bound_) finalize(dt);
-    else runner_.update_many(descriptors(dt), cuda_stream);
+    if (!bound_) finalize(dt, cuda_stream);
+    else runner_.update_many(descriptors(dt, cuda_stream), cuda_stream);
     runner_.step(cuda_stream);
 }
 
 void ClothBatch::assemble(float dt, std::uintptr_t cuda_stream) {
     if (!(dt > 0.0f)) throw std::invalid_argument("ClothBatch::assemble: dt must be positive");
-    if (!bound_) finalize(dt);
-    else runner_.update_many(descriptors(dt), cuda_stream);
+    if (!bound_) finalize(dt, cuda_stream);
+    else runner_.update_many(descriptors(dt, cuda_stream), cuda_stream);
     runner_.assemble(cuda_stream);
 }
 

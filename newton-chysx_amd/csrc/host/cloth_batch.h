@@ -37,7 +37,7 @@ public:
     void clear() noexcept;
     // Builds every instance's device arrays and binds the table (done
     // lazily by step() after membership changes).
-    void finalize(float dt = 1.0f / 60.0f);
+    void finalize(float dt = 1.0f / 60.0f, std::uintptr_t cuda_stream = 0);
     // One substep of every instance; asynchronous on the stream.
     void step(float dt, std::uintptr_t cuda_stream = 0);
     // The two halves of step(): element + gather assembly, then the
@@ -74,7 +74,7 @@ public:
     std::vector<double> last_relative_residuals(std::uintptr_t cuda_stream = 0);
 
 private:
-    std::vector<chx_instance> descriptors(float dt);
+    std::vector<chx_instance> descriptors(float dt, std::uintptr_t cuda_stream = 0);
 
     std::vector<ClothSimulator*> sims_;
     engine::BatchRunner runner_;

@@ -189,11 +189,11 @@ void ClothSimulator::set_bf16_blocks(bool on) {
 
 // cloth_simulator.cu:795-812: dx (the PCG warm start) is zeroed only when
 // its size changes.
-void ClothSimulator::resize_work_buffers(int n) {
+void ClothSimulator::resize_work_buffers(int n, std::uintptr_t stream) {
     if (dx_.gpu_size() != static_cast<std::size_t>(n)) {
         dx_.ensure_device(n);
-        dx_.zero_device();
-        check(chx_device_synchronize(), "cloth", "dx zero fill");
+        dx_.zero_device(stream);
+        check(stream ? chx_stream_synchronize(stream) : chx_device_synchronize(), "cloth", "dx zero fill");
     }
 }
 
@@ -214,14 +214,14 @@ void ClothSimulator::prepare_topology() {
     bending_.bind_hessian_layout(pattern_);
 }
 
-void ClothSimulator::ensure_hessian_topology() {
+void ClothSimulator::ensure_hessian_topology(std::uintptr_t stream) {
     const int N = buffers_.particle_count();
     if (N <= 0) return;
     prepare_topology();
-    inst_.build(source());
+    inst_.build(source(), stream);
     runner_.set_lanes_per_row(lanes_per_row_);
     runner_.set_bf16(bf16_);
-    runner_.bind_single(inst_.descriptor(source()));
+    runner_.bind_single(inst_.descriptor(source()), stream);
     packed_k_[0] = pins_.stiffness();
     packed_k_[1] = fem_stretch_.stiffness();
     packed_k_[2] = fem_shear_.stiffness();
@@ -259,13 +259,13 @@ engine::InstanceSource ClothSimulator::source() const {
 
 // Stiffness is read from the constraint sets at every launch in the
 // reference; re-pack the element arrays when one changed since packing.
-void ClothSimulator::check_elements_current() {
+void ClothSimulator::check_elements_current(std::uintptr_t stream) {
     const bool k_changed = packed_k_[0] != pins_.stiffness() || packed_k_[1] != fem_stretch_.stiffness() ||
                            packed_k_[2] != fem_shear_.stiffness() || packed_k_[3] != bending_.stiffness() ||
                            packed_stvk_[0] != fem_stvk_.mu() || packed_stvk_[1] != fem_stvk_.lambda() ||
                            packed_stvk_[2] != fem_stvk_.thickness();
     if (k_changed || pins_dirty_) {
-        inst_.update_elements(source());
+        inst_.update_elements(source(), stream);
         packed_k_[0] = pins_.stiffness();
         packed_k_[1] = fem_stretch_.stiffness();
         packed_k_[2] = fem_shear_.stiffness();
@@ -288,24 +288,24 @@ void ClothSimulator::step(float dt, std::uintptr_t cuda_stream) {
             "chysx::cloth::ClothSimulator::step: pos / vel spans must be set via set_external_buffers() before "
             "stepping.");
     if (!(dt > 0.0f)) throw std::invalid_argument("chysx::cloth::ClothSimulator::step: dt must be positive");
-    resize_work_buffers(n);
-    if (topology_dirty_ || H_num_block_rows_ != n) ensure_hessian_topology();
-    else check_elements_current();
+    resize_work_buffers(n, cuda_stream);
+    if (topology_dirty_ || H_num_block_rows_ != n) ensure_hessian_topology(cuda_stream);
+    else check_elements_current(cuda_stream);
     engine::InstanceSource s = source();
     s.dt = dt;
     runner_.update_single(inst_.descriptor(s));
     runner_.step(cuda_stream);
 }
 
-chx_instance ClothSimulator::prepare_instance(float dt) {
+chx_instance ClothSimulator::prepare_instance(float dt, std::uintptr_t cuda_stream) {
     const int n = buffers_.particle_count();
     if (n <= 0) throw std::runtime_error("ClothSimulator::prepare_instance: no particles bound");
     if (buffers_.pos.data() == nullptr || buffers_.vel.data() == nullptr)
         throw std::runtime_error("ClothSimulator::prepare_instance: pos / vel must be set via set_external_buffers()");
     if (!(dt > 0.0f)) throw std::invalid_argument("ClothSimulator::prepare_instance: dt must be positive");
-    resize_work_buffers(n);
-    if (topology_dirty_ || H_num_block_rows_ != n) ensure_hessian_topology();
-    else check_elements_current();
+    resize_work_buffers(n, cuda_stream);
+    if (topology_dirty_ || H_num_block_rows_ != n) ensure_hessian_topology(cuda_stream);
+    else check_elements_current(cuda_stream);
     engine::InstanceSource s = source();
     s.dt = dt;
     return inst_.descriptor(s);

@@ -160,18 +160,19 @@ public:
     // Builds (or refreshes) this cloth's device arrays without stepping and
     // returns the launch descriptor for substep `dt`; ClothBatch uses it to
     // advance this cloth as one instance of a batch.
-    chx_instance prepare_instance(float dt);
+    chx_instance prepare_instance(float dt, std::uintptr_t cuda_stream = 0);
     // Builds the Hessian pattern, scatter maps and device arrays now (the
     // first step would otherwise do it), so the set-up of a cloth can run
     // ahead of its first step, e.g. on another host thread.
-    void prepare(float dt) { (void)prepare_instance(dt); }
+    // With a stream, every set-up upload and wait is on that stream only.
+    void prepare(float dt, std::uintptr_t cuda_stream = 0) { (void)prepare_instance(dt, cuda_stream); }
     std::size_t device_bytes() const noexcept { return inst_.device_bytes() + dx_.gpu_size() * sizeof(math::Vec3f); }
 
 private:
-    void resize_work_buffers(int n);
-    void ensure_hessian_topology();
+    void resize_work_buffers(int n, std::uintptr_t stream = 0);
+    void ensure_hessian_topology(std::uintptr_t stream = 0);
     engine::InstanceSource source() const;
-    void check_elements_current();
+    void check_elements_current(std::uintptr_t stream = 0);
 
     ClothMaterial material_;
     ClothBuffers buffers_;

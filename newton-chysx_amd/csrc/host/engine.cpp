@@ -16,8 +16,8 @@ unsigned encode(unsigned kind, int a, int b, std::size_t id) {
     return (kind << 30) | (unsigned(a) << 28) | (unsigned(b) << 26) | static_cast<unsigned>(id);
 }
 
-template <class T> void upload(DeviceArray<T>& a, std::size_t& bytes) {
-    a.copy_to_device();
+template <class T> void upload(DeviceArray<T>& a, std::size_t& bytes, std::uintptr_t stream) {
+    a.copy_to_device(stream);
     bytes += a.gpu_size() * sizeof(T);
 }
 template <class T> void alloc(DeviceArray<T>& a, std::size_t n, std::size_t& bytes) {
@@ -26,6 +26,12 @@ template <class T> void alloc(DeviceArray<T>& a, std::size_t n, std::size_t& byt
 }
 
 int item_count(int n) { return (n + CHX_ITEM_SIZE - 1) / CHX_ITEM_SIZE; }
+
+// Waits for `stream` only, or for the whole device on the null stream.
+void sync_stream(std::uintptr_t stream, const char* what) {
+    if (stream == 0) check(chx_device_synchronize(), "engine", what);
+    else check(chx_stream_synchronize(stream), "engine", what);
+}
 
 }  // namespace
 
@@ -77,7 +83,7 @@ void InstanceData::pack_elements(const InstanceSource& s) {
     }
 }
 
-void InstanceData::build(const InstanceSource& s) {
+void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     if (!s.pattern || s.pattern->n != s.n) throw std::invalid_argument("chysx::engine: pattern missing or sized wrong");
     bytes_ = 0;
     n_ = s.n;
@@ -228,18 +234,18 @@ void InstanceData::build(const InstanceSource& s) {
         throw std::invalid_argument(
             "chysx::engine: a per-instance array would reach 4 GiB (at most 119,304,647 vertices or off-diagonal "
             "blocks per cloth); split the cloth into several instances");
-    upload(row_offsets_, bytes_);
-    upload(col_indices_, bytes_);
-    upload(slice_offsets_, bytes_);
-    upload(slice_entries_, bytes_);
-    upload(upper_offsets_, bytes_);
-    upload(block_slots_, bytes_);
-    upload(contrib_offsets_, bytes_);
-    upload(contribs_, bytes_);
-    upload(pins_, bytes_);
-    upload(tris_, bytes_);
-    upload(dihs_, bytes_);
-    upload(stvks_, bytes_);
+    upload(row_offsets_, bytes_, stream);
+    upload(col_indices_, bytes_, stream);
+    upload(slice_offsets_, bytes_, stream);
+    upload(slice_entries_, bytes_, stream);
+    upload(upper_offsets_, bytes_, stream);
+    upload(block_slots_, bytes_, stream);
+    upload(contrib_offsets_, bytes_, stream);
+    upload(contribs_, bytes_, stream);
+    upload(pins_, bytes_, stream);
+    upload(tris_, bytes_, stream);
+    upload(dihs_, bytes_, stream);
+    upload(stvks_, bytes_, stream);
     const std::size_t n = static_cast<std::size_t>(n_);
     alloc(pin_rec_, pins_.cpu_size() * CHX_PIN_REC, bytes_);
     alloc(tri_rec_, tris_.cpu_size() * CHX_TRI_REC, bytes_);
@@ -272,7 +278,7 @@ void InstanceData::build(const InstanceSource& s) {
             w[2 * st] = lo;
             w[2 * st + 1] = hi;
         }
-        upload(windows_, bytes_);
+        upload(windows_, bytes_, stream);
     }
     if (s.bf16_blocks) {
         alloc(diag_c_, 9 * n, bytes_);
@@ -284,15 +290,15 @@ void InstanceData::build(const InstanceSource& s) {
     built_ = true;
 }
 
-void InstanceData::update_elements(const InstanceSource& s) {
+void InstanceData::update_elements(const InstanceSource& s, std::uintptr_t stream) {
     const std::size_t nP = pins_.cpu_size(), nT = tris_.cpu_size(), nD = dihs_.cpu_size(), nV = stvks_.cpu_size();
     pack_elements(s);
     if (pins_.cpu_size() != nP || tris_.cpu_size() != nT || dihs_.cpu_size() != nD || stvks_.cpu_size() != nV)
         throw std::runtime_error("chysx::engine: element counts changed; rebuild the instance");
-    pins_.copy_to_device();
-    tris_.copy_to_device();
-    dihs_.copy_to_device();
-    stvks_.copy_to_device();
+    pins_.copy_to_device(stream);
+    tris_.copy_to_device(stream);
+    dihs_.copy_to_device(stream);
+    stvks_.copy_to_device(stream);
 }
 
 chx_instance InstanceData::descriptor(const InstanceSource& s) const {
@@ -360,22 +366,22 @@ void BatchRunner::set_lanes_per_row(int L) {
     desc_.lanes_per_row = L;
 }
 
-void BatchRunner::size_outputs(int n_instances, int n_tiles) {
+void BatchRunner::size_outputs(int n_instances, int n_tiles, std::uintptr_t stream) {
     const std::size_t I = static_cast<std::size_t>(std::max(n_instances, 1));
     if (inst_iters_.gpu_size() < I) {
         inst_iters_.ensure_device(I);
         inst_done_.ensure_device(I);
         inst_rr_.ensure_device(I);
         inst_bb_.ensure_device(I);
-        inst_iters_.zero_device();
-        inst_rr_.zero_device();
-        inst_bb_.zero_device();
+        inst_iters_.zero_device(stream);
+        inst_rr_.zero_device(stream);
+        inst_bb_.zero_device(stream);
     }
     const std::size_t P = static_cast<std::size_t>(CHX_TILE_PARTIALS) * std::max(n_tiles, 1);
     if (partials_.gpu_size() < P) partials_.ensure_device(P);
     if (sync_.gpu_size() == 0) {
         sync_.ensure_device(CHX_SYNC_WORDS);
-        sync_.zero_device();
+        sync_.zero_device(stream);
     }
     desc_.partials = partials_.gpu_data();
     desc_.inst_iters = inst_iters_.gpu_data();
@@ -429,7 +435,7 @@ void BatchRunner::set_tile_rows(int rows) {
     check(chx_device_synchronize(), "engine", "tile list rebuild");
 }
 
-void BatchRunner::build_tiles(std::vector<chx_instance>& ds) {
+void BatchRunner::build_tiles(std::vector<chx_instance>& ds, std::uintptr_t stream) {
     auto& ti = tiles_.host();
     ti.clear();
     for (std::size_t i = 0; i < ds.size(); ++i) {
@@ -441,15 +447,15 @@ void BatchRunner::build_tiles(std::vector<chx_instance>& ds) {
     }
     if (ti.size() / 4 > static_cast<std::size_t>(std::numeric_limits<int>::max()))
         throw std::runtime_error("chysx::engine: too many row tiles");
-    tiles_.copy_to_device();
+    tiles_.copy_to_device(stream);
     desc_.n_tiles = static_cast<int>(ti.size() / 4);
     desc_.tiles = tiles_.gpu_data();
     desc_.tile_rows = tile_rows_;
 }
 
-void BatchRunner::bind_single(const chx_instance& d) {
+void BatchRunner::bind_single(const chx_instance& d, std::uintptr_t stream) {
     std::vector<chx_instance> one{d};
-    build_tiles(one);
+    build_tiles(one, stream);
     desc_.n_instances = 1;
     desc_.n_elem_items = 0;
     desc_.n_slot_items = 0;
@@ -460,14 +466,14 @@ void BatchRunner::bind_single(const chx_instance& d) {
     desc_.inst = nullptr;
     desc_.elem_items = nullptr;
     desc_.slot_items = nullptr;
-    size_outputs(1, desc_.n_tiles);
-    check(chx_device_synchronize(), "engine", "bind");
+    size_outputs(1, desc_.n_tiles, stream);
+    sync_stream(stream, "bind");
 }
 
-void BatchRunner::bind_many(const std::vector<chx_instance>& ds) {
+void BatchRunner::bind_many(const std::vector<chx_instance>& ds, std::uintptr_t stream) {
     if (ds.empty()) throw std::invalid_argument("chysx::engine: empty batch");
     table_host_ = ds;
-    build_tiles(table_host_);
+    build_tiles(table_host_, stream);
     auto& ei = elem_items_.host();
     auto& si = slot_items_.host();
     ei.clear();
@@ -482,10 +488,10 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds) {
         const int slots = I.n_rows + I.nnz;
         for (int b = 0; b < slots; b += CHX_ITEM_SIZE) si.insert(si.end(), {inst, b, std::min(slots, b + CHX_ITEM_SIZE), 0});
     }
-    elem_items_.copy_to_device();
-    slot_items_.copy_to_device();
+    elem_items_.copy_to_device(stream);
+    slot_items_.copy_to_device(stream);
     table_.host() = table_host_;
-    table_.copy_to_device();
+    table_.copy_to_device(stream);
     desc_.n_instances = static_cast<int>(ds.size());
     desc_.n_elem_items = static_cast<int>(ei.size() / 4);
     desc_.n_slot_items = static_cast<int>(si.size() / 4);
@@ -496,8 +502,8 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds) {
     desc_.inst = table_.gpu_data();
     desc_.elem_items = elem_items_.gpu_data();
     desc_.slot_items = slot_items_.gpu_data();
-    size_outputs(desc_.n_instances, desc_.n_tiles);
-    check(chx_device_synchronize(), "engine", "bind");
+    size_outputs(desc_.n_instances, desc_.n_tiles, stream);
+    sync_stream(stream, "bind");
 }
 
 void BatchRunner::update_many(const std::vector<chx_instance>& ds, std::uintptr_t stream) {

@@ -48,9 +48,11 @@ public:
     InstanceData& operator=(const InstanceData&) = delete;
 
     // Builds every device array (slot tables must be bound to s.pattern).
-    void build(const InstanceSource& s);
+    // Uploads run on `stream`, which is synchronized (not the device), so
+    // an instance can be set up while other streams run batches.
+    void build(const InstanceSource& s, std::uintptr_t stream = 0);
     // Re-uploads the element arrays (stiffness or pin targets changed).
-    void update_elements(const InstanceSource& s);
+    void update_elements(const InstanceSource& s, std::uintptr_t stream = 0);
     // Descriptor for a launch: device arrays + the step scalars of `s`.
     chx_instance descriptor(const InstanceSource& s) const;
     bool built() const noexcept { return built_; }
@@ -97,9 +99,9 @@ public:
     BatchRunner& operator=(const BatchRunner&) = delete;
 
     // One instance: its descriptor travels in the kernel arguments.
-    void bind_single(const chx_instance& d);
+    void bind_single(const chx_instance& d, std::uintptr_t stream = 0);
     // Many instances: builds the work lists and uploads the table.
-    void bind_many(const std::vector<chx_instance>& ds);
+    void bind_many(const std::vector<chx_instance>& ds, std::uintptr_t stream = 0);
     // Per-step scalars / external pointers of the bound instances.
     void update_single(const chx_instance& d) {
         const int tb = desc_.single.tile_begin, te = desc_.single.tile_end;
@@ -150,9 +152,9 @@ public:
     void check_status(std::uintptr_t stream);
 
 private:
-    void size_outputs(int n_instances, int n_tiles);
+    void size_outputs(int n_instances, int n_tiles, std::uintptr_t stream = 0);
     // Tile list of the bound instances; sets their tile_begin / tile_end.
-    void build_tiles(std::vector<chx_instance>& ds);
+    void build_tiles(std::vector<chx_instance>& ds, std::uintptr_t stream = 0);
 
     chx_cloth_batch desc_{};
     int lanes_ = 8;
