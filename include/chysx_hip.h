@@ -72,6 +72,17 @@ typedef struct chx_stvk {
     int pad0, pad1;
 } chx_stvk;
 
+/* Static contact primitives (ChysX/src/collision/static_contact.h:82-110):
+ * a plane dot(n, x) + d = 0 with unit n pointing to the free side, and an
+ * oriented box (centre, half extents, unit frame columns ex, ey, ez). */
+typedef struct chx_plane { float nx, ny, nz, d; } chx_plane;
+typedef struct chx_box {
+    float cx, cy, cz;
+    float hx, hy, hz;
+    float ex[3], ey[3], ez[3];
+    float pad;
+} chx_box;
+
 /* Record sizes (floats) written per element by the element pass. */
 #define CHX_PIN_REC 4
 #define CHX_TRI_REC 24
@@ -154,6 +165,8 @@ typedef struct chx_instance {
     const int* slice_entries;        /* [n_padded] (col << 6) | jj of every entry slot       */
     const int* upper_offsets;        /* [ceil(n_rows/CHX_SLICE_ROWS)+1] first value slot     */
     const int* block_slots;          /* [nnz] value slot of CSR block k (-1 - slot: lower)   */
+    const chx_plane* planes;         /* static contact primitives, or NULL                   */
+    const chx_box* boxes;
     int n_rows, nnz, n_pins, n_tris, n_dihs;
     int tile_begin;                  /* this instance's row tiles in the batch tile list     */
     int tile_end;
@@ -164,6 +177,11 @@ typedef struct chx_instance {
     float damping;                   /* v *= exp(-damping*dt) when > 0                       */
     float gx, gy, gz;                /* gravity                                              */
     int max_iter;                    /* PCG iteration cap                                    */
+    int n_planes, n_boxes;           /* static contact: active when contact_stiffness > 0    */
+    float contact_thickness;         /* h: penalty on depth = h - signed distance > 0        */
+    float contact_stiffness;         /* k [N/m]                                              */
+    float contact_friction;          /* Coulomb mu (0: off)                                  */
+    float contact_friction_eps;      /* slip regularisation eps_u [m]                        */
     double tol;                      /* stop at ||r|| <= tol*||b||; <= 0: run max_iter       */
 } chx_instance;
 

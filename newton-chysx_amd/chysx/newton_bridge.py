@@ -21,9 +21,14 @@ the same construction order and step contract over torch tensors:
   from state_in when they differ, rebinds the pointers to state_out and
   steps on the caller's current torch stream.
 
-Features of the reference solver that are not on the implicit cloth step's
-path (self-collision, static / SDF / mesh-body contact, untangling,
-tetrahedral FEM, the VBD solver) raise ``NotImplementedError`` when enabled.
+``static_contact_enabled`` registers the model's world-static planes and
+boxes (``shape_type`` / ``shape_body`` / ``shape_transform`` /
+``shape_scale``, Newton's layout: a plane's normal is its local +Z, a box's
+frame columns are its rotation's columns) with the simulator's static
+contact set (solver_chysx.py:550-640).  Features of the reference solver
+that are not on the implicit cloth step's path (self-collision, SDF /
+mesh-body contact, untangling, tetrahedral FEM, the VBD solver) raise
+``NotImplementedError`` when enabled.
 """
 from __future__ import annotations
 
@@ -31,6 +36,21 @@ from dataclasses import dataclass
 from typing import Sequence
 
 import numpy as np
+
+
+# newton.GeoType values of the primitives the static contact set takes
+GEO_PLANE = 1
+GEO_BOX = 7
+
+
+def quat_to_matrix(q) -> np.ndarray:
+    """Rotation matrix of a unit quaternion (x, y, z, w)."""
+    x, y, z, w = (float(c) for c in q)
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)],
+    ])
 
 
 @dataclass
@@ -50,6 +70,16 @@ class ClothModel:
     particle_inv_mass: object | None
     tri_indices: np.ndarray          # (M, 3) int32
     gravity: tuple = (0.0, 0.0, -9.81)
+    # Newton's collision shapes (numpy): type (GEO_PLANE / GEO_BOX ...), owning
+    # body (-1: world-static), transform (px, py, pz, qx, qy, qz, qw), scale
+    shape_type: np.ndarray | None = None
+    shape_body: np.ndarray | None = None
+    shape_transform: np.ndarray | None = None
+    shape_scale: np.ndarray | None = None
+
+    @property
+    def shape_count(self) -> int:
+        return 0 if self.shape_type is None else int(len(self.shape_type))
 
     @property
     def particle_count(self) -> int:
@@ -103,12 +133,13 @@ class SolverChysX:
                  bending_stiffness: float = 0.0, pin_indices: Sequence[int] | None = None, pin_stiffness: float = 1.0e6,
                  pcg_iterations: int = 50, surface_density: float | None = None,
                  self_collision_enabled: bool = False, static_contact_enabled: bool = False,
+                 static_contact_thickness: float = 0.0, static_contact_stiffness: float = 1.0e4,
+                 static_contact_friction: float = 0.0, static_contact_friction_epsilon: float = 1.0e-4,
                  untangle_enabled: bool = False, tet_fem_enabled: bool = False, solver_type: str = "pcg",
                  **_unused):
         import chysx
 
         for flag, name in ((self_collision_enabled, "self_collision_enabled"),
-                           (static_contact_enabled, "static_contact_enabled"),
                            (untangle_enabled, "untangle_enabled"), (tet_fem_enabled, "tet_fem_enabled")):
             if flag:
                 raise NotImplementedError(f"SolverChysX: {name} is not part of this build (implicit cloth step only)")
@@ -157,6 +188,41 @@ class SolverChysX:
             q0 = model.particle_q.detach().cpu().numpy().reshape(-1, 3)
             self._sim.set_pins(idx, np.ascontiguousarray(q0[idx], np.float32), float(pin_stiffness))
             self._pin_indices_np = idx
+        if static_contact_enabled and n > 0:
+            if static_contact_thickness <= 0.0:
+                raise ValueError("SolverChysX(static_contact_enabled=True) requires static_contact_thickness > 0")
+            if static_contact_friction_epsilon <= 0.0:
+                raise ValueError("SolverChysX: static_contact_friction_epsilon must be positive")
+            self._sim.set_static_contact_thickness(float(static_contact_thickness))
+            self._sim.set_static_contact_stiffness(float(static_contact_stiffness))
+            self._sim.set_static_contact_friction(float(static_contact_friction))
+            self._sim.set_static_contact_friction_epsilon(float(static_contact_friction_epsilon))
+            self._register_static_shapes()
+
+    def _register_static_shapes(self) -> None:
+        """World-static planes and boxes of the model into the contact set:
+        a plane's normal is its transform's local +Z, d = -dot(n, position);
+        a box's frame columns are its rotation's columns, half extents its
+        scale."""
+        m = self.model
+        if m.shape_count == 0:
+            return
+        body = m.shape_body if m.shape_body is not None else np.full(m.shape_count, -1)
+        xf = np.asarray(m.shape_transform, np.float64).reshape(m.shape_count, 7)
+        scale = np.asarray(m.shape_scale, np.float64).reshape(m.shape_count, 3)
+        for s in range(m.shape_count):
+            if int(body[s]) != -1:
+                continue  # moving shapes are not static primitives
+            R = quat_to_matrix(xf[s, 3:7])
+            pos = xf[s, :3]
+            t = int(m.shape_type[s])
+            if t == GEO_PLANE:
+                nrm = R[:, 2]
+                self._sim.add_static_plane(n=nrm.astype(np.float32), d=float(-np.dot(nrm, pos)))
+            elif t == GEO_BOX:
+                self._sim.add_static_box(center=pos.astype(np.float32), half_ext=scale[s].astype(np.float32),
+                                         ex=R[:, 0].astype(np.float32), ey=R[:, 1].astype(np.float32),
+                                         ez=R[:, 2].astype(np.float32))
 
     @property
     def simulator(self):

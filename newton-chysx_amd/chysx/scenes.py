@@ -51,6 +51,9 @@ class ClothSpec:
     name: str = "cloth"
     pos_start: np.ndarray | None = None  # positions after set-up (None: = pos)
     stvk: tuple | None = None       # StVK membrane (mu [Pa], lambda [Pa], thickness [m]), or None
+    planes: tuple = ()              # static planes ((nx, ny, nz), d)
+    boxes: tuple = ()               # static boxes (center, half_ext, ex, ey, ez)
+    contact: tuple | None = None    # (thickness, stiffness, friction, friction_eps), or None
     meta: dict = field(default_factory=dict)
 
     @property
@@ -125,6 +128,18 @@ def plain_small_stvk(seed: int = 0, n: int = 32, E: float = 1.0e5, nu: float = 0
     spec.stvk = (mu, lam, THICKNESS)
     spec.name = f"grid{n}_stvk"
     return spec
+
+
+def drape(seed: int = 5, n: int = 32, friction: float = 0.0) -> ClothSpec:
+    """The 32x32 cloth (no pins) dropped 2 cm onto a static table: an
+    oriented box (top at y = 0.98, 0.5 x 0.5 m, turned 0.3 rad about y)
+    over a ground plane y = 0.5; contact thickness 5 mm, stiffness 1e4 N/m,
+    Coulomb friction `friction` (eps 1e-4 m)."""
+    spec = plain_small(seed=seed, n=n)
+    c, s_ = math.cos(0.3), math.sin(0.3)
+    box = ((0.5, 0.95, 0.5), (0.25, 0.03, 0.25), (c, 0.0, -s_), (0.0, 1.0, 0.0), (s_, 0.0, c))
+    return replace(spec, pins=np.zeros(0, np.int32), planes=(((0.0, 1.0, 0.0), -0.5),), boxes=(box,),
+                   contact=(5.0e-3, 1.0e4, friction, 1.0e-4), name=f"grid{n}_drape")
 
 
 def plain_large(seed: int = 1, n: int = 256) -> ClothSpec:
@@ -251,6 +266,16 @@ def setup_simulator(sim, spec: ClothSpec, pos_ptr: int, vel_ptr: int, inv_mass_p
         sim.build_bending_from_current_positions(stiffness=float(spec.k_bend), cuda_stream=stream)
     if len(spec.pins):
         sim.set_pins(spec.pins, spec.pos[spec.pins], float(spec.k_pin))
+    for n, d in spec.planes:
+        sim.add_static_plane(np.asarray(n, np.float32), float(d))
+    for c, h, ex, ey, ez in spec.boxes:
+        sim.add_static_box(*(np.asarray(a, np.float32) for a in (c, h, ex, ey, ez)))
+    if spec.contact is not None:
+        th, k, mu, eps = spec.contact
+        sim.set_static_contact_thickness(float(th))
+        sim.set_static_contact_stiffness(float(k))
+        sim.set_static_contact_friction(float(mu))
+        sim.set_static_contact_friction_epsilon(float(eps))
     return sim
 
 

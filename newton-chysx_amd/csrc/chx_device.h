@@ -222,6 +222,10 @@ struct Inst {
     gp<uint16_t> values16;
     gp<const int> windows;
     gp<const int> slice_offsets, slice_entries, upper_offsets, block_slots;
+    gp<const chx_plane> planes;
+    gp<const chx_box> boxes;
+    int n_planes, n_boxes;
+    float contact_thickness, contact_stiffness, contact_friction, contact_friction_eps;
     gp<const chx_stvk> stvks;
     gp<float> stvk_rec;
     int n_stvks, n_padded, n_upper;
@@ -266,6 +270,14 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.block_slots = glb(d->block_slots);
     I.n_padded = d->n_padded;
     I.n_upper = d->n_upper;
+    I.planes = glb(d->planes);
+    I.boxes = glb(d->boxes);
+    I.n_planes = d->n_planes;
+    I.n_boxes = d->n_boxes;
+    I.contact_thickness = d->contact_thickness;
+    I.contact_stiffness = d->contact_stiffness;
+    I.contact_friction = d->contact_friction;
+    I.contact_friction_eps = d->contact_friction_eps;
     I.stvks = glb(d->stvks);
     I.stvk_rec = glb(d->stvk_rec);
     I.n_stvks = d->n_stvks;

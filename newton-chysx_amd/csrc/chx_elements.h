@@ -203,4 +203,38 @@ __device__ __forceinline__ DihFrame dihedral_frame(f3 v0, f3 v1, f3 v2, f3 v3) {
     return F;
 }
 
+// ---- static plane / box contact ------------------------------------------
+// Oriented-box signed distance (positive outside, the nearest face's gap
+// inside) and outward unit normal (ChysX/src/collision/static_contact.cu:50-96).
+__device__ __forceinline__ float box_signed_dist(f3 p, f3 c, f3 h, f3 ex, f3 ey, f3 ez, f3& n) {
+    const f3 rel = sub(p, c);
+    const float qx = dot(rel, ex), qy = dot(rel, ey), qz = dot(rel, ez);
+    const float dx = fabsf(qx) - h.x, dy = fabsf(qy) - h.y, dz = fabsf(qz) - h.z;
+    if (dx <= 0.0f && dy <= 0.0f && dz <= 0.0f) {
+        if (dx >= dy && dx >= dz) {
+            n = (qx >= 0.0f) ? ex : mul(ex, -1.0f);
+            return dx;
+        }
+        if (dy >= dz) {
+            n = (qy >= 0.0f) ? ey : mul(ey, -1.0f);
+            return dy;
+        }
+        n = (qz >= 0.0f) ? ez : mul(ez, -1.0f);
+        return dz;
+    }
+    const float cx = dx > 0.0f ? dx : 0.0f, cy = dy > 0.0f ? dy : 0.0f, cz = dz > 0.0f ? dz : 0.0f;
+    const float dist = sqrtf(cx * cx + cy * cy + cz * cz);
+    const float inv = 1.0f / fmaxf(dist, 1.0e-30f);
+    const float sx = qx > 0.0f ? cx : -cx, sy = qy > 0.0f ? cy : -cy, sz = qz > 0.0f ? cz : -cz;
+    n = add(add(mul(ex, sx * inv), mul(ey, sy * inv)), mul(ez, sz * inv));
+    return dist;
+}
+
+// Slip ramp of the lagged Coulomb friction: 1/u past eps, linear from 2/eps
+// at rest to 1/eps at eps (ChysX/src/collision/friction.cuh:23-34).
+__device__ __forceinline__ float ipc_f1_over_x(float u, float eps) {
+    if (u > eps) return 1.0f / u;
+    return (-u / eps + 2.0f) / eps;
+}
+
 }  // namespace chx

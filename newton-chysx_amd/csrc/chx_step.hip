@@ -192,10 +192,95 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
         const f3 xt = mk(p.x + dt * v.x + dt * dt * I.gx, p.y + dt * v.y + dt * dt * I.gy,
                          p.z + dt * v.z + dt * dt * I.gz);
         const float mi = m * inv_dt2;
-        st3g(I.rhs, row, mk(mi * (xt.x - p.x) - grad.x, mi * (xt.y - p.y) - grad.y, mi * (xt.z - p.z) - grad.z));
+        // Static plane / box contact at x_n, after the elastic gradients
+        // (cloth_simulator.cu:997-1008; static_contact.cu:106-216): the
+        // deepest primitive's penalty gradient -k depth n and, with
+        // friction, -alpha u_t (the reference's sign) join the gradient.
+        f3 cn = mk(0.f, 0.f, 0.f), ut = mk(0.f, 0.f, 0.f);
+        float depth = 0.0f, unorm = 0.0f;
+        const bool contact = I.contact_stiffness > 0.0f && (I.n_planes > 0 || I.n_boxes > 0);
+        if (contact) {
+            const float th = I.contact_thickness;
+            for (int q = 0; q < I.n_planes; ++q) {
+                const gp<const float> pl = rec_at(gp<const float>(I.planes), unsigned(q), 4u);
+                const f3 nn = mk(pl[0], pl[1], pl[2]);
+                const float d = th - (dot(nn, p) + pl[3]);
+                if (d > depth) {
+                    depth = d;
+                    cn = nn;
+                }
+            }
+            for (int q = 0; q < I.n_boxes; ++q) {
+                const gp<const float> b = rec_at(gp<const float>(I.boxes), unsigned(q), 16u);
+                f3 nn;
+                const float d = th - box_signed_dist(p, mk(b[0], b[1], b[2]), mk(b[3], b[4], b[5]),
+                                                     mk(b[6], b[7], b[8]), mk(b[9], b[10], b[11]),
+                                                     mk(b[12], b[13], b[14]), nn);
+                if (d > depth) {
+                    depth = d;
+                    cn = nn;
+                }
+            }
+            if (depth > 0.0f) {
+                const f3 dxv = mul(v, dt);
+                ut = sub(dxv, mul(cn, dot(cn, dxv)));
+                unorm = sqrtf(ut.x * ut.x + ut.y * ut.y + ut.z * ut.z);
+                const float kd = -I.contact_stiffness * depth;
+                grad = mk(grad.x + kd * cn.x, grad.y + kd * cn.y, grad.z + kd * cn.z);
+                if (I.contact_friction > 0.0f) {
+                    const float fn = I.contact_stiffness * depth;
+                    const float alpha = I.contact_friction * fn * ipc_f1_over_x(unorm, I.contact_friction_eps);
+                    grad = mk(grad.x + -alpha * ut.x, grad.y + -alpha * ut.y, grad.z + -alpha * ut.z);
+                }
+            }
+        }
+        f3 b = mk(mi * (xt.x - p.x) - grad.x, mi * (xt.y - p.y) - grad.y, mi * (xt.z - p.z) - grad.z);
+        if (depth > 0.0f && I.contact_friction > 0.0f) {
+            // Coulomb-cone projection of the assembled force
+            // (cloth_simulator.cu:1073-1078; static_contact.cu:299-351)
+            const float push = I.contact_stiffness * depth;
+            const f3 F0 = mk(b.x - push * cn.x, b.y - push * cn.y, b.z - push * cn.z);
+            const float f0n = F0.x * cn.x + F0.y * cn.y + F0.z * cn.z;
+            const float fnm = fmaxf(0.0f, -f0n);
+            if (fnm > 0.0f) {
+                const f3 F0t = mk(F0.x - f0n * cn.x, F0.y - f0n * cn.y, F0.z - f0n * cn.z);
+                const float ft2 = F0t.x * F0t.x + F0t.y * F0t.y + F0t.z * F0t.z;
+                if (ft2 > 0.0f) {
+                    const float ft = sqrtf(ft2);
+                    const float cone = I.contact_friction * fnm;
+                    if (ft <= cone) {
+                        b = sub(b, F0t);
+                    } else {
+                        const float shrink = cone / ft;
+                        b = mk(b.x - shrink * F0t.x, b.y - shrink * F0t.y, b.z - shrink * F0t.z);
+                    }
+                }
+            }
+        }
+        st3g(I.rhs, row, b);
         h[0] += mi;
         h[4] += mi;
         h[8] += mi;
+        if (depth > 0.0f) {
+            // contact diagonal after the inertia (cloth_simulator.cu:1189-1193;
+            // static_contact.cu:236-296): k n n^T + alpha (I - n n^T)
+            const float k = I.contact_stiffness, nx = cn.x, ny = cn.y, nz = cn.z;
+            float a00 = k * nx * nx, a01 = k * nx * ny, a02 = k * nx * nz;
+            float a11 = k * ny * ny, a12 = k * ny * nz, a22 = k * nz * nz;
+            if (I.contact_friction > 0.0f) {
+                const float fn = k * depth;
+                const float alpha = I.contact_friction * fn * ipc_f1_over_x(unorm, I.contact_friction_eps);
+                a00 += alpha * (1.0f - nx * nx);
+                a01 += alpha * (-nx * ny);
+                a02 += alpha * (-nx * nz);
+                a11 += alpha * (1.0f - ny * ny);
+                a12 += alpha * (-ny * nz);
+                a22 += alpha * (1.0f - nz * nz);
+            }
+            h[0] += a00; h[1] += a01; h[2] += a02;
+            h[3] += a01; h[4] += a11; h[5] += a12;
+            h[6] += a02; h[7] += a12; h[8] += a22;
+        }
         // Planar blocks: component k of block row `row` at diag[k*n + row].
 #pragma unroll
         for (unsigned k = 0; k < 9; ++k) stx(I.diag, k * n + row, h[k]);
@@ -1140,6 +1225,8 @@ void validate_batch(const chx_cloth_batch* b) {
                 "chysx: sliced block layout smaller than the pattern, over 4 GiB or over 2^26 rows");
         if (b->bf16_blocks && I.n_rows > 0)
             require(I.diag_c && (I.values16 || I.nnz == 0), "chysx: BF16 mode needs the BF16 block copies");
+        require(I.n_planes >= 0 && I.n_boxes >= 0 && (I.n_planes == 0 || I.planes) && (I.n_boxes == 0 || I.boxes),
+                "chysx: static contact primitives missing");
     } else {
         require(b->tiles != nullptr || b->n_tiles == 0, "chysx: batch without a tile list");
         require(b->elem_items != nullptr || b->n_elem_items == 0, "chysx: batch without an element list");

@@ -85,6 +85,50 @@ PYBIND11_MODULE(_chysx_native, m) {
             },
             py::arg("indices"), py::arg("targets"), py::arg("stiffness") = 1.0e6f)
         .def("clear_pins", &ClothSimulator::clear_pins)
+        // ---- static plane / box contact (reference bindings.cpp:594-700) ----
+        .def(
+            "add_static_plane",
+            [](ClothSimulator& self, FArr n, float d) {
+                if (n.ndim() != 1 || n.shape(0) != 3)
+                    throw std::invalid_argument("ClothSimulator.add_static_plane: n must be a (3,) float32 vector");
+                chysx::collision::PlaneShape p;
+                p.n = chysx::math::Vec3f(n.at(0), n.at(1), n.at(2));
+                p.d = d;
+                self.add_static_plane(p);
+            },
+            py::arg("n"), py::arg("d"),
+            "Adds a static plane dot(n, x) + d == 0; n points to the side the cloth stays on.")
+        .def(
+            "add_static_box",
+            [](ClothSimulator& self, FArr center, FArr half_ext, FArr ex, FArr ey, FArr ez) {
+                auto v3 = [](FArr& a, const char* name) {
+                    if (a.ndim() != 1 || a.shape(0) != 3)
+                        throw std::invalid_argument(std::string("ClothSimulator.add_static_box: ") + name +
+                                                    " must be a (3,) float32 vector");
+                    return chysx::math::Vec3f(a.at(0), a.at(1), a.at(2));
+                };
+                chysx::collision::BoxShape b;
+                b.center = v3(center, "center");
+                b.half_ext = v3(half_ext, "half_ext");
+                b.ex = v3(ex, "ex");
+                b.ey = v3(ey, "ey");
+                b.ez = v3(ez, "ez");
+                self.add_static_box(b);
+            },
+            py::arg("center"), py::arg("half_ext"), py::arg("ex"), py::arg("ey"), py::arg("ez"),
+            "Adds a static oriented box (unit frame columns ex, ey, ez; half extents along them).")
+        .def("clear_static_shapes", &ClothSimulator::clear_static_shapes)
+        .def("set_static_contact_thickness", &ClothSimulator::set_static_contact_thickness, py::arg("thickness"))
+        .def("static_contact_thickness", &ClothSimulator::static_contact_thickness)
+        .def("set_static_contact_stiffness", &ClothSimulator::set_static_contact_stiffness, py::arg("stiffness"))
+        .def("static_contact_stiffness", &ClothSimulator::static_contact_stiffness)
+        .def("set_static_contact_friction", &ClothSimulator::set_static_contact_friction, py::arg("mu"))
+        .def("static_contact_friction", &ClothSimulator::static_contact_friction)
+        .def("set_static_contact_friction_epsilon", &ClothSimulator::set_static_contact_friction_epsilon,
+             py::arg("eps_u"))
+        .def("static_contact_friction_epsilon", &ClothSimulator::static_contact_friction_epsilon)
+        .def("static_plane_count", &ClothSimulator::static_plane_count)
+        .def("static_box_count", &ClothSimulator::static_box_count)
         .def("num_pins", [](const ClothSimulator& s) { return s.pins().size(); })
         .def(
             "update_pin_targets",

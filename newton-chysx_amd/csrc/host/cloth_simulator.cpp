@@ -232,6 +232,7 @@ void ClothSimulator::ensure_hessian_topology(std::uintptr_t stream) {
     H_num_block_rows_ = N;
     topology_dirty_ = false;
     pins_dirty_ = false;
+    shapes_dirty_ = false;  // build() uploaded the shape list
 }
 
 engine::InstanceSource ClothSimulator::source() const {
@@ -254,7 +255,39 @@ engine::InstanceSource ClothSimulator::source() const {
     s.max_iter = pcg_max_iterations_;
     s.tol = pcg_tolerance_;
     s.bf16_blocks = bf16_;
+    s.planes = &planes_;
+    s.boxes = &boxes_;
+    s.contact_thickness = contact_thickness_;
+    s.contact_stiffness = contact_stiffness_;
+    s.contact_friction = contact_friction_;
+    s.contact_friction_eps = contact_friction_eps_;
     return s;
+}
+
+void ClothSimulator::add_static_plane(const collision::PlaneShape& p) {
+    planes_.push_back(chx_plane{p.n.x, p.n.y, p.n.z, p.d});
+    shapes_dirty_ = true;
+}
+
+void ClothSimulator::add_static_box(const collision::BoxShape& b) {
+    chx_box c{};
+    c.cx = b.center.x; c.cy = b.center.y; c.cz = b.center.z;
+    c.hx = b.half_ext.x; c.hy = b.half_ext.y; c.hz = b.half_ext.z;
+    const math::Vec3f* ax[3] = {&b.ex, &b.ey, &b.ez};
+    float* out[3] = {c.ex, c.ey, c.ez};
+    for (int a = 0; a < 3; ++a) {
+        out[a][0] = ax[a]->x;
+        out[a][1] = ax[a]->y;
+        out[a][2] = ax[a]->z;
+    }
+    boxes_.push_back(c);
+    shapes_dirty_ = true;
+}
+
+void ClothSimulator::clear_static_shapes() {
+    planes_.clear();
+    boxes_.clear();
+    shapes_dirty_ = true;
 }
 
 // Stiffness is read from the constraint sets at every launch in the
@@ -274,6 +307,10 @@ void ClothSimulator::check_elements_current(std::uintptr_t stream) {
         packed_stvk_[1] = fem_stvk_.lambda();
         packed_stvk_[2] = fem_stvk_.thickness();
         pins_dirty_ = false;
+    }
+    if (shapes_dirty_) {
+        inst_.update_shapes(source(), stream);
+        shapes_dirty_ = false;
     }
 }
 

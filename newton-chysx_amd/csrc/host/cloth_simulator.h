@@ -7,9 +7,10 @@
 // and step(dt, stream) runs one implicit-Euler substep asynchronously on the
 // caller's stream.  The device work goes through the C ABI in chysx_hip.h.
 //
-// Out of scope here (the reference's contact, self-collision, tetrahedral
-// FEM and VBD branches are off by default there): the corresponding
-// methods are absent, and set_solver_type(VBD) makes step() throw.
+// Static plane / box contact is included; out of scope here (the
+// reference's self-collision, SDF / mesh contact, tetrahedral FEM and VBD
+// branches are off by default there): the corresponding methods are absent,
+// and set_solver_type(VBD) makes step() throw.
 #pragma once
 
 #include <cstdint>
@@ -21,6 +22,21 @@
 #include "topology.h"
 
 namespace chysx {
+namespace collision {
+
+// Static contact primitives (ChysX/src/collision/static_contact.h:82-110):
+// plane dot(n, x) + d = 0 with unit n pointing to the free side; oriented
+// box with centre, half extents and unit frame columns ex, ey, ez.
+struct PlaneShape {
+    math::Vec3f n;
+    float d = 0.0f;
+};
+struct BoxShape {
+    math::Vec3f center, half_ext, ex, ey, ez;
+};
+
+}  // namespace collision
+
 namespace cloth {
 
 // ChysX/src/cloth/cloth_material.h:18-43.
@@ -71,6 +87,25 @@ public:
     void update_pin_targets(const math::Vec3f* host_targets, int n_pins, std::uintptr_t cuda_stream = 0);
     constraint::PinConstraint& pins() noexcept { return pins_; }
     const constraint::PinConstraint& pins() const noexcept { return pins_; }
+
+    // ---- static plane / box contact (cloth_simulator.h:383-455) ------------
+    // Penalty k/2 max(0, h - d)^2 against the deepest primitive per particle
+    // at x_n, Gauss-Newton diagonal block, optional lagged Coulomb friction;
+    // diagonal and right-hand side only (no BSR change).  Active when the
+    // stiffness is positive and a primitive is registered.
+    void add_static_plane(const collision::PlaneShape& p);
+    void add_static_box(const collision::BoxShape& b);
+    void clear_static_shapes();
+    void set_static_contact_thickness(float t) noexcept { contact_thickness_ = t; }
+    float static_contact_thickness() const noexcept { return contact_thickness_; }
+    void set_static_contact_stiffness(float k) noexcept { contact_stiffness_ = k; }
+    float static_contact_stiffness() const noexcept { return contact_stiffness_; }
+    void set_static_contact_friction(float mu) noexcept { contact_friction_ = mu; }
+    float static_contact_friction() const noexcept { return contact_friction_; }
+    void set_static_contact_friction_epsilon(float eps_u) noexcept { contact_friction_eps_ = eps_u; }
+    float static_contact_friction_epsilon() const noexcept { return contact_friction_eps_; }
+    int static_plane_count() const noexcept { return static_cast<int>(planes_.size()); }
+    int static_box_count() const noexcept { return static_cast<int>(boxes_.size()); }
 
     // ---- mesh and elements --------------------------------------------------
     void set_mesh(const math::Vec3i* host_triangles, int n_triangles);
@@ -191,6 +226,11 @@ private:
     int H_num_block_rows_ = 0;
     bool topology_dirty_ = true;
     bool pins_dirty_ = false;
+    std::vector<chx_plane> planes_;
+    std::vector<chx_box> boxes_;
+    bool shapes_dirty_ = false;
+    float contact_thickness_ = 0.0f, contact_stiffness_ = 0.0f;
+    float contact_friction_ = 0.0f, contact_friction_eps_ = 1.0e-4f;
     float packed_k_[4] = {0.f, 0.f, 0.f, 0.f};
     int pcg_max_iterations_ = 50;
     double pcg_tolerance_ = 0.0;

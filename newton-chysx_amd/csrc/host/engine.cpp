@@ -280,6 +280,7 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
         }
         upload(windows_, bytes_, stream);
     }
+    update_shapes(s, stream);
     if (s.bf16_blocks) {
         alloc(diag_c_, 9 * n, bytes_);
         alloc(values16_, 9 * std::max<std::size_t>(static_cast<std::size_t>(nup_), 1), bytes_);
@@ -288,6 +289,13 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
         values16_.release();
     }
     built_ = true;
+}
+
+void InstanceData::update_shapes(const InstanceSource& s, std::uintptr_t stream) {
+    planes_.host() = s.planes ? *s.planes : std::vector<chx_plane>{};
+    boxes_.host() = s.boxes ? *s.boxes : std::vector<chx_box>{};
+    if (planes_.cpu_size() > 0) planes_.copy_to_device(stream);
+    if (boxes_.cpu_size() > 0) boxes_.copy_to_device(stream);
 }
 
 void InstanceData::update_elements(const InstanceSource& s, std::uintptr_t stream) {
@@ -338,6 +346,14 @@ chx_instance InstanceData::descriptor(const InstanceSource& s) const {
     I.block_slots = block_slots_.gpu_data();
     I.n_padded = npad_;
     I.n_upper = nup_;
+    I.planes = planes_.cpu_size() > 0 ? planes_.gpu_data() : nullptr;
+    I.boxes = boxes_.cpu_size() > 0 ? boxes_.gpu_data() : nullptr;
+    I.n_planes = static_cast<int>(planes_.cpu_size());
+    I.n_boxes = static_cast<int>(boxes_.cpu_size());
+    I.contact_thickness = s.contact_thickness;
+    I.contact_stiffness = s.contact_stiffness;
+    I.contact_friction = s.contact_friction;
+    I.contact_friction_eps = s.contact_friction_eps;
     I.n_rows = n_;
     I.nnz = nnz_;
     I.n_pins = static_cast<int>(pins_.cpu_size());

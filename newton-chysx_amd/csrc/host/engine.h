@@ -39,6 +39,11 @@ struct InstanceSource {
     int max_iter = 50;
     double tol = 0.0;
     bool bf16_blocks = false;  // keep BF16 copies of the Hessian blocks
+    // static contact primitives and parameters (active when stiffness > 0)
+    const std::vector<chx_plane>* planes = nullptr;
+    const std::vector<chx_box>* boxes = nullptr;
+    float contact_thickness = 0.0f, contact_stiffness = 0.0f;
+    float contact_friction = 0.0f, contact_friction_eps = 1.0e-4f;
 };
 
 class InstanceData {
@@ -53,6 +58,8 @@ public:
     void build(const InstanceSource& s, std::uintptr_t stream = 0);
     // Re-uploads the element arrays (stiffness or pin targets changed).
     void update_elements(const InstanceSource& s, std::uintptr_t stream = 0);
+    // Re-uploads the static contact primitives (the shape list changed).
+    void update_shapes(const InstanceSource& s, std::uintptr_t stream = 0);
     // Descriptor for a launch: device arrays + the step scalars of `s`.
     chx_instance descriptor(const InstanceSource& s) const;
     bool built() const noexcept { return built_; }
@@ -90,6 +97,8 @@ private:
     DeviceArray<uint16_t> values16_;
     DeviceArray<int> windows_;  // per CHX_STAGE_ROWS-row stage: {min column, max column}
     DeviceArray<int> slice_offsets_, slice_entries_, upper_offsets_, block_slots_;  // sliced symmetric layout
+    DeviceArray<chx_plane> planes_;
+    DeviceArray<chx_box> boxes_;
 };
 
 class BatchRunner {

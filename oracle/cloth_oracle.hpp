@@ -465,6 +465,84 @@ template <class T> struct StvkSet {
     std::vector<int> slots;
 };
 
+// ---------------------------------------------------------------------------
+// Static plane / box contact: penalty E = k/2 max(0, h - d)^2 on the deepest
+// primitive per particle, Gauss-Newton diagonal block, optional IPC-style
+// lagged Coulomb friction (ChysX/src/collision/static_contact.h:13-60,
+// static_contact.cu:41-366, friction.cuh:23-34).
+// ---------------------------------------------------------------------------
+template <class T> struct PlaneShape { V3<T> n; T d = T(0); };  // dot(n, x) + d = 0
+template <class T> struct BoxShape { V3<T> c, h, ex, ey, ez; };   // centre, half extents, frame columns
+
+// static_contact.cu:41-48
+template <class T> inline T plane_signed_dist(const V3<T>& p, const PlaneShape<T>& s, V3<T>& n) {
+    n = s.n;
+    return dot(s.n, p) + s.d;
+}
+
+// static_contact.cu:50-96: oriented-box signed distance (positive outside,
+// the face distance inside) and its outward normal.
+template <class T> inline T box_signed_dist(const V3<T>& p, const BoxShape<T>& b, V3<T>& n) {
+    const V3<T> rel = p - b.c;
+    const T qx = dot(rel, b.ex), qy = dot(rel, b.ey), qz = dot(rel, b.ez);
+    const T dx = std::fabs(qx) - b.h.x, dy = std::fabs(qy) - b.h.y, dz = std::fabs(qz) - b.h.z;
+    if (dx <= T(0) && dy <= T(0) && dz <= T(0)) {
+        if (dx >= dy && dx >= dz) {
+            n = (qx >= T(0)) ? b.ex : b.ex * T(-1);
+            return dx;
+        }
+        if (dy >= dz) {
+            n = (qy >= T(0)) ? b.ey : b.ey * T(-1);
+            return dy;
+        }
+        n = (qz >= T(0)) ? b.ez : b.ez * T(-1);
+        return dz;
+    }
+    const T cx = dx > T(0) ? dx : T(0), cy = dy > T(0) ? dy : T(0), cz = dz > T(0) ? dz : T(0);
+    const T dist = std::sqrt(cx * cx + cy * cy + cz * cz);
+    const T inv = T(1) / std::max(dist, T(1.0e-30f));
+    const T sx = qx > T(0) ? cx : -cx, sy = qy > T(0) ? cy : -cy, sz = qz > T(0) ? cz : -cz;
+    n = b.ex * (sx * inv) + b.ey * (sy * inv) + b.ez * (sz * inv);
+    return dist;
+}
+
+// friction.cuh:23-34: 1/u past eps, linear ramp 2/eps -> 1/eps below.
+template <class T> inline T ipc_f1_over_x(T u, T eps) {
+    if (u > eps) return T(1) / u;
+    return (-u / eps + T(2)) / eps;
+}
+
+template <class T> struct StaticContact {
+    std::vector<PlaneShape<T>> planes;
+    std::vector<BoxShape<T>> boxes;
+    T thickness = T(0), stiffness = T(0), friction = T(0), eps = T(1.0e-4f);
+    bool active() const { return stiffness > T(0) && (!planes.empty() || !boxes.empty()); }
+    // static_contact.cu:106-160: deepest penetration (planes, then boxes;
+    // a later primitive wins only when strictly deeper) and the lagged
+    // tangential slip u_t = (v dt) - n (n . v dt).
+    void detect(const V3<T>& x, const V3<T>& v, T dt, V3<T>& n, T& depth, V3<T>& ut, T& unorm) const {
+        depth = T(0);
+        n = V3<T>();
+        for (const auto& pl : planes) {
+            V3<T> nn;
+            const T d = thickness - plane_signed_dist(x, pl, nn);
+            if (d > depth) { depth = d; n = nn; }
+        }
+        for (const auto& b : boxes) {
+            V3<T> nn;
+            const T d = thickness - box_signed_dist(x, b, nn);
+            if (d > depth) { depth = d; n = nn; }
+        }
+        ut = V3<T>();
+        unorm = T(0);
+        if (depth > T(0) && dt > T(0)) {
+            const V3<T> dxv = v * dt;
+            ut = dxv - n * dot(n, dxv);
+            unorm = std::sqrt(ut.x * ut.x + ut.y * ut.y + ut.z * ut.z);
+        }
+    }
+};
+
 template <class T> struct Sim {
     int n = 0;
     std::vector<V3<T>> pos, vel;
@@ -484,6 +562,7 @@ template <class T> struct Sim {
     std::vector<V3<T>> pin_target;
     T k_pin = T(1.0e6f);
     std::vector<int> pin_slots;
+    StaticContact<T> contact;
 
     bool topology_dirty = true;
     int topo_n = 0;
@@ -812,6 +891,30 @@ template <class T> struct Sim {
         // gradient at x_n and rhs (:914-1061, :127-142)
         std::vector<V3<T>> grad;
         gradient(x_n, grad);
+        // static contact at x_n after the elastic gradients (:997-1008;
+        // static_contact.cu:175-216): g += -k depth n, and with friction
+        // g += -alpha u_t (the reference's sign: the force side gets +alpha u_t)
+        const bool contact_on = contact.active();
+        std::vector<V3<T>> c_n, c_ut;
+        std::vector<T> c_depth, c_unorm;
+        if (contact_on) {
+            c_n.resize(n); c_ut.resize(n); c_depth.resize(n); c_unorm.resize(n);
+            for (int i = 0; i < n; ++i) {
+                contact.detect(x_n[i], vel[i], dt, c_n[i], c_depth[i], c_ut[i], c_unorm[i]);
+                if (c_depth[i] <= T(0)) continue;
+                const T kd = -contact.stiffness * c_depth[i];
+                grad[i].x += kd * c_n[i].x;
+                grad[i].y += kd * c_n[i].y;
+                grad[i].z += kd * c_n[i].z;
+                if (contact.friction > T(0)) {
+                    const T fn = contact.stiffness * c_depth[i];
+                    const T alpha = contact.friction * fn * ipc_f1_over_x(c_unorm[i], contact.eps);
+                    grad[i].x += -alpha * c_ut[i].x;
+                    grad[i].y += -alpha * c_ut[i].y;
+                    grad[i].z += -alpha * c_ut[i].z;
+                }
+            }
+        }
         const T inv_dt2 = T(1) / (dt * dt);
         rhs.resize(n);
         for (int i = 0; i < n; ++i) {
@@ -820,12 +923,59 @@ template <class T> struct Sim {
                            m * (x_t[i].y - x_n[i].y) - grad[i].y,
                            m * (x_t[i].z - x_n[i].z) - grad[i].z);
         }
+        // Coulomb-cone projection of the assembled rhs (:1073-1078;
+        // static_contact.cu:299-351)
+        if (contact_on && contact.friction > T(0)) {
+            for (int i = 0; i < n; ++i) {
+                if (c_depth[i] <= T(0)) continue;
+                const V3<T> cn = c_n[i];
+                const T push = contact.stiffness * c_depth[i];
+                const V3<T> F0(rhs[i].x - push * cn.x, rhs[i].y - push * cn.y, rhs[i].z - push * cn.z);
+                const T f0n = F0.x * cn.x + F0.y * cn.y + F0.z * cn.z;
+                const T fnm = std::max(T(0), -f0n);
+                if (fnm <= T(0)) continue;
+                const V3<T> F0t(F0.x - f0n * cn.x, F0.y - f0n * cn.y, F0.z - f0n * cn.z);
+                const T ft2 = F0t.x * F0t.x + F0t.y * F0t.y + F0t.z * F0t.z;
+                if (ft2 <= T(0)) continue;
+                const T ft = std::sqrt(ft2);
+                const T cone = contact.friction * fnm;
+                if (ft <= cone) {
+                    rhs[i].x -= F0t.x; rhs[i].y -= F0t.y; rhs[i].z -= F0t.z;
+                } else {
+                    const T shrink = cone / ft;
+                    rhs[i].x -= shrink * F0t.x; rhs[i].y -= shrink * F0t.y; rhs[i].z -= shrink * F0t.z;
+                }
+            }
+        }
         if (topology_dirty || topo_n != n) ensure_topology();
         // Hessian at x_n + inertia diagonal (:1129-1146, :199-210)
         hessian(x_n);
         for (int i = 0; i < n; ++i) {
             const T d = mass[i] * inv_dt2;
             diag[i].m[0] += d; diag[i].m[4] += d; diag[i].m[8] += d;
+        }
+        // static contact diagonal after the inertia (:1189-1193;
+        // static_contact.cu:236-296): k n n^T + alpha (I - n n^T)
+        if (contact_on) {
+            for (int i = 0; i < n; ++i) {
+                if (c_depth[i] <= T(0)) continue;
+                const T k = contact.stiffness, nx = c_n[i].x, ny = c_n[i].y, nz = c_n[i].z;
+                T a00 = k * nx * nx, a01 = k * nx * ny, a02 = k * nx * nz;
+                T a11 = k * ny * ny, a12 = k * ny * nz, a22 = k * nz * nz;
+                if (contact.friction > T(0)) {
+                    const T alpha = contact.friction * (k * c_depth[i]) * ipc_f1_over_x(c_unorm[i], contact.eps);
+                    a00 += alpha * (T(1) - nx * nx);
+                    a01 += alpha * (-nx * ny);
+                    a02 += alpha * (-nx * nz);
+                    a11 += alpha * (T(1) - ny * ny);
+                    a12 += alpha * (-ny * nz);
+                    a22 += alpha * (T(1) - nz * nz);
+                }
+                M3<T>& A = diag[i];
+                A.m[0] += a00; A.m[1] += a01; A.m[2] += a02;
+                A.m[3] += a01; A.m[4] += a11; A.m[5] += a12;
+                A.m[6] += a02; A.m[7] += a12; A.m[8] += a22;
+            }
         }
         const int it = pcg(rhs, dx);
         // finalize (:222-244)

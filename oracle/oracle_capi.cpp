@@ -120,6 +120,24 @@ int orc_stvk_element(int precision, const double* x, const double* dinv, double 
     });
 }
 
+// ---- static contact primitives (pinning tests) --------------------------------
+// box: centre, half extents, ex, ey, ez (15 doubles).  Returns the signed
+// distance in *d and the outward normal in n[3].
+int orc_box_sdf(const double* p, const double* box, double* d, double* n) {
+    return guarded([&] {
+        BoxShape<double> b;
+        b.c = V3<double>(box[0], box[1], box[2]);
+        b.h = V3<double>(box[3], box[4], box[5]);
+        b.ex = V3<double>(box[6], box[7], box[8]);
+        b.ey = V3<double>(box[9], box[10], box[11]);
+        b.ez = V3<double>(box[12], box[13], box[14]);
+        V3<double> nn;
+        *d = box_signed_dist(V3<double>(p[0], p[1], p[2]), b, nn);
+        n[0] = nn.x; n[1] = nn.y; n[2] = nn.z;
+    });
+}
+double orc_ipc_f1_over_x(double u, double eps) { return ipc_f1_over_x(u, eps); }
+
 // ---- simulator ---------------------------------------------------------------
 void* orc_sim_create(int precision) {
     auto* h = new Handle;
@@ -172,6 +190,42 @@ int orc_sim_build_bending(void* hv, double k) {
 int orc_sim_set_pins(void* hv, const int* idx, const double* targets, int n, double k) {
     auto* h = static_cast<Handle*>(hv);
     return guarded([&] { DISPATCH(h, { using T = decltype(S.damping); S.set_pins(idx, targets, n, T(float(k))); }); });
+}
+int orc_sim_add_plane(void* hv, const double* nrm, double d) {
+    auto* h = static_cast<Handle*>(hv);
+    return guarded([&] {
+        DISPATCH(h, {
+            using T = decltype(S.damping);
+            PlaneShape<T> pl;
+            pl.n = V3<T>(T(float(nrm[0])), T(float(nrm[1])), T(float(nrm[2])));
+            pl.d = T(float(d));
+            S.contact.planes.push_back(pl);
+        });
+    });
+}
+int orc_sim_add_box(void* hv, const double* box) {
+    auto* h = static_cast<Handle*>(hv);
+    return guarded([&] {
+        DISPATCH(h, {
+            using T = decltype(S.damping);
+            auto v = [&](int o) { return V3<T>(T(float(box[o])), T(float(box[o + 1])), T(float(box[o + 2]))); };
+            BoxShape<T> b;
+            b.c = v(0); b.h = v(3); b.ex = v(6); b.ey = v(9); b.ez = v(12);
+            S.contact.boxes.push_back(b);
+        });
+    });
+}
+int orc_sim_set_contact(void* hv, double thickness, double stiffness, double friction, double eps) {
+    auto* h = static_cast<Handle*>(hv);
+    return guarded([&] {
+        DISPATCH(h, {
+            using T = decltype(S.damping);
+            S.contact.thickness = T(float(thickness));
+            S.contact.stiffness = T(float(stiffness));
+            S.contact.friction = T(float(friction));
+            S.contact.eps = T(float(eps));
+        });
+    });
 }
 int orc_sim_set_pcg(void* hv, int max_iter, double tol) {
     auto* h = static_cast<Handle*>(hv);

@@ -46,7 +46,7 @@ def lib():
             "orc_sim_last_stats", "orc_sim_sizes", "orc_sim_get_pattern", "orc_sim_get_slots",
             "orc_sim_get_dihedrals", "orc_sim_get_edges", "orc_sim_get_rest_shape", "orc_sim_get_system",
             "orc_sim_set_dx", "orc_sim_energy", "orc_sim_gradient", "orc_sim_hessian", "orc_sim_build_stvk",
-            "orc_stvk_element",
+            "orc_stvk_element", "orc_box_sdf", "orc_sim_add_plane", "orc_sim_add_box", "orc_sim_set_contact",
         ):
             getattr(L, name).restype = _I
         L.orc_sim_step.argtypes = [_P, _D]
@@ -59,6 +59,12 @@ def lib():
         L.orc_sim_set_material.argtypes = [_P, _D, _D, _D, _D]
         L.orc_sim_set_pcg.argtypes = [_P, _I, _D]
         L.orc_sim_run.argtypes = [_P, _D, _I, _P, _P]
+        L.orc_box_sdf.argtypes = [_P, _P, _P, _P]
+        L.orc_ipc_f1_over_x.restype = _D
+        L.orc_ipc_f1_over_x.argtypes = [_D, _D]
+        L.orc_sim_add_plane.argtypes = [_P, _P, _D]
+        L.orc_sim_add_box.argtypes = [_P, _P]
+        L.orc_sim_set_contact.argtypes = [_P, _D, _D, _D, _D]
         _lib = L
     return _lib
 
@@ -147,6 +153,18 @@ class OracleSim:
         i = _i32(idx)
         t = _f64(targets).reshape(-1, 3)
         _check(self.L.orc_sim_set_pins(self.h, _ptr(i), _ptr(t), _I(len(i)), _D(k)))
+
+    # static plane / box contact (ChysX/src/collision/static_contact.h)
+    def add_static_plane(self, n, d):
+        nn = _f64(n).reshape(3)
+        _check(self.L.orc_sim_add_plane(self.h, _ptr(nn), _D(d)))
+
+    def add_static_box(self, center, half_ext, ex, ey, ez):
+        b = np.concatenate([_f64(a).reshape(3) for a in (center, half_ext, ex, ey, ez)])
+        _check(self.L.orc_sim_add_box(self.h, _ptr(b)))
+
+    def set_static_contact(self, thickness, stiffness, friction=0.0, friction_epsilon=1.0e-4):
+        _check(self.L.orc_sim_set_contact(self.h, thickness, stiffness, friction, friction_epsilon))
 
     def set_pcg(self, max_iter=50, tol=0.0):
         _check(self.L.orc_sim_set_pcg(self.h, max_iter, tol))
@@ -296,3 +314,17 @@ def stvk_element(x, dinv, muA, lamA, exact=False, precision=64):
     _check(lib().orc_stvk_element(int(precision), _ptr(x), _ptr(dinv), float(muA), float(lamA), int(bool(exact)),
                                   ctypes.byref(E), _ptr(g), _ptr(H)))
     return E.value, g.reshape(3, 3), H.reshape(3, 3, 3, 3)
+
+
+def box_sdf(p, center, half_ext, ex=(1.0, 0.0, 0.0), ey=(0.0, 1.0, 0.0), ez=(0.0, 0.0, 1.0)):
+    """(signed distance, outward normal) of point p to an oriented box (FP64)."""
+    b = np.concatenate([_f64(a).reshape(3) for a in (center, half_ext, ex, ey, ez)])
+    pp = _f64(p).reshape(3)
+    d = np.zeros(1)
+    n = np.zeros(3)
+    _check(lib().orc_box_sdf(_ptr(pp), _ptr(b), _ptr(d), _ptr(n)))
+    return float(d[0]), n
+
+
+def ipc_f1_over_x(u, eps):
+    return float(lib().orc_ipc_f1_over_x(float(u), float(eps)))

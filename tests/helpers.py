@@ -25,6 +25,12 @@ def oracle_from_spec(spec, precision=32, pcg=None):
         o.build_bending(spec.k_bend)
     if len(spec.pins):
         o.set_pins(spec.pins, spec.pos[spec.pins].astype(np.float64), spec.k_pin)
+    for n_, d_ in getattr(spec, "planes", ()) or ():
+        o.add_static_plane(n_, d_)
+    for box in getattr(spec, "boxes", ()) or ():
+        o.add_static_box(*box)
+    if getattr(spec, "contact", None) is not None:
+        o.set_static_contact(*spec.contact)
     if spec.pos_start is not None:
         _, v = o.state()
         o.set_state(spec.pos_start.astype(np.float64), v, o.inv_mass())

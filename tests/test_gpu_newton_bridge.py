@@ -71,3 +71,37 @@ def test_unsupported_features_raise(gpu):
         SolverChysX(model, solver_type="vbd")
     with pytest.raises(ValueError):
         SolverChysX(model, pin_indices=[100])
+
+
+def test_table_drop_example(gpu):
+    """example_chysx_cloth_drop.py's scene without self-collision (not part
+    of this build): a 21x21, 1 m cloth dropped 0.5 m onto a 0.6 m table over
+    a ground plane through static_contact_enabled; the example's test_final
+    checks after 2 s, plus: the centre rests on the table top and the
+    overhanging edges hang below it."""
+    from chysx.newton_bridge import GEO_BOX, GEO_PLANE, SolverChysX, cloth_grid_model
+
+    dim, size = 21, 1.0
+    cell = size / (dim - 1)
+    top = 0.5
+    model = cloth_grid_model(pos=(-0.5 * size, -0.5 * size, top + 0.5), dim_x=dim - 1, dim_y=dim - 1, cell_x=cell,
+                             cell_y=cell, mass=0.0)
+    model.shape_type = np.array([GEO_BOX, GEO_PLANE])
+    model.shape_body = np.array([-1, -1])
+    model.shape_transform = np.array([[0.0, 0.0, top - 0.025, 0, 0, 0, 1], [0, 0, 0, 0, 0, 0, 1]], np.float64)
+    model.shape_scale = np.array([[0.3, 0.3, 0.025], [1.0, 1.0, 1.0]])
+    thickness = 0.5 * cell
+    solver = SolverChysX(model, damping=0.05, fem_stretch_stiffness=5.0e2, fem_shear_stiffness=5.0e2,
+                         bending_stiffness=1.0e-4, pcg_iterations=30, surface_density=0.3,
+                         static_contact_enabled=True, static_contact_thickness=thickness,
+                         static_contact_stiffness=1.0e4)
+    assert solver.simulator.static_box_count() == 1 and solver.simulator.static_plane_count() == 1
+    q, qd = _run(solver, model, 200, 1.0 / 100)
+    assert np.isfinite(q).all() and np.isfinite(qd).all()
+    assert np.abs(q).max() <= 5.0
+    assert q[:, 2].min() >= -1.5 * thickness
+    assert np.linalg.norm(qd, axis=1).max() <= 25.0
+    centre = (dim // 2) * dim + dim // 2
+    assert top - 1.5 * thickness <= q[centre, 2] <= top + 1.5 * thickness
+    corners = [0, dim - 1, dim * (dim - 1), dim * dim - 1]
+    assert np.all(q[corners, 2] < top - 0.05)
