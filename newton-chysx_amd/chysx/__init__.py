@@ -30,7 +30,8 @@ ClothMaterial = _chysx_native.ClothMaterial
 ClothSimulator = _chysx_native.ClothSimulator
 ClothBatch = _chysx_native.ClothBatch
 topology = _chysx_native.topology
+io = _chysx_native.io
 
 __version__ = "0.1.0"
 
-__all__ = ["ClothBatch", "ClothMaterial", "ClothSimulator", "topology", "__version__"]
+__all__ = ["ClothBatch", "ClothMaterial", "ClothSimulator", "io", "topology", "__version__"]

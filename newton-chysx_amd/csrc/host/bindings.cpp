@@ -10,6 +10,7 @@
 #include <stdexcept>
 
 #include "cloth_batch.h"
+#include "mesh_io.h"
 #include "cloth_simulator.h"
 #include "sparse_solver.h"
 #include "topology.h"
@@ -306,6 +307,100 @@ PYBIND11_MODULE(_chysx_native, m) {
              py::arg("cuda_stream") = 0);
 
     // ---- host topology (no device needed) ------------------------------------
+    // ---- frame writers (ChysX/src/io/obj_io.h, bgeo_writer.h) -------------
+    py::module_ io = m.def_submodule("io", "OBJ and Houdini .bgeo frame writers of the headless exporter");
+    auto tri_check = [](const FArr& pos, const IArr& tri, const char* fn) {
+        if (pos.ndim() != 2 || pos.shape(1) != 3) throw std::invalid_argument(std::string(fn) + ": positions must be (N, 3)");
+        if (tri.ndim() != 2 || tri.shape(1) != 3) throw std::invalid_argument(std::string(fn) + ": triangles must be (T, 3)");
+        const int n = static_cast<int>(pos.shape(0));
+        const int* t = tri.data();
+        for (py::ssize_t k = 0; k < tri.size(); ++k)
+            if (t[k] < 0 || t[k] >= n) throw std::invalid_argument(std::string(fn) + ": triangle index out of range");
+    };
+    io.def(
+        "load_obj",
+        [](const std::string& path) {
+            chysx::io::ObjMesh mesh;
+            if (!chysx::io::load_obj(path, mesh)) throw std::runtime_error("load_obj: cannot open " + path);
+            py::array_t<float> p({static_cast<py::ssize_t>(mesh.positions.size() / 3), static_cast<py::ssize_t>(3)});
+            py::array_t<int> t({static_cast<py::ssize_t>(mesh.triangles.size() / 3), static_cast<py::ssize_t>(3)});
+            if (!mesh.positions.empty()) std::memcpy(p.mutable_data(), mesh.positions.data(), mesh.positions.size() * 4);
+            if (!mesh.triangles.empty()) std::memcpy(t.mutable_data(), mesh.triangles.data(), mesh.triangles.size() * 4);
+            return py::make_tuple(p, t);
+        },
+        py::arg("path"), "(positions (N, 3) float32, triangles (T, 3) int32, 0-based)");
+    io.def(
+        "save_obj",
+        [tri_check](const std::string& path, FArr positions, IArr triangles) {
+            tri_check(positions, triangles, "save_obj");
+            if (!chysx::io::save_obj(path, positions.data(), static_cast<int>(positions.shape(0)), triangles.data(),
+                                     static_cast<int>(triangles.shape(0))))
+                throw std::runtime_error("save_obj: cannot write " + path);
+        },
+        py::arg("path"), py::arg("positions"), py::arg("triangles"));
+    io.def(
+        "write_bgeo",
+        [tri_check](const std::string& path, FArr positions, IArr triangles, py::object point_colors,
+                    py::object face_colors) {
+            tri_check(positions, triangles, "write_bgeo");
+            FArr pc, fc;
+            const float* pcp = nullptr;
+            const float* fcp = nullptr;
+            if (!point_colors.is_none()) {
+                pc = point_colors.cast<FArr>();
+                if (pc.ndim() != 2 || pc.shape(0) != positions.shape(0) || pc.shape(1) != 3)
+                    throw std::invalid_argument("write_bgeo: point_colors must be (N, 3)");
+                pcp = pc.data();
+            }
+            if (!face_colors.is_none()) {
+                fc = face_colors.cast<FArr>();
+                if (fc.ndim() != 2 || fc.shape(0) != triangles.shape(0) || fc.shape(1) != 3)
+                    throw std::invalid_argument("write_bgeo: face_colors must be (T, 3)");
+                fcp = fc.data();
+            }
+            if (!chysx::io::BgeoWriter::write(path, positions.data(), static_cast<int>(positions.shape(0)),
+                                              triangles.data(), static_cast<int>(triangles.shape(0)), pcp, fcp))
+                throw std::runtime_error("write_bgeo: cannot write " + path);
+        },
+        py::arg("path"), py::arg("positions"), py::arg("triangles"), py::arg("point_colors") = py::none(),
+        py::arg("face_colors") = py::none());
+    io.def(
+        "write_bgeo_multi",
+        [tri_check](const std::string& path, py::list pieces) {
+            std::vector<FArr> keep_p;
+            std::vector<IArr> keep_t;
+            std::vector<chysx::io::BgeoMeshPiece> ps;
+            for (py::handle h : pieces) {
+                py::tuple t = h.cast<py::tuple>();
+                if (t.size() != 3) throw std::invalid_argument("write_bgeo_multi: pieces are (positions, triangles, rgb)");
+                keep_p.push_back(t[0].cast<FArr>());
+                keep_t.push_back(t[1].cast<IArr>());
+                tri_check(keep_p.back(), keep_t.back(), "write_bgeo_multi");
+                auto rgb = t[2].cast<std::vector<float>>();
+                if (rgb.size() != 3) throw std::invalid_argument("write_bgeo_multi: colour must have 3 components");
+                ps.push_back({keep_p.back().data(), static_cast<int>(keep_p.back().shape(0)), keep_t.back().data(),
+                              static_cast<int>(keep_t.back().shape(0)), rgb[0], rgb[1], rgb[2]});
+            }
+            if (!chysx::io::BgeoWriter::write_multi(path, ps))
+                throw std::runtime_error("write_bgeo_multi: cannot write " + path);
+        },
+        py::arg("path"), py::arg("pieces"));
+    io.def(
+        "box_mesh",
+        [](FArr center, FArr half_ext) {
+            if (center.size() != 3 || half_ext.size() != 3) throw std::invalid_argument("box_mesh: (3,) vectors");
+            std::vector<float> p;
+            std::vector<int> t;
+            chysx::io::generate_box_mesh(center.data()[0], center.data()[1], center.data()[2], half_ext.data()[0],
+                                         half_ext.data()[1], half_ext.data()[2], p, t);
+            py::array_t<float> pa({static_cast<py::ssize_t>(8), static_cast<py::ssize_t>(3)});
+            py::array_t<int> ta({static_cast<py::ssize_t>(12), static_cast<py::ssize_t>(3)});
+            std::memcpy(pa.mutable_data(), p.data(), p.size() * 4);
+            std::memcpy(ta.mutable_data(), t.data(), t.size() * 4);
+            return py::make_tuple(pa, ta);
+        },
+        py::arg("center"), py::arg("half_ext"));
+
     py::module_ topo = m.def_submodule("topology", "Host topology of the cloth step (bit-exact integer work)");
     topo.def(
         "build_edges",

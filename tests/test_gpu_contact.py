@@ -65,3 +65,25 @@ def test_contact_api_surface(gpu):
     # the cloth now falls freely: no vertex is pushed up
     p, v = dc.state()
     assert np.all(v[:, 1] <= 0.0)
+
+
+def test_headless_export(gpu, tmp_path):
+    """chysx.export writes each frame as a merged .bgeo (cloth + table box)
+    and per-piece OBJ; the .bgeo's cloth points are the GPU state's bits."""
+    from chysx import export, scenes
+    from test_mesh_io import _attribute, _decode, _pairs
+
+    spec = scenes.drape()
+    paths = export.export_scene(spec, str(tmp_path), frames=3)
+    assert len(paths) == 3 * 3
+    g = _decode(str(tmp_path / "frame_0002.bgeo"))
+    assert g["pointcount"] == spec.n + 8 and g["primitivecount"] == len(spec.tris) + 12
+    P = _attribute(_pairs(g["attributes"])["pointattributes"][0])[2]["rawpagedata"].reshape(-1, 3)
+    ref = scenes.DeviceCloth(spec)
+    for _ in range(30):  # 3 frames of 1/60 s at dt = 1/600
+        ref.step()
+    assert np.array_equal(P[:spec.n], ref.state()[0].astype(np.float32))
+    import chysx
+
+    p_obj, t_obj = chysx.io.load_obj(str(tmp_path / "piece1_0002.obj"))
+    assert p_obj.shape == (8, 3) and t_obj.shape == (12, 3)
