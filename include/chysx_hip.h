@@ -83,6 +83,18 @@ typedef struct chx_box {
     float pad;
 } chx_box;
 
+typedef struct chx_contact_op {
+    const int* pairs;                /* [max_contacts][4] particle indices               */
+    const float* weights;            /* [max_contacts][8]: w0..w3, nx, ny, nz, depth     */
+    const int* count;                /* device int: active contacts (clamped to          */
+                                     /* max_contacts), or NULL for none                  */
+    const float* slips;              /* [max_contacts][4] (ux, uy, uz, |u_t|), or NULL   */
+    int max_contacts;
+    float stiffness;
+    float friction_mu;               /* 0: normal block only                             */
+    float friction_eps;              /* slip regularisation eps_u [m]                    */
+} chx_contact_op;
+
 /* Record sizes (floats) written per element by the element pass. */
 #define CHX_PIN_REC 4
 #define CHX_TRI_REC 24
@@ -220,6 +232,10 @@ typedef struct chx_cloth_batch {
     double* inst_bb;                 /* [n_instances] ||b||^2                         */
     int* inst_done;                  /* [n_instances] scratch                         */
     unsigned* sync;                  /* [CHX_SYNC_WORDS] grid barrier + control words */
+    chx_contact_op contact;          /* contact sidecar of a row-major solve (pairs     */
+                                     /* NULL: none); its per-row index below           */
+    const int* contact_rows;         /* [n_rows+1] first index entry of each row        */
+    const int* contact_entries;      /* 4 c + slot, ascending per row                   */
     unsigned long long* trace;       /* optional solver trace, or NULL: entry e =      */
                                      /* {100 MHz real-time stamp, active instances} at */
                                      /* barrier e (0: after set-up, 2k+1 / 2k+2: after */
@@ -304,6 +320,34 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
                        const float* values, const float* b, float* x, int max_iterations, double tol,
                        int lanes_per_row, void* workspace, size_t workspace_bytes, int* iterations,
                        double* rr, double* bb, uintptr_t stream);
+
+/* ---- contact SpMV sidecar ----------------------------------------------------
+ * A COO operator added to the block-CSR system the PCG iterates against,
+ * replacing collision::ContactSpMVOp / bake_contact_diag / apply_contact_spmv
+ * (ChysX/src/collision/contact_spmv.h:59-135, contact_spmv.cu:46-268) and the
+ * contact argument of PCGSolver::solve (pcg_solver.h:126-131).  Contact c
+ * couples particles pairs[c][0..3] with weights w0..w3, normal n and depth:
+ * H_c = k n n^T + a_c (I - n n^T), a_c = mu k depth f1(|u_t|) (friction, when
+ * mu > 0 and slips are given), C[i][j] = sum_c w_i w_j H_c.  The bake adds the
+ * same-slot terms to the diagonal blocks, the SpMV the cross-slot terms.  Each
+ * call gathers per row from an inverted contact index built on the device in
+ * fixed order: no float atomics, results reproducible run to run. */
+
+/* Bytes of device workspace the contact calls need (inverted index). */
+int chx_contact_workspace_size(int n_rows, int max_contacts, size_t* bytes);
+/* diag (row-major 3x3 blocks, [n_rows][9]) += alpha * same-slot contact terms. */
+int chx_contact_bake_diag(int n_rows, float* diag, const chx_contact_op* op, float alpha, void* workspace,
+                          size_t workspace_bytes, uintptr_t stream);
+/* y += alpha * cross-slot contact terms * x (AoS float3 [n_rows*3]). */
+int chx_contact_spmv(int n_rows, const chx_contact_op* op, const float* x, float* y, float alpha, void* workspace,
+                     size_t workspace_bytes, uintptr_t stream);
+/* chx_pcg_solve_bsr3 on (A + C_off): the contact cross-slot terms join the
+ * residual and every SpMV (the caller bakes the same-slot terms into diag). */
+int chx_pcg_solve_bsr3_contact(int n_rows, const float* diag, const int* row_offsets, const int* col_indices,
+                               const float* values, const float* b, float* x, int max_iterations, double tol,
+                               int lanes_per_row, void* workspace, size_t workspace_bytes,
+                               const chx_contact_op* op, void* contact_workspace, size_t contact_workspace_bytes,
+                               int* iterations, double* rr, double* bb, uintptr_t stream);
 
 /* ---- the implicit cloth substep -------------------------------------------- */
 /* One implicit-Euler substep of every instance of a packed batch:

@@ -41,6 +41,11 @@ _SIGS = {
     "chx_spmv_bsr3": (_I, [_I, _P, _P, _P, _P, _P, _P, _F, _F, _I, _U]),
     "chx_pcg_workspace_size": (_I, [_I, _P]),
     "chx_pcg_solve_bsr3": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _D, _I, _P, _S, _P, _P, _P, _U]),
+    "chx_contact_workspace_size": (_I, [_I, _I, _P]),
+    "chx_contact_bake_diag": (_I, [_I, _P, _P, _F, _P, _S, _U]),
+    "chx_contact_spmv": (_I, [_I, _P, _P, _P, _F, _P, _S, _U]),
+    "chx_pcg_solve_bsr3_contact": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _D, _I, _P, _S, _P, _P, _S, _P, _P, _P,
+                                        _U]),
     "chx_cloth_step": (_I, [_P, _U]),
     "chx_cloth_assemble": (_I, [_P, _U]),
     "chx_cloth_solve": (_I, [_P, _I, _U]),
@@ -108,6 +113,44 @@ def pcg_solve_bsr3(n, diag, row_offsets, col_indices, values, b, x, max_iteratio
     check(lib().chx_pcg_solve_bsr3(n, diag, row_offsets, col_indices, values, b, x, max_iterations, tol,
                                    lanes_per_row, workspace, workspace_bytes, *args, stream), "pcg_solve_bsr3")
     return (it.value, rr.value, bb.value) if sync else None
+
+
+class ContactOp(ctypes.Structure):
+    """chx_contact_op: device pointers of the contact sidecar (pairs [m][4]
+    int32, weights [m][8] float32, count one device int32, slips [m][4]
+    float32 or 0) and its scalars."""
+    _fields_ = [("pairs", _P), ("weights", _P), ("count", _P), ("slips", _P), ("max_contacts", _I),
+                ("stiffness", _F), ("friction_mu", _F), ("friction_eps", _F)]
+
+
+def contact_workspace_size(n, max_contacts) -> int:
+    out = ctypes.c_size_t()
+    check(lib().chx_contact_workspace_size(n, max_contacts, ctypes.byref(out)), "contact_workspace_size")
+    return out.value
+
+
+def contact_bake_diag(n, diag, op: ContactOp, alpha, workspace, workspace_bytes, stream=0):
+    """diag ([n][9] row-major blocks) += alpha * same-slot contact terms."""
+    check(lib().chx_contact_bake_diag(n, diag, ctypes.byref(op), alpha, workspace, workspace_bytes, stream),
+          "contact_bake_diag")
+
+
+def contact_spmv(n, op: ContactOp, x, y, alpha, workspace, workspace_bytes, stream=0):
+    """y += alpha * cross-slot contact terms * x."""
+    check(lib().chx_contact_spmv(n, ctypes.byref(op), x, y, alpha, workspace, workspace_bytes, stream),
+          "contact_spmv")
+
+
+def pcg_solve_bsr3_contact(n, diag, row_offsets, col_indices, values, b, x, max_iterations, tol, workspace,
+                           workspace_bytes, op: ContactOp, contact_workspace, contact_workspace_bytes,
+                           lanes_per_row=8, stream=0):
+    """Block-Jacobi PCG on (A + C_off); returns (iterations, rr, bb)."""
+    it, rr, bb = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    check(lib().chx_pcg_solve_bsr3_contact(n, diag, row_offsets, col_indices, values, b, x, max_iterations, tol,
+                                           lanes_per_row, workspace, workspace_bytes, ctypes.byref(op),
+                                           contact_workspace, contact_workspace_bytes, ctypes.byref(it),
+                                           ctypes.byref(rr), ctypes.byref(bb), stream), "pcg_solve_bsr3_contact")
+    return it.value, rr.value, bb.value
 
 
 def device_info():

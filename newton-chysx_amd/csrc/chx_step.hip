@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <mutex>
 
+#include "chx_contact.h"
 #include "chx_device.h"
 #include "chx_elements.h"
 #include "chx_internal.h"
@@ -837,12 +838,29 @@ __device__ __forceinline__ Slot instance_slot(const Inst& I, int k, double rr, d
 // the FP32 matrix), M^-1 = inverse(D) (setup only), z = M^-1 r, and p = z
 // when pdst is given (setup); tile partials <r,z>, <r,r> (and <b,b> at
 // setup).  M^-1 is stored planar.
+// fin(row, A x, x_row) of a tile SpMV, with the row's contact cross-slot
+// terms added to A x when a row-major solve carries a contact sidecar
+// (pcg_solver.cu:159-160, 180-181: every A x of the iteration is (A + C) x).
+// src(j) is the SpMV's source vector read from global memory.
+template <bool SL, class Src, class Fin>
+__device__ __forceinline__ auto plus_contact(const chx_cloth_batch& B, const Inst& I, Src src, Fin fin) {
+    return [&B, &I, src, fin](unsigned row, f3 Ax, f3 xr) {
+        if constexpr (!SL) {
+            if (B.contact_rows != nullptr)
+                Ax = add(Ax, contact_row(B.contact, B.contact_rows, B.contact_entries, row, unsigned(I.n_rows), 1.0f,
+                                         src));
+        }
+        fin(row, Ax, xr);
+    };
+}
+
 template <int L, bool ST, bool SL, class MatR>
-__device__ __forceinline__ void residual_tile(const Inst& I, MatR mat, int rb, int re, bool setup, gp<float> pdst,
-                                              PcgLds<ST>& S, double (&acc)[3]) {
+__device__ __forceinline__ void residual_tile(const chx_cloth_batch& B, const Inst& I, MatR mat, int rb, int re,
+                                              bool setup, gp<float> pdst, PcgLds<ST>& S, double (&acc)[3]) {
     const gp<const float> xv = I.x;
     const unsigned n = unsigned(I.n_rows);
-    tile_spmv<L, ST, SL>(I, mat, rb, re, S, [&](unsigned j) { return ld3g(xv, j); }, [&](unsigned row, f3 Ax, f3) {
+    auto xs = [&](unsigned j) { return ld3g(xv, j); };
+    tile_spmv<L, ST, SL>(I, mat, rb, re, S, xs, plus_contact<SL>(B, I, xs, [&](unsigned row, f3 Ax, f3) {
         const f3 b = ld3g(I.rhs, row);
         const f3 r = sub(b, Ax);
         float mi[9];
@@ -863,7 +881,7 @@ __device__ __forceinline__ void residual_tile(const Inst& I, MatR mat, int rb, i
         acc[0] += dot64(r, z);
         acc[1] += dot64(r, r);
         acc[2] += dot64(b, b);
-    });
+    }));
 }
 
 // Matrix variants of the PCG kernel: the view of the inner SpMV (A) and of
@@ -930,7 +948,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         if (threadIdx.x == 0 && t == I.tile_begin)
             __hip_atomic_store(&B.inst_done[inst], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         double acc[3] = {0.0, 0.0, 0.0};
-        residual_tile<L, ST, SL>(I, PcgMats<V>::r(I), rb, re, true, I.p0, S, acc);
+        residual_tile<L, ST, SL>(B, I, PcgMats<V>::r(I), rb, re, true, I.p0, S, acc);
         block_sum(acc, S.red);
         if (threadIdx.x == 0) {
             const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
@@ -985,7 +1003,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                     // True residual r = b - A32 x, z = M^-1 r and their sums
                     // become the parity cur^1 partials, marked true.
                     double acc[3] = {0.0, 0.0, 0.0};
-                    residual_tile<L, ST, SL>(I, PcgMats<V>::r(I), rb, re, false, gp<float>(nullptr), S, acc);
+                    residual_tile<L, ST, SL>(B, I, PcgMats<V>::r(I), rb, re, false, gp<float>(nullptr), S, acc);
                     block_sum(acc, S.red);
                     if (threadIdx.x == 0) {
                         const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
@@ -1004,7 +1022,8 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                     acc[0] += dot64(pr, Ap);
                 };
                 if (first) {
-                    tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(pk, jj); }, fin);
+                    auto pj = [&](unsigned jj) { return ld3g(pk, jj); };
+                    tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, pj, plus_contact<SL>(B, I, pj, fin));
                 } else if (BF16 && true_res) {
                     // restart after a refinement slot: p_k = z_k
                     tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(z, jj); }, fin);
@@ -1014,7 +1033,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                         const f3 zj = ld3g(z, jj), pp = ld3g(pprev, jj);
                         return mk(fmaf(beta, pp.x, zj.x), fmaf(beta, pp.y, zj.y), fmaf(beta, pp.z, zj.z));
                     };
-                    tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, pj, fin);
+                    tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, pj, plus_contact<SL>(B, I, pj, fin));
                 }
                 block_sum(acc, S.red);
                 if (threadIdx.x == 0) rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS))[P_SIG] = acc[0];
@@ -1329,11 +1348,19 @@ int chx_pcg_workspace_size(int n_rows, size_t* bytes) {
     });
 }
 
-int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, const int* col_indices,
-                       const float* values, const float* b, float* x, int max_iterations, double tol, int lanes_per_row,
-                       void* workspace, size_t workspace_bytes, int* iterations, double* rr, double* bb,
-                       uintptr_t stream) {
-    return guarded([&] {
+}  // extern "C"
+
+namespace chx {
+namespace {
+
+// The standalone row-major PCG (pcg_solver.cu:134-215), with the contact
+// sidecar's cross-slot terms in every A x when `op` is active
+// (ContactSpMVOp::active, contact_spmv.h:81-83).
+void solve_bsr3(int n_rows, const float* diag, const int* row_offsets, const int* col_indices, const float* values,
+                const float* b, float* x, int max_iterations, double tol, int lanes_per_row, void* workspace,
+                size_t workspace_bytes, const chx_contact_op* op, void* contact_workspace,
+                size_t contact_workspace_bytes, int* iterations, double* rr, double* bb, uintptr_t stream) {
+    {
         require(n_rows >= 0, "pcg: negative row count");
         require(max_iterations >= 0, "pcg: negative iteration cap");
         if (n_rows == 0) {
@@ -1387,6 +1414,13 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
         B.sync = reinterpret_cast<unsigned*>(take(SYNC_WORDS * sizeof(unsigned)));
         const hipStream_t s = as_stream(stream);
         pcg_fn(lanes_per_row, 0);  // validates L
+        if (op != nullptr && op->max_contacts > 0 && op->stiffness > 0.0f && op->pairs != nullptr &&
+            op->count != nullptr) {
+            const ContactIndex ix = build_contact_index(n_rows, *op, contact_workspace, contact_workspace_bytes, s);
+            B.contact = *op;
+            B.contact_rows = ix.rows;
+            B.contact_entries = ix.entries;
+        }
         reset_sync_kernel<<<1, kBlock, 0, s>>>(B.sync);
         hip_check(hipGetLastError(), "reset_sync_kernel launch");
         launch_pcg(B, 0, s, true);
@@ -1405,6 +1439,34 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
             if (rr) *rr = r2;
             if (bb) *bb = b2;
         }
+    }
+}
+
+}  // namespace
+}  // namespace chx
+
+extern "C" {
+
+int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, const int* col_indices,
+                       const float* values, const float* b, float* x, int max_iterations, double tol, int lanes_per_row,
+                       void* workspace, size_t workspace_bytes, int* iterations, double* rr, double* bb,
+                       uintptr_t stream) {
+    return guarded([&] {
+        solve_bsr3(n_rows, diag, row_offsets, col_indices, values, b, x, max_iterations, tol, lanes_per_row,
+                   workspace, workspace_bytes, nullptr, nullptr, 0, iterations, rr, bb, stream);
+    });
+}
+
+int chx_pcg_solve_bsr3_contact(int n_rows, const float* diag, const int* row_offsets, const int* col_indices,
+                               const float* values, const float* b, float* x, int max_iterations, double tol,
+                               int lanes_per_row, void* workspace, size_t workspace_bytes,
+                               const chx_contact_op* op, void* contact_workspace, size_t contact_workspace_bytes,
+                               int* iterations, double* rr, double* bb, uintptr_t stream) {
+    return guarded([&] {
+        require(op != nullptr, "chx_pcg_solve_bsr3_contact: null contact operator");
+        solve_bsr3(n_rows, diag, row_offsets, col_indices, values, b, x, max_iterations, tol, lanes_per_row,
+                   workspace, workspace_bytes, op, contact_workspace, contact_workspace_bytes, iterations, rr, bb,
+                   stream);
     });
 }
 

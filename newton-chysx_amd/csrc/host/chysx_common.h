@@ -36,13 +36,15 @@ using Vec2i = Vec2<int>;
 using Vec3i = Vec3<int>;
 using Vec4i = Vec4<int>;
 using Vec3f = Vec3<float>;
+using Vec4f = Vec4<float>;
 struct Mat3f {
     float data[9]{};
 };
 struct Mat2f {
     float data[4]{};
 };
-static_assert(sizeof(Vec3f) == 12 && sizeof(Vec3i) == 12 && sizeof(Vec4i) == 16 && sizeof(Mat3f) == 36,
+static_assert(sizeof(Vec3f) == 12 && sizeof(Vec3i) == 12 && sizeof(Vec4i) == 16 && sizeof(Vec4f) == 16 &&
+                  sizeof(Mat3f) == 36,
               "layout must match the reference");
 }  // namespace math
 

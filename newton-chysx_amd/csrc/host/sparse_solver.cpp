@@ -2,6 +2,7 @@
 
 #include <cmath>
 #include <stdexcept>
+#include <string>
 
 namespace chysx {
 namespace sparse {
@@ -38,6 +39,72 @@ void spmv(const BlockCSR3& A, DeviceSpan<math::Vec3f> x, DeviceSpan<math::Vec3f>
 
 }  // namespace sparse
 
+namespace collision {
+namespace {
+
+chx_contact_op to_abi(const ContactSpMVOp& op) {
+    chx_contact_op c{};
+    c.pairs = reinterpret_cast<const int*>(op.pairs);
+    c.weights = reinterpret_cast<const float*>(op.weights);
+    c.count = op.count_dev;
+    c.slips = op.friction_active() ? reinterpret_cast<const float*>(op.slips) : nullptr;
+    c.max_contacts = op.active() ? op.max_contacts : 0;
+    c.stiffness = op.stiffness;
+    c.friction_mu = op.friction_active() ? op.friction_mu : 0.0f;
+    c.friction_eps = op.friction_epsilon;
+    return c;
+}
+
+// Runs `call(workspace, bytes)` with the op's workspace, or a temporary one
+// freed after `cuda_stream` drains.
+template <class Call>
+void with_workspace(const ContactSpMVOp& op, int n, std::uintptr_t cuda_stream, const char* what, Call call) {
+    const std::size_t need = op.workspace_bytes(n);
+    if (op.workspace != nullptr) {
+        if (op.workspace_size < need) throw std::invalid_argument(std::string("chysx::collision::") + what +
+                                                                  ": ContactSpMVOp workspace too small");
+        call(op.workspace, op.workspace_size);
+        return;
+    }
+    DeviceArray<unsigned char> tmp;
+    tmp.ensure_device(need);
+    call(static_cast<void*>(tmp.gpu_data()), tmp.gpu_size());
+    check(chx_stream_synchronize(cuda_stream), "collision", what);
+}
+
+}  // namespace
+
+std::size_t ContactSpMVOp::workspace_bytes(int n_particles) const {
+    std::size_t bytes = 0;
+    check(chx_contact_workspace_size(n_particles, max_contacts > 0 ? max_contacts : 0, &bytes), "collision",
+          "contact workspace size");
+    return bytes;
+}
+
+void bake_contact_diag(math::Mat3f* diag_blocks, int n_particles, const ContactSpMVOp& op, float alpha,
+                       std::uintptr_t cuda_stream) {
+    if (!op.active() || n_particles <= 0 || op.count_dev == nullptr) return;
+    const chx_contact_op c = to_abi(op);
+    with_workspace(op, n_particles, cuda_stream, "bake_contact_diag", [&](void* ws, std::size_t bytes) {
+        check(chx_contact_bake_diag(n_particles, reinterpret_cast<float*>(diag_blocks), &c, alpha, ws, bytes,
+                                    cuda_stream),
+              "collision", "bake_contact_diag");
+    });
+}
+
+void apply_contact_spmv(const ContactSpMVOp& op, const math::Vec3f* x, math::Vec3f* y, int n_particles, float alpha,
+                        std::uintptr_t cuda_stream) {
+    if (!op.active() || n_particles <= 0 || op.count_dev == nullptr) return;
+    const chx_contact_op c = to_abi(op);
+    with_workspace(op, n_particles, cuda_stream, "apply_contact_spmv", [&](void* ws, std::size_t bytes) {
+        check(chx_contact_spmv(n_particles, &c, reinterpret_cast<const float*>(x), reinterpret_cast<float*>(y), alpha,
+                               ws, bytes, cuda_stream),
+              "collision", "apply_contact_spmv");
+    });
+}
+
+}  // namespace collision
+
 namespace solver {
 
 void PCGSolver::initialize(int num_block_rows) {
@@ -50,7 +117,7 @@ void PCGSolver::initialize(int num_block_rows) {
 }
 
 int PCGSolver::solve(const sparse::BlockCSR3& A, DeviceSpan<math::Vec3f> b, DeviceSpan<math::Vec3f> x,
-                     const PCGParams& params, std::uintptr_t cuda_stream) {
+                     const PCGParams& params, std::uintptr_t cuda_stream, collision::ContactSpMVOp contact) {
     const int n = A.num_block_rows();
     if (n == 0) return 0;
     if (static_cast<int>(b.size()) < n || static_cast<int>(x.size()) < n)
@@ -62,12 +129,31 @@ int PCGSolver::solve(const sparse::BlockCSR3& A, DeviceSpan<math::Vec3f> b, Devi
     initialize(n);
     int it = 0;
     double rr = 0.0, bb = 0.0;
-    check(chx_pcg_solve_bsr3(n, reinterpret_cast<const float*>(A.diag.gpu_data()), A.row_offsets.gpu_data(),
-                             A.col_indices.gpu_data(), reinterpret_cast<const float*>(A.values.gpu_data()),
-                             reinterpret_cast<const float*>(b.data()), reinterpret_cast<float*>(x.data()),
-                             params.max_iterations, params.tolerance, params.lanes_per_row, workspace_.gpu_data(),
-                             workspace_.gpu_size(), &it, &rr, &bb, cuda_stream),
-          "solver::PCGSolver", "solve");
+    if (contact.active() && contact.count_dev != nullptr) {
+        const chx_contact_op c = collision::to_abi(contact);
+        void* cws = contact.workspace;
+        std::size_t cbytes = contact.workspace_size;
+        const std::size_t need = contact.workspace_bytes(n);
+        if (cws == nullptr || cbytes < need) {
+            contact_workspace_.ensure_device(need);
+            cws = contact_workspace_.gpu_data();
+            cbytes = contact_workspace_.gpu_size();
+        }
+        check(chx_pcg_solve_bsr3_contact(
+                  n, reinterpret_cast<const float*>(A.diag.gpu_data()), A.row_offsets.gpu_data(),
+                  A.col_indices.gpu_data(), reinterpret_cast<const float*>(A.values.gpu_data()),
+                  reinterpret_cast<const float*>(b.data()), reinterpret_cast<float*>(x.data()), params.max_iterations,
+                  params.tolerance, params.lanes_per_row, workspace_.gpu_data(), workspace_.gpu_size(), &c, cws,
+                  cbytes, &it, &rr, &bb, cuda_stream),
+              "solver::PCGSolver", "solve");
+    } else {
+        check(chx_pcg_solve_bsr3(n, reinterpret_cast<const float*>(A.diag.gpu_data()), A.row_offsets.gpu_data(),
+                                 A.col_indices.gpu_data(), reinterpret_cast<const float*>(A.values.gpu_data()),
+                                 reinterpret_cast<const float*>(b.data()), reinterpret_cast<float*>(x.data()),
+                                 params.max_iterations, params.tolerance, params.lanes_per_row, workspace_.gpu_data(),
+                                 workspace_.gpu_size(), &it, &rr, &bb, cuda_stream),
+              "solver::PCGSolver", "solve");
+    }
     last_rel_residual_ = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
     return it;
 }

@@ -1,6 +1,7 @@
-// chysx::sparse::BlockCSR3 / spmv and chysx::solver::PCGSolver — drop-ins
-// for ChysX/src/sparse/block_csr.h:59-148 and
-// ChysX/src/solver/pcg_solver.h:79-167 over the C ABI.
+// chysx::sparse::BlockCSR3 / spmv, chysx::collision::ContactSpMVOp and
+// chysx::solver::PCGSolver — drop-ins for
+// ChysX/src/sparse/block_csr.h:59-148, ChysX/src/collision/contact_spmv.h:
+// 14-135 and ChysX/src/solver/pcg_solver.h:79-167 over the C ABI.
 #pragma once
 
 #include <cstdint>
@@ -45,6 +46,52 @@ void spmv(const BlockCSR3& A, DeviceSpan<math::Vec3f> x, DeviceSpan<math::Vec3f>
 
 }  // namespace sparse
 
+namespace collision {
+
+// One contact's weights, normal and depth (ChysX/src/collision/
+// self_collision.h, ContactWeights): 8 floats.
+struct ContactWeights {
+    float w0 = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f;
+    float nx = 0.f, ny = 0.f, nz = 0.f;
+    float depth = 0.f;
+};
+static_assert(sizeof(ContactWeights) == 32, "8 floats");
+
+// The contact SpMV sidecar (contact_spmv.h:14-50).  Contact c adds
+// sum_{s,t} w_s w_t H_c to the blocks (pairs[c][s], pairs[c][t]) with
+// H_c = k n n^T + a_c (I - n n^T), a_c = mu k depth f1(|u_t|).
+struct ContactSpMVOp {
+    const math::Vec4i* pairs = nullptr;
+    const ContactWeights* weights = nullptr;
+    const int* count_dev = nullptr;  // device int: active contacts
+    int max_contacts = 0;
+    float stiffness = 0.0f;
+    const math::Vec4f* slips = nullptr;  // (u_t, |u_t|) per contact, or null
+    float friction_mu = 0.0f;
+    float friction_epsilon = 1.0e-4f;
+    // Addition: device workspace of at least workspace_bytes() bytes for the
+    // per-row contact index.  Null: bake / apply allocate a temporary and
+    // wait for `cuda_stream` before freeing it.
+    void* workspace = nullptr;
+    std::size_t workspace_size = 0;
+
+    bool active() const noexcept { return max_contacts > 0 && stiffness > 0.0f && pairs != nullptr; }
+    bool friction_active() const noexcept { return active() && friction_mu > 0.0f && slips != nullptr; }
+    // Device workspace the calls need for `n_particles` rows.
+    std::size_t workspace_bytes(int n_particles) const;
+};
+
+// diag_blocks[i] += alpha * sum_c w_i^2 H_c (contact_spmv.h:71-96).
+// Deterministic: each row sums its contacts in (contact, slot) order.
+void bake_contact_diag(math::Mat3f* diag_blocks, int n_particles, const ContactSpMVOp& op, float alpha,
+                       std::uintptr_t cuda_stream);
+
+// y += alpha * C_off x: the cross-slot terms (contact_spmv.h:98-132).
+void apply_contact_spmv(const ContactSpMVOp& op, const math::Vec3f* x, math::Vec3f* y, int n_particles, float alpha,
+                        std::uintptr_t cuda_stream);
+
+}  // namespace collision
+
 namespace solver {
 
 struct PCGParams {
@@ -61,9 +108,12 @@ public:
 
     void initialize(int num_block_rows);
     // Solves A x = b in place (x is the warm start); returns the number of
-    // iterations performed (== max_iterations when tolerance <= 0).
+    // iterations performed (== max_iterations when tolerance <= 0).  An
+    // active `contact` turns every product into (A + C_off) x; its same-slot
+    // terms are the caller's to bake into A.diag (pcg_solver.h:111-131).
     int solve(const sparse::BlockCSR3& A, DeviceSpan<math::Vec3f> b, DeviceSpan<math::Vec3f> x,
-              const PCGParams& params = PCGParams{}, std::uintptr_t cuda_stream = 0);
+              const PCGParams& params = PCGParams{}, std::uintptr_t cuda_stream = 0,
+              collision::ContactSpMVOp contact = {});
     // Preconditioner-weighted residual of the last solve: <r, z> is not kept;
     // this returns ||r||_2 / ||b||_2 of the final iterate (FP64).
     double last_residual() const noexcept { return last_rel_residual_; }
@@ -71,6 +121,7 @@ public:
 private:
     int n_ = 0;
     DeviceArray<unsigned char> workspace_;
+    DeviceArray<unsigned char> contact_workspace_;
     double last_rel_residual_ = 0.0;
 };
 

@@ -543,6 +543,64 @@ template <class T> struct StaticContact {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Contact SpMV sidecar (ChysX/src/collision/contact_spmv.h:12-52,
+// contact_spmv.cu:46-268): contact c couples particles id[0..3] with
+// weights w[0..3], normal n, depth; H_c = k (n n^T) + a_c (I - n n^T) with
+// a_c = mu k depth f1(|u_t|) when friction is on.  The full operator is
+// C[i][j] = sum_c w_i w_j H_c over the contact's slots; bake adds the
+// diagonal terms (i == j slots) to the diagonal blocks, apply adds the
+// off-diagonal ones (i != j slots) to y.  FP64 sums in contact order.
+// ---------------------------------------------------------------------------
+struct ContactOp {
+    const int* pairs = nullptr;     // [m][4]
+    const double* weights = nullptr;  // [m][8]: w0..w3, nx, ny, nz, depth
+    const double* slips = nullptr;  // [m][4] or null
+    int count = 0;
+    double k = 0.0, mu = 0.0, eps = 1.0e-4;
+    void block(int c, double H[9]) const {
+        const double* w = weights + 8 * c;
+        const double nx = w[4], ny = w[5], nz = w[6];
+        double a = 0.0;
+        if (mu > 0.0 && slips) a = mu * (k * w[7]) * ipc_f1_over_x(slips[4 * c + 3], eps);
+        const double nn[3] = {nx, ny, nz};
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q) H[3 * r + q] = (k - a) * nn[r] * nn[q] + (r == q ? a : 0.0);
+    }
+    // diag[i] += alpha sum w_i^2 H_c
+    void bake(double* diag, double alpha) const {
+        for (int c = 0; c < count; ++c) {
+            double H[9];
+            block(c, H);
+            for (int s = 0; s < 4; ++s) {
+                const double wi = weights[8 * c + s];
+                if (wi == 0.0) continue;
+                for (int e = 0; e < 9; ++e) diag[9 * pairs[4 * c + s] + e] += alpha * wi * wi * H[e];
+            }
+        }
+    }
+    // y[i] += alpha sum_{s != t} w_s w_t H_c x[id_t]
+    void apply(const double* x, double* y, double alpha) const {
+        for (int c = 0; c < count; ++c) {
+            double H[9];
+            block(c, H);
+            for (int s = 0; s < 4; ++s) {
+                const double ws = weights[8 * c + s];
+                if (ws == 0.0) continue;
+                double t[3] = {0.0, 0.0, 0.0};
+                for (int u = 0; u < 4; ++u) {
+                    if (u == s) continue;
+                    const double wu = weights[8 * c + u];
+                    for (int r = 0; r < 3; ++r) t[r] += wu * x[3 * pairs[4 * c + u] + r];
+                }
+                const int i = pairs[4 * c + s];
+                for (int r = 0; r < 3; ++r)
+                    y[3 * i + r] += alpha * ws * (H[3 * r] * t[0] + H[3 * r + 1] * t[1] + H[3 * r + 2] * t[2]);
+            }
+        }
+    }
+};
+
 template <class T> struct Sim {
     int n = 0;
     std::vector<V3<T>> pos, vel;
@@ -829,13 +887,30 @@ template <class T> struct Sim {
         return (b > 1e-37 || b < -1e-37) ? T(a / b) : T(0);
     }
 
+    // Contact sidecar of the standalone solve (pcg_solver.h:111-131): when
+    // set, every product is (A + C_off) x (pcg_solver.cu:159-160, 180-181).
+    const ContactOp* sidecar = nullptr;
+    void product(const std::vector<V3<T>>& src, std::vector<V3<T>>& dst) {
+        spmv(src, dst);
+        if (!sidecar) return;
+        std::vector<double> xs(3 * size_t(n)), ys(3 * size_t(n), 0.0);
+        for (int i = 0; i < n; ++i) {
+            xs[3 * i] = double(src[i].x);
+            xs[3 * i + 1] = double(src[i].y);
+            xs[3 * i + 2] = double(src[i].z);
+        }
+        sidecar->apply(xs.data(), ys.data(), 1.0);
+        for (int i = 0; i < n; ++i)
+            dst[i] = V3<T>(dst[i].x + T(ys[3 * i]), dst[i].y + T(ys[3 * i + 1]), dst[i].z + T(ys[3 * i + 2]));
+    }
+
     // Block-Jacobi PCG, warm-started from x: ChysX/src/solver/pcg_solver.cu:134-215.
     int pcg(const std::vector<V3<T>>& b, std::vector<V3<T>>& x) {
         std::vector<M3<T>> Minv(n);
         std::vector<V3<T>> r(n), z(n), p(n), q(n), t(n);
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) Minv[i] = inverse(diag[i]);
-        spmv(x, t);
+        product(x, t);
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) {
             r[i] = b[i] - t[i];
@@ -850,7 +925,7 @@ template <class T> struct Sim {
         int it = 0;
         for (; it < max_iter; ++it) {
             if (use_tol && rr <= thresh) break;
-            spmv(p, q);
+            product(p, q);
             const double sigma = dot64(p, q);
             const T alpha = guarded_div(rho, sigma);
 #pragma omp parallel for schedule(static)

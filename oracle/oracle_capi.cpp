@@ -138,6 +138,23 @@ int orc_box_sdf(const double* p, const double* box, double* d, double* n) {
 }
 double orc_ipc_f1_over_x(double u, double eps) { return ipc_f1_over_x(u, eps); }
 
+// ---- contact SpMV sidecar -----------------------------------------------------
+// pairs [m][4], weights [m][8], slips [m][4] or null; diag [n][9], x / y [n][3].
+int orc_contact_bake(int count, const int* pairs, const double* weights, const double* slips, double k, double mu,
+                     double eps, double alpha, double* diag) {
+    return guarded([&] {
+        ContactOp op{pairs, weights, slips, count, k, mu, eps};
+        op.bake(diag, alpha);
+    });
+}
+int orc_contact_apply(int count, const int* pairs, const double* weights, const double* slips, double k, double mu,
+                      double eps, double alpha, const double* x, double* y) {
+    return guarded([&] {
+        ContactOp op{pairs, weights, slips, count, k, mu, eps};
+        op.apply(x, y, alpha);
+    });
+}
+
 // ---- simulator ---------------------------------------------------------------
 void* orc_sim_create(int precision) {
     auto* h = new Handle;
@@ -332,12 +349,15 @@ int orc_sim_hessian(void* hv, double* diag, double* values) {
 }
 
 // Standalone block-Jacobi PCG on a caller-provided BSR system (warm start in x).
-int orc_pcg_solve(int precision, int n, const int* row_offsets, const int* cols, const double* diag,
-                  const double* values, const double* b, double* x, int max_iter, double tol, int* iters,
-                  double* rr, double* bb) {
+// The standalone PCG on (A + C_off) when `contact` is given (the caller
+// baked the same-slot terms into diag).
+static int pcg_solve(int precision, int n, const int* row_offsets, const int* cols, const double* diag,
+                     const double* values, const double* b, double* x, int max_iter, double tol, int* iters,
+                     double* rr, double* bb, const ContactOp* contact) {
     auto run = [&](auto tag) {
         using T = decltype(tag);
         Sim<T> S;
+        S.sidecar = contact;
         S.n = n;
         S.pat.n = n;
         S.pat.row_offsets.assign(row_offsets, row_offsets + n + 1);
@@ -360,6 +380,20 @@ int orc_pcg_solve(int precision, int n, const int* row_offsets, const int* cols,
         put3(xv, x);
     };
     return guarded([&] { if (precision == 64) run(double{}); else run(float{}); });
+}
+
+int orc_pcg_solve(int precision, int n, const int* row_offsets, const int* cols, const double* diag,
+                  const double* values, const double* b, double* x, int max_iter, double tol, int* iters,
+                  double* rr, double* bb) {
+    return pcg_solve(precision, n, row_offsets, cols, diag, values, b, x, max_iter, tol, iters, rr, bb, nullptr);
+}
+
+int orc_pcg_solve_contact(int precision, int n, const int* row_offsets, const int* cols, const double* diag,
+                          const double* values, const double* b, double* x, int max_iter, double tol, int count,
+                          const int* pairs, const double* weights, const double* slips, double k, double mu,
+                          double eps, int* iters, double* rr, double* bb) {
+    const ContactOp op{pairs, weights, slips, count, k, mu, eps};
+    return pcg_solve(precision, n, row_offsets, cols, diag, values, b, x, max_iter, tol, iters, rr, bb, &op);
 }
 
 // y = A x with the oracle's SpMV.

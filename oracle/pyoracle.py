@@ -47,6 +47,7 @@ def lib():
             "orc_sim_get_dihedrals", "orc_sim_get_edges", "orc_sim_get_rest_shape", "orc_sim_get_system",
             "orc_sim_set_dx", "orc_sim_energy", "orc_sim_gradient", "orc_sim_hessian", "orc_sim_build_stvk",
             "orc_stvk_element", "orc_box_sdf", "orc_sim_add_plane", "orc_sim_add_box", "orc_sim_set_contact",
+            "orc_contact_bake", "orc_contact_apply", "orc_pcg_solve_contact",
         ):
             getattr(L, name).restype = _I
         L.orc_sim_step.argtypes = [_P, _D]
@@ -65,6 +66,10 @@ def lib():
         L.orc_sim_add_plane.argtypes = [_P, _P, _D]
         L.orc_sim_add_box.argtypes = [_P, _P]
         L.orc_sim_set_contact.argtypes = [_P, _D, _D, _D, _D]
+        L.orc_contact_bake.argtypes = [_I, _P, _P, _P, _D, _D, _D, _D, _P]
+        L.orc_contact_apply.argtypes = [_I, _P, _P, _P, _D, _D, _D, _D, _P, _P]
+        L.orc_pcg_solve_contact.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _I, _D, _I, _P, _P, _P, _D, _D, _D,
+                                            _P, _P, _P]
         _lib = L
     return _lib
 
@@ -283,6 +288,28 @@ def pcg_solve(row_offsets, cols, diag, values, b, x0, max_iter, tol=0.0, precisi
     return x, it.value, rr.value, bb.value
 
 
+def contact_pcg_solve(row_offsets, cols, diag, values, b, x0, max_iter, tol, pairs, weights, k, slips=None, mu=0.0,
+                      eps=1e-4, precision=32):
+    """The standalone PCG on (A + C_off) (diag already holds the baked
+    same-slot terms); returns (x, iters, rr, bb)."""
+    ro = _i32(row_offsets)
+    ci = _i32(cols) if len(cols) else np.zeros(1, np.int32)
+    dg = _f64(diag).reshape(-1, 9)
+    vl = _f64(values).reshape(-1, 9) if len(values) else np.zeros((1, 9))
+    bb_ = _f64(b).reshape(-1, 3)
+    x = _f64(x0).reshape(-1, 3).copy()
+    pr = _i32(pairs).reshape(-1, 4)
+    w = _f64(weights).reshape(-1, 8)
+    sl = None if slips is None else _f64(slips).reshape(-1, 4)
+    it, rr, bb = _I(), _D(), _D()
+    n = len(ro) - 1
+    _check(lib().orc_pcg_solve_contact(_I(precision), _I(n), _ptr(ro), _ptr(ci), _ptr(dg), _ptr(vl), _ptr(bb_),
+                                       _ptr(x), _I(max_iter), _D(tol), _I(len(pr)), _ptr(pr), _ptr(w),
+                                       None if sl is None else _ptr(sl), _D(k), _D(mu), _D(eps), ctypes.byref(it),
+                                       ctypes.byref(rr), ctypes.byref(bb)))
+    return x, it.value, rr.value, bb.value
+
+
 def spmv(row_offsets, cols, diag, values, x, precision=32):
     ro = _i32(row_offsets)
     ci = _i32(cols) if len(cols) else np.zeros(1, np.int32)
@@ -328,3 +355,26 @@ def box_sdf(p, center, half_ext, ex=(1.0, 0.0, 0.0), ey=(0.0, 1.0, 0.0), ez=(0.0
 
 def ipc_f1_over_x(u, eps):
     return float(lib().orc_ipc_f1_over_x(float(u), float(eps)))
+
+
+def contact_bake(diag, pairs, weights, k, slips=None, mu=0.0, eps=1e-4, alpha=1.0):
+    """diag (n, 3, 3) += alpha * sum_c w_i^2 H_c (FP64, contact order)."""
+    d = np.ascontiguousarray(diag, np.float64).copy()
+    pr = _i32(pairs).reshape(-1, 4)
+    w = _f64(weights).reshape(-1, 8)
+    sl = None if slips is None else _f64(slips).reshape(-1, 4)
+    _check(lib().orc_contact_bake(_I(len(pr)), _ptr(pr), _ptr(w), None if sl is None else _ptr(sl), _D(k), _D(mu),
+                                  _D(eps), _D(alpha), _ptr(d)))
+    return d
+
+
+def contact_apply(x, pairs, weights, k, slips=None, mu=0.0, eps=1e-4, alpha=1.0, y=None):
+    """y (n, 3) + alpha * C_off x (FP64, contact order)."""
+    xx = _f64(x).reshape(-1, 3)
+    out = np.zeros_like(xx) if y is None else _f64(y).reshape(-1, 3).copy()
+    pr = _i32(pairs).reshape(-1, 4)
+    w = _f64(weights).reshape(-1, 8)
+    sl = None if slips is None else _f64(slips).reshape(-1, 4)
+    _check(lib().orc_contact_apply(_I(len(pr)), _ptr(pr), _ptr(w), None if sl is None else _ptr(sl), _D(k), _D(mu),
+                                   _D(eps), _D(alpha), _ptr(xx), _ptr(out)))
+    return out
