@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--workload", choices=["batched", "serving"], default="batched",
                     help="batched (default; includes a short serving leg) or the full serving stream")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg of the batched workload")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager substeps instead of hipGraph replays of the captured substep")
     ap.add_argument("--serving-rate", type=float, default=None)
     ap.add_argument("--serving-duration", type=float, default=None)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -213,8 +215,11 @@ def main():
         log(f"autotune picked staging={int(tuned['lds_staging'])} lanes={tuned['lanes_per_row']} "
             f"grid={tuned['grid_blocks']} tile_rows={tuned['tile_rows']}")
 
+    # The timed substeps replay one hipGraph (element pass, gather pass,
+    # persistent PCG + finalize), captured during the warm-up.
+    batch.set_graph(not args.no_graph)
     with torch.cuda.stream(stream):
-        for _ in range(args.warmup):
+        for _ in range(max(1, args.warmup)):
             batch.step(dt, sh)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -222,7 +227,10 @@ def main():
             batch.step(dt, sh)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-    log(f"timed {args.steps} batched substeps: {wall / args.steps * 1e3:.2f} ms each")
+    graph_captures = batch.graph_captures()
+    batch.set_graph(False)
+    log(f"timed {args.steps} batched substeps ({'hipGraph replays' if not args.no_graph else 'eager'}): "
+        f"{wall / args.steps * 1e3:.2f} ms each")
     iters = np.array(batch.last_pcg_iterations(sh))
     res = np.array(batch.last_relative_residuals(sh))
     value = n_rows * args.steps / wall
@@ -339,7 +347,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "batched: 512 cloth grids, side logU[16,128] vertices (seed 2), BW98 + bending, "
-                               "PCG tol 1e-6 cap 100, one fused substep per step",
+                               "PCG tol 1e-6 cap 100, one fused substep per step"
+                               + ("" if args.no_graph else ", replayed from one hipGraph"),
+                   "hipgraph": not args.no_graph, "graph_captures": graph_captures,
                    "instances": len(specs), "vertices": n_rows, "off_diag_blocks": int(sizes[:, 1].sum()),
                    "pcg_tol": 1e-6, "pcg_cap": 100, "lanes_per_row": args.lanes, "grid_blocks": args.grid, "tile_rows": args.tile_rows,
                    "lds_staging": not args.no_lds_staging,

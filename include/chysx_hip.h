@@ -368,6 +368,21 @@ int chx_cloth_solve(const chx_cloth_batch* batch, int finalize, uintptr_t stream
  * timeout inside the persistent solver) and clears it; synchronous. */
 int chx_batch_check(const chx_cloth_batch* batch, uintptr_t stream);
 
+/* ---- hipGraph-captured substep ---------------------------------------------
+ * No counterpart in the reference's cloth step (its PCGSolver captures only
+ * its own iteration loop, pcg_solver.h:146-166).  Captures the device work of
+ * chx_cloth_step for `batch` (element pass, gather pass, persistent PCG with
+ * finalize: kernels only, no memset or copy nodes) on `stream` (not the null
+ * stream) and instantiates it.  The graph keeps the descriptor's values of
+ * capture time: per-step instance scalars and pointers read from the device
+ * instance table (batch->inst) may change between replays; a change of the
+ * descriptor itself (a single inline instance, pointers, sizes, launch shape)
+ * needs a new capture. */
+int chx_cloth_step_graph(const chx_cloth_batch* batch, uintptr_t stream, void** graph_exec);
+/* Replays a captured substep on `stream`. */
+int chx_graph_launch(void* graph_exec, uintptr_t stream);
+int chx_graph_destroy(void* graph_exec);
+
 #ifdef __cplusplus
 }
 #endif

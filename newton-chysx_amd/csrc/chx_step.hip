@@ -1496,6 +1496,48 @@ int chx_cloth_step(const chx_cloth_batch* batch, uintptr_t stream) {
     });
 }
 
+int chx_cloth_step_graph(const chx_cloth_batch* batch, uintptr_t stream, void** graph_exec) {
+    return guarded([&] {
+        require(graph_exec != nullptr, "chx_cloth_step_graph: null output");
+        *graph_exec = nullptr;
+        validate_batch(batch);
+        const hipStream_t s = as_stream(stream);
+        require(s != nullptr, "chx_cloth_step_graph: capture needs a stream other than the null stream");
+        // Device limits are queried (and cached) before the capture starts.
+        (void)max_coresident(batch->lanes_per_row, batch->bf16_blocks != 0 ? 2 : 1, batch->no_lds_staging == 0);
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipGraph_t g = nullptr;
+        try {
+            assemble(*batch, s);
+            launch_pcg(*batch, 1, s);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g != nullptr) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "hipGraphInstantiate");
+        *graph_exec = ex;
+    });
+}
+
+int chx_graph_launch(void* graph_exec, uintptr_t stream) {
+    return guarded([&] {
+        require(graph_exec != nullptr, "chx_graph_launch: null graph");
+        hip_check(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), as_stream(stream)), "hipGraphLaunch");
+    });
+}
+
+int chx_graph_destroy(void* graph_exec) {
+    return guarded([&] {
+        if (graph_exec != nullptr)
+            hip_check(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)), "hipGraphExecDestroy");
+    });
+}
+
 int chx_batch_check(const chx_cloth_batch* batch, uintptr_t stream) {
     return guarded([&] {
         require(batch != nullptr && batch->sync != nullptr, "chx_batch_check: null batch");

@@ -286,6 +286,10 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("set_tile_rows", &chysx::cloth::ClothBatch::set_tile_rows, py::arg("rows"))
         .def("set_lds_staging", &chysx::cloth::ClothBatch::set_lds_staging, py::arg("enabled"))
         .def("lds_staging", &chysx::cloth::ClothBatch::lds_staging)
+        .def("set_graph", &chysx::cloth::ClothBatch::set_graph, py::arg("enabled"),
+             "Replay substeps from a hipGraph (needs a non-null cuda_stream in step()).")
+        .def("graph", &chysx::cloth::ClothBatch::graph)
+        .def("graph_captures", &chysx::cloth::ClothBatch::graph_captures)
         .def("set_trace", &chysx::cloth::ClothBatch::set_trace, py::arg("entries"))
         .def(
             "last_trace",

@@ -50,7 +50,8 @@ void ClothBatch::step(float dt, std::uintptr_t cuda_stream) {
     if (!(dt > 0.0f)) throw std::invalid_argument("ClothBatch::step: dt must be positive");
     if (!bound_) finalize(dt, cuda_stream);
     else runner_.update_many(descriptors(dt, cuda_stream), cuda_stream);
-    runner_.step(cuda_stream);
+    if (graph_) runner_.step_graph(cuda_stream);
+    else runner_.step(cuda_stream);
 }
 
 void ClothBatch::assemble(float dt, std::uintptr_t cuda_stream) {

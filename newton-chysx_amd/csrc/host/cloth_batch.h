@@ -56,6 +56,12 @@ public:
     // Rows per PCG tile: a multiple of CHX_STAGE_ROWS (256).
     void set_tile_rows(int rows) { runner_.set_tile_rows(rows); }
     // SpMV form of the PCG launch: LDS-staged (default) or global gathers.
+    // Substeps replayed from a hipGraph of the element pass, gather pass and
+    // persistent PCG (captured on the first step and again after any change
+    // of the batch or its launch shape); needs a non-null stream.
+    void set_graph(bool on) { graph_ = on; }
+    bool graph() const noexcept { return graph_; }
+    int graph_captures() const noexcept { return runner_.graph_captures(); }
     void set_lds_staging(bool on) { runner_.set_lds_staging(on); }
     bool lds_staging() const noexcept { return runner_.lds_staging(); }
     // Solver trace of the persistent PCG launch: capacity in barrier entries
@@ -80,6 +86,7 @@ private:
     engine::BatchRunner runner_;
     bool bound_ = false;
     bool bf16_ = false;
+    bool graph_ = false;
 };
 
 }  // namespace cloth

@@ -544,6 +544,24 @@ void BatchRunner::update_many(const std::vector<chx_instance>& ds, std::uintptr_
 }
 
 void BatchRunner::step(std::uintptr_t stream) { check(chx_cloth_step(&desc_, stream), "cloth::ClothSimulator", "step"); }
+
+void BatchRunner::step_graph(std::uintptr_t stream) {
+    if (stream == 0) throw std::invalid_argument("chysx::cloth::ClothBatch: graph replay needs a non-null stream");
+    if (graph_ == nullptr || std::memcmp(&graph_desc_, &desc_, sizeof(chx_cloth_batch)) != 0) {
+        // A replaced graph may still run on the stream: wait before dropping it.
+        check(chx_stream_synchronize(stream), "cloth::ClothBatch", "graph capture");
+        drop_graph();
+        check(chx_cloth_step_graph(&desc_, stream, &graph_), "cloth::ClothBatch", "graph capture");
+        std::memcpy(&graph_desc_, &desc_, sizeof(chx_cloth_batch));
+        ++graph_captures_;
+    }
+    check(chx_graph_launch(graph_, stream), "cloth::ClothBatch", "graph replay");
+}
+
+void BatchRunner::drop_graph() noexcept {
+    if (graph_ != nullptr) (void)chx_graph_destroy(graph_);
+    graph_ = nullptr;
+}
 void BatchRunner::assemble(std::uintptr_t stream) {
     check(chx_cloth_assemble(&desc_, stream), "cloth::ClothSimulator", "assemble");
 }

@@ -104,6 +104,7 @@ private:
 class BatchRunner {
 public:
     BatchRunner() = default;
+    ~BatchRunner() { drop_graph(); }
     BatchRunner(const BatchRunner&) = delete;
     BatchRunner& operator=(const BatchRunner&) = delete;
 
@@ -121,6 +122,12 @@ public:
     void update_many(const std::vector<chx_instance>& ds, std::uintptr_t stream);
 
     void step(std::uintptr_t stream);
+    // The same substep replayed from a hipGraph, captured on first use and
+    // again whenever the descriptor changed since the capture (`stream` must
+    // not be the null stream).
+    void step_graph(std::uintptr_t stream);
+    void drop_graph() noexcept;
+    int graph_captures() const noexcept { return graph_captures_; }
     void assemble(std::uintptr_t stream);
     void solve(bool finalize, std::uintptr_t stream);
     double energy(int kind_mask, std::uintptr_t stream);
@@ -166,6 +173,9 @@ private:
     void build_tiles(std::vector<chx_instance>& ds, std::uintptr_t stream = 0);
 
     chx_cloth_batch desc_{};
+    void* graph_ = nullptr;           // hipGraphExec_t of one substep
+    chx_cloth_batch graph_desc_{};    // the descriptor it was captured with
+    int graph_captures_ = 0;
     int lanes_ = 8;
     int grid_ = 0;
     int tile_rows_ = CHX_TILE_ROWS;
