@@ -26,9 +26,13 @@
  * every per-instance array of a cloth) must stay below 4 GiB, i.e. at most
  * 119,304,647 block rows or off-diagonal blocks per matrix.
  * The matrices a cloth instance assembles for itself (chx_instance diag,
- * values, minv) are stored planar instead: nine planes, one per block
- * component, so that a wavefront reads one component of consecutive blocks
- * as consecutive words.
+ * values, minv, diag_c, values16) and the preconditioner of
+ * chx_pcg_solve_bsr3's workspace are stored grouped planar instead: blocks in
+ * groups of CHX_BLOCK_GROUP (64), each group as nine planes of 64 words, i.e.
+ * component c (row-major r*3+col) of block i at
+ * [((i / 64) * 9 + c) * 64 + i % 64], arrays rounded up to whole groups.  A
+ * wavefront reads one component of 64 consecutive blocks as 256 contiguous
+ * bytes, and the nine components of a group are one contiguous run.
  */
 #ifndef CHYSX_HIP_H_
 #define CHYSX_HIP_H_
@@ -118,6 +122,8 @@ typedef struct chx_contact_op {
 
 /* Rows per slice of the sliced layout of the off-diagonal blocks. */
 #define CHX_SLICE_ROWS 64
+/* Blocks per group of the grouped planar block layout. */
+#define CHX_BLOCK_GROUP 64
 
 /* One cloth instance: every array is the instance's own device memory, with
  * indices local to the instance, so the off-diagonal pattern is exactly the
@@ -145,9 +151,9 @@ typedef struct chx_instance {
     const float* inv_mass;           /* external inverse masses, or NULL (mass 1 kg)         */
     const int* row_offsets;          /* [n_rows+1]                                           */
     const int* col_indices;          /* [nnz]                                                */
-    float* diag;                     /* [9][n_rows] planar: component c (row-major r*3+col)  */
-                                     /* of block row i at diag[c*n_rows + i]                 */
-    float* values;                   /* [9][n_upper] planar over the upper value slots       */
+    float* diag;                     /* grouped planar over the block rows (9 * n_rows       */
+                                     /* rounded up to 64 blocks)                             */
+    float* values;                   /* grouped planar over the n_upper upper value slots    */
     const chx_pin* pins;
     const chx_tri* tris;             /* stretch elements, then shear elements                */
     const chx_dih* dihs;
@@ -163,11 +169,11 @@ typedef struct chx_instance {
     float* p0;
     float* p1;
     float* q;
-    float* minv;                     /* [9][n_rows] planar block-Jacobi preconditioner       */
+    float* minv;                     /* grouped planar block-Jacobi preconditioner           */
     float* diag_c;                   /* BF16 precision mode, else NULL: the inner SpMV's     */
-    uint16_t* values16;              /* matrix, planar FP32 compensated diagonal             */
-                                     /* D + sum_j (V_ij - bf16(V_ij)) [9][n_rows] and planar */
-                                     /* BF16 upper-block copies [9][n_upper]                 */
+    uint16_t* values16;              /* matrix, grouped planar FP32 compensated diagonal     */
+                                     /* D + sum_j (V_ij - bf16(V_ij)) and grouped planar     */
+                                     /* BF16 upper-block copies                              */
     const chx_stvk* stvks;           /* StVK membrane elements, or NULL                      */
     float* stvk_rec;                 /* [n_stvks*CHX_STVK_REC]                               */
     const int* windows;              /* [ceil(n_rows/CHX_STAGE_ROWS)][2]: smallest and       */

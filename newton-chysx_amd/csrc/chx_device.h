@@ -150,40 +150,46 @@ struct Mat32G {
     }
 };
 
-// Planar blocks: component c (row-major r*3+col) of block i at p[c*stride + i],
-// nine planes per array.  One load instruction of a wavefront then reads one
-// component of consecutive blocks, i.e. consecutive words, instead of a
-// 36-byte stride that spreads each instruction over ~9x more cache lines.
+// Grouped planar blocks: 3x3 blocks in groups of 64, each group stored as
+// nine planes of 64 words — component c (row-major r*3+col) of block i at
+// p[((i / 64) * 9 + c) * 64 + i % 64] (CHX_BLOCK_GROUP).  One load
+// instruction of a wavefront reads one component of 64 consecutive blocks
+// (256 contiguous bytes, not a 36-byte stride over ~9x more cache lines), and
+// the nine components of a group are one contiguous 2304-byte run, so the
+// nine loads of a block land in one DRAM page instead of nine planes far
+// apart.  Arrays hold whole groups (the block count rounded up to 64).
+static_assert(CHX_BLOCK_GROUP == 64, "bp() indexes groups of 64 blocks");
+__device__ __forceinline__ unsigned bp(unsigned c, unsigned i) {
+    return (((i >> 6) * 9u + c) << 6) | (i & 63u);
+}
 template <class T>
-__device__ __forceinline__ f3 mv9p(gp<const T> m, unsigned stride, unsigned i, f3 v) {
+__device__ __forceinline__ f3 mv9p(gp<const T> m, unsigned i, f3 v) {
     float a[9];
 #pragma unroll
-    for (unsigned c = 0; c < 9; ++c) a[c] = to_f32(ldx(m, c * stride + i));
+    for (unsigned c = 0; c < 9; ++c) a[c] = to_f32(ldx(m, bp(c, i)));
     return mv9(a, v);
 }
 // Block i, or its transpose when tr (lower blocks of the symmetric layout
 // are their upper partners' transposes): the same loads, swapped operands.
 template <class T>
-__device__ __forceinline__ f3 mv9p_t(gp<const T> m, unsigned stride, unsigned i, f3 v, bool tr) {
+__device__ __forceinline__ f3 mv9p_t(gp<const T> m, unsigned i, f3 v, bool tr) {
     float a[9];
 #pragma unroll
-    for (unsigned c = 0; c < 9; ++c) a[c] = to_f32(ldx(m, c * stride + i));
+    for (unsigned c = 0; c < 9; ++c) a[c] = to_f32(ldx(m, bp(c, i)));
     const float b[9] = {a[0], tr ? a[3] : a[1], tr ? a[6] : a[2], tr ? a[1] : a[3], a[4],
                         tr ? a[7] : a[5], tr ? a[2] : a[6], tr ? a[5] : a[7], a[8]};
     return mv9(b, v);
 }
 template <class T>
 struct MatP {
-    gp<const T> d;
-    gp<const T> v;
-    unsigned n;    // plane stride of d (block rows)
-    unsigned nnz;  // plane stride of v (off-diagonal blocks)
-    __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv9p(d, n, row, x); }
-    __device__ __forceinline__ f3 off(unsigned k, f3 x) const { return mv9p(v, nnz, k, x); }
-    __device__ __forceinline__ f3 off_t(unsigned k, f3 x, bool tr) const { return mv9p_t(v, nnz, k, x, tr); }
+    gp<const T> d;  // diagonal blocks (block rows)
+    gp<const T> v;  // off-diagonal blocks (value slots)
+    __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv9p(d, row, x); }
+    __device__ __forceinline__ f3 off(unsigned k, f3 x) const { return mv9p(v, k, x); }
+    __device__ __forceinline__ f3 off_t(unsigned k, f3 x, bool tr) const { return mv9p_t(v, k, x, tr); }
     __device__ __forceinline__ void diag_block(unsigned row, float (&m)[9]) const {
 #pragma unroll
-        for (unsigned c = 0; c < 9; ++c) m[c] = to_f32(ldx(d, c * n + row));
+        for (unsigned c = 0; c < 9; ++c) m[c] = to_f32(ldx(d, bp(c, row)));
     }
 };
 using MatP32 = MatP<float>;
@@ -193,11 +199,9 @@ using MatP32 = MatP<float>;
 struct MatP16 {
     gp<const float> d;
     gp<const uint16_t> v;
-    unsigned n;
-    unsigned nnz;
-    __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv9p(d, n, row, x); }
-    __device__ __forceinline__ f3 off(unsigned k, f3 x) const { return mv9p(v, nnz, k, x); }
-    __device__ __forceinline__ f3 off_t(unsigned k, f3 x, bool tr) const { return mv9p_t(v, nnz, k, x, tr); }
+    __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv9p(d, row, x); }
+    __device__ __forceinline__ f3 off(unsigned k, f3 x) const { return mv9p(v, k, x); }
+    __device__ __forceinline__ f3 off_t(unsigned k, f3 x, bool tr) const { return mv9p_t(v, k, x, tr); }
 };
 
 __device__ __forceinline__ double dot64(f3 a, f3 b) {

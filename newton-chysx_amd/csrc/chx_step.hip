@@ -282,16 +282,16 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
             h[3] += a01; h[4] += a11; h[5] += a12;
             h[6] += a02; h[7] += a12; h[8] += a22;
         }
-        // Planar blocks: component k of block row `row` at diag[k*n + row].
+        // Grouped planar blocks (bp): component k of block row `row`.
 #pragma unroll
-        for (unsigned k = 0; k < 9; ++k) stx(I.diag, k * n + row, h[k]);
+        for (unsigned k = 0; k < 9; ++k) stx(I.diag, bp(k, row), h[k]);
     } else {
-        const unsigned slot = unsigned(bslot), np = unsigned(I.n_upper);
+        const unsigned slot = unsigned(bslot);
 #pragma unroll
-        for (unsigned k = 0; k < 9; ++k) stx(I.values, k * np + slot, h[k]);
+        for (unsigned k = 0; k < 9; ++k) stx(I.values, bp(k, slot), h[k]);
         if (I.values16 != nullptr) {
 #pragma unroll
-            for (unsigned k = 0; k < 9; ++k) stx(I.values16, k * np + slot, f32_to_bf16(h[k]));
+            for (unsigned k = 0; k < 9; ++k) stx(I.values16, bp(k, slot), f32_to_bf16(h[k]));
         }
     }
 }
@@ -312,11 +312,10 @@ __global__ __launch_bounds__(kBlock) void compensate_kernel(chx_cloth_batch B) {
     const Inst I = inst_at(B, inst);
     const unsigned n = unsigned(I.n_rows), row = unsigned(ss);
     if (row >= n) return;
-    const unsigned np = unsigned(I.n_upper);
     const unsigned k0 = ldx(I.row_offsets, row), k1 = ldx(I.row_offsets, row + 1u);
     float d[9];
 #pragma unroll
-    for (unsigned c = 0; c < 9; ++c) d[c] = ldx(gp<const float>(I.diag), c * n + row);
+    for (unsigned c = 0; c < 9; ++c) d[c] = ldx(gp<const float>(I.diag), bp(c, row));
     for (unsigned k = k0; k < k1; ++k) {
         // lower blocks: the partner's rounding, transposed
         const int bs = ldx(I.block_slots, k);
@@ -325,12 +324,12 @@ __global__ __launch_bounds__(kBlock) void compensate_kernel(chx_cloth_batch B) {
 #pragma unroll
         for (unsigned c = 0; c < 9; ++c) {
             const unsigned cs = tr ? (c % 3u) * 3u + c / 3u : c;
-            const float v = ldx(gp<const float>(I.values), cs * np + slot);
+            const float v = ldx(gp<const float>(I.values), bp(cs, slot));
             d[c] += v - bf16_to_f32(f32_to_bf16(v));
         }
     }
 #pragma unroll
-    for (unsigned c = 0; c < 9; ++c) stx(I.diag_c, c * n + row, d[c]);
+    for (unsigned c = 0; c < 9; ++c) stx(I.diag_c, bp(c, row), d[c]);
 }
 
 __global__ void reset_sync_kernel(unsigned* sync) {
@@ -837,7 +836,7 @@ __device__ __forceinline__ Slot instance_slot(const Inst& I, int k, double rr, d
 // Setup / refinement step of one tile: r = b - A32 x (true residual with
 // the FP32 matrix), M^-1 = inverse(D) (setup only), z = M^-1 r, and p = z
 // when pdst is given (setup); tile partials <r,z>, <r,r> (and <b,b> at
-// setup).  M^-1 is stored planar.
+// setup).  M^-1 is stored grouped planar (bp).
 // fin(row, A x, x_row) of a tile SpMV, with the row's contact cross-slot
 // terms added to A x when a row-major solve carries a contact sidecar
 // (pcg_solver.cu:159-160, 180-181: every A x of the iteration is (A + C) x).
@@ -858,7 +857,6 @@ template <int L, bool ST, bool SL, class MatR>
 __device__ __forceinline__ void residual_tile(const chx_cloth_batch& B, const Inst& I, MatR mat, int rb, int re,
                                               bool setup, gp<float> pdst, PcgLds<ST>& S, double (&acc)[3]) {
     const gp<const float> xv = I.x;
-    const unsigned n = unsigned(I.n_rows);
     auto xs = [&](unsigned j) { return ld3g(xv, j); };
     tile_spmv<L, ST, SL>(I, mat, rb, re, S, xs, plus_contact<SL>(B, I, xs, [&](unsigned row, f3 Ax, f3) {
         const f3 b = ld3g(I.rhs, row);
@@ -869,10 +867,10 @@ __device__ __forceinline__ void residual_tile(const chx_cloth_batch& B, const In
             mat.diag_block(row, d9);
             inverse9(d9, mi);
 #pragma unroll
-            for (unsigned k = 0; k < 9; ++k) stx(I.minv, k * n + row, mi[k]);
+            for (unsigned k = 0; k < 9; ++k) stx(I.minv, bp(k, row), mi[k]);
         } else {
 #pragma unroll
-            for (unsigned k = 0; k < 9; ++k) mi[k] = ldx(gp<const float>(I.minv), k * n + row);
+            for (unsigned k = 0; k < 9; ++k) mi[k] = ldx(gp<const float>(I.minv), bp(k, row));
         }
         const f3 z = mv9(mi, r);
         st3g(I.r, row, r);
@@ -887,8 +885,8 @@ __device__ __forceinline__ void residual_tile(const chx_cloth_batch& B, const In
 // Matrix variants of the PCG kernel: the view of the inner SpMV (A) and of
 // the exact residual / preconditioner set-up (R).
 //   V = 0  row-major 3x3 blocks (standalone chx_pcg_solve_bsr3)
-//   V = 1  planar FP32 blocks (cloth instances)
-//   V = 2  planar BF16 copies for the inner SpMV, planar FP32 for refinement
+//   V = 1  grouped planar FP32 blocks (cloth instances)
+//   V = 2  grouped planar BF16 copies for the inner SpMV, FP32 for refinement
 template <int V>
 struct PcgMats;
 template <>
@@ -899,17 +897,17 @@ struct PcgMats<0> {
 template <>
 struct PcgMats<1> {
     static __device__ __forceinline__ MatP32 a(const Inst& I) {
-        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.n_upper)};
+        return MatP32{I.diag, I.values};
     }
     static __device__ __forceinline__ MatP32 r(const Inst& I) { return a(I); }
 };
 template <>
 struct PcgMats<2> {
     static __device__ __forceinline__ MatP16 a(const Inst& I) {
-        return MatP16{I.diag_c, I.values16, unsigned(I.n_rows), unsigned(I.n_upper)};
+        return MatP16{I.diag_c, I.values16};
     }
     static __device__ __forceinline__ MatP32 r(const Inst& I) {
-        return MatP32{I.diag, I.values, unsigned(I.n_rows), unsigned(I.n_upper)};
+        return MatP32{I.diag, I.values};
     }
 };
 
@@ -1064,14 +1062,13 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                     continue;
                 const float alpha = guarded_div(rho, sigma);
                 const gp<const float> pk = cur ? I.p1 : I.p0;
-                const unsigned n = unsigned(I.n_rows);
                 double acc[2] = {0.0, 0.0};
                 for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
                     const f3 p = ld3g(pk, row), qq = ld3g(I.q, row);
                     f3 x = ld3g(I.x, row), r = ld3g(I.r, row);
                     x = mk(fmaf(alpha, p.x, x.x), fmaf(alpha, p.y, x.y), fmaf(alpha, p.z, x.z));
                     r = mk(fmaf(-alpha, qq.x, r.x), fmaf(-alpha, qq.y, r.y), fmaf(-alpha, qq.z, r.z));
-                    const f3 zz = mv9p(gp<const float>(I.minv), n, row, r);
+                    const f3 zz = mv9p(gp<const float>(I.minv), row, r);
                     st3g(I.x, row, x);
                     st3g(I.r, row, r);
                     st3g(I.z, row, zz);
@@ -1343,7 +1340,8 @@ int chx_pcg_workspace_size(int n_rows, size_t* bytes) {
         const size_t n = size_t(n_rows);
         const size_t tiles = (n + CHX_TILE_ROWS - 1) / CHX_TILE_ROWS;
         auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
-        *bytes = 5 * al(12 * n) + al(36 * n) + al(8 * CHX_TILE_PARTIALS * tiles) + al(sizeof(int)) +
+        const size_t ng = (n + CHX_BLOCK_GROUP - 1) / CHX_BLOCK_GROUP * CHX_BLOCK_GROUP;  // minv: whole groups
+        *bytes = 5 * al(12 * n) + al(36 * ng) + al(8 * CHX_TILE_PARTIALS * tiles) + al(sizeof(int)) +
                  2 * al(sizeof(double)) + al(sizeof(int)) + al(SYNC_WORDS * sizeof(unsigned));
     });
 }
@@ -1400,7 +1398,7 @@ void solve_bsr3(int n_rows, const float* diag, const int* row_offsets, const int
         I.p0 = reinterpret_cast<float*>(take(12 * n));
         I.p1 = reinterpret_cast<float*>(take(12 * n));
         I.q = reinterpret_cast<float*>(take(12 * n));
-        I.minv = reinterpret_cast<float*>(take(36 * n));
+        I.minv = reinterpret_cast<float*>(take(36 * ((n + CHX_BLOCK_GROUP - 1) / CHX_BLOCK_GROUP * CHX_BLOCK_GROUP)));
         I.tile_begin = 0;
         I.tile_end = tiles;
         I.dt = 1.0f;

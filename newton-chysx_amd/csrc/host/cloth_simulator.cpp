@@ -365,22 +365,30 @@ double ClothSimulator::last_relative_residual(std::uintptr_t cuda_stream) {
 }
 
 namespace {
-// Device blocks are planar (component c of block i at p[c*count + i]); the
+// Device blocks are grouped planar (component c of block i at
+// p[((i / 64) * 9 + c) * 64 + i % 64], whole groups of CHX_BLOCK_GROUP); the
 // reference's debug taps return row-major 3x3 blocks.
+inline std::size_t grouped(int c, std::size_t i) {
+    const std::size_t G = CHX_BLOCK_GROUP;
+    return ((i / G) * 9 + static_cast<std::size_t>(c)) * G + i % G;
+}
+inline std::size_t group_floats(std::size_t count) {
+    return 9 * ((count + CHX_BLOCK_GROUP - 1) / CHX_BLOCK_GROUP * CHX_BLOCK_GROUP);
+}
 template <class T>
 void copy_planar_blocks(T* out, const T* dev, int count, const char* what) {
-    std::vector<T> planes(static_cast<std::size_t>(9) * count);
+    std::vector<T> planes(group_floats(static_cast<std::size_t>(count)));
     check(chx_memcpy_d2h(planes.data(), dev, sizeof(T) * planes.size(), 0), "cloth", what);
     for (int i = 0; i < count; ++i)
         for (int c = 0; c < 9; ++c)
-            out[static_cast<std::size_t>(9) * i + c] = planes[static_cast<std::size_t>(c) * count + i];
+            out[static_cast<std::size_t>(9) * i + c] = planes[grouped(c, static_cast<std::size_t>(i))];
 }
-// Off-diagonal blocks live once per symmetric pair (upper blocks, planes of
+// Off-diagonal blocks live once per symmetric pair (upper blocks over
 // n_upper value slots); out[k] = the block of CSR index k, row-major: the
 // stored upper block, or its partner's transpose for a lower one.
 template <class T>
 void copy_sliced_blocks(T* out, const T* dev, int n_upper, const std::vector<int>& slots, int nnz, const char* what) {
-    std::vector<T> planes(static_cast<std::size_t>(9) * n_upper);
+    std::vector<T> planes(group_floats(static_cast<std::size_t>(n_upper)));
     check(chx_memcpy_d2h(planes.data(), dev, sizeof(T) * planes.size(), 0), "cloth", what);
     for (int k = 0; k < nnz; ++k) {
         const bool lower = slots[k] < 0;
@@ -388,7 +396,7 @@ void copy_sliced_blocks(T* out, const T* dev, int n_upper, const std::vector<int
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) {
                 const int c = lower ? 3 * b + a : 3 * a + b;
-                out[static_cast<std::size_t>(9) * k + 3 * a + b] = planes[static_cast<std::size_t>(c) * n_upper + slot];
+                out[static_cast<std::size_t>(9) * k + 3 * a + b] = planes[grouped(c, slot)];
             }
     }
 }

@@ -226,7 +226,7 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     const auto fits = [](std::size_t count, std::size_t bytes_each) {
         return count * bytes_each < (std::size_t(1) << 32);
     };
-    if (!fits(static_cast<std::size_t>(n_), 36) || !fits(static_cast<std::size_t>(nup_), 36) ||
+    if (!fits(static_cast<std::size_t>(n_) + CHX_BLOCK_GROUP, 36) || !fits(static_cast<std::size_t>(nup_), 36) ||
         !fits(static_cast<std::size_t>(npad_), 4) ||
         !fits(tris_.cpu_size(), sizeof(float) * CHX_TRI_REC) || !fits(dihs_.cpu_size(), sizeof(float) * CHX_DIH_REC) ||
         !fits(pins_.cpu_size(), sizeof(chx_pin)) || !fits(ent.size(), sizeof(unsigned)) ||
@@ -247,19 +247,22 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     upload(dihs_, bytes_, stream);
     upload(stvks_, bytes_, stream);
     const std::size_t n = static_cast<std::size_t>(n_);
+    // grouped planar block arrays hold whole groups of CHX_BLOCK_GROUP blocks
+    const std::size_t ng = (n + CHX_BLOCK_GROUP - 1) / CHX_BLOCK_GROUP * CHX_BLOCK_GROUP;
+    const std::size_t nug = std::max<std::size_t>(static_cast<std::size_t>(nup_), CHX_BLOCK_GROUP);
     alloc(pin_rec_, pins_.cpu_size() * CHX_PIN_REC, bytes_);
     alloc(tri_rec_, tris_.cpu_size() * CHX_TRI_REC, bytes_);
     alloc(dih_rec_, dihs_.cpu_size() * CHX_DIH_REC, bytes_);
     alloc(stvk_rec_, stvks_.cpu_size() * CHX_STVK_REC, bytes_);
-    alloc(diag_, 9 * n, bytes_);
-    alloc(values_, 9 * std::max<std::size_t>(static_cast<std::size_t>(nup_), 1), bytes_);
+    alloc(diag_, 9 * ng, bytes_);
+    alloc(values_, 9 * nug, bytes_);
     alloc(rhs_, 3 * n, bytes_);
     alloc(r_, 3 * n, bytes_);
     alloc(z_, 3 * n, bytes_);
     alloc(p0_, 3 * n, bytes_);
     alloc(p1_, 3 * n, bytes_);
     alloc(q_, 3 * n, bytes_);
-    alloc(minv_, 9 * n, bytes_);
+    alloc(minv_, 9 * ng, bytes_);
     // Column window of every stage of CHX_STAGE_ROWS rows (its own rows
     // included): the rows of the source vector its SpMV reads.
     {
@@ -282,8 +285,8 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     }
     update_shapes(s, stream);
     if (s.bf16_blocks) {
-        alloc(diag_c_, 9 * n, bytes_);
-        alloc(values16_, 9 * std::max<std::size_t>(static_cast<std::size_t>(nup_), 1), bytes_);
+        alloc(diag_c_, 9 * ng, bytes_);
+        alloc(values16_, 9 * nug, bytes_);
     } else {
         diag_c_.release();
         values16_.release();
