@@ -42,6 +42,71 @@ def b_setup(n, nnz):
     return b_spmv(n, nnz) + 168 * n
 
 
+def kernel_breakdown(cloth, dt, stream, lanes, reps=5):
+    """Per-kernel times and algorithmic rates of one instance's substep:
+    element pass (x_n 12N + triangle 52T + dihedral 20D + pin 16P bytes),
+    gather pass (predictor 56N + rhs 28N + diagonal 36N + blocks 36 nnz), and
+    the persistent PCG launch split by its solver trace into the SpMV phase
+    (B_spmv + 36N per iteration: the folded p update) and the vector /
+    reduction phase (120N per iteration), plus the whole launch."""
+    import numpy as np
+    import torch
+
+    import chysx
+
+    sh = stream.cuda_stream
+    sim = cloth.sim
+    N, nnz = sim.num_particles(), sim.num_off_diag_blocks()
+    T, D, P = sim.num_fem_stretch_triangles(), sim.num_bending_dihedrals(), sim.num_pins()
+    b = chysx.ClothBatch()
+    b.set_lanes_per_row(lanes)
+    b.add_instance(sim)
+    b.set_trace(1024)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    te, tg, ts, ta, tb, ks = [], [], [], [], [], []
+    with torch.cuda.stream(stream):
+        b.step(dt, sh)
+        for _ in range(reps):
+            ev[0].record(stream)
+            b.element_pass(dt, sh)
+            ev[1].record(stream)
+            b.gather_pass(sh)
+            ev[2].record(stream)
+            b.solve(sh)
+            ev[3].record(stream)
+            ev[3].synchronize()
+            te.append(ev[0].elapsed_time(ev[1]))
+            tg.append(ev[1].elapsed_time(ev[2]))
+            ts.append(ev[2].elapsed_time(ev[3]))
+            tr = np.array(b.last_trace(sh)).astype(np.int64)
+            d = np.diff(tr[:, 0]) / 100.0  # us (100 MHz stamps)
+            ta.append(float(d[0::2].sum()))
+            tb.append(float(d[1::2].sum()))
+            ks.append(int(b.last_pcg_iterations(sh)[0]))
+    b.set_trace(0)
+    k = float(np.mean(ks))
+    rate = lambda nbytes, ms: round(nbytes / (ms * 1e-3) / 1e9, 1)
+    e_ms, g_ms, s_ms = float(np.median(te)), float(np.median(tg)), float(np.median(ts))
+    a_us, bv_us = float(np.median(ta)), float(np.median(tb))
+    e_b = 12 * N + 52 * T + 20 * D + 16 * P
+    g_b = 120 * N + 36 * nnz
+    a_b = k * (b_spmv(N, nnz) + 36 * N)
+    v_b = k * 120 * N
+    s_b = b_setup(N, nnz) + k * b_iter(N, nnz) + 48 * N
+    return {
+        "vertices": N, "off_diag_blocks": nnz, "triangles": T, "dihedrals": D, "pins": P, "pcg_iterations": k,
+        "element_pass": {"ms": round(e_ms, 4), "bytes": e_b, "GB_per_s": rate(e_b, e_ms)},
+        "gather_pass": {"ms": round(g_ms, 4), "bytes": g_b, "GB_per_s": rate(g_b, g_ms)},
+        "pcg_spmv_phase": {"us_per_iteration": round(a_us / max(k, 1.0), 2), "bytes": a_b,
+                           "GB_per_s": rate(a_b, a_us * 1e-3)},
+        "pcg_vector_phase": {"us_per_iteration": round(bv_us / max(k, 1.0), 2), "bytes": v_b,
+                             "GB_per_s": rate(v_b, bv_us * 1e-3)},
+        "pcg_launch": {"ms": round(s_ms, 4), "bytes": s_b, "GB_per_s": rate(s_b, s_ms)},
+        "note": "algorithmic bytes (reference counts); phase times from the solver's per-barrier trace, barrier "
+                "waits included; a 256x256 working set fits the 256 MB Infinity Cache",
+    }
+
+
 def load_traffic():
     """HBM bytes per PCG launch from the committed rocprofv3 --pmc summary."""
     path = os.path.join(REPO, "profiles", "pcg_traffic.json")
@@ -322,6 +387,11 @@ def main():
         plain_large = {"value": big.spec.n * reps / bw, "unit": "vertex-substeps/s", "vertices": big.spec.n,
                        "ms_per_substep": bw / reps * 1e3, "pcg_iterations": big.sim.last_pcg_iterations(sh)}
         log(f"plain 256x256: {plain_large['ms_per_substep']:.2f} ms/substep")
+        plain_large["kernels"] = kernel_breakdown(big, dt, stream, args.lanes)
+        kb = plain_large["kernels"]
+        log("plain 256x256 kernels: " + ", ".join(
+            f"{name} {kb[name]['GB_per_s']} GB/s" for name in
+            ("element_pass", "gather_pass", "pcg_spmv_phase", "pcg_vector_phase", "pcg_launch")))
     serving_leg = None
     if not args.no_serving:
         # A sustainable stream (about 70 % of the batched rate) from the

@@ -369,6 +369,10 @@ int chx_cloth_step(const chx_cloth_batch* batch, uintptr_t stream);
  * system before solving (debug_dump_last_solve) or time the parts. */
 int chx_cloth_assemble(const chx_cloth_batch* batch, uintptr_t stream);
 int chx_cloth_solve(const chx_cloth_batch* batch, int finalize, uintptr_t stream);
+/* The two halves of chx_cloth_assemble, for callers that time the element
+ * pass (forces + Hessian records) and the gather pass (assembly) apart. */
+int chx_cloth_element_pass(const chx_cloth_batch* batch, uintptr_t stream);
+int chx_cloth_gather_pass(const chx_cloth_batch* batch, uintptr_t stream);
 
 /* Reads the device status word of the batch (non-zero after a grid-barrier
  * timeout inside the persistent solver) and clears it; synchronous. */

@@ -49,6 +49,8 @@ _SIGS = {
     "chx_cloth_step": (_I, [_P, _U]),
     "chx_cloth_assemble": (_I, [_P, _U]),
     "chx_cloth_solve": (_I, [_P, _I, _U]),
+    "chx_cloth_element_pass": (_I, [_P, _U]),
+    "chx_cloth_gather_pass": (_I, [_P, _U]),
     "chx_batch_check": (_I, [_P, _U]),
     "chx_cloth_step_graph": (_I, [_P, _U, _P]),
     "chx_graph_launch": (_I, [_P, _U]),

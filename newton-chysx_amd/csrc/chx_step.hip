@@ -520,6 +520,9 @@ struct PcgShared {
     int wsum[kBlock / kWave];
     int nact;
     int ctile[kChunk];  // the tiles of the chunk at hand, -1 past the end
+    int cinst[kChunk];  // their instances and row ranges (tile records)
+    int crb[kChunk];
+    int cre[kChunk];
 };
 
 // Stage arrays of the LDS-staged SpMV.
@@ -675,10 +678,17 @@ __device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const do
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
     double done = 0.0, ntl = 1.0;
     if (t >= 0) {
-        int inst, rb, re;
-        get_tile(B, t, inst, rb, re);
-        const Inst I = inst_at(B, inst);
-        for (int u = I.tile_begin + l; u < I.tile_end; u += 16) {
+        const int inst = S.cinst[m];
+        int tb, te;
+        if (B.inst != nullptr && B.n_instances <= kCompactMax) {  // tile ranges in LDS
+            tb = S.tb[inst];
+            te = tb + S.tc[inst];
+        } else {
+            const Inst I = inst_at(B, inst);
+            tb = I.tile_begin;
+            te = I.tile_end;
+        }
+        for (int u = tb + l; u < te; u += 16) {
             const gp<const double> p = rec_at(part, unsigned(u), unsigned(CHX_TILE_PARTIALS));
             s0 += p[f0];
             s1 += p[f1];
@@ -687,7 +697,7 @@ __device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const do
             if (F4) s4 += p[f4];
         }
         if (l == 0) done = __hip_atomic_load(&B.inst_done[inst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1.0 : 0.0;
-        ntl = double(I.tile_end - I.tile_begin);
+        ntl = double(te - tb);
     }
     s0 = seg_sum<16>(s0);
     s1 = seg_sum<16>(s1);
@@ -798,10 +808,26 @@ __device__ __forceinline__ bool next_chunk(const chx_cloth_batch& B, PcgShared& 
     const int T = compact ? S.nact : B.n_tiles;
     if (threadIdx.x < kChunk) {
         const int a = list_entry(T, c * kChunk + int(threadIdx.x));
-        S.ctile[threadIdx.x] = (a < 0) ? -1 : (compact ? compact_tile(S, B.n_instances, a) : a);
+        const int t = (a < 0) ? -1 : (compact ? compact_tile(S, B.n_instances, a) : a);
+        S.ctile[threadIdx.x] = t;
+        // the chunk's tile records, loaded once for the phase's tile loop
+        if (t >= 0) {
+            int inst, rb, re;
+            get_tile(B, t, inst, rb, re);
+            S.cinst[threadIdx.x] = inst;
+            S.crb[threadIdx.x] = rb;
+            S.cre[threadIdx.x] = re;
+        }
     }
     __syncthreads();
     return S.ctile[0] >= 0;
+}
+
+// Tile record m of the chunk at hand (uniform values in scalar registers).
+__device__ __forceinline__ void chunk_tile(const PcgShared& S, int m, int& inst, int& rb, int& re) {
+    inst = __builtin_amdgcn_readfirstlane(S.cinst[m]);
+    rb = __builtin_amdgcn_readfirstlane(S.crb[m]);
+    re = __builtin_amdgcn_readfirstlane(S.cre[m]);
 }
 
 // What an instance does in iteration slot k.  `true_res`: the residual at
@@ -978,7 +1004,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 const int t = S.ctile[m];
                 if (t < 0) break;
                 int inst, rb, re;
-                get_tile(B, t, inst, rb, re);
+                chunk_tile(S, m, inst, rb, re);
                 const Inst I = inst_at(B, inst);
                 const double rho = S.sc[m][0], rho_prev = S.sc[m][1], rr = S.sc[m][2], bb = S.sc[m][3];
                 const bool finished = S.sc[m][4] != 0.0;
@@ -1054,7 +1080,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 const int t = S.ctile[m];
                 if (t < 0) break;
                 int inst, rb, re;
-                get_tile(B, t, inst, rb, re);
+                chunk_tile(S, m, inst, rb, re);
                 const Inst I = inst_at(B, inst);
                 const double rho = S.sc[m][0], sigma = S.sc[m][1], rr = S.sc[m][2], bb = S.sc[m][3];
                 const int rstate = BF16 ? int(S.sc[m][5] + 0.5) : 1;
@@ -1265,12 +1291,15 @@ int slot_items(const chx_cloth_batch& B) {
     return item_count(B.single.n_rows + B.single.nnz);
 }
 
-void assemble(const chx_cloth_batch& B, hipStream_t stream) {
+void element_pass(const chx_cloth_batch& B, hipStream_t stream) {
     const int ne = elem_items(B);
     if (ne > 0) {
         element_kernel<<<ne, kBlock, 0, stream>>>(B);
         hip_check(hipGetLastError(), "element_kernel launch");
     }
+}
+
+void gather_pass(const chx_cloth_batch& B, hipStream_t stream) {
     const int ns = slot_items(B);
     if (ns > 0) {
         gather_kernel<<<ns, kBlock, 0, stream>>>(B);
@@ -1283,6 +1312,11 @@ void assemble(const chx_cloth_batch& B, hipStream_t stream) {
         reset_sync_kernel<<<1, kBlock, 0, stream>>>(B.sync);
         hip_check(hipGetLastError(), "reset_sync_kernel launch");
     }
+}
+
+void assemble(const chx_cloth_batch& B, hipStream_t stream) {
+    element_pass(B, stream);
+    gather_pass(B, stream);
 }
 
 }  // namespace
@@ -1465,6 +1499,20 @@ int chx_pcg_solve_bsr3_contact(int n_rows, const float* diag, const int* row_off
         solve_bsr3(n_rows, diag, row_offsets, col_indices, values, b, x, max_iterations, tol, lanes_per_row,
                    workspace, workspace_bytes, op, contact_workspace, contact_workspace_bytes, iterations, rr, bb,
                    stream);
+    });
+}
+
+int chx_cloth_element_pass(const chx_cloth_batch* batch, uintptr_t stream) {
+    return guarded([&] {
+        validate_batch(batch);
+        element_pass(*batch, as_stream(stream));
+    });
+}
+
+int chx_cloth_gather_pass(const chx_cloth_batch* batch, uintptr_t stream) {
+    return guarded([&] {
+        validate_batch(batch);
+        gather_pass(*batch, as_stream(stream));
     });
 }
 

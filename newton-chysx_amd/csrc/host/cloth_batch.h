@@ -44,6 +44,10 @@ public:
     // persistent PCG solve with the fused finalize.
     void assemble(float dt, std::uintptr_t cuda_stream = 0);
     void solve(std::uintptr_t cuda_stream = 0);
+    // The two halves of assemble(), for timing them apart: the element pass
+    // (forces + Hessian records at x_n) and the gather pass (assembly).
+    void element_pass(float dt, std::uintptr_t cuda_stream = 0);
+    void gather_pass(std::uintptr_t cuda_stream = 0);
 
     int num_instances() const noexcept { return static_cast<int>(sims_.size()); }
     int num_rows() const noexcept;

@@ -568,6 +568,12 @@ void BatchRunner::drop_graph() noexcept {
 void BatchRunner::assemble(std::uintptr_t stream) {
     check(chx_cloth_assemble(&desc_, stream), "cloth::ClothSimulator", "assemble");
 }
+void BatchRunner::element_pass(std::uintptr_t stream) {
+    check(chx_cloth_element_pass(&desc_, stream), "cloth::ClothSimulator", "element pass");
+}
+void BatchRunner::gather_pass(std::uintptr_t stream) {
+    check(chx_cloth_gather_pass(&desc_, stream), "cloth::ClothSimulator", "gather pass");
+}
 void BatchRunner::solve(bool finalize, std::uintptr_t stream) {
     check(chx_cloth_solve(&desc_, finalize ? 1 : 0, stream), "solver::PCGSolver", "solve");
 }

@@ -129,6 +129,8 @@ public:
     void drop_graph() noexcept;
     int graph_captures() const noexcept { return graph_captures_; }
     void assemble(std::uintptr_t stream);
+    void element_pass(std::uintptr_t stream);
+    void gather_pass(std::uintptr_t stream);
     void solve(bool finalize, std::uintptr_t stream);
     double energy(int kind_mask, std::uintptr_t stream);
 
