@@ -109,7 +109,7 @@ typedef struct chx_contact_op {
 #define CHX_TILE_PARTIALS 8
 
 /* 32-bit words of the grid-barrier / control area `sync` of a batch. */
-#define CHX_SYNC_WORDS 448
+#define CHX_SYNC_WORDS 960
 
 /* Default rows per PCG row tile (the unit of workgroup assignment and of the
  * FP64 partial sums; the batch may use any multiple of CHX_STAGE_ROWS),

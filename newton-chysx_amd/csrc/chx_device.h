@@ -418,8 +418,9 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double* lds) {
 // generation does not contend with arrivals:
 //   sync[SYNC_COUNT] arrivals, sync[SYNC_GEN] generation, sync[SYNC_ABORT]
 //   abort flag, sync[SYNC_ACTIVE] active-instance counter, sync[SYNC_STATUS]
-//   status word, then the arrival count seen by a workgroup that gave up and
-//   the grid size.
+//   status word, then, from a workgroup that gave up, the number of complete
+//   workgroup groups, the grid size and the workgroups waiting in the
+//   incomplete groups.
 enum : int {
     SYNC_COUNT = 0,
     SYNC_GEN = 64,
@@ -428,10 +429,14 @@ enum : int {
     SYNC_STATUS = 256,
     SYNC_SEEN = 257,
     SYNC_GRID = 258,
-    SYNC_SNAP = 320,  // two snapshots of SYNC_ACTIVE, 64 words apart
+    SYNC_WAITING = 259,
+    SYNC_SNAP = 320,    // two snapshots of SYNC_ACTIVE, 64 words apart
+    SYNC_XCOUNT = 448,  // arrival counters of the 8 workgroup groups, 64 words apart
     SYNC_WORDS = CHX_SYNC_WORDS
 };
-static_assert(SYNC_SNAP + 64 < CHX_SYNC_WORDS, "sync area too small");
+constexpr unsigned kSyncGroups = 8;  // workgroups g = x (mod 8): one XCD each
+static_assert(SYNC_SNAP + 64 < SYNC_XCOUNT && SYNC_XCOUNT + 64 * kSyncGroups <= CHX_SYNC_WORDS,
+              "sync area too small");
 // The barrier is self-resetting: the last arriving workgroup zeroes the
 // count and bumps the generation, so consecutive launches reuse it.  Every
 // wait is bounded in time: a workgroup that has waited kWaitTicks of the
@@ -465,9 +470,15 @@ __device__ __forceinline__ void backoff(unsigned& level) {
 // acquire per workgroup, both from thread 0 (the workgroup barrier orders the
 // other waves' accesses around them), arrives with a relaxed atomic, and
 // polls with relaxed loads -- an acquiring poll would invalidate the L2 on
-// every probe.  Each wave first drains its own outstanding stores.  The last
-// arriver acquires the other arrivals before it releases the generation, so
-// every workgroup's writes reach every other one.
+// every probe.  Each wave first drains its own outstanding stores.
+//
+// Arrival is two-level: agent-scope atomics on one word are performed one
+// after another at the memory side (~23 ns each), so one counter for G
+// workgroups costs ~G x 23 ns.  Workgroup g arrives at the counter of its
+// group g mod 8 (the dispatcher's XCD); the group's last arriver resets it,
+// acquires the group's arrivals, releases them on, and arrives at the top
+// counter; the last of the groups acquires all of them before it releases
+// the generation, so every workgroup's writes reach every other one.
 //
 // With snap = 0 or 1 the last arriver also copies SYNC_ACTIVE into snapshot
 // slot `snap` before it releases the generation, and every workgroup reads
@@ -485,9 +496,19 @@ __device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks, int 
         int ok = 1;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const unsigned gen = ld_relaxed(&sync[SYNC_GEN]);
-        const unsigned arrived =
-            __hip_atomic_fetch_add(&sync[SYNC_COUNT], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
+        const unsigned x = blockIdx.x % kSyncGroups;
+        const unsigned groups = nblocks < kSyncGroups ? nblocks : kSyncGroups;
+        const unsigned gsize = (nblocks - x + kSyncGroups - 1u) / kSyncGroups;  // workgroups of group x
+        unsigned* xc = &sync[SYNC_XCOUNT + 64 * x];
+        bool last = false;
+        if (__hip_atomic_fetch_add(xc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1u) {
+            st_relaxed(xc, 0u);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            last = __hip_atomic_fetch_add(&sync[SYNC_COUNT], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                   groups - 1u;
+        }
+        if (last) {
             st_relaxed(&sync[SYNC_COUNT], 0u);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             if (snap >= 0) st_relaxed(&sync[SYNC_SNAP + 64 * snap], ld_relaxed(&sync[SYNC_ACTIVE]));
@@ -502,7 +523,10 @@ __device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks, int 
                     break;
                 }
                 if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                    unsigned waiting = 0;
+                    for (unsigned q = 0; q < kSyncGroups; ++q) waiting += ld_relaxed(&sync[SYNC_XCOUNT + 64 * q]);
                     st_relaxed(&sync[SYNC_SEEN], ld_relaxed(&sync[SYNC_COUNT]));
+                    st_relaxed(&sync[SYNC_WAITING], waiting);
                     st_relaxed(&sync[SYNC_GRID], nblocks);
                     st_relaxed(&sync[SYNC_ABORT], 1u);
                     st_relaxed(&sync[SYNC_STATUS], 1u);

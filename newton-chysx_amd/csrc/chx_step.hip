@@ -688,13 +688,32 @@ __device__ __forceinline__ void chunk_sums(const chx_cloth_batch& B, gp<const do
             tb = I.tile_begin;
             te = I.tile_end;
         }
-        for (int u = tb + l; u < te; u += 16) {
-            const gp<const double> p = rec_at(part, unsigned(u), unsigned(CHX_TILE_PARTIALS));
-            s0 += p[f0];
-            s1 += p[f1];
-            s2 += p[f2];
-            s3 += p[f3_];
-            if (F4) s4 += p[f4];
+        // Four of the lane's tiles per pass: their loads are issued together,
+        // then added in ascending tile order (the same sums as one at a time).
+        for (int u0 = tb + l; u0 < te; u0 += 64) {
+            double v[4][5];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int u = u0 + 16 * q;
+                if (u < te) {
+                    const gp<const double> p = rec_at(part, unsigned(u), unsigned(CHX_TILE_PARTIALS));
+                    v[q][0] = p[f0];
+                    v[q][1] = p[f1];
+                    v[q][2] = p[f2];
+                    v[q][3] = p[f3_];
+                    v[q][4] = F4 ? p[f4] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (u0 + 16 * q < te) {
+                    s0 += v[q][0];
+                    s1 += v[q][1];
+                    s2 += v[q][2];
+                    s3 += v[q][3];
+                    if (F4) s4 += v[q][4];
+                }
+            }
         }
         if (l == 0) done = __hip_atomic_load(&B.inst_done[inst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1.0 : 0.0;
         ntl = double(te - tb);
@@ -1594,9 +1613,13 @@ int chx_batch_check(const chx_cloth_batch* batch, uintptr_t stream) {
         if (words[SYNC_STATUS] != 0u) {
             reset_sync_kernel<<<1, kBlock, 0, s>>>(batch->sync);
             hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(reset)");
-            throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out (" +
+            const unsigned grid = words[SYNC_GRID];
+            throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out (grid of " +
+                                            std::to_string(grid) + " workgroups: " +
                                             std::to_string(words[SYNC_SEEN]) + " of " +
-                                            std::to_string(words[SYNC_GRID]) + " workgroups arrived)");
+                                            std::to_string(grid < kSyncGroups ? grid : kSyncGroups) +
+                                            " groups complete, " + std::to_string(words[SYNC_WAITING]) +
+                                            " workgroups waiting in the others)");
         }
     });
 }
