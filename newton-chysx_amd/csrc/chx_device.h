@@ -494,8 +494,15 @@ __device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks, int 
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // The generation is read before the release fence, whose wait for all
+        // outstanding memory operations completes the read before this
+        // workgroup arrives.  (A read issued after the fence could still be in
+        // flight -- queued behind the pollers of the same word -- when the
+        // last arrival bumps the generation; it would return the next
+        // generation and this workgroup would wait one barrier too long,
+        // stalling all others at the next barrier.)
         const unsigned gen = ld_relaxed(&sync[SYNC_GEN]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const unsigned x = blockIdx.x % kSyncGroups;
         const unsigned groups = nblocks < kSyncGroups ? nblocks : kSyncGroups;
         const unsigned gsize = (nblocks - x + kSyncGroups - 1u) / kSyncGroups;  // workgroups of group x
