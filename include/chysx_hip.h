@@ -160,7 +160,7 @@ typedef struct chx_instance {
     float* pin_rec;                  /* [n_pins*CHX_PIN_REC]                                 */
     float* tri_rec;                  /* [n_tris*CHX_TRI_REC]                                 */
     float* dih_rec;                  /* [n_dihs*CHX_DIH_REC]                                 */
-    const unsigned* contrib_offsets; /* [n_rows+nnz+1]                                       */
+    const unsigned* contrib_offsets; /* [n_rows+n_upper+1]: diagonal blocks, then value slots */
     const unsigned* contribs;        /* gather-map entries                                   */
     float* rhs;                      /* b [n_rows*3]                                         */
     float* x;                        /* dx: warm start in, solution out [n_rows*3]           */

@@ -369,7 +369,7 @@ __device__ __forceinline__ void get_slot_item(const chx_cloth_batch& B, int w, i
     }
     inst = 0;
     b = w * CHX_ITEM_SIZE;
-    e = min(B.single.n_rows + B.single.nnz, b + CHX_ITEM_SIZE);
+    e = min(B.single.n_rows + B.single.n_upper, b + CHX_ITEM_SIZE);
 }
 
 // ---- wave64 / workgroup reductions (fixed order => deterministic) ---------

@@ -123,11 +123,9 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
     const Inst I = inst_at(B, inst);
     const unsigned n = unsigned(I.n_rows);
     const bool is_diag = s < n;
-    // Symmetric sliced layout: the upper block of CSR index e lives in value
-    // slot block_slots[e]; lower blocks (their partners' transposes) are not
-    // assembled.
-    const int bslot = is_diag ? 0 : ldx(I.block_slots, s - n);
-    if (bslot < 0) return;
+    // Slots n.. are the value slots of the symmetric sliced layout (upper
+    // blocks, in storage order, padding included); lower blocks are their
+    // partners' transposes and are not assembled.
     const unsigned c0 = ldx(I.contrib_offsets, s), c1 = ldx(I.contrib_offsets, s + 1u);
     float h[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     f3 grad = mk(0.f, 0.f, 0.f);
@@ -286,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
 #pragma unroll
         for (unsigned k = 0; k < 9; ++k) stx(I.diag, bp(k, row), h[k]);
     } else {
-        const unsigned slot = unsigned(bslot);
+        const unsigned slot = s - n;
 #pragma unroll
         for (unsigned k = 0; k < 9; ++k) stx(I.values, bp(k, slot), h[k]);
         if (I.values16 != nullptr) {
@@ -1307,7 +1305,7 @@ int elem_items(const chx_cloth_batch& B) {
 }
 int slot_items(const chx_cloth_batch& B) {
     if (B.inst != nullptr) return B.n_slot_items;
-    return item_count(B.single.n_rows + B.single.nnz);
+    return item_count(B.single.n_rows + B.single.n_upper);
 }
 
 void element_pass(const chx_cloth_batch& B, hipStream_t stream) {

@@ -92,61 +92,6 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     col_indices_.host() = s.pattern->col_indices;
     pack_elements(s);
 
-    // Gather map.  Slots: diagonal block of row r -> r, off-diagonal block k
-    // -> n + k.  Entries are appended in element order (pins, stretch,
-    // shear, StVK, bending; element; a; b) — the order of the reference's
-    // launches, StVK (no reference counterpart) after the BW98 triangles —
-    // so every slot sums its terms in that order.
-    const std::size_t S = static_cast<std::size_t>(n_) + nnz_;
-    auto slot_of = [&](int slot) -> std::size_t {
-        return slot < 0 ? static_cast<std::size_t>(-slot - 1) : static_cast<std::size_t>(n_) + slot;
-    };
-    auto walk = [&](auto&& visit) {
-        if (s.pins) {
-            const auto& sl = s.pins->hessian_slots();
-            if (static_cast<int>(sl.size()) != s.pins->size()) throw std::runtime_error("chysx::engine: pin slots not bound");
-            for (int c = 0; c < s.pins->size(); ++c) visit(slot_of(sl[c]), CHX_CONTRIB_PIN, 0, 0, std::size_t(c));
-        }
-        std::size_t tri_base = 0;
-        for (const constraint::TriangleStretchConstraint* ts : {s.stretch, s.shear}) {
-            if (!ts) continue;
-            const auto& sl = ts->hessian_slots();
-            if (sl.size() != 9u * ts->size()) throw std::runtime_error("chysx::engine: triangle slots not bound");
-            for (int t = 0; t < ts->size(); ++t)
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b) visit(slot_of(sl[9 * t + 3 * a + b]), CHX_CONTRIB_TRI, a, b, tri_base + t);
-            tri_base += ts->size();
-        }
-        if (s.stvk) {
-            const auto& sl = s.stvk->hessian_slots();
-            if (sl.size() != 9u * s.stvk->size()) throw std::runtime_error("chysx::engine: StVK slots not bound");
-            for (int t = 0; t < s.stvk->size(); ++t)
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b)
-                        visit(slot_of(sl[9 * t + 3 * a + b]), CHX_CONTRIB_STVK, a, b, std::size_t(t));
-        }
-        if (s.bending) {
-            const auto& sl = s.bending->hessian_slots();
-            if (sl.size() != 16u * s.bending->size()) throw std::runtime_error("chysx::engine: dihedral slots not bound");
-            for (int e = 0; e < s.bending->size(); ++e)
-                for (int a = 0; a < 4; ++a)
-                    for (int b = 0; b < 4; ++b)
-                        visit(slot_of(sl[16 * e + 4 * a + b]), CHX_CONTRIB_DIH, a, b, std::size_t(e));
-        }
-    };
-    std::vector<unsigned long long> count(S + 1, 0);
-    walk([&](std::size_t g, unsigned, int, int, std::size_t) { ++count[g + 1]; });
-    for (std::size_t k = 0; k < S; ++k) count[k + 1] += count[k];
-    if (count[S] > std::numeric_limits<unsigned>::max())
-        throw std::runtime_error("chysx::engine: gather map exceeds 2^32 entries");
-    auto& off = contrib_offsets_.host();
-    off.resize(S + 1);
-    for (std::size_t k = 0; k <= S; ++k) off[k] = static_cast<unsigned>(count[k]);
-    auto& ent = contribs_.host();
-    ent.assign(count[S], 0u);
-    std::vector<unsigned> fill(off.begin(), off.end() - 1);
-    walk([&](std::size_t g, unsigned kind, int a, int b, std::size_t id) { ent[fill[g]++] = encode(kind, a, b, id); });
-
     // Symmetric sliced layout (slice s = rows 64s..64s+63).  Entries: the
     // j-th off-diagonal column of row 64s+l at entry slot
     // slice_offsets[s] + 64 j + l, encoded (col << 6) | jj with jj the
@@ -221,6 +166,73 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
             }
         }
     }
+
+
+    // Gather map.  Slots: diagonal block of row r -> r, upper off-diagonal
+    // block of CSR index k -> n + its value slot (block_slots[k]), so the
+    // gather writes value slots in order; lower blocks (their partners'
+    // transposes) are not assembled and get no entries.  Entries are
+    // appended in element order (pins, stretch, shear, StVK, bending;
+    // element; a; b) — the order of the reference's launches, StVK (no
+    // reference counterpart) after the BW98 triangles — so every slot sums
+    // its terms in that order.
+    const std::size_t S = static_cast<std::size_t>(n_) + static_cast<std::size_t>(nup_);
+    constexpr std::size_t kNoSlot = ~std::size_t(0);
+    const auto& bslots = block_slots_.host();
+    auto slot_of = [&](int slot) -> std::size_t {
+        if (slot < 0) return static_cast<std::size_t>(-slot - 1);
+        const int v = bslots[static_cast<std::size_t>(slot)];
+        return v < 0 ? kNoSlot : static_cast<std::size_t>(n_) + static_cast<std::size_t>(v);
+    };
+    auto walk = [&](auto&& visit) {
+        if (s.pins) {
+            const auto& sl = s.pins->hessian_slots();
+            if (static_cast<int>(sl.size()) != s.pins->size()) throw std::runtime_error("chysx::engine: pin slots not bound");
+            for (int c = 0; c < s.pins->size(); ++c) visit(slot_of(sl[c]), CHX_CONTRIB_PIN, 0, 0, std::size_t(c));
+        }
+        std::size_t tri_base = 0;
+        for (const constraint::TriangleStretchConstraint* ts : {s.stretch, s.shear}) {
+            if (!ts) continue;
+            const auto& sl = ts->hessian_slots();
+            if (sl.size() != 9u * ts->size()) throw std::runtime_error("chysx::engine: triangle slots not bound");
+            for (int t = 0; t < ts->size(); ++t)
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) visit(slot_of(sl[9 * t + 3 * a + b]), CHX_CONTRIB_TRI, a, b, tri_base + t);
+            tri_base += ts->size();
+        }
+        if (s.stvk) {
+            const auto& sl = s.stvk->hessian_slots();
+            if (sl.size() != 9u * s.stvk->size()) throw std::runtime_error("chysx::engine: StVK slots not bound");
+            for (int t = 0; t < s.stvk->size(); ++t)
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        visit(slot_of(sl[9 * t + 3 * a + b]), CHX_CONTRIB_STVK, a, b, std::size_t(t));
+        }
+        if (s.bending) {
+            const auto& sl = s.bending->hessian_slots();
+            if (sl.size() != 16u * s.bending->size()) throw std::runtime_error("chysx::engine: dihedral slots not bound");
+            for (int e = 0; e < s.bending->size(); ++e)
+                for (int a = 0; a < 4; ++a)
+                    for (int b = 0; b < 4; ++b)
+                        visit(slot_of(sl[16 * e + 4 * a + b]), CHX_CONTRIB_DIH, a, b, std::size_t(e));
+        }
+    };
+    std::vector<unsigned long long> count(S + 1, 0);
+    walk([&](std::size_t g, unsigned, int, int, std::size_t) {
+        if (g != kNoSlot) ++count[g + 1];
+    });
+    for (std::size_t k = 0; k < S; ++k) count[k + 1] += count[k];
+    if (count[S] > std::numeric_limits<unsigned>::max())
+        throw std::runtime_error("chysx::engine: gather map exceeds 2^32 entries");
+    auto& off = contrib_offsets_.host();
+    off.resize(S + 1);
+    for (std::size_t k = 0; k <= S; ++k) off[k] = static_cast<unsigned>(count[k]);
+    auto& ent = contribs_.host();
+    ent.assign(count[S], 0u);
+    std::vector<unsigned> fill(off.begin(), off.end() - 1);
+    walk([&](std::size_t g, unsigned kind, int a, int b, std::size_t id) {
+        if (g != kNoSlot) ent[fill[g]++] = encode(kind, a, b, id);
+    });
 
     // The kernels address every per-instance array with 32-bit byte offsets.
     const auto fits = [](std::size_t count, std::size_t bytes_each) {
@@ -504,7 +516,7 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds, std::uintptr_t 
         for (int kind = 0; kind < 4; ++kind)
             for (int b = 0; b < counts[kind]; b += CHX_ITEM_SIZE)
                 ei.insert(ei.end(), {inst, kind, b, std::min(counts[kind], b + CHX_ITEM_SIZE)});
-        const int slots = I.n_rows + I.nnz;
+        const int slots = I.n_rows + I.n_upper;  // gather slots: diagonal blocks, then value slots
         for (int b = 0; b < slots; b += CHX_ITEM_SIZE) si.insert(si.end(), {inst, b, std::min(slots, b + CHX_ITEM_SIZE), 0});
     }
     elem_items_.copy_to_device(stream);
