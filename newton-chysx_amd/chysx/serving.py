@@ -10,9 +10,12 @@ A job moves through four states:
 
   waiting    arrived, not admitted yet;
   preparing  admitted first come first served while its estimated HBM
-             footprint fits the budget; its set-up (topology, Hessian pattern,
-             scatter maps, device arrays) runs on a worker thread while the
-             GPU keeps stepping the running jobs;
+             footprint fits the budget and fewer than `max_preparing` jobs
+             are in set-up (by default twice the executor's set-up workers,
+             so that under overload late arrivals wait without holding HBM);
+             its set-up (topology, Hessian pattern, scatter maps, device
+             arrays) runs on a worker thread while the GPU keeps stepping
+             the running jobs;
   running    in the batch; it joins at the next substep boundary once its
              set-up is done;
   finished   after its last substep; it leaves the batch and frees its memory.
@@ -109,6 +112,7 @@ class GpuExecutor:
         for _ in range(max(1, prepare_workers)):
             self.prep_streams.put(torch.cuda.Stream())
         self.keep_results = keep_results
+        self.prepare_workers = prepare_workers
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=prepare_workers) if prepare_workers > 0 else None
 
     def _build(self, spec, dt):
@@ -161,10 +165,14 @@ class GpuExecutor:
 
 class ServingEngine:
     def __init__(self, executor, hbm_budget_bytes: float = 200e9, max_concurrency: int = 1 << 30,
-                 clock=time.perf_counter, sleep=time.sleep):
+                 clock=time.perf_counter, sleep=time.sleep, max_preparing: int | None = None):
         self.ex = executor
         self.budget = float(hbm_budget_bytes)
         self.max_conc = max_concurrency
+        if max_preparing is None:
+            workers = getattr(executor, "prepare_workers", None)
+            max_preparing = 2 * workers if workers else 1 << 30
+        self.max_prep = max(1, int(max_preparing))
         self.clock = clock
         self.sleep = sleep
 
@@ -201,8 +209,9 @@ class ServingEngine:
                 break
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            # admission: FCFS while the estimated footprint fits
-            while waiting and len(active) + len(preparing) < self.max_conc:
+            # admission: FCFS while the estimated footprint fits and the
+            # set-up queue has room
+            while waiting and len(active) + len(preparing) < self.max_conc and len(preparing) < self.max_prep:
                 req = waiting[0]
                 est = BYTES_PER_VERTEX_ESTIMATE * req.spec.n
                 if (active or preparing) and in_use + est > self.budget:

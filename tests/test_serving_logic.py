@@ -91,3 +91,43 @@ def test_horizon_stops_and_releases():
     rep = eng.run(reqs, dt=1 / 600, horizon_s=0.1)
     assert rep.completed < len(reqs)
     assert ex.active == []
+
+
+class SetupExecutor(FakeExecutor):
+    """Set-up takes `setup` seconds of the simulated clock per job, on
+    `prepare_workers` workers."""
+
+    def __init__(self, clock, setup=0.05, prepare_workers=2, **kw):
+        super().__init__(clock, **kw)
+        self.setup, self.prepare_workers = setup, prepare_workers
+        self.in_setup = 0
+        self.max_in_setup = 0
+
+    def begin(self, req, dt):
+        req.ready_at = self.clock.t + self.setup
+        self.in_setup += 1
+        self.max_in_setup = max(self.max_in_setup, self.in_setup)
+
+    def ready(self, req):
+        return self.clock.t >= req.ready_at
+
+    def admit(self, req, dt=None):
+        self.in_setup -= 1
+        super().admit(req, dt)
+
+
+def test_setup_queue_is_bounded():
+    """Under overload at most twice the set-up workers are preparing at
+    once (the rest wait without holding HBM); jobs still run FCFS and all
+    complete."""
+    clock = FakeClock()
+    ex = SetupExecutor(clock, setup=0.05, prepare_workers=2)
+    eng = serving.ServingEngine(ex, hbm_budget_bytes=1e12, clock=clock, sleep=clock.sleep)
+    reqs = _requests(count=60, seed=3)
+    rep = eng.run(reqs, dt=1 / 600)
+    assert rep.completed == len(reqs)
+    assert ex.max_in_setup <= 4
+    assert ex.admitted == sorted(ex.admitted)
+    eng2 = serving.ServingEngine(SetupExecutor(FakeClock(), prepare_workers=2), hbm_budget_bytes=1e12,
+                                 max_preparing=1)
+    assert eng2.max_prep == 1
