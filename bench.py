@@ -212,6 +212,7 @@ def main():
     ap.add_argument("--workload", choices=["batched", "serving"], default="batched",
                     help="batched (default; includes a short serving leg) or the full serving stream")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg of the batched workload")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-blocks leg of the batched workload")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager substeps instead of hipGraph replays of the captured substep")
     ap.add_argument("--serving-rate", type=float, default=None)
@@ -330,6 +331,35 @@ def main():
                 "mean_launch_ms": round(mean_ms, 3), "algorithmic_bytes_per_launch": float(np.mean(pcg_bytes))}
 
     log(f"PCG launch {mean_ms:.3f} ms, {achieved:.0f} GB/s algorithmic")
+
+    bf16_leg = None
+    if not args.no_bf16:
+        # The same batch with BF16-stored off-diagonal blocks (compensated
+        # FP32 diagonal, one outer refinement against the FP32 residual):
+        # same substeps, same tolerance and cap.
+        for c in cloths:
+            c.sim.set_bf16_blocks(True)
+        batch.set_bf16_blocks(True)
+        batch.set_graph(not args.no_graph)
+        with torch.cuda.stream(stream):
+            for _ in range(max(1, args.warmup)):
+                batch.step(dt, sh)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                batch.step(dt, sh)
+            torch.cuda.synchronize()
+            bw16 = time.perf_counter() - t0
+        it16 = np.array(batch.last_pcg_iterations(sh))
+        batch.set_graph(False)
+        bf16_leg = {"value": n_rows * args.steps / bw16, "unit": "vertex-substeps/s",
+                    "ms_per_step": bw16 / args.steps * 1e3,
+                    "pcg_iterations": {"mean": float(it16.mean()), "max": int(it16.max())},
+                    "vs_fp32": round((n_rows * args.steps / bw16) / value, 3)}
+        log(f"BF16 blocks: {bf16_leg['ms_per_step']:.2f} ms per substep, {bf16_leg['vs_fp32']}x the FP32 rate")
+        batch.set_bf16_blocks(False)
+        for c in cloths:
+            c.sim.set_bf16_blocks(False)
     copy_probe = None
     if args.copy_probe_mib > 0:
         # A plain streaming kernel of known size (read B, write B bytes):
@@ -428,6 +458,7 @@ def main():
         "max_relative_residual": float(res.max()),
         "plain": plain,
         "plain_large": plain_large,
+        "bf16_blocks": bf16_leg,
         "autotune": tuned,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": roofline,
