@@ -221,7 +221,7 @@ def main():
                     help="also run an elementwise copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "bench.py measures one GPU; multi-GPU runs are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

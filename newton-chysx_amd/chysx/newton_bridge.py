@@ -142,9 +142,9 @@ class SolverChysX:
         for flag, name in ((self_collision_enabled, "self_collision_enabled"),
                            (untangle_enabled, "untangle_enabled"), (tet_fem_enabled, "tet_fem_enabled")):
             if flag:
-                raise NotImplementedError(f"SolverChysX: {name} is not part of this build (implicit cloth step only)")
+                raise NotImplementedError(f"SolverChysX: {name} is not implemented by this library (implicit cloth step only)")
         if solver_type.lower() != "pcg":
-            raise NotImplementedError("SolverChysX: only solver_type='pcg' is part of this build")
+            raise NotImplementedError("SolverChysX: only solver_type='pcg' is implemented by this library")
         if spring_stiffness != 0.0:
             import warnings
 

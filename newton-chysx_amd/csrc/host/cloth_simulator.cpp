@@ -319,7 +319,7 @@ void ClothSimulator::step(float dt, std::uintptr_t cuda_stream) {
     const int n = buffers_.particle_count();
     if (n <= 0) return;
     if (solver_type_ == SolverType::VBD)
-        throw std::runtime_error("chysx::cloth::ClothSimulator::step: the VBD solver is not part of this build");
+        throw std::runtime_error("chysx::cloth::ClothSimulator::step: the VBD solver is not implemented by this library");
     if (buffers_.pos.data() == nullptr || buffers_.vel.data() == nullptr)
         throw std::runtime_error(
             "chysx::cloth::ClothSimulator::step: pos / vel spans must be set via set_external_buffers() before "

@@ -74,8 +74,8 @@ def test_unsupported_features_raise(gpu):
 
 
 def test_table_drop_example(gpu):
-    """example_chysx_cloth_drop.py's scene without self-collision (not part
-    of this build): a 21x21, 1 m cloth dropped 0.5 m onto a 0.6 m table over
+    """example_chysx_cloth_drop.py's scene without self-collision (not
+    implemented here): a 21x21, 1 m cloth dropped 0.5 m onto a 0.6 m table over
     a ground plane through static_contact_enabled; the example's test_final
     checks after 2 s, plus: the centre rests on the table top and the
     overhanging edges hang below it."""
