@@ -74,3 +74,37 @@ def test_graph_needs_a_stream(gpu):
     b.set_graph(True)
     with pytest.raises(ValueError, match="non-null stream"):
         b.step(spec.dt, 0)
+
+
+def test_graph_with_static_contact_and_mixed_instances(gpu):
+    """A mix of drape (static plane / box contact with friction) and plain
+    instances: graph replays equal eager substeps bitwise; the contact
+    parameters travel in the instance table, so 40 replays need one
+    capture."""
+    import torch
+
+    import chysx
+    from chysx import scenes
+
+    specs = [scenes.drape(n=24, friction=0.3), scenes.drape(n=20), *_specs(count=2, seed=22)]
+    eager = [scenes.DeviceCloth(s) for s in specs]
+    graph = [scenes.DeviceCloth(s) for s in specs]
+    be, bg = chysx.ClothBatch(), chysx.ClothBatch()
+    for c in eager:
+        be.add_instance(c.sim)
+    for c in graph:
+        bg.add_instance(c.sim)
+    bg.set_graph(True)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    dt = specs[0].dt
+    print(f"graph contact mix: sizes {[s.n for s in specs]}", flush=True)
+    for step in range(40):
+        with torch.cuda.stream(stream):
+            be.step(dt, sh)
+            bg.step(dt, sh)
+        if step % 10 == 9:
+            stream.synchronize()
+            for (pe, ve), (pg, vg) in zip(_states(eager), _states(graph)):
+                assert np.array_equal(pe, pg) and np.array_equal(ve, vg), step
+    assert bg.graph_captures() == 1
