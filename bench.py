@@ -332,6 +332,29 @@ def main():
 
     log(f"PCG launch {mean_ms:.3f} ms, {achieved:.0f} GB/s algorithmic")
 
+    # The SpMV phase alone (q = A p with the folded p update) while every
+    # instance is iterating, from the solver's per-barrier trace of one more
+    # solve: algorithmic bytes B_spmv + 36N per iteration over its duration.
+    spmv_phase = None
+    batch.set_trace(1024)
+    with torch.cuda.stream(stream):
+        batch.assemble(dt, sh)
+        batch.solve(sh)
+        tr = np.array(batch.last_trace(sh)).astype(np.int64)
+    batch.set_trace(0)
+    n_all, nnz_all = int(sizes[:, 0].sum()), int(sizes[:, 1].sum())
+    full = [k for k in range((len(tr) - 1) // 2) if tr[2 * k + 1, 1] == len(specs)]
+    if full:
+        us = np.array([(tr[2 * k + 1, 0] - tr[2 * k, 0]) / 100.0 for k in full])
+        phase_bytes = b_spmv(n_all, nnz_all) + 36 * n_all
+        spmv_phase = {"iterations": len(full), "us_per_iteration": round(float(us.mean()), 1),
+                      "algorithmic_bytes": phase_bytes,
+                      "GB_per_s": round(phase_bytes / (float(us.mean()) * 1e-6) / 1e9, 1),
+                      "frac_of_peak": round(phase_bytes / (float(us.mean()) * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                      "note": "iterations with every instance solving; barrier wait included"}
+        log(f"SpMV phase at full activity: {spmv_phase['us_per_iteration']} us, {spmv_phase['GB_per_s']} GB/s "
+            f"algorithmic ({spmv_phase['frac_of_peak']:.3f} of peak)")
+
     bf16_leg = None
     if not args.no_bf16:
         # The same batch with BF16-stored off-diagonal blocks (compensated
@@ -459,6 +482,7 @@ def main():
         "plain": plain,
         "plain_large": plain_large,
         "bf16_blocks": bf16_leg,
+        "spmv_phase": spmv_phase,
         "autotune": tuned,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": roofline,
