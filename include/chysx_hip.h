@@ -32,7 +32,9 @@
  * component c (row-major r*3+col) of block i at
  * [((i / 64) * 9 + c) * 64 + i % 64], arrays rounded up to whole groups.  A
  * wavefront reads one component of 64 consecutive blocks as 256 contiguous
- * bytes, and the nine components of a group are one contiguous run.
+ * bytes, and the nine components of a group are one contiguous run.  A cloth
+ * instance's diagonal blocks and preconditioner are symmetric and keep six
+ * planes per group (xx, xy, xz, yy, yz, zz).
  */
 #ifndef CHYSX_HIP_H_
 #define CHYSX_HIP_H_
@@ -151,8 +153,9 @@ typedef struct chx_instance {
     const float* inv_mass;           /* external inverse masses, or NULL (mass 1 kg)         */
     const int* row_offsets;          /* [n_rows+1]                                           */
     const int* col_indices;          /* [nnz]                                                */
-    float* diag;                     /* grouped planar over the block rows (9 * n_rows       */
-                                     /* rounded up to 64 blocks)                             */
+    float* diag;                     /* symmetric: upper triangles (xx xy xz yy yz zz) in    */
+                                     /* groups of 64 blocks, six planes each; room for 9 *   */
+                                     /* n_rows rounded up to 64 blocks                       */
     float* values;                   /* grouped planar over the n_upper upper value slots    */
     const chx_pin* pins;
     const chx_tri* tris;             /* stretch elements, then shear elements                */
@@ -169,7 +172,7 @@ typedef struct chx_instance {
     float* p0;
     float* p1;
     float* q;
-    float* minv;                     /* grouped planar block-Jacobi preconditioner           */
+    float* minv;                     /* block-Jacobi preconditioner, symmetric like diag     */
     float* diag_c;                   /* BF16 precision mode, else NULL: the inner SpMV's     */
     uint16_t* values16;              /* matrix, grouped planar FP32 compensated diagonal     */
                                      /* D + sum_j (V_ij - bf16(V_ij)) and grouped planar     */

@@ -162,6 +162,37 @@ static_assert(CHX_BLOCK_GROUP == 64, "bp() indexes groups of 64 blocks");
 __device__ __forceinline__ unsigned bp(unsigned c, unsigned i) {
     return (((i >> 6) * 9u + c) << 6) | (i & 63u);
 }
+// Symmetric blocks (the cloth instances' diagonal blocks and block-Jacobi
+// preconditioner, symmetric by construction) keep only their upper
+// triangle: groups of 64 blocks as six planes xx, xy, xz, yy, yz, zz,
+// component q of block i at p[((i / 64) * 6 + q) * 64 + i % 64].
+__device__ __forceinline__ unsigned bp6(unsigned q, unsigned i) {
+    return (((i >> 6) * 6u + q) << 6) | (i & 63u);
+}
+// Row-major position of upper-triangle component q.
+__device__ __forceinline__ unsigned sym6_pos(unsigned q) { return q < 3u ? q : (q < 5u ? q + 1u : 8u); }
+template <class T>
+__device__ __forceinline__ void ld6p(gp<const T> m, unsigned i, float (&a)[9]) {
+    float u[6];
+#pragma unroll
+    for (unsigned q = 0; q < 6; ++q) u[q] = to_f32(ldx(m, bp6(q, i)));
+    a[0] = u[0];
+    a[1] = u[1];
+    a[2] = u[2];
+    a[3] = u[1];
+    a[4] = u[3];
+    a[5] = u[4];
+    a[6] = u[2];
+    a[7] = u[4];
+    a[8] = u[5];
+}
+template <class T>
+__device__ __forceinline__ f3 mv6p(gp<const T> m, unsigned i, f3 v) {
+    float a[9];
+    ld6p(m, i, a);
+    return mv9(a, v);
+}
+
 template <class T>
 __device__ __forceinline__ f3 mv9p(gp<const T> m, unsigned i, f3 v) {
     float a[9];
@@ -182,15 +213,12 @@ __device__ __forceinline__ f3 mv9p_t(gp<const T> m, unsigned i, f3 v, bool tr) {
 }
 template <class T>
 struct MatP {
-    gp<const T> d;  // diagonal blocks (block rows)
-    gp<const T> v;  // off-diagonal blocks (value slots)
-    __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv9p(d, row, x); }
+    gp<const T> d;  // symmetric diagonal blocks (block rows, bp6)
+    gp<const T> v;  // off-diagonal blocks (value slots, bp)
+    __device__ __forceinline__ f3 diag(unsigned row, f3 x) const { return mv6p(d, row, x); }
     __device__ __forceinline__ f3 off(unsigned k, f3 x) const { return mv9p(v, k, x); }
     __device__ __forceinline__ f3 off_t(unsigned k, f3 x, bool tr) const { return mv9p_t(v, k, x, tr); }
-    __device__ __forceinline__ void diag_block(unsigned row, float (&m)[9]) const {
-#pragma unroll
-        for (unsigned c = 0; c < 9; ++c) m[c] = to_f32(ldx(d, bp(c, row)));
-    }
+    __device__ __forceinline__ void diag_block(unsigned row, float (&m)[9]) const { ld6p(d, row, m); }
 };
 using MatP32 = MatP<float>;
 

@@ -280,9 +280,10 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
             h[3] += a01; h[4] += a11; h[5] += a12;
             h[6] += a02; h[7] += a12; h[8] += a22;
         }
-        // Grouped planar blocks (bp): component k of block row `row`.
+        // Symmetric grouped planar blocks (bp6): the upper triangle of the
+        // block row's diagonal block.
 #pragma unroll
-        for (unsigned k = 0; k < 9; ++k) stx(I.diag, bp(k, row), h[k]);
+        for (unsigned q = 0; q < 6; ++q) stx(I.diag, bp6(q, row), h[sym6_pos(q)]);
     } else {
         const unsigned slot = s - n;
 #pragma unroll
@@ -312,8 +313,7 @@ __global__ __launch_bounds__(kBlock) void compensate_kernel(chx_cloth_batch B) {
     if (row >= n) return;
     const unsigned k0 = ldx(I.row_offsets, row), k1 = ldx(I.row_offsets, row + 1u);
     float d[9];
-#pragma unroll
-    for (unsigned c = 0; c < 9; ++c) d[c] = ldx(gp<const float>(I.diag), bp(c, row));
+    ld6p(gp<const float>(I.diag), row, d);
     for (unsigned k = k0; k < k1; ++k) {
         // lower blocks: the partner's rounding, transposed
         const int bs = ldx(I.block_slots, k);
@@ -909,11 +909,20 @@ __device__ __forceinline__ void residual_tile(const chx_cloth_batch& B, const In
             float d9[9];
             mat.diag_block(row, d9);
             inverse9(d9, mi);
+            if constexpr (SL) {  // symmetric: the upper triangle
 #pragma unroll
-            for (unsigned k = 0; k < 9; ++k) stx(I.minv, bp(k, row), mi[k]);
+                for (unsigned q = 0; q < 6; ++q) stx(I.minv, bp6(q, row), mi[sym6_pos(q)]);
+            } else {
+#pragma unroll
+                for (unsigned k = 0; k < 9; ++k) stx(I.minv, bp(k, row), mi[k]);
+            }
         } else {
+            if constexpr (SL) {
+                ld6p(gp<const float>(I.minv), row, mi);
+            } else {
 #pragma unroll
-            for (unsigned k = 0; k < 9; ++k) mi[k] = ldx(gp<const float>(I.minv), bp(k, row));
+                for (unsigned k = 0; k < 9; ++k) mi[k] = ldx(gp<const float>(I.minv), bp(k, row));
+            }
         }
         const f3 z = mv9(mi, r);
         st3g(I.r, row, r);
@@ -1111,7 +1120,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                     f3 x = ld3g(I.x, row), r = ld3g(I.r, row);
                     x = mk(fmaf(alpha, p.x, x.x), fmaf(alpha, p.y, x.y), fmaf(alpha, p.z, x.z));
                     r = mk(fmaf(-alpha, qq.x, r.x), fmaf(-alpha, qq.y, r.y), fmaf(-alpha, qq.z, r.z));
-                    const f3 zz = mv9p(gp<const float>(I.minv), row, r);
+                    const f3 zz = SL ? mv6p(gp<const float>(I.minv), row, r) : mv9p(gp<const float>(I.minv), row, r);
                     st3g(I.x, row, x);
                     st3g(I.r, row, r);
                     st3g(I.z, row, zz);

@@ -383,6 +383,24 @@ void copy_planar_blocks(T* out, const T* dev, int count, const char* what) {
         for (int c = 0; c < 9; ++c)
             out[static_cast<std::size_t>(9) * i + c] = planes[grouped(c, static_cast<std::size_t>(i))];
 }
+// Symmetric blocks (the diagonal) keep their upper triangle: groups of 64
+// blocks as six planes xx, xy, xz, yy, yz, zz.
+template <class T>
+void copy_sym_blocks(T* out, const T* dev, int count, const char* what) {
+    const std::size_t G = CHX_BLOCK_GROUP;
+    const std::size_t groups = (static_cast<std::size_t>(count) + G - 1) / G;
+    std::vector<T> planes(6 * G * groups);
+    check(chx_memcpy_d2h(planes.data(), dev, sizeof(T) * planes.size(), 0), "cloth", what);
+    static const int pos[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int i = 0; i < count; ++i) {
+        const std::size_t ii = static_cast<std::size_t>(i);
+        for (int q = 0; q < 6; ++q) {
+            const T v = planes[((ii / G) * 6 + static_cast<std::size_t>(q)) * G + ii % G];
+            out[9 * ii + 3 * pos[q][0] + pos[q][1]] = v;
+            out[9 * ii + 3 * pos[q][1] + pos[q][0]] = v;
+        }
+    }
+}
 // Off-diagonal blocks live once per symmetric pair (upper blocks over
 // n_upper value slots); out[k] = the block of CSR index k, row-major: the
 // stored upper block, or its partner's transpose for a lower one.
@@ -407,7 +425,7 @@ void ClothSimulator::debug_copy_hessian_diag(float* out) const {
     if (n == 0) return;
     if (out == nullptr) throw std::invalid_argument("ClothSimulator::debug_copy_hessian_diag: out must be non-null");
     check(chx_device_synchronize(), "cloth", "device synchronize(debug_copy_hessian_diag)");
-    copy_planar_blocks(out, inst_.diag(), n, "diag copy");
+    copy_sym_blocks(out, inst_.diag(), n, "diag copy");
 }
 
 void ClothSimulator::debug_copy_bf16_matrix(float* out_diag_c, uint16_t* out_values16) const {
