@@ -104,7 +104,11 @@ private:
 class BatchRunner {
 public:
     BatchRunner() = default;
-    ~BatchRunner() { drop_graph(); }
+    ~BatchRunner() {
+        // a replay may still be running on the caller's stream
+        if (graph_ != nullptr) (void)chx_device_synchronize();
+        drop_graph();
+    }
     BatchRunner(const BatchRunner&) = delete;
     BatchRunner& operator=(const BatchRunner&) = delete;
 
