@@ -184,6 +184,9 @@ typedef struct chx_instance {
                                      /* CHX_STAGE_ROWS rows, or NULL (no LDS staging)        */
     const int* slice_offsets;        /* [ceil(n_rows/CHX_SLICE_ROWS)+1] first entry slot     */
     const int* slice_entries;        /* [n_padded] (col << 6) | jj of every entry slot       */
+    const uint16_t* slice_entries16; /* the same entries in 16 bits when every one fits,     */
+                                     /* else NULL: ((col - row + 2048) << 4) | jj, |col -    */
+                                     /* row| < 2048, jj < 16 (row = the slot's row)          */
     const int* upper_offsets;        /* [ceil(n_rows/CHX_SLICE_ROWS)+1] first value slot     */
     const int* block_slots;          /* [nnz] value slot of CSR block k (-1 - slot: lower)   */
     const chx_plane* planes;         /* static contact primitives, or NULL                   */
@@ -225,7 +228,8 @@ typedef struct chx_cloth_batch {
                                      /* accumulate), outer refinement on the FP32     */
                                      /* true residual when the inner residual meets   */
                                      /* the tolerance                                 */
-    int pad_bf16;
+    int entries16;                   /* 1: every instance has slice_entries16 and the PCG   */
+                                     /* reads entries in 16 bits                         */
     int tile_rows;                   /* rows per tile of the tile list (informational) */
     int no_lds_staging;              /* 1: the PCG SpMV gathers from global memory;   */
                                      /* 0: stages of CHX_STAGE_ROWS rows go through   */

@@ -254,6 +254,7 @@ struct Inst {
     gp<uint16_t> values16;
     gp<const int> windows;
     gp<const int> slice_offsets, slice_entries, upper_offsets, block_slots;
+    gp<const uint16_t> slice_entries16;
     gp<const chx_plane> planes;
     gp<const chx_box> boxes;
     int n_planes, n_boxes;
@@ -298,6 +299,7 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.windows = glb(d->windows);
     I.slice_offsets = glb(d->slice_offsets);
     I.slice_entries = glb(d->slice_entries);
+    I.slice_entries16 = glb(d->slice_entries16);
     I.upper_offsets = glb(d->upper_offsets);
     I.block_slots = glb(d->block_slots);
     I.n_padded = d->n_padded;

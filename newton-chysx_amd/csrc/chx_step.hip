@@ -450,7 +450,7 @@ constexpr unsigned kSlice = CHX_SLICE_ROWS;
 static_assert(CHX_SLICE_ROWS == kWave, "one lane per row of a slice");
 static_assert(CHX_STAGE_ROWS % CHX_SLICE_ROWS == 0, "stages hold whole slices");
 
-template <class Mat, class SlotAt, class EntAt, class UoffAt, class LenAt, class Src, class Fin>
+template <bool E16, class Mat, class SlotAt, class EntAt, class UoffAt, class LenAt, class Src, class Fin>
 __device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot_at, EntAt ent_at, UoffAt uoff_at,
                                             LenAt len_at, Src src, Fin fin) {
     constexpr int KS = kSliceBatch;
@@ -474,7 +474,10 @@ __device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot
             for (int u = 0; u < KS; ++u) {
                 const unsigned j = j0 + unsigned(u);
                 const unsigned e = unsigned(ent_at(base + (j < width ? j : j0) * kSlice + lane));
-                const unsigned col = e >> 6, jj = e & 63u;
+                // 16-bit entries hold col - row + 2048 (row = this lane's row
+                // of the slice, valid or not) and jj < 16.
+                const unsigned col = E16 ? unsigned(s0) + lane + (e >> 4) - 2048u : e >> 6;
+                const unsigned jj = E16 ? (e & 15u) : (e & 63u);
                 const bool lower = col < rowc;
                 const unsigned vs = lower ? uoff_at(col / kSlice) + (col & (kSlice - 1u)) + jj * kSlice
                                           : ubase + jj * kSlice;
@@ -548,9 +551,10 @@ __device__ __forceinline__ f3 lds3(const float* w, unsigned i) { return {w[3 * i
 // vector entry j.  Uniform across the workgroup.
 // SL: the column indices come from the sliced layout (slots of the stage's
 // slices) instead of the CSR order.
-template <bool SL, class SrcG>
+template <int EL, class SrcG>
 __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG srcg, PcgStageLds& S,
                                            unsigned& beg0, unsigned& lo) {
+    constexpr bool SL = EL != 0, E16 = EL == 2;
     if (I.windows == nullptr) return false;
     const unsigned rb0 = unsigned(ldx(I.row_offsets, unsigned(sb)));
     const unsigned b0 = SL ? unsigned(ldx(I.slice_offsets, unsigned(sb) / kSlice)) : rb0;
@@ -573,7 +577,8 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const unsigned k = it * 2048u + unsigned(u) * 256u + tid;
-                cv[u] = ldx(cols, b0 + (k < nnz_s ? k : 0u));
+                cv[u] = E16 ? int(ldx(I.slice_entries16, b0 + (k < nnz_s ? k : 0u)))
+                            : ldx(cols, b0 + (k < nnz_s ? k : 0u));
             }
         }
 #pragma unroll
@@ -586,7 +591,10 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const unsigned k = it * 2048u + unsigned(u) * 256u + tid;
-                if (k < nnz_s) S.ci[k] = cv[u];
+                if (k < nnz_s) {
+                    if (E16) reinterpret_cast<uint16_t*>(S.ci)[k] = uint16_t(cv[u]);
+                    else S.ci[k] = cv[u];
+                }
             }
         }
 #pragma unroll
@@ -614,14 +622,18 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
 // time, or straight from global memory; SL: sliced layout (cloth
 // instances), else CSR with L lanes per row (caller-provided row-major
 // blocks).
-template <int L, bool ST, bool SL, class Mat, class SrcG, class Fin>
+template <int L, bool ST, int EL, class Mat, class SrcG, class Fin>
 __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re, PcgLds<ST>& S, SrcG srcg,
                                           Fin fin) {
+    constexpr bool SL = EL != 0, E16 = EL == 2;
     auto global_rows = [&](int b, int e) {
         if constexpr (SL) {
-            spmv_sliced(
+            spmv_sliced<E16>(
                 mat, b, e, [&](unsigned sl) { return unsigned(ldx(I.slice_offsets, sl)); },
-                [&](unsigned k) { return ldx(I.slice_entries, k); },
+                [&](unsigned k) {
+                    if constexpr (E16) return unsigned(ldx(I.slice_entries16, k));
+                    else return unsigned(ldx(I.slice_entries, k));
+                },
                 [&](unsigned sl) { return unsigned(ldx(I.upper_offsets, sl)); },
                 [&](unsigned r) { return unsigned(ldx(I.row_offsets, r + 1u) - ldx(I.row_offsets, r)); }, srcg,
                 fin);
@@ -638,12 +650,15 @@ __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re
         for (int sb = rb; sb < re; sb += kStage) {
             const int se = min(re, sb + kStage);
             unsigned beg0 = 0, lo = 0;
-            if (stage_load<SL>(I, sb, se, srcg, T, beg0, lo)) {
+            if (stage_load<EL>(I, sb, se, srcg, T, beg0, lo)) {
                 if constexpr (SL) {
                     const unsigned s_lo = lo / kSlice;
-                    spmv_sliced(
+                    spmv_sliced<E16>(
                         mat, sb, se, [&](unsigned sl) { return unsigned(ldx(I.slice_offsets, sl)); },
-                        [&](unsigned k) { return T.ci[k - beg0]; },
+                        [&](unsigned k) {
+                            if constexpr (E16) return unsigned(reinterpret_cast<const uint16_t*>(T.ci)[k - beg0]);
+                            else return unsigned(T.ci[k - beg0]);
+                        },
                         [&](unsigned sl) { return unsigned(T.uo[sl - s_lo]); },
                         [&](unsigned r) { return unsigned(T.ro[r - unsigned(sb) + 1u] - T.ro[r - unsigned(sb)]); },
                         [&](unsigned j) { return lds3(T.win, j - lo); }, fin);
@@ -896,12 +911,13 @@ __device__ __forceinline__ auto plus_contact(const chx_cloth_batch& B, const Ins
     };
 }
 
-template <int L, bool ST, bool SL, class MatR>
+template <int L, bool ST, int EL, class MatR>
 __device__ __forceinline__ void residual_tile(const chx_cloth_batch& B, const Inst& I, MatR mat, int rb, int re,
                                               bool setup, gp<float> pdst, PcgLds<ST>& S, double (&acc)[3]) {
+    constexpr bool SL = EL != 0;
     const gp<const float> xv = I.x;
     auto xs = [&](unsigned j) { return ld3g(xv, j); };
-    tile_spmv<L, ST, SL>(I, mat, rb, re, S, xs, plus_contact<SL>(B, I, xs, [&](unsigned row, f3 Ax, f3) {
+    tile_spmv<L, ST, EL>(I, mat, rb, re, S, xs, plus_contact<SL>(B, I, xs, [&](unsigned row, f3 Ax, f3) {
         const f3 b = ld3g(I.rhs, row);
         const f3 r = sub(b, Ax);
         float mi[9];
@@ -963,6 +979,12 @@ struct PcgMats<2> {
     }
 };
 
+// V = 3 / 4: V = 1 / 2 with 16-bit slice entries (same matrix views).
+template <>
+struct PcgMats<3> : PcgMats<1> {};
+template <>
+struct PcgMats<4> : PcgMats<2> {};
+
 // Optional solver trace (workgroup 0, thread 0): real-time stamp and
 // active-instance count at barrier e.
 __device__ __forceinline__ void trace_point(const chx_cloth_batch& B, int e, unsigned active) {
@@ -975,8 +997,9 @@ __device__ __forceinline__ void trace_point(const chx_cloth_batch& B, int e, uns
 
 template <int L, int V, bool ST>
 __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int finalize) {
-    constexpr bool BF16 = V == 2;
-    constexpr bool SL = V != 0;  // cloth instances: sliced blocks
+    constexpr bool BF16 = V == 2 || V == 4;
+    constexpr bool SL = V != 0;                       // cloth instances: sliced blocks
+    constexpr int EL = V == 0 ? 0 : (V >= 3 ? 2 : 1);  // entry layout: CSR, sliced 32-bit, sliced 16-bit
     __shared__ PcgLds<ST> S;
     const int G = int(gridDim.x);
     unsigned* sync = B.sync;
@@ -998,7 +1021,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
         if (threadIdx.x == 0 && t == I.tile_begin)
             __hip_atomic_store(&B.inst_done[inst], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         double acc[3] = {0.0, 0.0, 0.0};
-        residual_tile<L, ST, SL>(B, I, PcgMats<V>::r(I), rb, re, true, I.p0, S, acc);
+        residual_tile<L, ST, EL>(B, I, PcgMats<V>::r(I), rb, re, true, I.p0, S, acc);
         block_sum(acc, S.red);
         if (threadIdx.x == 0) {
             const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
@@ -1053,7 +1076,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                     // True residual r = b - A32 x, z = M^-1 r and their sums
                     // become the parity cur^1 partials, marked true.
                     double acc[3] = {0.0, 0.0, 0.0};
-                    residual_tile<L, ST, SL>(B, I, PcgMats<V>::r(I), rb, re, false, gp<float>(nullptr), S, acc);
+                    residual_tile<L, ST, EL>(B, I, PcgMats<V>::r(I), rb, re, false, gp<float>(nullptr), S, acc);
                     block_sum(acc, S.red);
                     if (threadIdx.x == 0) {
                         const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
@@ -1073,17 +1096,17 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 };
                 if (first) {
                     auto pj = [&](unsigned jj) { return ld3g(pk, jj); };
-                    tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, pj, plus_contact<SL>(B, I, pj, fin));
+                    tile_spmv<L, ST, EL>(I, PcgMats<V>::a(I), rb, re, S, pj, plus_contact<SL>(B, I, pj, fin));
                 } else if (BF16 && true_res) {
                     // restart after a refinement slot: p_k = z_k
-                    tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(z, jj); }, fin);
+                    tile_spmv<L, ST, EL>(I, PcgMats<V>::a(I), rb, re, S, [&](unsigned jj) { return ld3g(z, jj); }, fin);
                 } else {
                     const float beta = guarded_div(rho, rho_prev);
                     auto pj = [&](unsigned jj) {
                         const f3 zj = ld3g(z, jj), pp = ld3g(pprev, jj);
                         return mk(fmaf(beta, pp.x, zj.x), fmaf(beta, pp.y, zj.y), fmaf(beta, pp.z, zj.z));
                     };
-                    tile_spmv<L, ST, SL>(I, PcgMats<V>::a(I), rb, re, S, pj, plus_contact<SL>(B, I, pj, fin));
+                    tile_spmv<L, ST, EL>(I, PcgMats<V>::a(I), rb, re, S, pj, plus_contact<SL>(B, I, pj, fin));
                 }
                 block_sum(acc, S.red);
                 if (threadIdx.x == 0) rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS))[P_SIG] = acc[0];
@@ -1199,11 +1222,17 @@ PcgFn pcg_fn_s(int L, int v) {
     }
     // Sliced layout (cloth instances): one lane per row, lanes_per_row
     // does not apply.
-    return v == 1 ? pcg_kernel<4, 1, ST> : pcg_kernel<4, 2, ST>;
+    switch (v) {
+        case 1: return pcg_kernel<4, 1, ST>;
+        case 2: return pcg_kernel<4, 2, ST>;
+        case 3: return pcg_kernel<4, 3, ST>;
+        default: return pcg_kernel<4, 4, ST>;
+    }
 }
 
-// v: matrix variant (0 row-major CSR, 1 sliced FP32, 2 sliced BF16 + FP32);
-// staged: LDS-staged SpMV or the global form.
+// v: matrix variant (0 row-major CSR, 1 sliced FP32, 2 sliced BF16 + FP32;
+// 3 / 4: 1 / 2 with 16-bit slice entries); staged: LDS-staged SpMV or the
+// global form.
 PcgFn pcg_fn(int L, int v = 1, bool staged = true) {
     if (L != 4 && L != 8 && L != 16)
         throw Error(CHX_ERR_INVALID_ARGUMENT, "chysx: lanes_per_row must be 4, 8 or 16");
@@ -1213,8 +1242,8 @@ PcgFn pcg_fn(int L, int v = 1, bool staged = true) {
 struct DeviceLimits {
     int device = -1;
     int cus = 0;
-    int max_blocks[6][17] = {{0}};  // [2 * v + staged][L]
-    int api_blocks[6][17] = {{0}};
+    int max_blocks[10][17] = {{0}};  // [2 * v + staged][L]
+    int api_blocks[10][17] = {{0}};
 };
 
 std::mutex g_lim_mu;
@@ -1256,7 +1285,7 @@ int max_coresident(int L, int v = 1, bool staged = true) {
 void launch_pcg(const chx_cloth_batch& B, int finalize, hipStream_t stream, bool row_major = false) {
     if (B.n_tiles <= 0) return;
     const int L = B.lanes_per_row;
-    const int v = row_major ? 0 : (B.bf16_blocks != 0 ? 2 : 1);
+    const int v = row_major ? 0 : (B.bf16_blocks != 0 ? 2 : 1) + (B.entries16 != 0 ? 2 : 0);
     const bool staged = B.no_lds_staging == 0;
     PcgFn fn = pcg_fn(L, v, staged);
     int grid = max_coresident(L, v, staged);
@@ -1285,7 +1314,8 @@ void validate_batch(const chx_cloth_batch* b) {
                 "chysx: negative size in instance");
         if (I.n_rows > 0)
             require(I.row_offsets && I.diag && I.rhs && I.x && I.r && I.z && I.p0 && I.p1 && I.q && I.minv &&
-                        I.slice_offsets && I.slice_entries && I.upper_offsets && (I.block_slots || I.nnz == 0),
+                        I.slice_offsets && I.slice_entries && I.upper_offsets && (I.block_slots || I.nnz == 0) &&
+                        (I.slice_entries16 || b->entries16 == 0),
                     "chysx: null array in instance");
         require(I.n_padded >= I.nnz && (I.n_padded % CHX_SLICE_ROWS) == 0 && (I.n_upper % CHX_SLICE_ROWS) == 0 &&
                     2 * size_t(I.n_upper) >= size_t(I.nnz) && size_t(I.n_upper) * 36u < (size_t(1) << 32) &&
@@ -1576,7 +1606,8 @@ int chx_cloth_step_graph(const chx_cloth_batch* batch, uintptr_t stream, void** 
         const hipStream_t s = as_stream(stream);
         require(s != nullptr, "chx_cloth_step_graph: capture needs a stream other than the null stream");
         // Device limits are queried (and cached) before the capture starts.
-        (void)max_coresident(batch->lanes_per_row, batch->bf16_blocks != 0 ? 2 : 1, batch->no_lds_staging == 0);
+        (void)max_coresident(batch->lanes_per_row, (batch->bf16_blocks != 0 ? 2 : 1) + (batch->entries16 != 0 ? 2 : 0),
+                             batch->no_lds_staging == 0);
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         hipGraph_t g = nullptr;
         try {

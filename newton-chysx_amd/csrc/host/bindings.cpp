@@ -260,6 +260,7 @@ PYBIND11_MODULE(_chysx_native, m) {
                  if (!w.empty()) std::memcpy(a.mutable_data(), w.data(), w.size() * sizeof(int));
                  return a;
              })
+        .def("debug_entries16", &ClothSimulator::debug_entries16)
         .def("debug_edges",
              [](const ClothSimulator& s) {
                  const auto& e = s.buffers().edges;

@@ -162,6 +162,9 @@ public:
     // {smallest, largest} column of each 256-row stage of the assembled
     // pattern (after the first step): the source rows its SpMV reads.
     const std::vector<int>& debug_stage_windows() const noexcept { return inst_.stage_windows(); }
+    // Whether the SpMV reads 16-bit sliced entries (row-relative column and
+    // block index) instead of 32-bit ones (after the first step).
+    bool debug_entries16() const noexcept { return inst_.entries16(); }
     int lanes_per_row() const noexcept { return lanes_per_row_; }
     // Precision of the solve: false = FP32 Hessian blocks (default); true =
     // the inner CG runs on BF16 copies of the off-diagonal blocks and an FP32

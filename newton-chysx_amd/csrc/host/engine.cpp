@@ -165,6 +165,27 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
                 }
             }
         }
+        // 16-bit form ((col - row + 2048) << 4 | jj, row = the slot's row),
+        // used when every entry fits: grids up to ~1000 vertices wide.
+        auto& s16 = slice_entries16_.host();
+        s16.assign(std::max<std::size_t>(total, 1), 0);
+        entries16_ = true;
+        for (int sl = 0; sl < slices && entries16_; ++sl) {
+            const int width = (so[sl + 1] - so[sl]) / S_;
+            for (int l = 0; l < S_ && entries16_; ++l) {
+                const int r = sl * S_ + l;
+                for (int j = 0; j < width; ++j) {
+                    const int slot = so[sl] + j * S_ + l;
+                    const int c = se[slot] >> 6, jj = se[slot] & 63, off = c - r;
+                    if (off < -2048 || off > 2047 || jj > 15) {
+                        entries16_ = false;
+                        break;
+                    }
+                    s16[slot] = static_cast<uint16_t>(((off + 2048) << 4) | jj);
+                }
+            }
+        }
+        if (!entries16_) s16.clear();
     }
 
 
@@ -250,6 +271,8 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     upload(col_indices_, bytes_, stream);
     upload(slice_offsets_, bytes_, stream);
     upload(slice_entries_, bytes_, stream);
+    if (entries16_) upload(slice_entries16_, bytes_, stream);
+    else slice_entries16_.release();
     upload(upper_offsets_, bytes_, stream);
     upload(block_slots_, bytes_, stream);
     upload(contrib_offsets_, bytes_, stream);
@@ -357,6 +380,7 @@ chx_instance InstanceData::descriptor(const InstanceSource& s) const {
     I.windows = windows_.gpu_data();
     I.slice_offsets = slice_offsets_.gpu_data();
     I.slice_entries = slice_entries_.gpu_data();
+    I.slice_entries16 = entries16_ ? slice_entries16_.gpu_data() : nullptr;
     I.upper_offsets = upper_offsets_.gpu_data();
     I.block_slots = block_slots_.gpu_data();
     I.n_padded = npad_;
@@ -484,6 +508,13 @@ void BatchRunner::build_tiles(std::vector<chx_instance>& ds, std::uintptr_t stre
     desc_.tile_rows = tile_rows_;
 }
 
+// 1 when every instance with rows carries 16-bit slice entries.
+static int entries16_all(const std::vector<chx_instance>& ds) {
+    for (const chx_instance& d : ds)
+        if (d.n_rows > 0 && d.slice_entries16 == nullptr) return 0;
+    return 1;
+}
+
 void BatchRunner::bind_single(const chx_instance& d, std::uintptr_t stream) {
     std::vector<chx_instance> one{d};
     build_tiles(one, stream);
@@ -493,6 +524,7 @@ void BatchRunner::bind_single(const chx_instance& d, std::uintptr_t stream) {
     desc_.lanes_per_row = lanes_;
     desc_.grid_blocks = grid_;
     desc_.bf16_blocks = bf16_ ? 1 : 0;
+    desc_.entries16 = entries16_all(one);
     desc_.single = one[0];
     desc_.inst = nullptr;
     desc_.elem_items = nullptr;
@@ -529,6 +561,7 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds, std::uintptr_t 
     desc_.lanes_per_row = lanes_;
     desc_.grid_blocks = grid_;
     desc_.bf16_blocks = bf16_ ? 1 : 0;
+    desc_.entries16 = entries16_all(table_host_);
     desc_.single = chx_instance{};
     desc_.inst = table_.gpu_data();
     desc_.elem_items = elem_items_.gpu_data();
@@ -549,6 +582,7 @@ void BatchRunner::update_many(const std::vector<chx_instance>& ds, std::uintptr_
             changed = true;
         }
     }
+    desc_.entries16 = entries16_all(table_host_);
     if (changed) {
         // The previous table may still be read by queued launches.
         check(chx_stream_synchronize(stream), "engine", "instance table update");

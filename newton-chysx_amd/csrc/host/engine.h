@@ -77,6 +77,9 @@ public:
     const uint16_t* values16() const noexcept { return values16_.gpu_data(); }
     const float* rhs() const noexcept { return rhs_.gpu_data(); }
     const std::vector<int>& stage_windows() const noexcept { return windows_.host(); }
+    // True when every sliced entry fits the 16-bit form (column within 2047
+    // rows of its row, block index below 16).
+    bool entries16() const noexcept { return entries16_; }
 
 private:
     void pack_elements(const InstanceSource& s);
@@ -97,6 +100,8 @@ private:
     DeviceArray<uint16_t> values16_;
     DeviceArray<int> windows_;  // per CHX_STAGE_ROWS-row stage: {min column, max column}
     DeviceArray<int> slice_offsets_, slice_entries_, upper_offsets_, block_slots_;  // sliced symmetric layout
+    DeviceArray<uint16_t> slice_entries16_;  // the entries in 16 bits, when every one fits (else empty)
+    bool entries16_ = false;
     DeviceArray<chx_plane> planes_;
     DeviceArray<chx_box> boxes_;
 };
@@ -119,6 +124,7 @@ public:
     // Per-step scalars / external pointers of the bound instances.
     void update_single(const chx_instance& d) {
         const int tb = desc_.single.tile_begin, te = desc_.single.tile_end;
+        desc_.entries16 = (d.n_rows > 0 && d.slice_entries16 == nullptr) ? 0 : 1;
         desc_.single = d;
         desc_.single.tile_begin = tb;
         desc_.single.tile_end = te;

@@ -88,6 +88,39 @@ def test_bf16_substeps_match_fp64_oracle(gpu):
           f"(FP64 oracle {np.mean(its64):.1f}), max excess {max(a - b for a, b in zip(its, its64))}")
 
 
+def test_bf16_scattered_numbering_matches_fp64_oracle(gpu):
+    """BF16 blocks on a randomly renumbered 64x64 grid, whose columns lie up
+    to 4095 rows from their rows (32-bit sliced entries, SpMV from global
+    memory): one-substep error from the oracle's state <= 1e-5."""
+    import torch
+
+    from chysx import scenes
+
+    base = scenes.plain_small(seed=6, n=64)
+    rng = np.random.default_rng(6)
+    perm = rng.permutation(base.n)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(base.n)
+    spec = scenes.ClothSpec(pos=base.pos[perm].copy(), tris=inv[base.tris].astype(np.int32),
+                            pins=np.sort(inv[base.pins]).astype(np.int32), k_stretch=base.k_stretch,
+                            k_shear=base.k_shear, k_bend=base.k_bend, density=base.density, name="scattered64")
+    dc = _bf16_cloth(spec)
+    o = helpers.oracle_from_spec(spec, precision=64)
+    for s in range(10):
+        po, vo = o.state()
+        dxo = o.system()[3] if s else np.zeros((spec.n, 3))
+        dc.pos.copy_(torch.from_numpy(po.astype(np.float32)).cuda())
+        dc.vel.copy_(torch.from_numpy(vo.astype(np.float32)).cuda())
+        dc.sim.debug_set_warm_start(dxo.astype(np.float32))
+        dc.step()
+        o.step(spec.dt)
+        assert not dc.sim.debug_entries16()
+        err = helpers.rel_maxnorm(dc.state()[0], o.state()[0])
+        assert err <= 1e-5, (s, err)
+        it, res = dc.sim.last_pcg_iterations(), dc.sim.last_relative_residual()
+        assert it == spec.max_iter or res <= spec.tol, (s, it, res)
+
+
 def test_bf16_batch_equals_alone_bitwise(gpu):
     import chysx
     from chysx import scenes

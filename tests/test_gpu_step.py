@@ -172,6 +172,30 @@ def test_scattered_numbering_uses_global_path(gpu):
         if s == 0:
             windows = np.asarray(dc.sim.debug_stage_windows())
             assert np.all(windows[:, 1] - windows[:, 0] + 1 > 2816), windows[:4]
+            assert not dc.sim.debug_entries16()  # columns up to 4095 rows away: 32-bit entries
+        assert helpers.rel_maxnorm(dc.state()[0], o.state()[0]) <= 1e-5, s
+        assert abs(dc.sim.last_pcg_iterations() - it_o) <= 1, s
+
+
+def test_grid_numbering_uses_16bit_entries(gpu):
+    """A grid numbered row by row keeps every column within 2047 rows of its
+    row, so the SpMV reads the 16-bit sliced entries; parity as above."""
+    import torch
+
+    from chysx import scenes
+
+    spec = scenes.plain_small(seed=7, n=64)
+    dc = _make(spec)
+    o = helpers.oracle_from_spec(spec)
+    for s in range(10):
+        po, vo = o.state()
+        dxo = o.system()[3] if s else np.zeros((spec.n, 3))
+        dc.pos.copy_(torch.from_numpy(po.astype(np.float32)).cuda())
+        dc.vel.copy_(torch.from_numpy(vo.astype(np.float32)).cuda())
+        dc.sim.debug_set_warm_start(dxo.astype(np.float32))
+        dc.step()
+        it_o = o.step(spec.dt)
+        assert dc.sim.debug_entries16()
         assert helpers.rel_maxnorm(dc.state()[0], o.state()[0]) <= 1e-5, s
         assert abs(dc.sim.last_pcg_iterations() - it_o) <= 1, s
 
