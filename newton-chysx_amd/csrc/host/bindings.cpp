@@ -287,6 +287,7 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("set_tile_rows", &chysx::cloth::ClothBatch::set_tile_rows, py::arg("rows"))
         .def("set_lds_staging", &chysx::cloth::ClothBatch::set_lds_staging, py::arg("enabled"))
         .def("lds_staging", &chysx::cloth::ClothBatch::lds_staging)
+        .def("entries16", &chysx::cloth::ClothBatch::entries16)
         .def("element_pass", &chysx::cloth::ClothBatch::element_pass, py::arg("dt"), py::arg("cuda_stream") = 0,
              py::call_guard<py::gil_scoped_release>())
         .def("gather_pass", &chysx::cloth::ClothBatch::gather_pass, py::arg("cuda_stream") = 0,

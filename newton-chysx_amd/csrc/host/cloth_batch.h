@@ -68,6 +68,7 @@ public:
     int graph_captures() const noexcept { return runner_.graph_captures(); }
     void set_lds_staging(bool on) { runner_.set_lds_staging(on); }
     bool lds_staging() const noexcept { return runner_.lds_staging(); }
+    bool entries16() const noexcept { return runner_.entries16(); }
     // Solver trace of the persistent PCG launch: capacity in barrier entries
     // (0: off); last_trace() returns {stamp in 100 MHz ticks, active
     // instances} per barrier of the last solve.

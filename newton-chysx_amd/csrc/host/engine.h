@@ -158,6 +158,9 @@ public:
     // straight from global memory (smaller LDS, more workgroups per CU).
     void set_lds_staging(bool on) { desc_.no_lds_staging = on ? 0 : 1; }
     bool lds_staging() const noexcept { return desc_.no_lds_staging == 0; }
+    // Whether the bound instances' SpMV reads 16-bit sliced entries (every
+    // instance has them) or 32-bit ones.
+    bool entries16() const noexcept { return desc_.entries16 != 0; }
     // Precision mode of the solve: FP32 blocks, or inner CG on the BF16
     // matrix with outer refinement on the FP32 true residual.
     void set_bf16(bool on) {

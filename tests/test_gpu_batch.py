@@ -42,10 +42,42 @@ def test_batch_equals_alone_bitwise(gpu):
         its_b = batch.last_pcg_iterations()
         its_a = [c.sim.last_pcg_iterations() for c in alone]
         assert its_b == its_a, (step, its_b, its_a)
+    assert batch.entries16()  # row-numbered grids: 16-bit sliced entries
     for a, m in zip(alone, mixed):
         pa, va = a.state()
         pm, vm = m.state()
         assert np.array_equal(pa, pm) and np.array_equal(va, vm)
+
+
+def test_32bit_entries_equal_16bit_bitwise(gpu):
+    """A row-numbered grid solved alone reads 16-bit sliced entries; batched
+    with a randomly renumbered grid (columns up to 4095 rows away) the whole
+    batch reads 32-bit entries.  Same blocks, same sums in the same order:
+    the grid's states and iteration counts agree bit for bit."""
+    import chysx
+    from chysx import scenes
+
+    grid = scenes.plain_small(seed=15, n=48)
+    base = scenes.plain_small(seed=16, n=64)
+    perm = np.random.default_rng(16).permutation(base.n)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(base.n)
+    scattered = scenes.ClothSpec(pos=base.pos[perm].copy(), tris=inv[base.tris].astype(np.int32),
+                                 pins=np.sort(inv[base.pins]).astype(np.int32), k_stretch=base.k_stretch,
+                                 k_shear=base.k_shear, k_bend=base.k_bend, density=base.density, name="scattered64")
+    alone = scenes.DeviceCloth(grid)
+    mixed = [scenes.DeviceCloth(grid), scenes.DeviceCloth(scattered)]
+    batch = chysx.ClothBatch()
+    for c in mixed:
+        batch.add_instance(c.sim)
+    for step in range(8):
+        alone.step()
+        batch.step(grid.dt)
+        assert alone.sim.debug_entries16() and not batch.entries16(), step
+        assert batch.last_pcg_iterations()[0] == alone.sim.last_pcg_iterations(), step
+    pa, va = alone.state()
+    pm, vm = mixed[0].state()
+    assert np.array_equal(pa, pm) and np.array_equal(va, vm)
 
 
 def test_batch_matches_oracle(gpu):
