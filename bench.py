@@ -318,16 +318,18 @@ def main():
     mean_ms = float(np.mean(pcg_ms))
     achieved = float(np.mean(pcg_bytes)) / (mean_ms * 1e-3) / 1e9
     traffic = load_traffic()
-    key = (f"{'global_' if args.no_lds_staging else ''}lanes{args.lanes}_grid{args.grid}_tile{args.tile_rows}"
-           f"_inst{len(specs)}")
+    entries16 = bool(batch.entries16())
+    key = (f"{'global_' if args.no_lds_staging else ''}{'' if entries16 else 'entries32_'}"
+           f"lanes{args.lanes}_grid{args.grid}_tile{args.tile_rows}_inst{len(specs)}")
     traffic_bytes = None
     if traffic and key in traffic.get("configs", {}):
         traffic_bytes = traffic["configs"][key]["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_bytes,
-                "kernel": "pcg_kernel<%d, %s> (persistent block-Jacobi PCG + finalize)"
-                          % (args.lanes, "global SpMV" if args.no_lds_staging else "LDS-staged SpMV"),
+                "kernel": "pcg_kernel<%d, %s, %s sliced entries> (persistent block-Jacobi PCG + finalize)"
+                          % (args.lanes, "global SpMV" if args.no_lds_staging else "LDS-staged SpMV",
+                             "16-bit" if entries16 else "32-bit"),
                 "mean_launch_ms": round(mean_ms, 3), "algorithmic_bytes_per_launch": float(np.mean(pcg_bytes))}
 
     log(f"PCG launch {mean_ms:.3f} ms, {achieved:.0f} GB/s algorithmic")
@@ -475,7 +477,7 @@ def main():
                    "hipgraph": not args.no_graph, "graph_captures": graph_captures,
                    "instances": len(specs), "vertices": n_rows, "off_diag_blocks": int(sizes[:, 1].sum()),
                    "pcg_tol": 1e-6, "pcg_cap": 100, "lanes_per_row": args.lanes, "grid_blocks": args.grid, "tile_rows": args.tile_rows,
-                   "lds_staging": not args.no_lds_staging,
+                   "lds_staging": not args.no_lds_staging, "sliced_entries": "16-bit" if entries16 else "32-bit",
                    "parallelism": "single GPU"},
         "pcg_iterations": {"mean": float(iters.mean()), "max": int(iters.max()), "min": int(iters.min())},
         "max_relative_residual": float(res.max()),
