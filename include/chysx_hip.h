@@ -56,7 +56,8 @@ enum {
 
 /* Contribution-list entry encoding (gather map of the deterministic
  * assembly): bits 31..30 element kind, 29..28 local row a, 27..26 local
- * column b, 25..0 element index in the packed element array. */
+ * column b, 25..0 the element's position in its assembly unit's list of
+ * elements of that kind. */
 #define CHX_CONTRIB_PIN 0u
 #define CHX_CONTRIB_TRI 1u
 #define CHX_CONTRIB_DIH 2u
@@ -65,7 +66,9 @@ enum {
 
 /* Packed element records (device arrays). Vertex indices are local to the
  * element's cloth instance. */
-typedef struct chx_pin { int v; int pad; float tx, ty, tz; float k; } chx_pin;
+/* Pins: vertex and stiffness; the targets are a separate AoS float3 array
+ * (chx_instance::pin_targets) so that update_pin_targets uploads only them. */
+typedef struct chx_pin { int v; float k; } chx_pin;
 typedef struct chx_tri { int v0, v1, v2, pad; float dm00, dm01, dm10, dm11; float kA; } chx_tri;
 typedef struct chx_dih { int v0, v1, v2, v3; float rest; float k; int pad0, pad1; } chx_dih;
 /* StVK membrane triangle (no reference counterpart; ClothMaterial's
@@ -101,14 +104,28 @@ typedef struct chx_contact_op {
     float friction_eps;              /* slip regularisation eps_u [m]                    */
 } chx_contact_op;
 
-/* Record sizes (floats) written per element by the element pass. */
-#define CHX_PIN_REC 4
-#define CHX_TRI_REC 24
-#define CHX_DIH_REC 16
-#define CHX_STVK_REC 36
+/* Assembly units.  The assembly kernel processes one unit per workgroup:
+ * a row range of one instance -- whole slices of CHX_SLICE_ROWS rows, up to
+ * four, or part of one slice -- whose elements' force / Hessian records
+ * (every element with a vertex in the range) it computes into LDS, then
+ * every block the unit owns: the diagonal blocks of its rows, then the upper
+ * value slots of its rows (slice by slice; block index jj, then row), each
+ * summing its contribution list.  Record floats per element in LDS and the
+ * LDS capacity a unit's records may use: */
+#define CHX_ASM_PIN_REC 4
+#define CHX_ASM_TRI_REC 22
+#define CHX_ASM_DIH_REC 14
+#define CHX_ASM_STVK_REC 31
+#define CHX_ASM_LDS_FLOATS 12288
+typedef struct chx_unit {
+    int row_begin, row_end;          /* rows [row_begin, row_end) of the instance             */
+    int elem_begin;                  /* first entry in unit_elems: pins, triangles, StVK,     */
+    int n_pins, n_tris, n_stvks, n_dihs; /* dihedrals, ascending element index per kind      */
+    int slot_begin;                  /* first of the unit's slots in contrib_offsets          */
+} chx_unit;
 
 /* Number of float64 partial sums kept per row tile by the PCG kernel. */
-#define CHX_TILE_PARTIALS 8
+#define CHX_TILE_PARTIALS 16
 
 /* 32-bit words of the grid-barrier / control area `sync` of a batch. */
 #define CHX_SYNC_WORDS 960
@@ -158,12 +175,12 @@ typedef struct chx_instance {
                                      /* n_rows rounded up to 64 blocks                       */
     float* values;                   /* grouped planar over the n_upper upper value slots    */
     const chx_pin* pins;
+    const float* pin_targets;        /* [n_pins*3] AoS float3 pin targets                    */
     const chx_tri* tris;             /* stretch elements, then shear elements                */
     const chx_dih* dihs;
-    float* pin_rec;                  /* [n_pins*CHX_PIN_REC]                                 */
-    float* tri_rec;                  /* [n_tris*CHX_TRI_REC]                                 */
-    float* dih_rec;                  /* [n_dihs*CHX_DIH_REC]                                 */
-    const unsigned* contrib_offsets; /* [n_rows+n_upper+1]: diagonal blocks, then value slots */
+    const chx_unit* units;           /* [n_units] assembly units, ascending rows             */
+    const int* unit_elems;           /* element indices of the units' lists                  */
+    const unsigned* contrib_offsets; /* [slots+1] per unit slot (units in order)             */
     const unsigned* contribs;        /* gather-map entries                                   */
     float* rhs;                      /* b [n_rows*3]                                         */
     float* x;                        /* dx: warm start in, solution out [n_rows*3]           */
@@ -172,13 +189,13 @@ typedef struct chx_instance {
     float* p0;
     float* p1;
     float* q;
+    float* q1;                       /* second q buffer of the single-barrier solve          */
     float* minv;                     /* block-Jacobi preconditioner, symmetric like diag     */
     float* diag_c;                   /* BF16 precision mode, else NULL: the inner SpMV's     */
     uint16_t* values16;              /* matrix, grouped planar FP32 compensated diagonal     */
                                      /* D + sum_j (V_ij - bf16(V_ij)) and grouped planar     */
                                      /* BF16 upper-block copies                              */
     const chx_stvk* stvks;           /* StVK membrane elements, or NULL                      */
-    float* stvk_rec;                 /* [n_stvks*CHX_STVK_REC]                               */
     const int* windows;              /* [ceil(n_rows/CHX_STAGE_ROWS)][2]: smallest and       */
                                      /* largest column (own rows included) of each stage of  */
                                      /* CHX_STAGE_ROWS rows, or NULL (no LDS staging)        */
@@ -197,6 +214,7 @@ typedef struct chx_instance {
     int n_stvks;
     int n_padded;                    /* entry slots (>= nnz)                                 */
     int n_upper;                     /* upper value slots (>= nnz / 2)                       */
+    int n_units;                     /* assembly units                                       */
     float dt;                        /* substep                                              */
     float damping;                   /* v *= exp(-damping*dt) when > 0                       */
     float gx, gy, gz;                /* gravity                                              */
@@ -212,16 +230,17 @@ typedef struct chx_instance {
 /* A batch of instances advanced together.  Work lists are device int4
  * arrays: row tiles {instance, row_begin, row_end, 0} (local rows, row_begin
  * a multiple of CHX_STAGE_ROWS, grouped by instance), element items {instance, kind
- * (0 pin, 1 triangle, 2 dihedral, 3 StVK triangle), begin, end} and slot items {instance,
- * begin, end, 0} (at most CHX_ITEM_SIZE each).  A batch with inst == NULL
- * holds one instance in `single` (passed by value in the kernel arguments);
- * its element and slot lists are implied by the counts, and so is its tile
- * list when `tiles` is NULL (CHX_TILE_ROWS rows per tile). */
+ * (0 pin, 1 triangle, 2 dihedral, 3 StVK triangle), begin, end} (at most
+ * CHX_ITEM_SIZE each; energy only) and unit items {instance, unit, 0, 0}.  A
+ * batch with inst == NULL holds one instance in `single` (passed by value in
+ * the kernel arguments); its element and unit lists are implied by the
+ * counts, and so is its tile list when `tiles` is NULL (CHX_TILE_ROWS rows
+ * per tile). */
 typedef struct chx_cloth_batch {
     int n_instances;
     int n_tiles;
     int n_elem_items;
-    int n_slot_items;
+    int n_unit_items;
     int lanes_per_row;               /* SpMV lanes per block row: 4, 8 or 16          */
     int grid_blocks;                 /* persistent PCG workgroups (0 = auto)          */
     int bf16_blocks;                 /* 1: inner CG on the BF16 matrix (FP32          */
@@ -238,30 +257,43 @@ typedef struct chx_cloth_batch {
     const chx_instance* inst;        /* device instance table, or NULL                */
     const int* tiles;
     const int* elem_items;
-    const int* slot_items;
+    const int* unit_items;
     double* partials;                /* [n_tiles*CHX_TILE_PARTIALS]                   */
     int* inst_iters;                 /* [n_instances] PCG iterations of the last solve */
     double* inst_rr;                 /* [n_instances] final ||r||^2                   */
     double* inst_bb;                 /* [n_instances] ||b||^2                         */
     int* inst_done;                  /* [n_instances] scratch                         */
+    double* inst_rho;                /* [n_instances] final <r, z>, or NULL           */
     unsigned* sync;                  /* [CHX_SYNC_WORDS] grid barrier + control words */
+    unsigned* status_host;           /* host-mapped word set to 1 when a solve aborts */
+                                     /* (grid-barrier timeout), or NULL; cleared by   */
+                                     /* chx_batch_check                               */
     chx_contact_op contact;          /* contact sidecar of a row-major solve (pairs     */
                                      /* NULL: none); its per-row index below           */
     const int* contact_rows;         /* [n_rows+1] first index entry of each row        */
     const int* contact_entries;      /* 4 c + slot, ascending per row                   */
     unsigned long long* trace;       /* optional solver trace, or NULL: entry e =      */
                                      /* {100 MHz real-time stamp, active instances} at */
-                                     /* barrier e (0: after set-up, 2k+1 / 2k+2: after */
-                                     /* phase A / B of iteration k), written by        */
-                                     /* workgroup 0                                    */
+                                     /* barrier e (0: after set-up; two-barrier form:  */
+                                     /* 2k+1 / 2k+2 after phase A / B of iteration k;  */
+                                     /* single-barrier form: k+1 after phase k),       */
+                                     /* written by workgroup 0                         */
     int trace_entries;               /* capacity of `trace` in entries (2 words each) */
-    int pad1;
+    unsigned wait_ticks;             /* bound of every grid-barrier wait in 100 MHz   */
+                                     /* ticks (0: 1e8, one second); past it the solve */
+                                     /* aborts                                        */
+    int pcg_form;                    /* FP32 cloth instances: 0 one grid barrier per   */
+                                     /* iteration (the two reductions merged), 1 two  */
+                                     /* barriers per iteration                        */
+    int pad2;
 } chx_cloth_batch;
 
 /* ---- errors and runtime ------------------------------------------------- */
 const char* chx_last_error_message(void);
 /* Number of visible devices; 0 when the HIP runtime finds none. */
 int chx_device_count(int* count);
+/* The calling thread's current device. */
+int chx_get_device(int* device);
 /* Compute units, the persistent PCG grid cap, and the runtime's occupancy
  * figure (workgroups per CU) for the PCG kernel on the current device. */
 int chx_device_info(int* cu_count, int* max_pcg_blocks, int* occupancy_blocks_per_cu);
@@ -273,6 +305,19 @@ int chx_pcg_grid_limit(int lanes_per_row, int bf16_blocks, int no_lds_staging, i
  * cuda_array.cu:41-90). */
 int chx_malloc(void** ptr, size_t bytes);
 int chx_free(void* ptr);
+/* Page-locked host memory, also mapped for the device (the pinned host
+ * mirror of CudaArray<T>, cuda_array.cu:41-62). */
+int chx_host_malloc(void** ptr, size_t bytes);
+int chx_host_free(void* ptr);
+/* Host-to-device copy queued on `stream` without waiting (`src` must be
+ * page-locked and stay untouched until the copy ran); on the null stream it
+ * waits, like the reference's cuda_copy_h2d (cuda_array.cu:69-79). */
+int chx_memcpy_h2d_async(void* dst, const void* src, size_t bytes, uintptr_t stream);
+/* Stream events (ordering host reuse of page-locked staging buffers). */
+int chx_event_create(void** event);
+int chx_event_destroy(void* event);
+int chx_event_record(void* event, uintptr_t stream);
+int chx_event_synchronize(void* event);
 /* Host-to-device copy ordered on `stream`; returns once `src` may be reused. */
 int chx_memcpy_h2d(void* dst, const void* src, size_t bytes, uintptr_t stream);
 /* Device-to-host copy; returns after the data is on the host. */
@@ -363,27 +408,33 @@ int chx_pcg_solve_bsr3_contact(int n_rows, const float* diag, const int* row_off
                                int* iterations, double* rr, double* bb, uintptr_t stream);
 
 /* ---- the implicit cloth substep -------------------------------------------- */
-/* One implicit-Euler substep of every instance of a packed batch:
- * element force/Hessian pass, deterministic gather assembly into the fixed
- * BSR pattern, persistent block-Jacobi PCG (warm start in batch->x) and
- * finalize into the instances' external pos/vel.  Replaces the device work
- * of ClothSimulator::step (ChysX/src/cloth/cloth_simulator.cu:814-1261).
- * `batch` is a host pointer to a descriptor of device pointers.
- * Asynchronous on `stream`. */
+/* One implicit-Euler substep of every instance of a packed batch: one
+ * assembly kernel (per unit: element forces and Hessian records into LDS,
+ * then the deterministic gather of every block and right-hand side the unit
+ * owns, into the fixed BSR pattern), the persistent block-Jacobi PCG (warm
+ * start in batch->x) and finalize into the instances' external pos/vel.
+ * Replaces the device work of ClothSimulator::step
+ * (ChysX/src/cloth/cloth_simulator.cu:814-1261).  `batch` is a host pointer
+ * to a descriptor of device pointers.  Asynchronous on `stream`. */
 int chx_cloth_step(const chx_cloth_batch* batch, uintptr_t stream);
 
 /* The two halves of chx_cloth_step, for callers that inspect the assembled
  * system before solving (debug_dump_last_solve) or time the parts. */
 int chx_cloth_assemble(const chx_cloth_batch* batch, uintptr_t stream);
 int chx_cloth_solve(const chx_cloth_batch* batch, int finalize, uintptr_t stream);
-/* The two halves of chx_cloth_assemble, for callers that time the element
- * pass (forces + Hessian records) and the gather pass (assembly) apart. */
-int chx_cloth_element_pass(const chx_cloth_batch* batch, uintptr_t stream);
-int chx_cloth_gather_pass(const chx_cloth_batch* batch, uintptr_t stream);
 
 /* Reads the device status word of the batch (non-zero after a grid-barrier
- * timeout inside the persistent solver) and clears it; synchronous. */
+ * timeout inside the persistent solver; it stays set across substeps until
+ * read here) and clears it and the host-mapped status word; synchronous. */
 int chx_batch_check(const chx_cloth_batch* batch, uintptr_t stream);
+
+/* Statistics of the last chx_pcg_solve_bsr3 call that used `workspace`:
+ * iterations, FP64 ||r||^2 and ||b||^2 of the final residual and the
+ * preconditioner-weighted residual <r, z> of the final iteration (the value
+ * PCGSolver::last_residual returns, pcg_solver.h:133-136,
+ * pcg_solver.cu:319-323).  Any output may be NULL; synchronous on `stream`. */
+int chx_pcg_last_stats(int n_rows, const void* workspace, size_t workspace_bytes, int* iterations, double* rr,
+                       double* bb, double* rho, uintptr_t stream);
 
 /* ---- hipGraph-captured substep ---------------------------------------------
  * No counterpart in the reference's cloth step (its PCGSolver captures only

@@ -4,8 +4,9 @@ Same public surface as the reference's ``chysx`` package for the cloth step
 (``ChysX/chysx/__init__.py:36-39``): :class:`ClothMaterial` and
 :class:`ClothSimulator`, whose ``step`` runs hand-written gfx950 HIP kernels
 through the C ABI in ``include/chysx_hip.h``.  Additions:
-:class:`ClothBatch` (many cloths per fused substep) and the host ``topology``
-functions.
+:class:`ClothBatch` (many cloths per fused substep), the standalone
+:class:`BlockCSR3` / :class:`PCGSolver` pair of the C++ API, and the host
+``topology`` functions.
 
 Particle buffers are borrowed device pointers (AoS float3), e.g.
 ``torch.Tensor.data_ptr()`` of a ``(n, 3)`` float32 tensor on ``cuda:0``::
@@ -29,9 +30,12 @@ from . import _chysx_native  # noqa: E402
 ClothMaterial = _chysx_native.ClothMaterial
 ClothSimulator = _chysx_native.ClothSimulator
 ClothBatch = _chysx_native.ClothBatch
+BlockCSR3 = _chysx_native.BlockCSR3
+PCGSolver = _chysx_native.PCGSolver
 topology = _chysx_native.topology
 io = _chysx_native.io
 
 __version__ = "0.1.0"
 
-__all__ = ["ClothBatch", "ClothMaterial", "ClothSimulator", "io", "topology", "__version__"]
+__all__ = ["BlockCSR3", "ClothBatch", "ClothMaterial", "ClothSimulator", "PCGSolver", "io", "topology",
+           "__version__"]

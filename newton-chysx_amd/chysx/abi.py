@@ -28,6 +28,14 @@ _SIGS = {
     "chx_pcg_grid_limit": (_I, [_I, _I, _I, _P]),
     "chx_malloc": (_I, [_P, _S]),
     "chx_free": (_I, [_P]),
+    "chx_get_device": (_I, [_P]),
+    "chx_host_malloc": (_I, [_P, _S]),
+    "chx_host_free": (_I, [_P]),
+    "chx_memcpy_h2d_async": (_I, [_P, _P, _S, _U]),
+    "chx_event_create": (_I, [_P]),
+    "chx_event_destroy": (_I, [_P]),
+    "chx_event_record": (_I, [_P, _U]),
+    "chx_event_synchronize": (_I, [_P]),
     "chx_memcpy_h2d": (_I, [_P, _P, _S, _U]),
     "chx_memcpy_d2h": (_I, [_P, _P, _S, _U]),
     "chx_memcpy_d2d": (_I, [_P, _P, _S, _U]),
@@ -49,9 +57,8 @@ _SIGS = {
     "chx_cloth_step": (_I, [_P, _U]),
     "chx_cloth_assemble": (_I, [_P, _U]),
     "chx_cloth_solve": (_I, [_P, _I, _U]),
-    "chx_cloth_element_pass": (_I, [_P, _U]),
-    "chx_cloth_gather_pass": (_I, [_P, _U]),
     "chx_batch_check": (_I, [_P, _U]),
+    "chx_pcg_last_stats": (_I, [_I, _P, _S, _P, _P, _P, _P, _U]),
     "chx_cloth_step_graph": (_I, [_P, _U, _P]),
     "chx_graph_launch": (_I, [_P, _U]),
     "chx_graph_destroy": (_I, [_P]),

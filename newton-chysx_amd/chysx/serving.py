@@ -26,8 +26,13 @@ else's.  Each job's result is identical alone and in any mix (the batch runs
 the same kernels on the job's own arrays with per-instance reductions).
 
 Reported: throughput (vertex-substeps/s), p50/p99 request latency (arrival
-to the end of the substep that finishes the job), queueing and set-up
-delays, peak concurrency and peak HBM in use.
+to the end of the substep that finishes the job) over the completed jobs and,
+when the server stops at a horizon, over every job that had arrived by then
+with the unfinished ones counted as censored at the horizon (their latency is
+at least horizon - arrival, so these percentiles are lower bounds), queueing
+and set-up delays, peak concurrency and peak HBM in use.  With `max_queue_s`
+the server sheds load: a job still waiting for admission that long after its
+arrival is rejected, and the rejection count is reported.
 """
 from __future__ import annotations
 
@@ -57,6 +62,7 @@ class Request:
     bytes: int = 0
     result: object = None
     future: object = None
+    rejected: bool = False         # shed at admission
 
 
 @dataclass
@@ -74,6 +80,10 @@ class ServingReport:
     peak_bytes: int
     substep_launches: int
     unfinished: int = 0
+    arrived: int = 0                              # arrivals by the end of the run
+    rejected: int = 0                             # shed at admission (max_queue_s)
+    p50_latency_censored_s: float | None = None   # over every arrived, not rejected job;
+    p99_latency_censored_s: float | None = None   # unfinished ones censored at the end
     latencies: list = field(default_factory=list)
 
     def as_dict(self):
@@ -87,7 +97,8 @@ class GpuExecutor:
     runs on `prepare_workers` host threads and its own stream."""
 
     def __init__(self, lanes_per_row: int = 8, stream=None, keep_results: bool = False, prepare_workers: int = 2,
-                 tile_rows: int | None = None, grid_blocks: int | None = None, lds_staging: bool = True):
+                 tile_rows: int | None = None, grid_blocks: int | None = None, lds_staging: bool = True,
+                 keep_only=None):
         import concurrent.futures
 
         import torch
@@ -112,6 +123,7 @@ class GpuExecutor:
         for _ in range(max(1, prepare_workers)):
             self.prep_streams.put(torch.cuda.Stream())
         self.keep_results = keep_results
+        self.keep_only = keep_only  # predicate on the request: whose final state to keep
         self.prepare_workers = prepare_workers
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=prepare_workers) if prepare_workers > 0 else None
 
@@ -146,7 +158,7 @@ class GpuExecutor:
 
     def retire(self, req: Request):
         cloth = req.handle
-        if self.keep_results:
+        if self.keep_results and (self.keep_only is None or self.keep_only(req)):
             req.result = cloth.state()
         self.batch.remove_instance(cloth.sim)
         req.handle = None
@@ -165,8 +177,10 @@ class GpuExecutor:
 
 class ServingEngine:
     def __init__(self, executor, hbm_budget_bytes: float = 200e9, max_concurrency: int = 1 << 30,
-                 clock=time.perf_counter, sleep=time.sleep, max_preparing: int | None = None):
+                 clock=time.perf_counter, sleep=time.sleep, max_preparing: int | None = None,
+                 max_queue_s: float | None = None):
         self.ex = executor
+        self.max_queue_s = max_queue_s
         self.budget = float(hbm_budget_bytes)
         self.max_conc = max_concurrency
         if max_preparing is None:
@@ -198,6 +212,7 @@ class ServingEngine:
         active: list[Request] = []
         done: list[Request] = []
         in_use = 0
+        rejected = 0
         peak_bytes = 0
         max_conc = 0
         launches = 0
@@ -209,6 +224,11 @@ class ServingEngine:
                 break
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
+            # load shedding: jobs that waited too long for admission leave
+            if self.max_queue_s is not None:
+                while waiting and now - waiting[0].arrival > self.max_queue_s:
+                    waiting.popleft().rejected = True
+                    rejected += 1
             # admission: FCFS while the estimated footprint fits and the
             # set-up queue has room
             while waiting and len(active) + len(preparing) < self.max_conc and len(preparing) < self.max_prep:
@@ -268,9 +288,15 @@ class ServingEngine:
         lat = [r.finished - r.arrival for r in done]
         qd = [r.admitted - r.arrival for r in done]
         sd = [r.joined - r.admitted for r in done]
+        # every job that arrived by the end and was not shed: finished ones
+        # with their latency, the others censored at the end of the run
+        arrived = [r for r in requests if r.arrival <= wall]
+        cens = [(r.finished if r.finished is not None else wall) - r.arrival
+                for r in arrived if not getattr(r, "rejected", False)]
         pct = lambda a, q: float(np.percentile(a, q)) if a else None  # noqa: E731
         return ServingReport(
             completed=len(done), submitted=len(requests), vertex_substeps=vs, wall_s=wall,
             throughput=vs / wall if wall > 0 else 0.0, p50_latency_s=pct(lat, 50), p99_latency_s=pct(lat, 99),
             p50_queue_s=pct(qd, 50), p50_setup_s=pct(sd, 50), max_concurrency=max_conc, peak_bytes=int(peak_bytes),
-            substep_launches=launches, unfinished=len(requests) - len(done), latencies=lat)
+            substep_launches=launches, unfinished=len(requests) - len(done), arrived=len(arrived), rejected=rejected,
+            p50_latency_censored_s=pct(cens, 50), p99_latency_censored_s=pct(cens, 99), latencies=lat)

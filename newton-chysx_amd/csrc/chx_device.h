@@ -245,11 +245,13 @@ struct Inst {
     gp<const int> row_offsets, col_indices;
     gp<float> diag, values;
     gp<const chx_pin> pins;
+    gp<const float> pin_targets;
     gp<const chx_tri> tris;
     gp<const chx_dih> dihs;
-    gp<float> pin_rec, tri_rec, dih_rec;
+    gp<const chx_unit> units;
+    gp<const int> unit_elems;
     gp<const unsigned> contrib_offsets, contribs;
-    gp<float> rhs, x, r, z, p0, p1, q, minv;
+    gp<float> rhs, x, r, z, p0, p1, q, q1, minv;
     gp<float> diag_c;
     gp<uint16_t> values16;
     gp<const int> windows;
@@ -260,8 +262,7 @@ struct Inst {
     int n_planes, n_boxes;
     float contact_thickness, contact_stiffness, contact_friction, contact_friction_eps;
     gp<const chx_stvk> stvks;
-    gp<float> stvk_rec;
-    int n_stvks, n_padded, n_upper;
+    int n_stvks, n_padded, n_upper, n_units;
     int n_rows, nnz, n_pins, n_tris, n_dihs, tile_begin, tile_end;
     float dt, damping, gx, gy, gz;
     int max_iter;
@@ -279,11 +280,11 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.diag = glb(d->diag);
     I.values = glb(d->values);
     I.pins = glb(d->pins);
+    I.pin_targets = glb(d->pin_targets);
     I.tris = glb(d->tris);
     I.dihs = glb(d->dihs);
-    I.pin_rec = glb(d->pin_rec);
-    I.tri_rec = glb(d->tri_rec);
-    I.dih_rec = glb(d->dih_rec);
+    I.units = glb(d->units);
+    I.unit_elems = glb(d->unit_elems);
     I.contrib_offsets = glb(d->contrib_offsets);
     I.contribs = glb(d->contribs);
     I.rhs = glb(d->rhs);
@@ -293,6 +294,7 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.p0 = glb(d->p0);
     I.p1 = glb(d->p1);
     I.q = glb(d->q);
+    I.q1 = glb(d->q1);
     I.minv = glb(d->minv);
     I.diag_c = glb(d->diag_c);
     I.values16 = glb(d->values16);
@@ -304,6 +306,7 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.block_slots = glb(d->block_slots);
     I.n_padded = d->n_padded;
     I.n_upper = d->n_upper;
+    I.n_units = d->n_units;
     I.planes = glb(d->planes);
     I.boxes = glb(d->boxes);
     I.n_planes = d->n_planes;
@@ -313,7 +316,6 @@ __device__ __forceinline__ Inst make_inst(P d) {
     I.contact_friction = d->contact_friction;
     I.contact_friction_eps = d->contact_friction_eps;
     I.stvks = glb(d->stvks);
-    I.stvk_rec = glb(d->stvk_rec);
     I.n_stvks = d->n_stvks;
     I.n_rows = d->n_rows;
     I.nnz = d->nnz;
@@ -389,17 +391,16 @@ __device__ __forceinline__ void get_elem_item(const chx_cloth_batch& B, int w, i
     e = min(n, b + CHX_ITEM_SIZE);
 }
 
-__device__ __forceinline__ void get_slot_item(const chx_cloth_batch& B, int w, int& inst, int& b, int& e) {
-    if (B.slot_items != nullptr) {
-        const gp<const int> p = rec_at(glb((const int*)B.slot_items), unsigned(w), 4u);
+// Assembly unit item w: {instance, unit}.
+__device__ __forceinline__ void get_unit_item(const chx_cloth_batch& B, int w, int& inst, int& u) {
+    if (B.unit_items != nullptr) {
+        const gp<const int> p = rec_at(glb((const int*)B.unit_items), unsigned(w), 4u);
         inst = p[0];
-        b = p[1];
-        e = p[2];
+        u = p[1];
         return;
     }
     inst = 0;
-    b = w * CHX_ITEM_SIZE;
-    e = min(B.single.n_rows + B.single.n_upper, b + CHX_ITEM_SIZE);
+    u = w;
 }
 
 // ---- wave64 / workgroup reductions (fixed order => deterministic) ---------
@@ -469,12 +470,23 @@ static_assert(SYNC_SNAP + 64 < SYNC_XCOUNT && SYNC_XCOUNT + 64 * kSyncGroups <= 
               "sync area too small");
 // The barrier is self-resetting: the last arriving workgroup zeroes the
 // count and bumps the generation, so consecutive launches reuse it.  Every
-// wait is bounded in time: a workgroup that has waited kWaitTicks of the
-// 100 MHz real-time counter raises the abort flag and all workgroups leave
-// the kernel, so a launch whose workgroups are not all resident ends with an
-// error instead of hanging.  Waiters back off exponentially so ~1000
-// workgroups polling one word do not starve the arrivals.
+// wait is bounded in time: a workgroup that has waited `ticks` of the
+// 100 MHz real-time counter (kWaitTicks unless the batch sets a bound)
+// raises the abort flag and the status word, sets the host-mapped status
+// word when there is one, and all workgroups leave the kernel, so a launch
+// whose workgroups are not all resident ends with an error instead of
+// hanging.  Waiters back off exponentially so ~1000 workgroups polling one
+// word do not starve the arrivals.
 constexpr unsigned long long kWaitTicks = 100000000ull;  // 1 s at 100 MHz
+
+struct Barrier {
+    unsigned* sync;                // CHX_SYNC_WORDS control words
+    unsigned* status_host;         // host-mapped abort report, or null
+    unsigned long long ticks;      // bound of every wait
+};
+__host__ __device__ inline Barrier make_barrier(const chx_cloth_batch& B) {
+    return Barrier{B.sync, B.status_host, B.wait_ticks ? (unsigned long long)B.wait_ticks : kWaitTicks};
+}
 
 __device__ __forceinline__ unsigned ld_relaxed(unsigned* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -516,8 +528,9 @@ __device__ __forceinline__ void backoff(unsigned& level) {
 // counter may already be changing in the next phase when a slow workgroup
 // leaves the barrier; the snapshot of this barrier is only rewritten two
 // barriers later, which needs that workgroup's arrival first.
-__device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks, int snap = -1,
+__device__ __forceinline__ bool grid_sync(const Barrier& bar, unsigned nblocks, int snap = -1,
                                           unsigned* active = nullptr) {
+    unsigned* sync = bar.sync;
     __shared__ int s_ok;
     __shared__ unsigned s_active;
     __builtin_amdgcn_s_waitcnt(0);
@@ -559,7 +572,7 @@ __device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks, int 
                     ok = 0;
                     break;
                 }
-                if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                if (__builtin_amdgcn_s_memrealtime() - t0 > bar.ticks) {
                     unsigned waiting = 0;
                     for (unsigned q = 0; q < kSyncGroups; ++q) waiting += ld_relaxed(&sync[SYNC_XCOUNT + 64 * q]);
                     st_relaxed(&sync[SYNC_SEEN], ld_relaxed(&sync[SYNC_COUNT]));
@@ -567,6 +580,8 @@ __device__ __forceinline__ bool grid_sync(unsigned* sync, unsigned nblocks, int 
                     st_relaxed(&sync[SYNC_GRID], nblocks);
                     st_relaxed(&sync[SYNC_ABORT], 1u);
                     st_relaxed(&sync[SYNC_STATUS], 1u);
+                    if (bar.status_host != nullptr)
+                        __hip_atomic_store(bar.status_host, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                     ok = 0;
                     break;
                 }

@@ -42,6 +42,13 @@ int chx_device_count(int* count) {
     });
 }
 
+int chx_get_device(int* device) {
+    return guarded([&] {
+        require(device != nullptr, "chx_get_device: null output");
+        hip_check(hipGetDevice(device), "hipGetDevice");
+    });
+}
+
 int chx_malloc(void** ptr, size_t bytes) {
     return guarded([&] {
         require(ptr != nullptr, "chx_malloc: null output pointer");
@@ -54,6 +61,59 @@ int chx_malloc(void** ptr, size_t bytes) {
 int chx_free(void* ptr) {
     return guarded([&] {
         if (ptr) hip_check(hipFree(ptr), "hipFree");
+    });
+}
+
+int chx_host_malloc(void** ptr, size_t bytes) {
+    return guarded([&] {
+        require(ptr != nullptr, "chx_host_malloc: null output pointer");
+        *ptr = nullptr;
+        if (bytes == 0) return;
+        hip_check(hipHostMalloc(ptr, bytes, hipHostMallocDefault), "hipHostMalloc");
+    });
+}
+
+int chx_host_free(void* ptr) {
+    return guarded([&] {
+        if (ptr) hip_check(hipHostFree(ptr), "hipHostFree");
+    });
+}
+
+int chx_memcpy_h2d_async(void* dst, const void* src, size_t bytes, uintptr_t stream) {
+    return guarded([&] {
+        if (bytes == 0) return;
+        require(dst && src, "chx_memcpy_h2d_async: null pointer");
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)), "hipMemcpyAsync(H2D)");
+        if (stream == 0) hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+    });
+}
+
+int chx_event_create(void** event) {
+    return guarded([&] {
+        require(event != nullptr, "chx_event_create: null output");
+        hipEvent_t e = nullptr;
+        hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreateWithFlags");
+        *event = e;
+    });
+}
+
+int chx_event_destroy(void* event) {
+    return guarded([&] {
+        if (event) hip_check(hipEventDestroy(static_cast<hipEvent_t>(event)), "hipEventDestroy");
+    });
+}
+
+int chx_event_record(void* event, uintptr_t stream) {
+    return guarded([&] {
+        require(event != nullptr, "chx_event_record: null event");
+        hip_check(hipEventRecord(static_cast<hipEvent_t>(event), as_stream(stream)), "hipEventRecord");
+    });
+}
+
+int chx_event_synchronize(void* event) {
+    return guarded([&] {
+        require(event != nullptr, "chx_event_synchronize: null event");
+        hip_check(hipEventSynchronize(static_cast<hipEvent_t>(event)), "hipEventSynchronize");
     });
 }
 

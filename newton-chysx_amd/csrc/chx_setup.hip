@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kBlock) void energy_kernel(chx_cloth_batch B, int m
     if (i < ie) {
         if (kind == 0 && (mask & 1)) {
             const chx_pin p = ldg(I.pins, unsigned(i));
-            const f3 d = sub(ld3(I.pos, p.v), mk(p.tx, p.ty, p.tz));
+            const f3 d = sub(ld3(I.pos, p.v), ld3g(I.pin_targets, unsigned(i)));
             E[0] = double(0.5f * p.k * dot(d, d));  // pin_constraint.cu:40-53
         } else if (kind == 1 && (mask & 2)) {
             const chx_tri t = ldg(I.tris, unsigned(i));

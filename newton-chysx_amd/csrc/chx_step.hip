@@ -1,27 +1,26 @@
 // C ABI: the implicit cloth substep on gfx950.
 //
-//   element pass   one thread per pin / triangle / dihedral: gathers x_n from
-//                  the instance's AoS positions and writes a compact force +
-//                  Hessian record (chx_elements.h).
-//   gather pass    one thread per BSR slot (diagonal slots, then off-diagonal
-//                  slots): sums the slot's contributions in the fixed order of
-//                  its contribution list, so the assembled matrix is
-//                  bitwise reproducible with no float atomics.  Diagonal slots
-//                  also form the right-hand side
+//   assembly       one workgroup per assembly unit (a row range of one
+//                  instance): the unit's element force / Hessian records are
+//                  computed into LDS from x_n, then one thread per block the
+//                  unit owns sums that block's contributions in the fixed
+//                  order of its contribution list, so the assembled matrix is
+//                  bitwise reproducible with no float atomics.  Diagonal
+//                  blocks also form the right-hand side
 //                  b = m/dt^2 (x~ - x_n) - grad E(x_n) and the inertia block
 //                  (cloth_simulator.cu:66-142,199-210).
 //   PCG            one persistent cooperative kernel per solve: block-Jacobi
-//                  PCG with L lanes per block row in the SpMV, FP64
-//                  per-tile partial sums reduced in a fixed order, two grid
-//                  barriers per iteration, per-instance convergence, then the
-//                  finalize x = x_n + dx, v = dx/dt (cloth_simulator.cu:222-244).
+//                  PCG, FP64 per-tile partial sums reduced in a fixed order,
+//                  one grid barrier per iteration (FP32 cloth instances) or
+//                  two (BF16 mode, caller-provided row-major systems),
+//                  per-instance convergence, then the finalize
+//                  x = x_n + dx, v = dx/dt (cloth_simulator.cu:222-244).
 //
 // Every instance owns its arrays (local indices); the batch gives work lists
-// of row tiles (CHX_TILE_ROWS rows of one instance), element items and slot
-// items (CHX_ITEM_SIZE entries of one instance).  Each row tile's partial
-// sums live in batch->partials[tile*8 + field]:
-//   0/1 rho (<r,z>) for even/odd iterations, 2/3 ||r||^2 even/odd,
-//   4 sigma (<p,Ap>), 5 ||b||^2.
+// of row tiles (CHX_TILE_ROWS rows of one instance), element items (energy)
+// and assembly units.  Each row tile's partial sums live in
+// batch->partials[tile * CHX_TILE_PARTIALS + field] (fields: P_* for the
+// two-barrier form, Q_* for the single-barrier form).
 #include <algorithm>
 #include <mutex>
 
@@ -48,135 +47,53 @@ __device__ __forceinline__ float guarded_div(double a, double b) {
     return (b > 1e-37 || b < -1e-37) ? float(a / b) : 0.0f;
 }
 
-// ---------------------------------------------------------------------------
-// Element pass
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void element_kernel(chx_cloth_batch B) {
-    int inst, kind, ib, ie;
-    get_elem_item(B, blockIdx.x, inst, kind, ib, ie);
-    const int ii = ib + int(threadIdx.x);
-    if (ii >= ie) return;
-    const unsigned i = unsigned(ii);
-    const Inst I = inst_at(B, inst);
-    const gp<const float> pos = I.pos;
-    if (kind == 0) {
-        const chx_pin p = ldg(I.pins, i);
-        const f3 x = ld3g(pos, unsigned(p.v));
-        const f3 g = mul(sub(x, mk(p.tx, p.ty, p.tz)), p.k);  // pin_constraint.cu:61-75
-        const gp<float> rec = rec_at(I.pin_rec, i, unsigned(CHX_PIN_REC));
-        rec[0] = g.x;
-        rec[1] = g.y;
-        rec[2] = g.z;
-        rec[3] = p.k;
-    } else if (kind == 1) {
-        const chx_tri t = ldg(I.tris, i);
-        float rec[CHX_TRI_REC];
-        tri_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00, t.dm01,
-                   t.dm10, t.dm11, t.kA, rec);
-        const gp<float> out = rec_at(I.tri_rec, i, unsigned(CHX_TRI_REC));
-#pragma unroll
-        for (int k = 0; k < CHX_TRI_REC; ++k) out[k] = rec[k];
-    } else if (kind == 3) {
-        const chx_stvk t = ldg(I.stvks, i);
-        float rec[CHX_STVK_REC];
-        stvk_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00, t.dm01,
-                    t.dm10, t.dm11, t.muA, t.lamA, rec);
-        const gp<float> out = rec_at(I.stvk_rec, i, unsigned(CHX_STVK_REC));
-#pragma unroll
-        for (int k = 0; k < CHX_STVK_REC; ++k) out[k] = rec[k];
-    } else {
-        const chx_dih d = ldg(I.dihs, i);
-        const DihFrame F = dihedral_frame(ld3g(pos, unsigned(d.v0)), ld3g(pos, unsigned(d.v1)),
-                                          ld3g(pos, unsigned(d.v2)), ld3g(pos, unsigned(d.v3)));
-        const gp<float> out = rec_at(I.dih_rec, i, unsigned(CHX_DIH_REC));
-        // An invalid frame contributes nothing (bending_constraint.cu:190-191,227-228).
-        const float c = F.valid ? d.k * (F.theta - d.rest) : 0.0f;
-        const float k = F.valid ? d.k : 0.0f;
-        out[0] = c;
-        out[1] = k;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const f3 a = F.valid ? F.a[j] : mk(0.0f, 0.0f, 0.0f);
-            out[2 + 3 * j] = a.x;
-            out[3 + 3 * j] = a.y;
-            out[4 + 3 * j] = a.z;
-        }
-        out[14] = 0.0f;
-        out[15] = 0.0f;
-    }
+// Resets the persistent solver's barrier / control words before a PCG launch
+// on the same stream.  The status word and the abort report next to it are
+// kept (all == false): a timed-out solve stays reported until
+// chx_batch_check reads and clears it.
+__device__ __forceinline__ void reset_sync(unsigned* sync, bool all) {
+    for (int k = threadIdx.x; k < SYNC_WORDS; k += blockDim.x)
+        if (all || k < SYNC_STATUS || k > SYNC_WAITING) sync[k] = 0u;
 }
 
 // ---------------------------------------------------------------------------
-// Gather (assembly) pass
+// Assembly: one kernel, one workgroup per assembly unit (a row range of one
+// instance, chx_unit).
+//
+//   phase 1  every element with a vertex in the unit's rows -- the unit's
+//            element list, ascending per kind -- gathers x_n from the
+//            instance's AoS positions and writes its force / Hessian record
+//            into LDS, field-major (record field f of local element e at
+//            [f * count + e]), so the gather below reads each record from
+//            LDS instead of from memory once per block it touches;
+//   phase 2  one thread per block the unit owns (the diagonal blocks of its
+//            rows, then its upper value slots) sums the block's
+//            contributions in the fixed order of its contribution list, so
+//            the assembled matrix is bitwise reproducible with no float
+//            atomics.  Diagonal blocks also form the right-hand side
+//            b = m/dt^2 (x~ - x_n) - grad E(x_n) and the inertia block
+//            (cloth_simulator.cu:66-142,199-210).
+//
+// Records hold the Hessian in factored form (chx_elements.h): all blocks of
+// an element come from one record.  The element formulas are the
+// reference's (triangle_stretch_constraint.cu:153-311,
+// bending_constraint.cu:180-245, pin_constraint.cu:61-92).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
-    if (blockIdx.x == 0) {
-        // Reset the persistent solver's barrier / control words for the PCG
-        // launch that follows on this stream.
-        for (int k = threadIdx.x; k < SYNC_WORDS; k += blockDim.x) B.sync[k] = 0u;
-    }
-    int inst, sb, se;
-    get_slot_item(B, blockIdx.x, inst, sb, se);
-    const int ss = sb + int(threadIdx.x);
-    if (ss >= se) return;
-    const unsigned s = unsigned(ss);
-    const Inst I = inst_at(B, inst);
-    const unsigned n = unsigned(I.n_rows);
-    const bool is_diag = s < n;
-    // Slots n.. are the value slots of the symmetric sliced layout (upper
-    // blocks, in storage order, padding included); lower blocks are their
-    // partners' transposes and are not assembled.
-    const unsigned c0 = ldx(I.contrib_offsets, s), c1 = ldx(I.contrib_offsets, s + 1u);
-    float h[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    f3 grad = mk(0.f, 0.f, 0.f);
-    for (unsigned c = c0; c < c1; ++c) {
-        const unsigned w = ldx(I.contribs, c);
-        const unsigned kind = w >> 30, a = (w >> 28) & 3u, b = (w >> 26) & 3u, id = w & CHX_CONTRIB_MAX_ELEMENT;
-        if (kind == CHX_CONTRIB_PIN) {
-            const gp<const float> r = rec_at(gp<const float>(I.pin_rec), id, unsigned(CHX_PIN_REC));
-            h[0] += r[3];
-            h[4] += r[3];
-            h[8] += r[3];
-            grad = add(grad, mk(r[0], r[1], r[2]));
-        } else if (kind == CHX_CONTRIB_TRI) {
-            const gp<const float> r = rec_at(gp<const float>(I.tri_rec), id, unsigned(CHX_TRI_REC));
-            const float su = r[18 + a] * r[18 + b];
-            const float sv = r[21 + a] * r[21 + b];
-            const float m0 = su * r[6] + sv * r[12], m1 = su * r[7] + sv * r[13], m2 = su * r[8] + sv * r[14];
-            const float m3 = su * r[9] + sv * r[15], m4 = su * r[10] + sv * r[16], m5 = su * r[11] + sv * r[17];
-            h[0] += m0; h[1] += m1; h[2] += m2;
-            h[3] += m1; h[4] += m3; h[5] += m4;
-            h[6] += m2; h[7] += m4; h[8] += m5;
-            if (is_diag && a == b) {
-                const float ua = r[18 + a], va = r[21 + a];
-                grad = add(grad, mk(ua * r[0] + va * r[3], ua * r[1] + va * r[4], ua * r[2] + va * r[5]));
-            }
-        } else if (kind == CHX_CONTRIB_STVK) {
-            const gp<const float> r = rec_at(gp<const float>(I.stvk_rec), id, unsigned(CHX_STVK_REC));
-            const float wua = r[27 + a], wva = r[30 + a], wub = r[27 + b], wvb = r[30 + b];
-            const float su = wua * wub, sv = wva * wvb, suv = wua * wvb, svu = wva * wub;
-            // Huu / Hvv symmetric (xx xy xz yy yz zz), Huv full; Hvu = Huv^T.
-            const float uu[9] = {r[6], r[7], r[8], r[7], r[9], r[10], r[8], r[10], r[11]};
-            const float vv[9] = {r[12], r[13], r[14], r[13], r[15], r[16], r[14], r[16], r[17]};
-#pragma unroll
-            for (int q = 0; q < 3; ++q)
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    h[3 * q + c] += su * uu[3 * q + c] + sv * vv[3 * q + c] + suv * r[18 + 3 * q + c] +
-                                    svu * r[18 + 3 * c + q];
-            if (is_diag && a == b)
-                grad = add(grad, mk(wua * r[0] + wva * r[3], wua * r[1] + wva * r[4], wua * r[2] + wva * r[5]));
-        } else {
-            const gp<const float> r = rec_at(gp<const float>(I.dih_rec), id, unsigned(CHX_DIH_REC));
-            const float k = r[1];
-            const f3 ai = mk(r[2 + 3 * a], r[3 + 3 * a], r[4 + 3 * a]);
-            const f3 aj = mk(r[2 + 3 * b], r[3 + 3 * b], r[4 + 3 * b]);
-            h[0] += (ai.x * aj.x) * k; h[1] += (ai.x * aj.y) * k; h[2] += (ai.x * aj.z) * k;
-            h[3] += (ai.y * aj.x) * k; h[4] += (ai.y * aj.y) * k; h[5] += (ai.y * aj.z) * k;
-            h[6] += (ai.z * aj.x) * k; h[7] += (ai.z * aj.y) * k; h[8] += (ai.z * aj.z) * k;
-            if (is_diag && a == b) grad = add(grad, mul(ai, r[0]));
-        }
-    }
+constexpr int kAsmFloats = CHX_ASM_LDS_FLOATS;
+// LDS record fields.  Triangle: Gu, Gv (0..5), Mu, Mv upper triangles
+// (6..17), Dm^-1 (18..21).  StVK: Gu, Gv (0..5), Huu, Hvv upper triangles
+// (6..17), Huv row-major (18..26), Dm^-1 (27..30).  Dihedral: k (theta -
+// theta0), k, a_0..a_3 (2..13).  Pin: k (x - t), k.
+static_assert(CHX_ASM_TRI_REC == 22 && CHX_ASM_STVK_REC == 31 && CHX_ASM_DIH_REC == 14 && CHX_ASM_PIN_REC == 4,
+              "record layouts");
+// Floats tri_record / stvk_record write (chx_elements.h): the LDS records
+// keep their first 18 / 27 and Dm^-1 in place of the shape derivatives.
+constexpr int kTriRec = 24, kStvkRec = 36;
+
+// Finish of one assembled slot: a diagonal block (row s) with the
+// predictor, right-hand side, inertia and static contact, or an upper value
+// slot s (planar FP32 block, and its BF16 copy in BF16 mode).
+__device__ __forceinline__ void finish_slot(const Inst& I, bool is_diag, unsigned s, float (&h)[9], f3 grad) {
     if (is_diag) {
         const unsigned row = s;
         const f3 p = ld3g(I.pos, row), v = ld3g(I.vel, row);
@@ -285,13 +202,181 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
 #pragma unroll
         for (unsigned q = 0; q < 6; ++q) stx(I.diag, bp6(q, row), h[sym6_pos(q)]);
     } else {
-        const unsigned slot = s - n;
+        const unsigned slot = s;
 #pragma unroll
         for (unsigned k = 0; k < 9; ++k) stx(I.values, bp(k, slot), h[k]);
         if (I.values16 != nullptr) {
 #pragma unroll
             for (unsigned k = 0; k < 9; ++k) stx(I.values16, bp(k, slot), f32_to_bf16(h[k]));
         }
+    }
+}
+
+struct AsmLds {
+    float rec[kAsmFloats];
+    int sl_first[4];   // slices of the unit: first value slot, lane range, width
+    int sl_l0[4];
+    int sl_cnt[4];
+    int sl_w[4];
+    int sl_end[4];     // cumulative upper slots through this slice
+};
+
+__global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
+    if (blockIdx.x == 0) reset_sync(B.sync, false);
+    __shared__ AsmLds L;
+    int inst, u;
+    get_unit_item(B, blockIdx.x, inst, u);
+    const Inst I = inst_at(B, inst);
+    const gp<const int> ud = rec_at((gp<const int>)I.units, unsigned(u), unsigned(sizeof(chx_unit) / sizeof(int)));
+    const int r0 = ud[0], r1 = ud[1], e0 = ud[2];
+    const int np = ud[3], nt = ud[4], nv = ud[5], nd = ud[6];
+    const unsigned slot0 = unsigned(ud[7]);
+    const int tid = int(threadIdx.x);
+    // the unit's slices (whole slices, up to four, or part of one)
+    const int s_lo = r0 / int(CHX_SLICE_ROWS), s_hi = (r1 - 1) / int(CHX_SLICE_ROWS);
+    const int ns = s_hi - s_lo + 1;
+    if (tid == 0) {
+        int acc = 0;
+        for (int q = 0; q < ns; ++q) {
+            const int sl = s_lo + q;
+            const int a = max(r0, sl * int(CHX_SLICE_ROWS)), b = min(r1, (sl + 1) * int(CHX_SLICE_ROWS));
+            const int uo0 = ldx(I.upper_offsets, unsigned(sl)), uo1 = ldx(I.upper_offsets, unsigned(sl + 1));
+            L.sl_first[q] = uo0;
+            L.sl_l0[q] = a - sl * int(CHX_SLICE_ROWS);
+            L.sl_cnt[q] = b - a;
+            L.sl_w[q] = (uo1 - uo0) / int(CHX_SLICE_ROWS);
+            acc += (b - a) * L.sl_w[q];
+            L.sl_end[q] = acc;
+        }
+    }
+    // ---- phase 1: records of the unit's elements into LDS ----
+    float* Rp = L.rec;
+    float* Rt = Rp + CHX_ASM_PIN_REC * np;
+    float* Rv = Rt + CHX_ASM_TRI_REC * nt;
+    float* Rd = Rv + CHX_ASM_STVK_REC * nv;
+    const gp<const float> pos = I.pos;
+    for (int e = tid; e < np + nt + nv + nd; e += kBlock) {
+        const unsigned id = unsigned(ldx(I.unit_elems, unsigned(e0 + e)));
+        if (e < np) {
+            const chx_pin pn = ldg(I.pins, id);
+            const f3 g = mul(sub(ld3g(pos, unsigned(pn.v)), ld3g(I.pin_targets, id)), pn.k);  // pin_constraint.cu:61-75
+            Rp[0 * np + e] = g.x;
+            Rp[1 * np + e] = g.y;
+            Rp[2 * np + e] = g.z;
+            Rp[3 * np + e] = pn.k;
+        } else if (e < np + nt) {
+            const int l = e - np;
+            const chx_tri t = ldg(I.tris, id);
+            float rec[kTriRec];
+            tri_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00,
+                       t.dm01, t.dm10, t.dm11, t.kA, rec);
+#pragma unroll
+            for (int f = 0; f < 18; ++f) Rt[f * nt + l] = rec[f];
+            Rt[18 * nt + l] = t.dm00;
+            Rt[19 * nt + l] = t.dm01;
+            Rt[20 * nt + l] = t.dm10;
+            Rt[21 * nt + l] = t.dm11;
+        } else if (e < np + nt + nv) {
+            const int l = e - np - nt;
+            const chx_stvk t = ldg(I.stvks, id);
+            float rec[kStvkRec];
+            stvk_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00,
+                        t.dm01, t.dm10, t.dm11, t.muA, t.lamA, rec);
+#pragma unroll
+            for (int f = 0; f < 27; ++f) Rv[f * nv + l] = rec[f];
+            Rv[27 * nv + l] = t.dm00;
+            Rv[28 * nv + l] = t.dm01;
+            Rv[29 * nv + l] = t.dm10;
+            Rv[30 * nv + l] = t.dm11;
+        } else {
+            const int l = e - np - nt - nv;
+            const chx_dih d = ldg(I.dihs, id);
+            const DihFrame F = dihedral_frame(ld3g(pos, unsigned(d.v0)), ld3g(pos, unsigned(d.v1)),
+                                              ld3g(pos, unsigned(d.v2)), ld3g(pos, unsigned(d.v3)));
+            // An invalid frame contributes nothing (bending_constraint.cu:190-191,227-228).
+            Rd[0 * nd + l] = F.valid ? d.k * (F.theta - d.rest) : 0.0f;
+            Rd[1 * nd + l] = F.valid ? d.k : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const f3 a = F.valid ? F.a[j] : mk(0.0f, 0.0f, 0.0f);
+                Rd[(2 + 3 * j) * nd + l] = a.x;
+                Rd[(3 + 3 * j) * nd + l] = a.y;
+                Rd[(4 + 3 * j) * nd + l] = a.z;
+            }
+        }
+    }
+    __syncthreads();
+    // ---- phase 2: one thread per owned slot ----
+    const int R = r1 - r0;
+    const int n_slots = R + L.sl_end[ns - 1];
+    for (int t = tid; t < n_slots; t += kBlock) {
+        const bool is_diag = t < R;
+        unsigned s;  // diagonal: row; upper: value slot
+        if (is_diag) {
+            s = unsigned(r0 + t);
+        } else {
+            int q = 0, v = t - R;
+            while (v >= L.sl_end[q]) ++q;
+            if (q > 0) v -= L.sl_end[q - 1];
+            const int jj = v / L.sl_cnt[q], l = L.sl_l0[q] + v % L.sl_cnt[q];
+            s = unsigned(L.sl_first[q] + jj * int(CHX_SLICE_ROWS) + l);
+        }
+        const unsigned c0 = ldx(I.contrib_offsets, slot0 + unsigned(t)), c1 = ldx(I.contrib_offsets, slot0 + unsigned(t) + 1u);
+        float h[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        f3 grad = mk(0.f, 0.f, 0.f);
+        for (unsigned c = c0; c < c1; ++c) {
+            const unsigned w = ldx(I.contribs, c);
+            const unsigned kind = w >> 30, a = (w >> 28) & 3u, b = (w >> 26) & 3u, l = w & CHX_CONTRIB_MAX_ELEMENT;
+            if (kind == CHX_CONTRIB_PIN) {
+                const float k = Rp[3 * np + l];
+                h[0] += k;
+                h[4] += k;
+                h[8] += k;
+                grad = add(grad, mk(Rp[l], Rp[np + l], Rp[2 * np + l]));
+            } else if (kind == CHX_CONTRIB_TRI) {
+                auto r = [&](int f) { return Rt[f * nt + int(l)]; };
+                const float d00 = r(18), d01 = r(19), d10 = r(20), d11 = r(21);
+                const float dwu[3] = {-d00 - d10, d00, d10}, dwv[3] = {-d01 - d11, d01, d11};
+                const float su = dwu[a] * dwu[b];
+                const float sv = dwv[a] * dwv[b];
+                const float m0 = su * r(6) + sv * r(12), m1 = su * r(7) + sv * r(13), m2 = su * r(8) + sv * r(14);
+                const float m3 = su * r(9) + sv * r(15), m4 = su * r(10) + sv * r(16), m5 = su * r(11) + sv * r(17);
+                h[0] += m0; h[1] += m1; h[2] += m2;
+                h[3] += m1; h[4] += m3; h[5] += m4;
+                h[6] += m2; h[7] += m4; h[8] += m5;
+                if (is_diag && a == b) {
+                    const float ua = dwu[a], va = dwv[a];
+                    grad = add(grad, mk(ua * r(0) + va * r(3), ua * r(1) + va * r(4), ua * r(2) + va * r(5)));
+                }
+            } else if (kind == CHX_CONTRIB_STVK) {
+                auto r = [&](int f) { return Rv[f * nv + int(l)]; };
+                const float d00 = r(27), d01 = r(28), d10 = r(29), d11 = r(30);
+                const float dwu[3] = {-d00 - d10, d00, d10}, dwv[3] = {-d01 - d11, d01, d11};
+                const float wua = dwu[a], wva = dwv[a], wub = dwu[b], wvb = dwv[b];
+                const float su = wua * wub, sv = wva * wvb, suv = wua * wvb, svu = wva * wub;
+                // Huu / Hvv symmetric (xx xy xz yy yz zz), Huv full; Hvu = Huv^T.
+                const float uu[9] = {r(6), r(7), r(8), r(7), r(9), r(10), r(8), r(10), r(11)};
+                const float vv[9] = {r(12), r(13), r(14), r(13), r(15), r(16), r(14), r(16), r(17)};
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        h[3 * q + c] += su * uu[3 * q + c] + sv * vv[3 * q + c] + suv * r(18 + 3 * q + c) +
+                                        svu * r(18 + 3 * c + q);
+                if (is_diag && a == b)
+                    grad = add(grad, mk(wua * r(0) + wva * r(3), wua * r(1) + wva * r(4), wua * r(2) + wva * r(5)));
+            } else {
+                auto r = [&](int f) { return Rd[f * nd + int(l)]; };
+                const float k = r(1);
+                const f3 ai = mk(r(2 + 3 * a), r(3 + 3 * a), r(4 + 3 * a));
+                const f3 aj = mk(r(2 + 3 * b), r(3 + 3 * b), r(4 + 3 * b));
+                h[0] += (ai.x * aj.x) * k; h[1] += (ai.x * aj.y) * k; h[2] += (ai.x * aj.z) * k;
+                h[3] += (ai.y * aj.x) * k; h[4] += (ai.y * aj.y) * k; h[5] += (ai.y * aj.z) * k;
+                h[6] += (ai.z * aj.x) * k; h[7] += (ai.z * aj.y) * k; h[8] += (ai.z * aj.z) * k;
+                if (is_diag && a == b) grad = add(grad, mul(ai, r(0)));
+            }
+        }
+        finish_slot(I, is_diag, s, h, grad);
     }
 }
 
@@ -302,37 +387,33 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(chx_cloth_batch B) {
 // it the rounding of the off-diagonal blocks perturbs the smooth modes --
 // the slowest ones for CG -- by O(eps * stiffness) against eigenvalues of
 // O(stiffness / n^2); with it the perturbation acts on differences
-// x_j - x_i only.  One thread per block row, over the gather's slot items.
+// x_j - x_i only.  One workgroup per row tile, one thread per block row.
 __global__ __launch_bounds__(kBlock) void compensate_kernel(chx_cloth_batch B) {
-    int inst, sb, se;
-    get_slot_item(B, blockIdx.x, inst, sb, se);
-    const int ss = sb + int(threadIdx.x);
-    if (ss >= se) return;
+    int inst, rb, re;
+    get_tile(B, blockIdx.x, inst, rb, re);
     const Inst I = inst_at(B, inst);
-    const unsigned n = unsigned(I.n_rows), row = unsigned(ss);
-    if (row >= n) return;
-    const unsigned k0 = ldx(I.row_offsets, row), k1 = ldx(I.row_offsets, row + 1u);
-    float d[9];
-    ld6p(gp<const float>(I.diag), row, d);
-    for (unsigned k = k0; k < k1; ++k) {
-        // lower blocks: the partner's rounding, transposed
-        const int bs = ldx(I.block_slots, k);
-        const bool tr = bs < 0;
-        const unsigned slot = unsigned(tr ? -1 - bs : bs);
+    for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
+        const unsigned k0 = ldx(I.row_offsets, row), k1 = ldx(I.row_offsets, row + 1u);
+        float d[9];
+        ld6p(gp<const float>(I.diag), row, d);
+        for (unsigned k = k0; k < k1; ++k) {
+            // lower blocks: the partner's rounding, transposed
+            const int bs = ldx(I.block_slots, k);
+            const bool tr = bs < 0;
+            const unsigned slot = unsigned(tr ? -1 - bs : bs);
 #pragma unroll
-        for (unsigned c = 0; c < 9; ++c) {
-            const unsigned cs = tr ? (c % 3u) * 3u + c / 3u : c;
-            const float v = ldx(gp<const float>(I.values), bp(cs, slot));
-            d[c] += v - bf16_to_f32(f32_to_bf16(v));
+            for (unsigned c = 0; c < 9; ++c) {
+                const unsigned cs = tr ? (c % 3u) * 3u + c / 3u : c;
+                const float v = ldx(gp<const float>(I.values), bp(cs, slot));
+                d[c] += v - bf16_to_f32(f32_to_bf16(v));
+            }
         }
-    }
 #pragma unroll
-    for (unsigned c = 0; c < 9; ++c) stx(I.diag_c, bp(c, row), d[c]);
+        for (unsigned c = 0; c < 9; ++c) stx(I.diag_c, bp(c, row), d[c]);
+    }
 }
 
-__global__ void reset_sync_kernel(unsigned* sync) {
-    for (int k = threadIdx.x; k < SYNC_WORDS; k += blockDim.x) sync[k] = 0u;
-}
+__global__ void reset_sync_kernel(unsigned* sync, int all) { reset_sync(sync, all != 0); }
 
 // ---------------------------------------------------------------------------
 // SpMV row loop: L lanes per block row, 64/L rows per wave and row group; R
@@ -511,8 +592,8 @@ static_assert(kCompactMax <= 4 * kBlock, "build_active scans 4 instances per thr
 
 // Shared by both SpMV forms.
 struct PcgShared {
-    double red[16];
-    double sc[kChunk][6];
+    double red[24];
+    double sc[kChunk][8];
     // Active tile list (batches of up to kCompactMax instances): instance i
     // owns tiles [tb[i], tb[i] + tc[i]); pre[i] = active tiles before it.
     int tb[kCompactMax];
@@ -796,7 +877,11 @@ __device__ __forceinline__ void load_tile_ranges(const chx_cloth_batch& B, PcgSh
 // Builds pre[] / nact from the done flags: instance i counts tc[i] tiles
 // while it is solving, 0 once finished.  The flags must not change while
 // any workgroup builds (every workgroup has to derive the same list).
-__device__ __forceinline__ void build_active(const chx_cloth_batch& B, PcgShared& S) {
+// phase >= 0 (single-barrier solve): done flags hold the phase an instance
+// stopped in, plus one; an instance stays on the list of the phase it stops
+// in, so workgroups that read its flag before or after it is set in this
+// phase derive the same list.
+__device__ __forceinline__ void build_active(const chx_cloth_batch& B, PcgShared& S, int phase = -1) {
     const int n = B.n_instances;
     const int base = int(threadIdx.x) * 4;
     int c[4];
@@ -805,8 +890,10 @@ __device__ __forceinline__ void build_active(const chx_cloth_batch& B, PcgShared
     for (int q = 0; q < 4; ++q) {
         const int i = base + q;
         c[q] = 0;
-        if (i < n && __hip_atomic_load(&B.inst_done[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-            c[q] = S.tc[i];
+        if (i < n) {
+            const int d = __hip_atomic_load(&B.inst_done[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (d == 0 || (phase >= 0 && d > phase)) c[q] = S.tc[i];
+        }
         sum += c[q];
     }
     // inclusive scan of `sum` over the workgroup: wave scan, then wave totals
@@ -995,6 +1082,26 @@ __device__ __forceinline__ void trace_point(const chx_cloth_batch& B, int e, uns
     }
 }
 
+// Finalize of every tile (cloth_simulator.cu:222-244): pos = x_n + dx,
+// vel = dx / dt, times exp(-damping dt) when damping > 0.
+__device__ __forceinline__ void finalize_tiles(const chx_cloth_batch& B, int G) {
+    for (int t = blockIdx.x; t < B.n_tiles; t += G) {
+        int inst, rb, re;
+        get_tile(B, t, inst, rb, re);
+        const Inst I = inst_at(B, inst);
+        const float inv_dt = 1.0f / I.dt;
+        const float decay = (I.damping > 0.0f) ? expf(-I.damping * I.dt) : 1.0f;
+        for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
+            const f3 d = ld3g(I.x, row);
+            const f3 p = ld3g(I.pos, row);
+            st3g(I.pos, row, add(p, d));
+            f3 v = mul(d, inv_dt);
+            if (I.damping > 0.0f) v = mul(v, decay);
+            st3g(I.vel, row, v);
+        }
+    }
+}
+
 template <int L, int V, bool ST>
 __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int finalize) {
     constexpr bool BF16 = V == 2 || V == 4;
@@ -1003,6 +1110,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
     __shared__ PcgLds<ST> S;
     const int G = int(gridDim.x);
     unsigned* sync = B.sync;
+    const Barrier bar = make_barrier(B);
     const gp<double> part = glb(B.partials);
     // BF16 mode: the inner iterations are CG on the BF16 matrix (compensated
     // FP32 diagonal) and test the recursively updated residual; when it meets
@@ -1038,7 +1146,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
     const bool compactable = B.inst != nullptr && B.n_instances <= kCompactMax;
     if (compactable) load_tile_ranges(B, S);
     bool compact = false;
-    if (!grid_sync(sync, G)) return;
+    if (!grid_sync(bar, G)) return;
     trace_point(B, 0, unsigned(B.n_instances));
 
     for (int k = 0;; ++k) {
@@ -1066,6 +1174,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                         B.inst_iters[inst] = k;
                         B.inst_rr[inst] = rr;
                         B.inst_bb[inst] = bb;
+                        if (B.inst_rho != nullptr) B.inst_rho[inst] = rho;
                         __hip_atomic_fetch_sub(&sync[SYNC_ACTIVE], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                     continue;
@@ -1113,7 +1222,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
             }
         }
         unsigned active = 0;
-        if (!grid_sync(sync, G, cur, &active)) return;
+        if (!grid_sync(bar, G, cur, &active)) return;
         trace_point(B, 2 * k + 1, active);
         if (active == 0u) break;
 
@@ -1159,28 +1268,261 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 }
             }
         }
-        if (!grid_sync(sync, G)) return;
+        if (!grid_sync(bar, G)) return;
         trace_point(B, 2 * k + 2, active);
     }
 
     // ---- finalize: x_{n+1} = x_n + dx, v = dx/dt (cloth_simulator.cu:222-244) --
-    if (finalize) {
-        for (int t = blockIdx.x; t < B.n_tiles; t += G) {
-            int inst, rb, re;
-            get_tile(B, t, inst, rb, re);
+    if (finalize) finalize_tiles(B, G);
+}
+
+// ---------------------------------------------------------------------------
+// Single-barrier form of the persistent PCG for the FP32 sliced cloth
+// instances.  Same recurrence as pcg_kernel -- and as the reference
+// (pcg_solver.cu:134-215) -- with each iteration's two global reductions
+// merged into one phase:
+//
+//   phase k: with alpha_{k-1} = rho_{k-1} / sigma_{k-1} and
+//            beta_{k-1} = rho_k / rho_{k-1},
+//              r_k = r_{k-1} - alpha q_{k-1},  z_k = M^-1 r_k,
+//              x_k = x_{k-1} + alpha p_{k-1},  p_k = z_k + beta p_{k-1},
+//              q_k = A p_k,
+//            tile partials rho_k = <r_k, z_k>, ||r_k||^2, sigma_k = <p_k, q_k>,
+//            delta_k = <q_k, z_k>, gamma_k = <q_k, M^-1 q_k>      -> barrier
+//
+// rho_k is needed for beta_{k-1} before the reduction over the r_k that the
+// same phase produces, so it comes from the exact identity (M^-1 symmetric)
+//   <r_k, z_k> = rho_{k-1} - 2 alpha delta_{k-1} + alpha^2 gamma_{k-1}
+// in FP64; every other scalar -- alpha_k's rho_k, the convergence test's
+// ||r_k||^2 -- is the exact sum over the stored FP32 vectors of the phase
+// before, so the identity is used for one step only and never accumulates.
+// A tile forms z_k and p_k on the fly for the source window of its SpMV
+// from r_{k-1}, q_{k-1}, p_{k-1} and M^-1 (double-buffered r and q), and
+// updates r, x of its own rows in the SpMV's row finish.  The convergence
+// test of iteration k runs at the start of phase k+1, on ||r_k||^2: the
+// iteration count, the stopping iterate x_k and the final statistics are
+// the two-barrier form's; the instance's last SpMV goes unused.  At the
+// iteration cap the last phase updates r and x only.
+// ---------------------------------------------------------------------------
+// Tile partials of the single-barrier form, per parity: rho, ||r||^2,
+// sigma, delta, gamma; then ||b||^2.
+enum { Q_RHO0 = 0, Q_RR0 = 2, Q_SIG0 = 4, Q_DEL0 = 6, Q_GAM0 = 8, Q_BB = 10 };
+static_assert(Q_BB < CHX_TILE_PARTIALS, "tile partials");
+
+// Sums over the instance of each chunk tile of partial fields f[0..NF-1]
+// into S.sc[m][0..NF-1] and the instance's raw done flag into S.sc[m][NF]
+// (same lane-strided order and butterfly as chunk_sums: the same bits in
+// every workgroup).
+template <int NF>
+__device__ __forceinline__ void chunk_sums_n(const chx_cloth_batch& B, gp<const double> part, const int (&f)[NF],
+                                             PcgShared& S) {
+    static_assert(NF + 1 <= 8, "S.sc holds 8 values per tile");
+    __syncthreads();
+    const int m = int(threadIdx.x) >> 4, l = int(threadIdx.x) & 15;
+    const int t = S.ctile[m];
+    double sum[NF];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) sum[q] = 0.0;
+    double done = 0.0;
+    if (t >= 0) {
+        const int inst = S.cinst[m];
+        int tb, te;
+        if (B.inst != nullptr && B.n_instances <= kCompactMax) {
+            tb = S.tb[inst];
+            te = tb + S.tc[inst];
+        } else {
             const Inst I = inst_at(B, inst);
-            const float inv_dt = 1.0f / I.dt;
-            const float decay = (I.damping > 0.0f) ? expf(-I.damping * I.dt) : 1.0f;
-            for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
-                const f3 d = ld3g(I.x, row);
-                const f3 p = ld3g(I.pos, row);
-                st3g(I.pos, row, add(p, d));
-                f3 v = mul(d, inv_dt);
-                if (I.damping > 0.0f) v = mul(v, decay);
-                st3g(I.vel, row, v);
+            tb = I.tile_begin;
+            te = I.tile_end;
+        }
+        for (int u0 = tb + l; u0 < te; u0 += 32) {
+            double v[2][NF];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int u = u0 + 16 * h;
+                if (u < te) {
+                    const gp<const double> p = rec_at(part, unsigned(u), unsigned(CHX_TILE_PARTIALS));
+#pragma unroll
+                    for (int q = 0; q < NF; ++q) v[h][q] = p[f[q]];
+                }
             }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (u0 + 16 * h < te) {
+#pragma unroll
+                    for (int q = 0; q < NF; ++q) sum[q] += v[h][q];
+                }
+        }
+        if (l == 0) done = double(__hip_atomic_load(&B.inst_done[inst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+#pragma unroll
+    for (int q = 0; q < NF; ++q) sum[q] = seg_sum<16>(sum[q]);
+    if (l == 0) {
+#pragma unroll
+        for (int q = 0; q < NF; ++q) S.sc[m][q] = sum[q];
+        S.sc[m][NF] = done;
+    }
+    __syncthreads();
+}
+
+template <int EL, bool ST>
+__global__ __launch_bounds__(kBlock) void pcg1_kernel(chx_cloth_batch B, int finalize) {
+    __shared__ PcgLds<ST> S;
+    const int G = int(gridDim.x);
+    unsigned* sync = B.sync;
+    const Barrier bar = make_barrier(B);
+    const gp<double> part = glb(B.partials);
+
+    // ---- setup: r_0 = b - A x, M^-1, z_0 = M^-1 r_0, p_0 = z_0; rho_0, ||r_0||^2, ||b||^2 ----
+    if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&sync[SYNC_ACTIVE], unsigned(B.n_instances));
+    if (blockIdx.x == 0 && B.trace != nullptr)
+        for (int e = threadIdx.x; e < 2 * B.trace_entries; e += blockDim.x) glb(B.trace)[e] = 0ull;
+    for (int t = blockIdx.x; t < B.n_tiles; t += G) {
+        int inst, rb, re;
+        get_tile(B, t, inst, rb, re);
+        const Inst I = inst_at(B, inst);
+        if (threadIdx.x == 0 && t == I.tile_begin)
+            __hip_atomic_store(&B.inst_done[inst], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        double acc[3] = {0.0, 0.0, 0.0};
+        residual_tile<4, ST, EL>(B, I, PcgMats<1>::r(I), rb, re, true, I.p0, S, acc);
+        block_sum(acc, S.red);
+        if (threadIdx.x == 0) {
+            const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
+            p[Q_RHO0 + 1] = acc[0];  // read by phase 0 as the previous parity
+            p[Q_RR0 + 1] = acc[1];
+            p[Q_BB] = acc[2];
         }
     }
+    const bool compactable = B.inst != nullptr && B.n_instances <= kCompactMax;
+    if (compactable) load_tile_ranges(B, S);
+    if (!grid_sync(bar, G)) return;
+    trace_point(B, 0, unsigned(B.n_instances));
+
+    for (int k = 0;; ++k) {
+        const int cur = k & 1, prv = cur ^ 1;
+        if (compactable) build_active(B, S, k);
+        const int fld[6] = {Q_RHO0 + prv, Q_SIG0 + prv, Q_DEL0 + prv, Q_GAM0 + prv, Q_RR0 + prv, Q_BB};
+        for (int c = 0; next_chunk(B, S, compactable, c); ++c) {
+            chunk_sums_n<6>(B, part, fld, S);
+            for (int m = 0; m < kChunk; ++m) {
+                const int t = S.ctile[m];
+                if (t < 0) break;
+                int inst, rb, re;
+                chunk_tile(S, m, inst, rb, re);
+                const Inst I = inst_at(B, inst);
+                const double rho = S.sc[m][0], sig = S.sc[m][1], del = S.sc[m][2], gam = S.sc[m][3];
+                const double rr = S.sc[m][4], bb = S.sc[m][5];
+                const int d = int(S.sc[m][6]);
+                if (d != 0 && d <= k) continue;  // stopped in an earlier phase
+                const int its = k > 0 ? k - 1 : 0;  // updates of x so far
+                if (its >= I.max_iter || (I.tol > 0.0 && rr <= I.tol * I.tol * bb)) {
+                    if (threadIdx.x == 0 && t == I.tile_begin && d == 0) {
+                        __hip_atomic_store(&B.inst_done[inst], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        B.inst_iters[inst] = its;
+                        B.inst_rr[inst] = rr;
+                        B.inst_bb[inst] = bb;
+                        if (B.inst_rho != nullptr) B.inst_rho[inst] = rho;
+                        __hip_atomic_fetch_sub(&sync[SYNC_ACTIVE], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    continue;
+                }
+                // buffers of parity q: r -> (r, z), q -> (q, q1), p -> (p0, p1)
+                const gp<float> r_prev = prv ? I.z : I.r, r_cur = cur ? I.z : I.r;
+                const gp<float> q_prev = prv ? I.q1 : I.q, q_cur = cur ? I.q1 : I.q;
+                const gp<float> p_prev = prv ? I.p1 : I.p0, p_cur = cur ? I.p1 : I.p0;
+                const gp<const float> minv = gp<const float>(I.minv);
+                double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // rho, rr, sigma, delta, gamma
+                if (k == 0) {
+                    // q_0 = A p_0 (z_0 = p_0); rho_0, ||r_0||^2 carried over
+                    auto pj = [&](unsigned j) { return ld3g(p_cur, j); };
+                    tile_spmv<4, ST, EL>(I, PcgMats<1>::a(I), rb, re, S, pj, [&](unsigned row, f3 Ap, f3 pr) {
+                        st3g(q_cur, row, Ap);
+                        acc[2] += dot64(pr, Ap);
+                        acc[3] += dot64(Ap, pr);
+                        acc[4] += dot64(Ap, mv6p(minv, row, Ap));
+                    });
+                } else {
+                    const float alpha = guarded_div(rho, sig);
+                    const double ad = double(alpha);
+                    // rho_k by the identity: its terms carry the FP32
+                    // rounding of the stored vectors, a few units of 2^-24
+                    // of their size.  Above 1e-5 of them (~100 such units)
+                    // rho_k, and so beta, is resolved to about 1%; below,
+                    // the direction restarts from z_k (beta = 0) instead of
+                    // taking a rounding-noise beta.  A tighter cut (1e-3)
+                    // restarts on ordinary fast-converging iterations and
+                    // costs the Krylov acceleration: 13 iterations instead
+                    // of 9 on a pin-driven substep of the 32x32 cloth.
+                    const double t1 = 2.0 * ad * del, t2 = ad * ad * gam;
+                    const double rho_k = rho - t1 + t2;
+                    const bool resolved = rho_k > 1.0e-5 * (fabs(rho) + fabs(t1) + fabs(t2));
+                    const float beta = resolved ? guarded_div(rho_k, rho) : 0.0f;
+                    // r_k and z_k of row j from the previous parity
+                    auto rz = [&](unsigned j, f3& rn, f3& zn) {
+                        const f3 r = ld3g(r_prev, j), q = ld3g(q_prev, j);
+                        rn = mk(fmaf(-alpha, q.x, r.x), fmaf(-alpha, q.y, r.y), fmaf(-alpha, q.z, r.z));
+                        zn = mv6p(minv, j, rn);
+                    };
+                    // own-row update: r_k, x_k; returns z_k
+                    auto own = [&](unsigned row, f3& zn) {
+                        f3 rn;
+                        rz(row, rn, zn);
+                        const f3 pp = ld3g(p_prev, row);
+                        f3 x = ld3g(I.x, row);
+                        x = mk(fmaf(alpha, pp.x, x.x), fmaf(alpha, pp.y, x.y), fmaf(alpha, pp.z, x.z));
+                        st3g(I.x, row, x);
+                        st3g(r_cur, row, rn);
+                        acc[0] += dot64(rn, zn);
+                        acc[1] += dot64(rn, rn);
+                    };
+                    if (k < I.max_iter) {
+                        auto pj = [&](unsigned j) {
+                            f3 rn, zn;
+                            rz(j, rn, zn);
+                            const f3 pp = ld3g(p_prev, j);
+                            return mk(fmaf(beta, pp.x, zn.x), fmaf(beta, pp.y, zn.y), fmaf(beta, pp.z, zn.z));
+                        };
+                        tile_spmv<4, ST, EL>(I, PcgMats<1>::a(I), rb, re, S, pj, [&](unsigned row, f3 Ap, f3 pr) {
+                            f3 zn;
+                            own(row, zn);
+                            st3g(p_cur, row, pr);
+                            st3g(q_cur, row, Ap);
+                            acc[2] += dot64(pr, Ap);
+                            acc[3] += dot64(Ap, zn);
+                            acc[4] += dot64(Ap, mv6p(minv, row, Ap));
+                        });
+                    } else {
+                        // at the cap: the last update only
+                        for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
+                            f3 zn;
+                            own(row, zn);
+                        }
+                    }
+                }
+                block_sum(acc, S.red);
+                if (threadIdx.x == 0) {
+                    const gp<double> p = rec_at(part, unsigned(t), unsigned(CHX_TILE_PARTIALS));
+                    if (k == 0) {
+                        p[Q_RHO0 + cur] = p[Q_RHO0 + prv];
+                        p[Q_RR0 + cur] = p[Q_RR0 + prv];
+                    } else {
+                        p[Q_RHO0 + cur] = acc[0];
+                        p[Q_RR0 + cur] = acc[1];
+                    }
+                    p[Q_SIG0 + cur] = acc[2];
+                    p[Q_DEL0 + cur] = acc[3];
+                    p[Q_GAM0 + cur] = acc[4];
+                }
+            }
+        }
+        unsigned active = 0;
+        if (!grid_sync(bar, G, cur, &active)) return;
+        trace_point(B, k + 1, active);
+        if (active == 0u) break;
+    }
+
+    // ---- finalize: x_{n+1} = x_n + dx, v = dx/dt (cloth_simulator.cu:222-244) --
+    if (finalize) finalize_tiles(B, G);
 }
 
 // ---------------------------------------------------------------------------
@@ -1211,8 +1553,21 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const float* __rest
 // ---------------------------------------------------------------------------
 using PcgFn = void (*)(chx_cloth_batch, int);
 
+// Bytes of a standalone solve's workspace for n block rows (the layout
+// carve_workspace cuts): r, z, p0, p1, q, M^-1 (whole groups), tile
+// partials, per-instance outputs and the barrier area, 256-byte aligned.
+constexpr size_t al256(size_t v) { return (v + 255) & ~size_t(255); }
+constexpr size_t pcg_workspace_bytes(size_t n) {
+    return 5 * al256(12 * n) + al256(36 * ((n + CHX_BLOCK_GROUP - 1) / CHX_BLOCK_GROUP * CHX_BLOCK_GROUP)) +
+           al256(8 * CHX_TILE_PARTIALS * ((n + CHX_TILE_ROWS - 1) / CHX_TILE_ROWS)) + al256(sizeof(int)) +
+           3 * al256(sizeof(double)) + al256(sizeof(int)) + al256(SYNC_WORDS * sizeof(unsigned));
+}
+
 template <bool ST>
 PcgFn pcg_fn_s(int L, int v) {
+    // single-barrier FP32 sliced forms: 16-bit / 32-bit entries
+    if (v == 5) return pcg1_kernel<2, ST>;
+    if (v == 6) return pcg1_kernel<1, ST>;
     if (v == 0) {
         switch (L) {
             case 4: return pcg_kernel<4, 0, ST>;
@@ -1231,8 +1586,9 @@ PcgFn pcg_fn_s(int L, int v) {
 }
 
 // v: matrix variant (0 row-major CSR, 1 sliced FP32, 2 sliced BF16 + FP32;
-// 3 / 4: 1 / 2 with 16-bit slice entries); staged: LDS-staged SpMV or the
-// global form.
+// 3 / 4: 1 / 2 with 16-bit slice entries; 5 / 6: the single-barrier FP32
+// form with 16-bit / 32-bit entries); staged: LDS-staged SpMV or the global
+// form.
 PcgFn pcg_fn(int L, int v = 1, bool staged = true) {
     if (L != 4 && L != 8 && L != 16)
         throw Error(CHX_ERR_INVALID_ARGUMENT, "chysx: lanes_per_row must be 4, 8 or 16");
@@ -1242,8 +1598,8 @@ PcgFn pcg_fn(int L, int v = 1, bool staged = true) {
 struct DeviceLimits {
     int device = -1;
     int cus = 0;
-    int max_blocks[10][17] = {{0}};  // [2 * v + staged][L]
-    int api_blocks[10][17] = {{0}};
+    int max_blocks[14][17] = {{0}};  // [2 * v + staged][L]
+    int api_blocks[14][17] = {{0}};
 };
 
 std::mutex g_lim_mu;
@@ -1282,10 +1638,17 @@ int max_coresident(int L, int v = 1, bool staged = true) {
 
 // row_major: the caller's blocks are row-major 3x3 (standalone solve);
 // cloth instances store planar blocks.
+// The PCG kernel variant of a launch (pcg_fn's v).
+int pcg_variant(const chx_cloth_batch& B, bool row_major) {
+    if (row_major) return 0;
+    if (B.bf16_blocks == 0 && B.pcg_form == 0) return B.entries16 != 0 ? 5 : 6;
+    return (B.bf16_blocks != 0 ? 2 : 1) + (B.entries16 != 0 ? 2 : 0);
+}
+
 void launch_pcg(const chx_cloth_batch& B, int finalize, hipStream_t stream, bool row_major = false) {
     if (B.n_tiles <= 0) return;
     const int L = B.lanes_per_row;
-    const int v = row_major ? 0 : (B.bf16_blocks != 0 ? 2 : 1) + (B.entries16 != 0 ? 2 : 0);
+    const int v = pcg_variant(B, row_major);
     const bool staged = B.no_lds_staging == 0;
     PcgFn fn = pcg_fn(L, v, staged);
     int grid = max_coresident(L, v, staged);
@@ -1300,21 +1663,23 @@ void launch_pcg(const chx_cloth_batch& B, int finalize, hipStream_t stream, bool
 
 void validate_batch(const chx_cloth_batch* b) {
     require(b != nullptr, "chysx: null batch descriptor");
-    require(b->n_instances >= 0 && b->n_tiles >= 0 && b->n_elem_items >= 0 && b->n_slot_items >= 0,
+    require(b->n_instances >= 0 && b->n_tiles >= 0 && b->n_elem_items >= 0 && b->n_unit_items >= 0,
             "chysx: negative size in batch descriptor");
     require(b->lanes_per_row == 4 || b->lanes_per_row == 8 || b->lanes_per_row == 16,
             "chysx: lanes_per_row must be 4, 8 or 16");
     require(b->sync != nullptr, "chysx: batch descriptor without a sync area");
     if (b->inst == nullptr) {
         require(b->n_instances == 1, "chysx: a batch without an instance table holds exactly one instance");
-        require(b->elem_items == nullptr && b->slot_items == nullptr,
-                "chysx: single-instance element and slot lists are implied");
+        require(b->elem_items == nullptr && b->unit_items == nullptr,
+                "chysx: single-instance element and unit lists are implied");
         const chx_instance& I = b->single;
         require(I.n_rows >= 0 && I.nnz >= 0 && I.n_pins >= 0 && I.n_tris >= 0 && I.n_dihs >= 0 && I.n_stvks >= 0,
                 "chysx: negative size in instance");
         if (I.n_rows > 0)
             require(I.row_offsets && I.diag && I.rhs && I.x && I.r && I.z && I.p0 && I.p1 && I.q && I.minv &&
+                        (I.q1 || b->bf16_blocks || b->pcg_form != 0) &&
                         I.slice_offsets && I.slice_entries && I.upper_offsets && (I.block_slots || I.nnz == 0) &&
+                        I.units && I.unit_elems && I.contrib_offsets && I.n_units > 0 &&
                         (I.slice_entries16 || b->entries16 == 0),
                     "chysx: null array in instance");
         require(I.n_padded >= I.nnz && (I.n_padded % CHX_SLICE_ROWS) == 0 && (I.n_upper % CHX_SLICE_ROWS) == 0 &&
@@ -1328,7 +1693,7 @@ void validate_batch(const chx_cloth_batch* b) {
     } else {
         require(b->tiles != nullptr || b->n_tiles == 0, "chysx: batch without a tile list");
         require(b->elem_items != nullptr || b->n_elem_items == 0, "chysx: batch without an element list");
-        require(b->slot_items != nullptr || b->n_slot_items == 0, "chysx: batch without a slot list");
+        require(b->unit_items != nullptr || b->n_unit_items == 0, "chysx: batch without a unit list");
     }
     if (b->n_tiles > 0) require(b->partials && b->inst_iters && b->inst_rr && b->inst_bb && b->inst_done,
                                 "chysx: batch without solver outputs");
@@ -1342,37 +1707,26 @@ int elem_items(const chx_cloth_batch& B) {
     return item_count(B.single.n_pins) + item_count(B.single.n_tris) + item_count(B.single.n_dihs) +
            item_count(B.single.n_stvks);
 }
-int slot_items(const chx_cloth_batch& B) {
-    if (B.inst != nullptr) return B.n_slot_items;
-    return item_count(B.single.n_rows + B.single.n_upper);
+int unit_items(const chx_cloth_batch& B) {
+    if (B.inst != nullptr) return B.n_unit_items;
+    return B.single.n_rows > 0 ? B.single.n_units : 0;
 }
 
-void element_pass(const chx_cloth_batch& B, hipStream_t stream) {
-    const int ne = elem_items(B);
-    if (ne > 0) {
-        element_kernel<<<ne, kBlock, 0, stream>>>(B);
-        hip_check(hipGetLastError(), "element_kernel launch");
-    }
-}
-
-void gather_pass(const chx_cloth_batch& B, hipStream_t stream) {
-    const int ns = slot_items(B);
-    if (ns > 0) {
-        gather_kernel<<<ns, kBlock, 0, stream>>>(B);
-        hip_check(hipGetLastError(), "gather_kernel launch");
-        if (B.bf16_blocks) {
-            compensate_kernel<<<ns, kBlock, 0, stream>>>(B);
+// The assembly kernel (it also resets the solver's control words for the
+// PCG launch that follows), then in BF16 mode the compensated diagonal.
+void assemble(const chx_cloth_batch& B, hipStream_t stream) {
+    const int nu = unit_items(B);
+    if (nu > 0) {
+        assemble_kernel<<<nu, kBlock, 0, stream>>>(B);
+        hip_check(hipGetLastError(), "assemble_kernel launch");
+        if (B.bf16_blocks && B.n_tiles > 0) {
+            compensate_kernel<<<B.n_tiles, kBlock, 0, stream>>>(B);
             hip_check(hipGetLastError(), "compensate_kernel launch");
         }
     } else {
-        reset_sync_kernel<<<1, kBlock, 0, stream>>>(B.sync);
+        reset_sync_kernel<<<1, kBlock, 0, stream>>>(B.sync, 0);
         hip_check(hipGetLastError(), "reset_sync_kernel launch");
     }
-}
-
-void assemble(const chx_cloth_batch& B, hipStream_t stream) {
-    element_pass(B, stream);
-    gather_pass(B, stream);
 }
 
 }  // namespace
@@ -1427,12 +1781,7 @@ int chx_spmv_bsr3(int n_rows, const float* diag, const int* row_offsets, const i
 int chx_pcg_workspace_size(int n_rows, size_t* bytes) {
     return guarded([&] {
         require(n_rows >= 0 && bytes, "chx_pcg_workspace_size: bad argument");
-        const size_t n = size_t(n_rows);
-        const size_t tiles = (n + CHX_TILE_ROWS - 1) / CHX_TILE_ROWS;
-        auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
-        const size_t ng = (n + CHX_BLOCK_GROUP - 1) / CHX_BLOCK_GROUP * CHX_BLOCK_GROUP;  // minv: whole groups
-        *bytes = 5 * al(12 * n) + al(36 * ng) + al(8 * CHX_TILE_PARTIALS * tiles) + al(sizeof(int)) +
-                 2 * al(sizeof(double)) + al(sizeof(int)) + al(SYNC_WORDS * sizeof(unsigned));
+        *bytes = pcg_workspace_bytes(size_t(n_rows));
     });
 }
 
@@ -1440,6 +1789,67 @@ int chx_pcg_workspace_size(int n_rows, size_t* bytes) {
 
 namespace chx {
 namespace {
+
+// The standalone solve's descriptor over its workspace (the layout
+// chx_pcg_workspace_size sizes): r, z, p0, p1, q, M^-1, tile partials, the
+// per-instance outputs and the barrier area, each 256-byte aligned.
+chx_cloth_batch carve_workspace(int n_rows, void* workspace) {
+    const size_t n = size_t(n_rows);
+    const int tiles = static_cast<int>((n + CHX_TILE_ROWS - 1) / CHX_TILE_ROWS);
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    char* w = static_cast<char*>(workspace);
+    auto take = [&](size_t bytes) {
+        char* p = w;
+        w += al(bytes);
+        return p;
+    };
+    chx_cloth_batch B{};
+    B.n_instances = 1;
+    B.n_tiles = tiles;
+    chx_instance& I = B.single;
+    I.n_rows = n_rows;
+    I.r = reinterpret_cast<float*>(take(12 * n));
+    I.z = reinterpret_cast<float*>(take(12 * n));
+    I.p0 = reinterpret_cast<float*>(take(12 * n));
+    I.p1 = reinterpret_cast<float*>(take(12 * n));
+    I.q = reinterpret_cast<float*>(take(12 * n));
+    I.minv = reinterpret_cast<float*>(take(36 * ((n + CHX_BLOCK_GROUP - 1) / CHX_BLOCK_GROUP * CHX_BLOCK_GROUP)));
+    I.tile_begin = 0;
+    I.tile_end = tiles;
+    B.partials = reinterpret_cast<double*>(take(8 * CHX_TILE_PARTIALS * size_t(tiles)));
+    B.inst_iters = reinterpret_cast<int*>(take(sizeof(int)));
+    B.inst_rr = reinterpret_cast<double*>(take(sizeof(double)));
+    B.inst_bb = reinterpret_cast<double*>(take(sizeof(double)));
+    B.inst_done = reinterpret_cast<int*>(take(sizeof(int)));
+    B.sync = reinterpret_cast<unsigned*>(take(SYNC_WORDS * sizeof(unsigned)));
+    B.inst_rho = reinterpret_cast<double*>(take(sizeof(double)));
+    return B;
+}
+
+// Synchronous read of a standalone solve's outputs; a timed-out solve is
+// reported (and its status cleared) here.
+void read_workspace_stats(const chx_cloth_batch& B, int* iterations, double* rr, double* bb, double* rho,
+                          hipStream_t s) {
+    int it = 0;
+    double r2 = 0.0, b2 = 0.0, rz = 0.0;
+    unsigned status = 0;
+    hip_check(hipMemcpyAsync(&it, B.inst_iters, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(iters)");
+    hip_check(hipMemcpyAsync(&r2, B.inst_rr, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(rr)");
+    hip_check(hipMemcpyAsync(&b2, B.inst_bb, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(bb)");
+    hip_check(hipMemcpyAsync(&rz, B.inst_rho, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(rho)");
+    hip_check(hipMemcpyAsync(&status, B.sync + SYNC_STATUS, sizeof(unsigned), hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(status)");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(pcg)");
+    if (status != 0u) {
+        reset_sync_kernel<<<1, kBlock, 0, s>>>(B.sync, 1);
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(reset)");
+        throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out");
+    }
+    if (iterations) *iterations = it;
+    if (rr) *rr = r2;
+    if (bb) *bb = b2;
+    if (rho) *rho = rz;
+}
 
 // The standalone row-major PCG (pcg_solver.cu:134-215), with the contact
 // sidecar's cross-slot terms in every A x when `op` is active
@@ -1462,44 +1872,18 @@ void solve_bsr3(int n_rows, const float* diag, const int* row_offsets, const int
         size_t need = 0;
         if (chx_pcg_workspace_size(n_rows, &need) != CHX_OK) throw Error(CHX_ERR_RUNTIME, chx_last_error_message());
         require(workspace_bytes >= need, "pcg: workspace too small (see chx_pcg_workspace_size)");
-        const size_t n = size_t(n_rows);
-        const int tiles = static_cast<int>((n + CHX_TILE_ROWS - 1) / CHX_TILE_ROWS);
-        auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
-        char* w = static_cast<char*>(workspace);
-        auto take = [&](size_t bytes) {
-            char* p = w;
-            w += al(bytes);
-            return p;
-        };
-        chx_cloth_batch B{};
-        B.n_instances = 1;
-        B.n_tiles = tiles;
+        chx_cloth_batch B = carve_workspace(n_rows, workspace);
         B.lanes_per_row = lanes_per_row;
         chx_instance& I = B.single;
-        I.n_rows = n_rows;
         I.row_offsets = row_offsets;
         I.col_indices = col_indices;
         I.diag = const_cast<float*>(diag);
         I.values = const_cast<float*>(values);
         I.rhs = const_cast<float*>(b);
         I.x = x;
-        I.r = reinterpret_cast<float*>(take(12 * n));
-        I.z = reinterpret_cast<float*>(take(12 * n));
-        I.p0 = reinterpret_cast<float*>(take(12 * n));
-        I.p1 = reinterpret_cast<float*>(take(12 * n));
-        I.q = reinterpret_cast<float*>(take(12 * n));
-        I.minv = reinterpret_cast<float*>(take(36 * ((n + CHX_BLOCK_GROUP - 1) / CHX_BLOCK_GROUP * CHX_BLOCK_GROUP)));
-        I.tile_begin = 0;
-        I.tile_end = tiles;
         I.dt = 1.0f;
         I.max_iter = max_iterations;
         I.tol = tol;
-        B.partials = reinterpret_cast<double*>(take(8 * CHX_TILE_PARTIALS * size_t(tiles)));
-        B.inst_iters = reinterpret_cast<int*>(take(sizeof(int)));
-        B.inst_rr = reinterpret_cast<double*>(take(sizeof(double)));
-        B.inst_bb = reinterpret_cast<double*>(take(sizeof(double)));
-        B.inst_done = reinterpret_cast<int*>(take(sizeof(int)));
-        B.sync = reinterpret_cast<unsigned*>(take(SYNC_WORDS * sizeof(unsigned)));
         const hipStream_t s = as_stream(stream);
         pcg_fn(lanes_per_row, 0);  // validates L
         if (op != nullptr && op->max_contacts > 0 && op->stiffness > 0.0f && op->pairs != nullptr &&
@@ -1509,24 +1893,13 @@ void solve_bsr3(int n_rows, const float* diag, const int* row_offsets, const int
             B.contact_rows = ix.rows;
             B.contact_entries = ix.entries;
         }
-        reset_sync_kernel<<<1, kBlock, 0, s>>>(B.sync);
+        // The workspace needs no initialisation: every control word is
+        // cleared, status included (an earlier solve's abort is reported by
+        // its own synchronous read or chx_pcg_last_stats before this call).
+        reset_sync_kernel<<<1, kBlock, 0, s>>>(B.sync, 1);
         hip_check(hipGetLastError(), "reset_sync_kernel launch");
         launch_pcg(B, 0, s, true);
-        if (iterations || rr || bb) {
-            int it = 0;
-            double r2 = 0.0, b2 = 0.0;
-            unsigned status = 0;
-            hip_check(hipMemcpyAsync(&it, B.inst_iters, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(iters)");
-            hip_check(hipMemcpyAsync(&r2, B.inst_rr, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(rr)");
-            hip_check(hipMemcpyAsync(&b2, B.inst_bb, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(bb)");
-            hip_check(hipMemcpyAsync(&status, B.sync + SYNC_STATUS, sizeof(unsigned), hipMemcpyDeviceToHost, s),
-                      "hipMemcpyAsync(status)");
-            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(pcg)");
-            if (status != 0u) throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out");
-            if (iterations) *iterations = it;
-            if (rr) *rr = r2;
-            if (bb) *bb = b2;
-        }
+        if (iterations || rr || bb) read_workspace_stats(B, iterations, rr, bb, nullptr, s);
     }
 }
 
@@ -1558,17 +1931,23 @@ int chx_pcg_solve_bsr3_contact(int n_rows, const float* diag, const int* row_off
     });
 }
 
-int chx_cloth_element_pass(const chx_cloth_batch* batch, uintptr_t stream) {
+int chx_pcg_last_stats(int n_rows, const void* workspace, size_t workspace_bytes, int* iterations, double* rr,
+                       double* bb, double* rho, uintptr_t stream) {
     return guarded([&] {
-        validate_batch(batch);
-        element_pass(*batch, as_stream(stream));
-    });
-}
-
-int chx_cloth_gather_pass(const chx_cloth_batch* batch, uintptr_t stream) {
-    return guarded([&] {
-        validate_batch(batch);
-        gather_pass(*batch, as_stream(stream));
+        require(n_rows >= 0, "chx_pcg_last_stats: negative row count");
+        if (n_rows == 0) {
+            if (iterations) *iterations = 0;
+            if (rr) *rr = 0.0;
+            if (bb) *bb = 0.0;
+            if (rho) *rho = 0.0;
+            return;
+        }
+        require(workspace != nullptr, "chx_pcg_last_stats: null workspace");
+        size_t need = 0;
+        if (chx_pcg_workspace_size(n_rows, &need) != CHX_OK) throw Error(CHX_ERR_RUNTIME, chx_last_error_message());
+        require(workspace_bytes >= need, "chx_pcg_last_stats: workspace too small");
+        const chx_cloth_batch B = carve_workspace(n_rows, const_cast<void*>(workspace));
+        read_workspace_stats(B, iterations, rr, bb, rho, as_stream(stream));
     });
 }
 
@@ -1583,7 +1962,7 @@ int chx_cloth_solve(const chx_cloth_batch* batch, int finalize, uintptr_t stream
     return guarded([&] {
         validate_batch(batch);
         const hipStream_t s = as_stream(stream);
-        reset_sync_kernel<<<1, kBlock, 0, s>>>(batch->sync);
+        reset_sync_kernel<<<1, kBlock, 0, s>>>(batch->sync, 0);
         hip_check(hipGetLastError(), "reset_sync_kernel launch");
         launch_pcg(*batch, finalize, s);
     });
@@ -1606,8 +1985,7 @@ int chx_cloth_step_graph(const chx_cloth_batch* batch, uintptr_t stream, void** 
         const hipStream_t s = as_stream(stream);
         require(s != nullptr, "chx_cloth_step_graph: capture needs a stream other than the null stream");
         // Device limits are queried (and cached) before the capture starts.
-        (void)max_coresident(batch->lanes_per_row, (batch->bf16_blocks != 0 ? 2 : 1) + (batch->entries16 != 0 ? 2 : 0),
-                             batch->no_lds_staging == 0);
+        (void)max_coresident(batch->lanes_per_row, pcg_variant(*batch, false), batch->no_lds_staging == 0);
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         hipGraph_t g = nullptr;
         try {
@@ -1649,8 +2027,10 @@ int chx_batch_check(const chx_cloth_batch* batch, uintptr_t stream) {
         hip_check(hipMemcpyAsync(words, batch->sync, sizeof(words), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(sync)");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(check)");
         if (words[SYNC_STATUS] != 0u) {
-            reset_sync_kernel<<<1, kBlock, 0, s>>>(batch->sync);
+            reset_sync_kernel<<<1, kBlock, 0, s>>>(batch->sync, 1);
             hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(reset)");
+            if (batch->status_host != nullptr)
+                __atomic_store_n(batch->status_host, 0u, __ATOMIC_SEQ_CST);
             const unsigned grid = words[SYNC_GRID];
             throw Error(CHX_ERR_DEVICE, "chysx::solver::PCGSolver: grid barrier timed out (grid of " +
                                             std::to_string(grid) + " workgroups: " +

@@ -194,6 +194,20 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("bf16_blocks", &ClothSimulator::bf16_blocks)
         .def("last_pcg_iterations", &ClothSimulator::last_pcg_iterations, py::arg("cuda_stream") = 0)
         .def("last_relative_residual", &ClothSimulator::last_relative_residual, py::arg("cuda_stream") = 0)
+        .def("set_barrier_timeout_ticks", &ClothSimulator::set_barrier_timeout_ticks, py::arg("ticks"),
+             "Bound of each grid-barrier wait of the solver in 100 MHz ticks (0: one second).")
+        .def("set_pcg_form", &ClothSimulator::set_pcg_form, py::arg("form"),
+             "0: one grid barrier per PCG iteration (default), 1: two")
+        .def("pcg_form", &ClothSimulator::pcg_form)
+        .def(
+            "debug_move_roundtrip",
+            [](ClothSimulator& s) {
+                // Moves the simulator into a temporary and back (move
+                // construction, then move assignment).
+                ClothSimulator tmp(std::move(s));
+                s = std::move(tmp);
+            },
+            "Moves this simulator out and back in (exercises the move constructor and assignment).")
         .def("num_particles", &ClothSimulator::num_particles)
         .def("num_off_diag_blocks", &ClothSimulator::num_off_diag_blocks)
         .def("debug_dump_last_solve",
@@ -288,10 +302,6 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("set_lds_staging", &chysx::cloth::ClothBatch::set_lds_staging, py::arg("enabled"))
         .def("lds_staging", &chysx::cloth::ClothBatch::lds_staging)
         .def("entries16", &chysx::cloth::ClothBatch::entries16)
-        .def("element_pass", &chysx::cloth::ClothBatch::element_pass, py::arg("dt"), py::arg("cuda_stream") = 0,
-             py::call_guard<py::gil_scoped_release>())
-        .def("gather_pass", &chysx::cloth::ClothBatch::gather_pass, py::arg("cuda_stream") = 0,
-             py::call_guard<py::gil_scoped_release>())
         .def("set_graph", &chysx::cloth::ClothBatch::set_graph, py::arg("enabled"),
              "Replay substeps from a hipGraph (needs a non-null cuda_stream in step()).")
         .def("graph", &chysx::cloth::ClothBatch::graph)
@@ -314,7 +324,67 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def("instance_sizes", &chysx::cloth::ClothBatch::instance_sizes)
         .def("last_pcg_iterations", &chysx::cloth::ClothBatch::last_pcg_iterations, py::arg("cuda_stream") = 0)
         .def("last_relative_residuals", &chysx::cloth::ClothBatch::last_relative_residuals,
-             py::arg("cuda_stream") = 0);
+             py::arg("cuda_stream") = 0)
+        .def("set_barrier_timeout_ticks", &chysx::cloth::ClothBatch::set_barrier_timeout_ticks, py::arg("ticks"))
+        .def("set_pcg_form", &chysx::cloth::ClothBatch::set_pcg_form, py::arg("form"),
+             "0: one grid barrier per PCG iteration (default), 1: two")
+        .def("pcg_form", &chysx::cloth::ClothBatch::pcg_form);
+
+    // ---- standalone block-CSR matrix and PCG solver --------------------------
+    // (ChysX/src/sparse/block_csr.h:59-135, ChysX/src/solver/pcg_solver.h:79-167)
+    py::class_<chysx::sparse::BlockCSR3>(m, "BlockCSR3")
+        .def(py::init<>())
+        .def(
+            "build_topology",
+            [](chysx::sparse::BlockCSR3& A, int n, IArr rows, IArr cols) {
+                if (rows.ndim() != 1 || cols.ndim() != 1 || rows.shape(0) != cols.shape(0))
+                    throw std::invalid_argument("BlockCSR3.build_topology: rows/cols must be 1-D of equal length");
+                A.build_topology(n, rows.data(), cols.data(), static_cast<int>(rows.shape(0)));
+            },
+            py::arg("num_block_rows"), py::arg("rows"), py::arg("cols"))
+        .def(
+            "upload",
+            [](chysx::sparse::BlockCSR3& A, FArr diag, FArr values) {
+                // Addition: host blocks (row-major 3x3) into the device arrays.
+                const auto n = static_cast<py::ssize_t>(A.num_block_rows());
+                const auto nnz = static_cast<py::ssize_t>(A.num_off_diag_blocks());
+                if (diag.size() != 9 * n || values.size() != 9 * nnz)
+                    throw std::invalid_argument("BlockCSR3.upload: diag (n, 3, 3) and values (nnz, 3, 3) expected");
+                A.diag.host().assign(reinterpret_cast<const chysx::math::Mat3f*>(diag.data()),
+                                     reinterpret_cast<const chysx::math::Mat3f*>(diag.data()) + n);
+                A.values.host().assign(reinterpret_cast<const chysx::math::Mat3f*>(values.data()),
+                                       reinterpret_cast<const chysx::math::Mat3f*>(values.data()) + nnz);
+                A.diag.copy_to_device();
+                A.values.copy_to_device();
+            },
+            py::arg("diag"), py::arg("values"))
+        .def("num_block_rows", &chysx::sparse::BlockCSR3::num_block_rows)
+        .def("num_off_diag_blocks", &chysx::sparse::BlockCSR3::num_off_diag_blocks)
+        .def("row_offsets", [](const chysx::sparse::BlockCSR3& A) { return to_array(A.pattern().row_offsets); })
+        .def("col_indices", [](const chysx::sparse::BlockCSR3& A) { return to_array(A.pattern().col_indices); });
+
+    py::class_<chysx::solver::PCGSolver>(m, "PCGSolver")
+        .def(py::init<>())
+        .def(
+            "solve",
+            [](chysx::solver::PCGSolver& S, const chysx::sparse::BlockCSR3& A, std::uintptr_t b_ptr,
+               std::uintptr_t x_ptr, int max_iterations, double tolerance, std::uintptr_t stream, int lanes) {
+                const auto n = static_cast<std::size_t>(A.num_block_rows());
+                chysx::solver::PCGParams prm;
+                prm.max_iterations = max_iterations;
+                prm.tolerance = tolerance;
+                prm.lanes_per_row = lanes;
+                py::gil_scoped_release release;
+                return S.solve(A, chysx::DeviceSpan<chysx::math::Vec3f>::from_raw(b_ptr, n),
+                               chysx::DeviceSpan<chysx::math::Vec3f>::from_raw(x_ptr, n), prm, stream);
+            },
+            py::arg("A"), py::arg("b_ptr"), py::arg("x_ptr"), py::arg("max_iterations") = 50,
+            py::arg("tolerance") = 0.0, py::arg("cuda_stream") = 0, py::arg("lanes_per_row") = 8,
+            "Asynchronous unless tolerance > 0; returns the iteration count (max_iterations without a tolerance).")
+        .def("last_residual", &chysx::solver::PCGSolver::last_residual,
+             "<r, z> of the last solve's final iteration (float, read on demand).")
+        .def("last_iterations", &chysx::solver::PCGSolver::last_iterations)
+        .def("last_relative_residual", &chysx::solver::PCGSolver::last_relative_residual);
 
     // ---- host topology (no device needed) ------------------------------------
     // ---- frame writers (ChysX/src/io/obj_io.h, bgeo_writer.h) -------------

@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -140,6 +141,68 @@ private:
     T* d_ = nullptr;
     std::size_t dn_ = 0;
     std::vector<T> host_;
+};
+
+// Page-locked staging for small asynchronous host-to-device uploads (the
+// reference's pinned host mirrors, cuda_array.cu:41-62): a ring of two
+// buffers, each reused only after the copy that read it has run on its
+// stream, so the caller's data may change as soon as upload() returns and
+// the host waits only when it is two uploads ahead of the device.
+class HostStager {
+public:
+    HostStager() = default;
+    ~HostStager() { release(); }
+    HostStager(const HostStager&) = delete;
+    HostStager& operator=(const HostStager&) = delete;
+    HostStager(HostStager&& o) noexcept { *this = std::move(o); }
+    HostStager& operator=(HostStager&& o) noexcept {
+        if (this != &o) {
+            release();
+            for (int i = 0; i < 2; ++i) {
+                slot_[i] = o.slot_[i];
+                o.slot_[i] = Slot{};
+            }
+            next_ = o.next_;
+        }
+        return *this;
+    }
+    void upload(void* dst, const void* src, std::size_t bytes, std::uintptr_t stream) {
+        if (bytes == 0) return;
+        Slot& s = slot_[next_];
+        next_ ^= 1;
+        if (s.event == nullptr) check(chx_event_create(&s.event), "memory", "staging event");
+        if (s.pending) check(chx_event_synchronize(s.event), "memory", "staging buffer reuse");
+        s.pending = false;
+        if (s.cap < bytes) {
+            if (s.buf) check(chx_host_free(s.buf), "memory", "staging buffer");
+            s.buf = nullptr;
+            s.cap = 0;
+            check(chx_host_malloc(&s.buf, bytes), "memory", "staging buffer");
+            s.cap = bytes;
+        }
+        std::memcpy(s.buf, src, bytes);
+        check(chx_memcpy_h2d_async(dst, s.buf, bytes, stream), "memory", "host-to-device copy");
+        check(chx_event_record(s.event, stream), "memory", "staging event");
+        s.pending = true;
+    }
+    void release() noexcept {
+        for (Slot& s : slot_) {
+            if (s.pending && s.event) (void)chx_event_synchronize(s.event);
+            if (s.buf) (void)chx_host_free(s.buf);
+            if (s.event) (void)chx_event_destroy(s.event);
+            s = Slot{};
+        }
+    }
+
+private:
+    struct Slot {
+        void* buf = nullptr;
+        std::size_t cap = 0;
+        void* event = nullptr;
+        bool pending = false;
+    };
+    Slot slot_[2];
+    int next_ = 0;
 };
 
 }  // namespace chysx

@@ -61,18 +61,6 @@ void ClothBatch::assemble(float dt, std::uintptr_t cuda_stream) {
     runner_.assemble(cuda_stream);
 }
 
-void ClothBatch::element_pass(float dt, std::uintptr_t cuda_stream) {
-    if (!(dt > 0.0f)) throw std::invalid_argument("ClothBatch::element_pass: dt must be positive");
-    if (!bound_) finalize(dt, cuda_stream);
-    else runner_.update_many(descriptors(dt, cuda_stream), cuda_stream);
-    runner_.element_pass(cuda_stream);
-}
-
-void ClothBatch::gather_pass(std::uintptr_t cuda_stream) {
-    if (!bound_) throw std::runtime_error("ClothBatch::gather_pass: call element_pass() first");
-    runner_.gather_pass(cuda_stream);
-}
-
 void ClothBatch::solve(std::uintptr_t cuda_stream) {
     if (!bound_) throw std::runtime_error("ClothBatch::solve: call assemble() first");
     runner_.solve(true, cuda_stream);

@@ -44,10 +44,6 @@ public:
     // persistent PCG solve with the fused finalize.
     void assemble(float dt, std::uintptr_t cuda_stream = 0);
     void solve(std::uintptr_t cuda_stream = 0);
-    // The two halves of assemble(), for timing them apart: the element pass
-    // (forces + Hessian records at x_n) and the gather pass (assembly).
-    void element_pass(float dt, std::uintptr_t cuda_stream = 0);
-    void gather_pass(std::uintptr_t cuda_stream = 0);
 
     int num_instances() const noexcept { return static_cast<int>(sims_.size()); }
     int num_rows() const noexcept;
@@ -82,6 +78,12 @@ public:
         runner_.set_bf16(on);
     }
     std::vector<int> last_pcg_iterations(std::uintptr_t cuda_stream = 0);
+    // Bound of each grid-barrier wait of the solver in 100 MHz ticks (0: one
+    // second); an aborted solve is reported by the next step or status read.
+    void set_barrier_timeout_ticks(unsigned ticks) { runner_.set_barrier_timeout_ticks(ticks); }
+    // Persistent PCG form of the FP32 solve (ClothSimulator::set_pcg_form).
+    void set_pcg_form(int form) { runner_.set_pcg_form(form); }
+    int pcg_form() const noexcept { return runner_.pcg_form(); }
     std::vector<double> last_relative_residuals(std::uintptr_t cuda_stream = 0);
 
 private:

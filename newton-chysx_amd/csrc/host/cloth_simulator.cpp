@@ -6,9 +6,15 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 namespace chysx {
 namespace cloth {
+
+// Move-only, as the reference (cloth_simulator.h:72-75).
+static_assert(std::is_move_constructible<ClothSimulator>::value && std::is_move_assignable<ClothSimulator>::value &&
+                  !std::is_copy_constructible<ClothSimulator>::value,
+              "ClothSimulator is move-only");
 
 void ClothSimulator::set_external_buffers(std::uintptr_t pos_ptr, std::uintptr_t vel_ptr, int particle_count,
                                           std::uintptr_t inv_mass_ptr, std::uintptr_t force_ptr) noexcept {
@@ -34,15 +40,21 @@ void ClothSimulator::clear_pins() noexcept {
     topology_dirty_ = true;
 }
 
-// cloth_simulator.cu:294-313
-void ClothSimulator::update_pin_targets(const math::Vec3f* host_targets, int n_pins, std::uintptr_t /*cuda_stream*/) {
+// cloth_simulator.cu:294-313: the targets go to the device at once, on the
+// caller's stream, through a page-locked staging buffer (no wait unless the
+// host is two updates ahead of the stream; the null stream waits).  The pin
+// set and the pattern are unchanged, so a captured step graph reads the new
+// targets on its next replay.
+void ClothSimulator::update_pin_targets(const math::Vec3f* host_targets, int n_pins, std::uintptr_t cuda_stream) {
     if (host_targets == nullptr || n_pins <= 0) return;
     if (pins_.size() != n_pins)
         throw std::invalid_argument(
             "ClothSimulator::update_pin_targets: n_pins does not match the currently installed pin set; call "
             "set_pins(...) to change the pin set instead.");
     std::memcpy(pins_.targets().data(), host_targets, sizeof(math::Vec3f) * n_pins);
-    pins_dirty_ = true;
+    // Not built yet (or rebuilt on the next step): the build uploads them.
+    if (!topology_dirty_ && H_num_block_rows_ > 0)
+        inst_.upload_pin_targets(reinterpret_cast<const float*>(host_targets), n_pins, cuda_stream);
 }
 
 // cloth_simulator.cu:315-377 (edge list; the self-collision topology the
@@ -231,7 +243,6 @@ void ClothSimulator::ensure_hessian_topology(std::uintptr_t stream) {
     packed_stvk_[2] = fem_stvk_.thickness();
     H_num_block_rows_ = N;
     topology_dirty_ = false;
-    pins_dirty_ = false;
     shapes_dirty_ = false;  // build() uploaded the shape list
 }
 
@@ -297,7 +308,7 @@ void ClothSimulator::check_elements_current(std::uintptr_t stream) {
                            packed_k_[2] != fem_shear_.stiffness() || packed_k_[3] != bending_.stiffness() ||
                            packed_stvk_[0] != fem_stvk_.mu() || packed_stvk_[1] != fem_stvk_.lambda() ||
                            packed_stvk_[2] != fem_stvk_.thickness();
-    if (k_changed || pins_dirty_) {
+    if (k_changed) {
         inst_.update_elements(source(), stream);
         packed_k_[0] = pins_.stiffness();
         packed_k_[1] = fem_stretch_.stiffness();
@@ -306,7 +317,6 @@ void ClothSimulator::check_elements_current(std::uintptr_t stream) {
         packed_stvk_[0] = fem_stvk_.mu();
         packed_stvk_[1] = fem_stvk_.lambda();
         packed_stvk_[2] = fem_stvk_.thickness();
-        pins_dirty_ = false;
     }
     if (shapes_dirty_) {
         inst_.update_shapes(source(), stream);

@@ -69,6 +69,10 @@ public:
     ClothSimulator() = default;
     ClothSimulator(const ClothSimulator&) = delete;
     ClothSimulator& operator=(const ClothSimulator&) = delete;
+    // Move-only, as the reference (cloth_simulator.h:72-75).  A simulator
+    // registered with a ClothBatch must not be moved while it is a member.
+    ClothSimulator(ClothSimulator&&) = default;
+    ClothSimulator& operator=(ClothSimulator&&) = default;
 
     // ---- material ------------------------------------------------------
     void set_material(const ClothMaterial& m) noexcept { material_ = m; }
@@ -174,6 +178,17 @@ public:
     void set_bf16_blocks(bool on);
     bool bf16_blocks() const noexcept { return bf16_; }
     int last_pcg_iterations(std::uintptr_t cuda_stream = 0);
+    // Bound of every grid-barrier wait of the persistent solver in 100 MHz
+    // ticks (0: one second); a solve that exceeds it stops, leaves pos / vel
+    // unchanged and is reported as a RuntimeError by the next step or status
+    // read.
+    void set_barrier_timeout_ticks(unsigned ticks) { runner_.set_barrier_timeout_ticks(ticks); }
+    // Persistent PCG form of the FP32 solve: 0 (default) one grid barrier
+    // per iteration, the iteration's two reductions merged; 1 two barriers
+    // per iteration.  Same recurrence and stopping rule; BF16 mode always
+    // uses the two-barrier form.
+    void set_pcg_form(int form) { runner_.set_pcg_form(form); }
+    int pcg_form() const noexcept { return runner_.pcg_form(); }
     double last_relative_residual(std::uintptr_t cuda_stream = 0);
 
     // ---- diagnostics -----------------------------------------------------------------
@@ -228,7 +243,6 @@ private:
     engine::BatchRunner runner_;
     int H_num_block_rows_ = 0;
     bool topology_dirty_ = true;
-    bool pins_dirty_ = false;
     std::vector<chx_plane> planes_;
     std::vector<chx_box> boxes_;
     bool shapes_dirty_ = false;

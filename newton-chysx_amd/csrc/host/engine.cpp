@@ -41,10 +41,12 @@ void sync_stream(std::uintptr_t stream, const char* what) {
 
 void InstanceData::pack_elements(const InstanceSource& s) {
     auto& P = pins_.host();
+    auto& PT = pin_targets_.host();
     auto& T = tris_.host();
     auto& D = dihs_.host();
     auto& V = stvks_.host();
     P.clear();
+    PT.clear();
     T.clear();
     D.clear();
     V.clear();
@@ -52,7 +54,8 @@ void InstanceData::pack_elements(const InstanceSource& s) {
         const float k = s.pins->stiffness();
         for (int c = 0; c < s.pins->size(); ++c) {
             const auto& t = s.pins->targets()[c];
-            P.push_back(chx_pin{s.pins->indices()[c], 0, t.x, t.y, t.z, k});
+            P.push_back(chx_pin{s.pins->indices()[c], k});
+            PT.insert(PT.end(), {t.x, t.y, t.z});
         }
     }
     for (const constraint::TriangleStretchConstraint* ts : {s.stretch, s.shear}) {
@@ -80,6 +83,197 @@ void InstanceData::pack_elements(const InstanceSource& s) {
             const auto& v = s.bending->indices()[e];
             D.push_back(chx_dih{v.x, v.y, v.z, v.w, s.bending->rest_angles()[e], k, 0, 0});
         }
+    }
+}
+
+// Assembly units (chx_unit): the rows are cut into units whose elements'
+// LDS records fit CHX_ASM_LDS_FLOATS -- up to four whole slices, or parts of
+// one slice when a slice alone does not fit -- and every unit gets the list
+// of the elements with a vertex in its rows (ascending per kind) and the
+// contribution lists of its slots (diagonal blocks of its rows, then its
+// upper value slots slice by slice, block index jj, then row), each entry's
+// element index replaced by the element's position in the unit's list of
+// that kind.  goff / gent: the per-slot lists over the slot order
+// [diagonal blocks 0..n) + [value slots n..n+n_upper) with global element
+// indices.
+void InstanceData::build_units(const std::vector<unsigned>& goff, const std::vector<unsigned>& gent) {
+    const int n = n_;
+    constexpr int kKinds = 4;  // CHX_CONTRIB_PIN, _TRI, _DIH, _STVK
+    static_assert(CHX_CONTRIB_PIN == 0 && CHX_CONTRIB_TRI == 1 && CHX_CONTRIB_DIH == 2 && CHX_CONTRIB_STVK == 3,
+                  "kind codes index the per-kind tables");
+    const int rec_floats[kKinds] = {CHX_ASM_PIN_REC, CHX_ASM_TRI_REC, CHX_ASM_DIH_REC, CHX_ASM_STVK_REC};
+    const std::size_t counts[kKinds] = {pins_.cpu_size(), tris_.cpu_size(), dihs_.cpu_size(), stvks_.cpu_size()};
+    // vertex -> incident elements, per kind (CSR)
+    std::vector<int> voff[kKinds], vel[kKinds];
+    auto incidence = [&](int kind, int per, auto vertex_of) {
+        auto& off = voff[kind];
+        auto& el = vel[kind];
+        off.assign(static_cast<std::size_t>(n) + 1, 0);
+        for (std::size_t e = 0; e < counts[kind]; ++e)
+            for (int j = 0; j < per; ++j) ++off[static_cast<std::size_t>(vertex_of(e, j)) + 1];
+        for (int v = 0; v < n; ++v) off[v + 1] += off[v];
+        el.assign(static_cast<std::size_t>(off[n]), 0);
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (std::size_t e = 0; e < counts[kind]; ++e)
+            for (int j = 0; j < per; ++j) el[fill[vertex_of(e, j)]++] = static_cast<int>(e);
+    };
+    incidence(CHX_CONTRIB_PIN, 1, [&](std::size_t e, int) { return pins_.host()[e].v; });
+    incidence(CHX_CONTRIB_TRI, 3, [&](std::size_t e, int j) {
+        const chx_tri& t = tris_.host()[e];
+        return j == 0 ? t.v0 : (j == 1 ? t.v1 : t.v2);
+    });
+    incidence(CHX_CONTRIB_DIH, 4, [&](std::size_t e, int j) {
+        const chx_dih& d = dihs_.host()[e];
+        return j == 0 ? d.v0 : (j == 1 ? d.v1 : (j == 2 ? d.v2 : d.v3));
+    });
+    incidence(CHX_CONTRIB_STVK, 3, [&](std::size_t e, int j) {
+        const chx_stvk& t = stvks_.host()[e];
+        return j == 0 ? t.v0 : (j == 1 ? t.v1 : t.v2);
+    });
+    // stamp[k][e]: the unit that holds element e; mark[k][e]: trial token
+    std::vector<int> stamp[kKinds], mark[kKinds];
+    for (int k = 0; k < kKinds; ++k) {
+        stamp[k].assign(counts[k], -1);
+        mark[k].assign(counts[k], -1);
+    }
+    int token = 0;
+    // LDS floats the records of rows [ra, rb) add to unit uid
+    auto added = [&](int ra, int rb, int uid) {
+        ++token;
+        long floats = 0;
+        for (int r = ra; r < rb; ++r)
+            for (int k = 0; k < kKinds; ++k)
+                for (int q = voff[k][r]; q < voff[k][r + 1]; ++q) {
+                    const int e = vel[k][q];
+                    if (stamp[k][e] != uid && mark[k][e] != token) {
+                        mark[k][e] = token;
+                        floats += rec_floats[k];
+                    }
+                }
+        return floats;
+    };
+    auto commit = [&](int ra, int rb, int uid) {
+        for (int r = ra; r < rb; ++r)
+            for (int k = 0; k < kKinds; ++k)
+                for (int q = voff[k][r]; q < voff[k][r + 1]; ++q) stamp[k][vel[k][q]] = uid;
+    };
+    const long budget = CHX_ASM_LDS_FLOATS;
+    const int SR = CHX_SLICE_ROWS;
+    const int slices = (n + SR - 1) / SR;
+    std::vector<std::pair<int, int>> ranges;
+    int u_r0 = -1, u_slices = 0;
+    long u_floats = 0;
+    for (int sl = 0; sl < slices; ++sl) {
+        const int sa = sl * SR, sb = std::min(n, sa + SR);
+        const int uid = static_cast<int>(ranges.size());
+        if (u_r0 >= 0) {
+            const long add = added(sa, sb, uid);
+            if (u_slices < 4 && u_floats + add <= budget) {
+                commit(sa, sb, uid);
+                u_floats += add;
+                ++u_slices;
+                continue;
+            }
+            ranges.emplace_back(u_r0, sa);
+            u_r0 = -1;
+        }
+        const int nid = static_cast<int>(ranges.size());
+        const long whole = added(sa, sb, nid);
+        if (whole <= budget) {
+            commit(sa, sb, nid);
+            u_r0 = sa;
+            u_floats = whole;
+            u_slices = 1;
+            continue;
+        }
+        // the slice alone does not fit: split it by rows
+        for (int r = sa; r < sb;) {
+            const int pid = static_cast<int>(ranges.size());
+            const int p0 = r;
+            long pf = 0;
+            while (r < sb) {
+                const long add = added(r, r + 1, pid);
+                if (pf + add > budget) break;
+                commit(r, r + 1, pid);
+                pf += add;
+                ++r;
+            }
+            if (r == p0)
+                throw std::invalid_argument(
+                    "chysx::engine: the elements of one vertex exceed an assembly unit's LDS records "
+                    "(CHX_ASM_LDS_FLOATS)");
+            ranges.emplace_back(p0, r);
+        }
+    }
+    if (u_r0 >= 0) ranges.emplace_back(u_r0, n);
+
+    const auto& uo = upper_offsets_.host();
+    auto& units = units_.host();
+    auto& uel = unit_elems_.host();
+    auto& off = contrib_offsets_.host();
+    auto& ent = contribs_.host();
+    units.clear();
+    uel.clear();
+    off.assign(1, 0u);
+    ent.clear();
+    ent.reserve(gent.size());
+    std::vector<int> local[kKinds];
+    for (int k = 0; k < kKinds; ++k) local[k].assign(counts[k], -1);
+    std::vector<int> list[kKinds];
+    for (std::size_t u = 0; u < ranges.size(); ++u) {
+        const int r0 = ranges[u].first, r1 = ranges[u].second;
+        chx_unit U{};
+        U.row_begin = r0;
+        U.row_end = r1;
+        U.elem_begin = static_cast<int>(uel.size());
+        U.slot_begin = static_cast<int>(off.size() - 1);
+        ++token;
+        for (int k = 0; k < kKinds; ++k) list[k].clear();
+        for (int r = r0; r < r1; ++r)
+            for (int k = 0; k < kKinds; ++k)
+                for (int q = voff[k][r]; q < voff[k][r + 1]; ++q) {
+                    const int e = vel[k][q];
+                    if (mark[k][e] != token) {
+                        mark[k][e] = token;
+                        list[k].push_back(e);
+                    }
+                }
+        long floats = 0;
+        const int order[kKinds] = {CHX_CONTRIB_PIN, CHX_CONTRIB_TRI, CHX_CONTRIB_STVK, CHX_CONTRIB_DIH};
+        for (int k : order) {
+            std::sort(list[k].begin(), list[k].end());
+            for (std::size_t i = 0; i < list[k].size(); ++i) local[k][list[k][i]] = static_cast<int>(i);
+            uel.insert(uel.end(), list[k].begin(), list[k].end());
+            floats += static_cast<long>(list[k].size()) * rec_floats[k];
+        }
+        if (floats > budget) throw std::logic_error("chysx::engine: assembly unit over its LDS capacity");
+        U.n_pins = static_cast<int>(list[CHX_CONTRIB_PIN].size());
+        U.n_tris = static_cast<int>(list[CHX_CONTRIB_TRI].size());
+        U.n_stvks = static_cast<int>(list[CHX_CONTRIB_STVK].size());
+        U.n_dihs = static_cast<int>(list[CHX_CONTRIB_DIH].size());
+        auto emit = [&](std::size_t g) {
+            for (unsigned c = goff[g]; c < goff[g + 1]; ++c) {
+                const unsigned w = gent[c];
+                const unsigned kind = w >> 30;
+                const int e = static_cast<int>(w & CHX_CONTRIB_MAX_ELEMENT);
+                const int l = local[kind][static_cast<std::size_t>(e)];
+                if (l < 0 || stamp[kind][static_cast<std::size_t>(e)] < 0)
+                    throw std::logic_error("chysx::engine: contribution outside its assembly unit");
+                ent.push_back((w & ~CHX_CONTRIB_MAX_ELEMENT) | static_cast<unsigned>(l));
+            }
+            off.push_back(static_cast<unsigned>(ent.size()));
+        };
+        for (int r = r0; r < r1; ++r) emit(static_cast<std::size_t>(r));
+        for (int sl = r0 / SR; sl <= (r1 - 1) / SR; ++sl) {
+            const int a = std::max(r0, sl * SR), b = std::min(r1, (sl + 1) * SR);
+            const int width = (uo[sl + 1] - uo[sl]) / SR;
+            for (int jj = 0; jj < width; ++jj)
+                for (int r = a; r < b; ++r)
+                    emit(static_cast<std::size_t>(n) + static_cast<std::size_t>(uo[sl] + jj * SR + (r - sl * SR)));
+        }
+        for (int k = 0; k < kKinds; ++k)
+            for (int e : list[k]) local[k][e] = -1;
+        units.push_back(U);
     }
 }
 
@@ -245,15 +439,18 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     for (std::size_t k = 0; k < S; ++k) count[k + 1] += count[k];
     if (count[S] > std::numeric_limits<unsigned>::max())
         throw std::runtime_error("chysx::engine: gather map exceeds 2^32 entries");
-    auto& off = contrib_offsets_.host();
-    off.resize(S + 1);
-    for (std::size_t k = 0; k <= S; ++k) off[k] = static_cast<unsigned>(count[k]);
-    auto& ent = contribs_.host();
-    ent.assign(count[S], 0u);
-    std::vector<unsigned> fill(off.begin(), off.end() - 1);
-    walk([&](std::size_t g, unsigned kind, int a, int b, std::size_t id) {
-        if (g != kNoSlot) ent[fill[g]++] = encode(kind, a, b, id);
-    });
+    // per-slot lists with global element indices (slot order as above)
+    std::vector<unsigned> goff(S + 1);
+    for (std::size_t k = 0; k <= S; ++k) goff[k] = static_cast<unsigned>(count[k]);
+    std::vector<unsigned> gent(count[S], 0u);
+    {
+        std::vector<unsigned> fill(goff.begin(), goff.end() - 1);
+        walk([&](std::size_t g, unsigned kind, int a, int b, std::size_t id) {
+            if (g != kNoSlot) gent[fill[g]++] = encode(kind, a, b, id);
+        });
+    }
+    build_units(goff, gent);
+    const auto& ent = contribs_.host();
 
     // The kernels address every per-instance array with 32-bit byte offsets.
     const auto fits = [](std::size_t count, std::size_t bytes_each) {
@@ -261,9 +458,9 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     };
     if (!fits(static_cast<std::size_t>(n_) + CHX_BLOCK_GROUP, 36) || !fits(static_cast<std::size_t>(nup_), 36) ||
         !fits(static_cast<std::size_t>(npad_), 4) ||
-        !fits(tris_.cpu_size(), sizeof(float) * CHX_TRI_REC) || !fits(dihs_.cpu_size(), sizeof(float) * CHX_DIH_REC) ||
+        !fits(tris_.cpu_size(), sizeof(chx_tri)) || !fits(dihs_.cpu_size(), sizeof(chx_dih)) ||
         !fits(pins_.cpu_size(), sizeof(chx_pin)) || !fits(ent.size(), sizeof(unsigned)) ||
-        !fits(stvks_.cpu_size(), sizeof(float) * CHX_STVK_REC))
+        !fits(stvks_.cpu_size(), sizeof(chx_stvk)) || !fits(unit_elems_.cpu_size(), sizeof(int)))
         throw std::invalid_argument(
             "chysx::engine: a per-instance array would reach 4 GiB (at most 119,304,647 vertices or off-diagonal "
             "blocks per cloth); split the cloth into several instances");
@@ -277,7 +474,10 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     upload(block_slots_, bytes_, stream);
     upload(contrib_offsets_, bytes_, stream);
     upload(contribs_, bytes_, stream);
+    upload(units_, bytes_, stream);
+    upload(unit_elems_, bytes_, stream);
     upload(pins_, bytes_, stream);
+    upload(pin_targets_, bytes_, stream);
     upload(tris_, bytes_, stream);
     upload(dihs_, bytes_, stream);
     upload(stvks_, bytes_, stream);
@@ -285,10 +485,6 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     // grouped planar block arrays hold whole groups of CHX_BLOCK_GROUP blocks
     const std::size_t ng = (n + CHX_BLOCK_GROUP - 1) / CHX_BLOCK_GROUP * CHX_BLOCK_GROUP;
     const std::size_t nug = std::max<std::size_t>(static_cast<std::size_t>(nup_), CHX_BLOCK_GROUP);
-    alloc(pin_rec_, pins_.cpu_size() * CHX_PIN_REC, bytes_);
-    alloc(tri_rec_, tris_.cpu_size() * CHX_TRI_REC, bytes_);
-    alloc(dih_rec_, dihs_.cpu_size() * CHX_DIH_REC, bytes_);
-    alloc(stvk_rec_, stvks_.cpu_size() * CHX_STVK_REC, bytes_);
     alloc(diag_, 9 * ng, bytes_);
     alloc(values_, 9 * nug, bytes_);
     alloc(rhs_, 3 * n, bytes_);
@@ -297,6 +493,7 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     alloc(p0_, 3 * n, bytes_);
     alloc(p1_, 3 * n, bytes_);
     alloc(q_, 3 * n, bytes_);
+    alloc(q1_, 3 * n, bytes_);
     alloc(minv_, 9 * ng, bytes_);
     // Column window of every stage of CHX_STAGE_ROWS rows (its own rows
     // included): the rows of the source vector its SpMV reads.
@@ -342,9 +539,18 @@ void InstanceData::update_elements(const InstanceSource& s, std::uintptr_t strea
     if (pins_.cpu_size() != nP || tris_.cpu_size() != nT || dihs_.cpu_size() != nD || stvks_.cpu_size() != nV)
         throw std::runtime_error("chysx::engine: element counts changed; rebuild the instance");
     pins_.copy_to_device(stream);
+    pin_targets_.copy_to_device(stream);
     tris_.copy_to_device(stream);
     dihs_.copy_to_device(stream);
     stvks_.copy_to_device(stream);
+}
+
+void InstanceData::upload_pin_targets(const float* host_xyz, int n, std::uintptr_t stream) {
+    if (n <= 0) return;
+    if (static_cast<std::size_t>(n) * 3 != pin_targets_.cpu_size() || pin_targets_.gpu_size() != pin_targets_.cpu_size())
+        throw std::invalid_argument("chysx::engine: pin target count differs from the built pin set");
+    std::memcpy(pin_targets_.cpu_data(), host_xyz, sizeof(float) * 3 * static_cast<std::size_t>(n));
+    stager_.upload(pin_targets_.gpu_data(), host_xyz, sizeof(float) * 3 * static_cast<std::size_t>(n), stream);
 }
 
 chx_instance InstanceData::descriptor(const InstanceSource& s) const {
@@ -358,13 +564,13 @@ chx_instance InstanceData::descriptor(const InstanceSource& s) const {
     I.diag = diag_.gpu_data();
     I.values = values_.gpu_data();
     I.pins = pins_.gpu_data();
+    I.pin_targets = pin_targets_.gpu_data();
     I.tris = tris_.gpu_data();
     I.dihs = dihs_.gpu_data();
-    I.pin_rec = pin_rec_.gpu_data();
-    I.tri_rec = tri_rec_.gpu_data();
-    I.dih_rec = dih_rec_.gpu_data();
+    I.units = units_.gpu_data();
+    I.unit_elems = unit_elems_.gpu_data();
+    I.n_units = static_cast<int>(units_.cpu_size());
     I.stvks = stvks_.gpu_data();
-    I.stvk_rec = stvk_rec_.gpu_data();
     I.contrib_offsets = contrib_offsets_.gpu_data();
     I.contribs = contribs_.gpu_data();
     I.rhs = rhs_.gpu_data();
@@ -374,6 +580,7 @@ chx_instance InstanceData::descriptor(const InstanceSource& s) const {
     I.p0 = p0_.gpu_data();
     I.p1 = p1_.gpu_data();
     I.q = q_.gpu_data();
+    I.q1 = q1_.gpu_data();
     I.minv = minv_.gpu_data();
     I.diag_c = s.bf16_blocks ? diag_c_.gpu_data() : nullptr;
     I.values16 = s.bf16_blocks ? values16_.gpu_data() : nullptr;
@@ -428,9 +635,17 @@ void BatchRunner::size_outputs(int n_instances, int n_tiles, std::uintptr_t stre
         inst_done_.ensure_device(I);
         inst_rr_.ensure_device(I);
         inst_bb_.ensure_device(I);
+        inst_rho_.ensure_device(I);
         inst_iters_.zero_device(stream);
         inst_rr_.zero_device(stream);
         inst_bb_.zero_device(stream);
+        inst_rho_.zero_device(stream);
+    }
+    if (status_host_ == nullptr) {
+        void* p = nullptr;
+        check(chx_host_malloc(&p, 64), "engine", "status word");
+        status_host_ = static_cast<unsigned*>(p);
+        *status_host_ = 0u;
     }
     const std::size_t P = static_cast<std::size_t>(CHX_TILE_PARTIALS) * std::max(n_tiles, 1);
     if (partials_.gpu_size() < P) partials_.ensure_device(P);
@@ -443,7 +658,52 @@ void BatchRunner::size_outputs(int n_instances, int n_tiles, std::uintptr_t stre
     desc_.inst_rr = inst_rr_.gpu_data();
     desc_.inst_bb = inst_bb_.gpu_data();
     desc_.inst_done = inst_done_.gpu_data();
+    desc_.inst_rho = inst_rho_.gpu_data();
     desc_.sync = sync_.gpu_data();
+    desc_.status_host = status_host_;
+}
+
+void BatchRunner::release() noexcept {
+    // a replay may still be running on the caller's stream
+    if (graph_ != nullptr) (void)chx_device_synchronize();
+    drop_graph();
+    if (status_host_ != nullptr) {
+        // a queued solve may still report into the word
+        (void)chx_device_synchronize();
+        (void)chx_host_free(status_host_);
+        status_host_ = nullptr;
+    }
+}
+
+BatchRunner& BatchRunner::operator=(BatchRunner&& o) noexcept {
+    if (this == &o) return *this;
+    release();
+    desc_ = o.desc_;
+    graph_ = o.graph_;
+    graph_desc_ = o.graph_desc_;
+    graph_captures_ = o.graph_captures_;
+    lanes_ = o.lanes_;
+    grid_ = o.grid_;
+    tile_rows_ = o.tile_rows_;
+    bf16_ = o.bf16_;
+    table_host_ = std::move(o.table_host_);
+    table_ = std::move(o.table_);
+    tiles_ = std::move(o.tiles_);
+    elem_items_ = std::move(o.elem_items_);
+    unit_items_ = std::move(o.unit_items_);
+    partials_ = std::move(o.partials_);
+    inst_rr_ = std::move(o.inst_rr_);
+    inst_bb_ = std::move(o.inst_bb_);
+    inst_rho_ = std::move(o.inst_rho_);
+    inst_iters_ = std::move(o.inst_iters_);
+    inst_done_ = std::move(o.inst_done_);
+    sync_ = std::move(o.sync_);
+    trace_ = std::move(o.trace_);
+    status_host_ = o.status_host_;
+    o.graph_ = nullptr;
+    o.status_host_ = nullptr;
+    o.desc_ = chx_cloth_batch{};
+    return *this;
 }
 
 void BatchRunner::set_trace(int entries) {
@@ -520,7 +780,7 @@ void BatchRunner::bind_single(const chx_instance& d, std::uintptr_t stream) {
     build_tiles(one, stream);
     desc_.n_instances = 1;
     desc_.n_elem_items = 0;
-    desc_.n_slot_items = 0;
+    desc_.n_unit_items = 0;
     desc_.lanes_per_row = lanes_;
     desc_.grid_blocks = grid_;
     desc_.bf16_blocks = bf16_ ? 1 : 0;
@@ -528,7 +788,7 @@ void BatchRunner::bind_single(const chx_instance& d, std::uintptr_t stream) {
     desc_.single = one[0];
     desc_.inst = nullptr;
     desc_.elem_items = nullptr;
-    desc_.slot_items = nullptr;
+    desc_.unit_items = nullptr;
     size_outputs(1, desc_.n_tiles, stream);
     sync_stream(stream, "bind");
 }
@@ -538,7 +798,7 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds, std::uintptr_t 
     table_host_ = ds;
     build_tiles(table_host_, stream);
     auto& ei = elem_items_.host();
-    auto& si = slot_items_.host();
+    auto& si = unit_items_.host();
     ei.clear();
     si.clear();
     for (std::size_t i = 0; i < table_host_.size(); ++i) {
@@ -548,16 +808,16 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds, std::uintptr_t 
         for (int kind = 0; kind < 4; ++kind)
             for (int b = 0; b < counts[kind]; b += CHX_ITEM_SIZE)
                 ei.insert(ei.end(), {inst, kind, b, std::min(counts[kind], b + CHX_ITEM_SIZE)});
-        const int slots = I.n_rows + I.n_upper;  // gather slots: diagonal blocks, then value slots
-        for (int b = 0; b < slots; b += CHX_ITEM_SIZE) si.insert(si.end(), {inst, b, std::min(slots, b + CHX_ITEM_SIZE), 0});
+        if (I.n_rows > 0)
+            for (int u = 0; u < I.n_units; ++u) si.insert(si.end(), {inst, u, 0, 0});
     }
     elem_items_.copy_to_device(stream);
-    slot_items_.copy_to_device(stream);
+    unit_items_.copy_to_device(stream);
     table_.host() = table_host_;
     table_.copy_to_device(stream);
     desc_.n_instances = static_cast<int>(ds.size());
     desc_.n_elem_items = static_cast<int>(ei.size() / 4);
-    desc_.n_slot_items = static_cast<int>(si.size() / 4);
+    desc_.n_unit_items = static_cast<int>(si.size() / 4);
     desc_.lanes_per_row = lanes_;
     desc_.grid_blocks = grid_;
     desc_.bf16_blocks = bf16_ ? 1 : 0;
@@ -565,7 +825,7 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds, std::uintptr_t 
     desc_.single = chx_instance{};
     desc_.inst = table_.gpu_data();
     desc_.elem_items = elem_items_.gpu_data();
-    desc_.slot_items = slot_items_.gpu_data();
+    desc_.unit_items = unit_items_.gpu_data();
     size_outputs(desc_.n_instances, desc_.n_tiles, stream);
     sync_stream(stream, "bind");
 }
@@ -592,10 +852,14 @@ void BatchRunner::update_many(const std::vector<chx_instance>& ds, std::uintptr_
     }
 }
 
-void BatchRunner::step(std::uintptr_t stream) { check(chx_cloth_step(&desc_, stream), "cloth::ClothSimulator", "step"); }
+void BatchRunner::step(std::uintptr_t stream) {
+    check_host_status(stream);
+    check(chx_cloth_step(&desc_, stream), "cloth::ClothSimulator", "step");
+}
 
 void BatchRunner::step_graph(std::uintptr_t stream) {
     if (stream == 0) throw std::invalid_argument("chysx::cloth::ClothBatch: graph replay needs a non-null stream");
+    check_host_status(stream);
     if (graph_ == nullptr || std::memcmp(&graph_desc_, &desc_, sizeof(chx_cloth_batch)) != 0) {
         // A replaced graph may still run on the stream: wait before dropping it.
         check(chx_stream_synchronize(stream), "cloth::ClothBatch", "graph capture");
@@ -614,13 +878,8 @@ void BatchRunner::drop_graph() noexcept {
 void BatchRunner::assemble(std::uintptr_t stream) {
     check(chx_cloth_assemble(&desc_, stream), "cloth::ClothSimulator", "assemble");
 }
-void BatchRunner::element_pass(std::uintptr_t stream) {
-    check(chx_cloth_element_pass(&desc_, stream), "cloth::ClothSimulator", "element pass");
-}
-void BatchRunner::gather_pass(std::uintptr_t stream) {
-    check(chx_cloth_gather_pass(&desc_, stream), "cloth::ClothSimulator", "gather pass");
-}
 void BatchRunner::solve(bool finalize, std::uintptr_t stream) {
+    check_host_status(stream);
     check(chx_cloth_solve(&desc_, finalize ? 1 : 0, stream), "solver::PCGSolver", "solve");
 }
 double BatchRunner::energy(int kind_mask, std::uintptr_t stream) {

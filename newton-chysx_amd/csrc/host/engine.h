@@ -12,6 +12,7 @@
 #pragma once
 
 #include <cstdint>
+#include <stdexcept>
 #include <vector>
 
 #include "chysx_common.h"
@@ -51,13 +52,20 @@ public:
     InstanceData() = default;
     InstanceData(const InstanceData&) = delete;
     InstanceData& operator=(const InstanceData&) = delete;
+    InstanceData(InstanceData&&) = default;
+    InstanceData& operator=(InstanceData&&) = default;
 
     // Builds every device array (slot tables must be bound to s.pattern).
     // Uploads run on `stream`, which is synchronized (not the device), so
     // an instance can be set up while other streams run batches.
     void build(const InstanceSource& s, std::uintptr_t stream = 0);
-    // Re-uploads the element arrays (stiffness or pin targets changed).
+    // Re-uploads the element arrays (a stiffness changed).
     void update_elements(const InstanceSource& s, std::uintptr_t stream = 0);
+    // Uploads new pin targets (n AoS float3) on `stream` without waiting
+    // (cloth_simulator.cu:294-313): the element pass reads them from the
+    // instance's target array, so queued and captured substeps that run
+    // after the copy see them.
+    void upload_pin_targets(const float* host_xyz, int n, std::uintptr_t stream);
     // Re-uploads the static contact primitives (the shape list changed).
     void update_shapes(const InstanceSource& s, std::uintptr_t stream = 0);
     // Descriptor for a launch: device arrays + the step scalars of `s`.
@@ -71,6 +79,8 @@ public:
     const std::vector<int>& block_slots() const noexcept { return block_slots_.host(); }
     std::size_t device_bytes() const noexcept { return bytes_; }
     std::size_t contributions() const noexcept { return contribs_.cpu_size(); }
+    // Assembly units (host copies): row ranges and element counts.
+    const std::vector<chx_unit>& units() const noexcept { return units_.host(); }
     const float* diag() const noexcept { return diag_.gpu_data(); }
     const float* values() const noexcept { return values_.gpu_data(); }
     const float* diag_c() const noexcept { return diag_c_.gpu_data(); }
@@ -83,19 +93,23 @@ public:
 
 private:
     void pack_elements(const InstanceSource& s);
+    void build_units(const std::vector<unsigned>& goff, const std::vector<unsigned>& gent);
 
     bool built_ = false;
     int n_ = 0, nnz_ = 0, npad_ = 0, nup_ = 0;
     std::size_t bytes_ = 0;
     DeviceArray<int> row_offsets_, col_indices_;
     DeviceArray<chx_pin> pins_;
+    DeviceArray<float> pin_targets_;  // AoS float3 per pin
+    HostStager stager_;
     DeviceArray<chx_tri> tris_;
     DeviceArray<chx_dih> dihs_;
     DeviceArray<chx_stvk> stvks_;
-    DeviceArray<float> stvk_rec_;
-    DeviceArray<float> pin_rec_, tri_rec_, dih_rec_, diag_, values_;
+    DeviceArray<float> diag_, values_;
+    DeviceArray<chx_unit> units_;
+    DeviceArray<int> unit_elems_;
     DeviceArray<unsigned> contrib_offsets_, contribs_;
-    DeviceArray<float> rhs_, r_, z_, p0_, p1_, q_, minv_;
+    DeviceArray<float> rhs_, r_, z_, p0_, p1_, q_, q1_, minv_;
     DeviceArray<float> diag_c_;
     DeviceArray<uint16_t> values16_;
     DeviceArray<int> windows_;  // per CHX_STAGE_ROWS-row stage: {min column, max column}
@@ -109,13 +123,11 @@ private:
 class BatchRunner {
 public:
     BatchRunner() = default;
-    ~BatchRunner() {
-        // a replay may still be running on the caller's stream
-        if (graph_ != nullptr) (void)chx_device_synchronize();
-        drop_graph();
-    }
+    ~BatchRunner() { release(); }
     BatchRunner(const BatchRunner&) = delete;
     BatchRunner& operator=(const BatchRunner&) = delete;
+    BatchRunner(BatchRunner&& o) noexcept { *this = std::move(o); }
+    BatchRunner& operator=(BatchRunner&& o) noexcept;
 
     // One instance: its descriptor travels in the kernel arguments.
     void bind_single(const chx_instance& d, std::uintptr_t stream = 0);
@@ -131,6 +143,8 @@ public:
     }
     void update_many(const std::vector<chx_instance>& ds, std::uintptr_t stream);
 
+    // Launches one substep.  Raises (after synchronising) when an earlier
+    // solve of this runner aborted at a grid barrier.
     void step(std::uintptr_t stream);
     // The same substep replayed from a hipGraph, captured on first use and
     // again whenever the descriptor changed since the capture (`stream` must
@@ -139,8 +153,6 @@ public:
     void drop_graph() noexcept;
     int graph_captures() const noexcept { return graph_captures_; }
     void assemble(std::uintptr_t stream);
-    void element_pass(std::uintptr_t stream);
-    void gather_pass(std::uintptr_t stream);
     void solve(bool finalize, std::uintptr_t stream);
     double energy(int kind_mask, std::uintptr_t stream);
 
@@ -181,9 +193,25 @@ public:
     // Synchronous read-backs.
     void read_stats(std::vector<int>& iters, std::vector<double>& rr, std::vector<double>& bb, std::uintptr_t stream);
     void check_status(std::uintptr_t stream);
+    // Bound of every grid-barrier wait of the persistent solver in 100 MHz
+    // ticks (0: one second).  A solve whose wait exceeds it stops and is
+    // reported by the next step or status read.
+    void set_barrier_timeout_ticks(unsigned ticks) { desc_.wait_ticks = ticks; }
+    // Solver form of FP32 instances: 0 one grid barrier per iteration
+    // (default), 1 two barriers per iteration.
+    void set_pcg_form(int form) {
+        if (form != 0 && form != 1) throw std::invalid_argument("chysx::engine: pcg form must be 0 or 1");
+        desc_.pcg_form = form;
+    }
+    int pcg_form() const noexcept { return desc_.pcg_form; }
 
 private:
     void size_outputs(int n_instances, int n_tiles, std::uintptr_t stream = 0);
+    // Raises when the host-mapped status word reports an aborted solve.
+    void check_host_status(std::uintptr_t stream) {
+        if (status_host_ != nullptr && *static_cast<volatile unsigned*>(status_host_) != 0u) check_status(stream);
+    }
+    void release() noexcept;
     // Tile list of the bound instances; sets their tile_begin / tile_end.
     void build_tiles(std::vector<chx_instance>& ds, std::uintptr_t stream = 0);
 
@@ -197,8 +225,9 @@ private:
     bool bf16_ = false;
     std::vector<chx_instance> table_host_;
     DeviceArray<chx_instance> table_;
-    DeviceArray<int> tiles_, elem_items_, slot_items_;
-    DeviceArray<double> partials_, inst_rr_, inst_bb_;
+    DeviceArray<int> tiles_, elem_items_, unit_items_;
+    DeviceArray<double> partials_, inst_rr_, inst_bb_, inst_rho_;
+    unsigned* status_host_ = nullptr;  // page-locked, device-mapped
     DeviceArray<int> inst_iters_, inst_done_;
     DeviceArray<unsigned> sync_;
     DeviceArray<unsigned long long> trace_;

@@ -1,11 +1,22 @@
 #include "sparse_solver.h"
 
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 namespace chysx {
 namespace sparse {
+
+// Move-only, as the reference (block_csr.h:67-68, pcg_solver.h:88-89).
+static_assert(std::is_move_constructible<BlockCSR3>::value && std::is_move_assignable<BlockCSR3>::value &&
+                  !std::is_copy_constructible<BlockCSR3>::value,
+              "BlockCSR3 is move-only");
+static_assert(std::is_move_constructible<solver::PCGSolver>::value &&
+                  std::is_move_assignable<solver::PCGSolver>::value &&
+                  !std::is_copy_constructible<solver::PCGSolver>::value,
+              "PCGSolver is move-only");
 
 void BlockCSR3::build_topology(int num_block_rows, const int* host_rows, const int* host_cols, int num_pairs) {
     pattern_ = topology::build_pattern(num_block_rows, host_rows, host_cols, num_pairs);
@@ -55,8 +66,46 @@ chx_contact_op to_abi(const ContactSpMVOp& op) {
     return c;
 }
 
-// Runs `call(workspace, bytes)` with the op's workspace, or a temporary one
-// freed after `cuda_stream` drains.
+// Library-kept contact workspaces, one per (device, stream): calls on one
+// stream run in order, so they may share its workspace; calls on different
+// streams never share one.  A workspace only grows (waiting for its stream
+// before the old one is freed) and is held until the process ends -- it is
+// deliberately never freed at exit, when the runtime may be gone.
+struct StreamWorkspace {
+    int device;
+    std::uintptr_t stream;
+    void* ptr;
+    std::size_t bytes;
+};
+std::mutex g_ws_mu;
+std::vector<StreamWorkspace>* g_ws = new std::vector<StreamWorkspace>();
+
+void* stream_workspace(std::size_t need, std::uintptr_t stream, std::size_t* bytes) {
+    int dev = 0;
+    check(chx_get_device(&dev), "collision", "current device");
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    for (StreamWorkspace& w : *g_ws) {
+        if (w.device != dev || w.stream != stream) continue;
+        if (w.bytes < need) {
+            check(stream ? chx_stream_synchronize(stream) : chx_device_synchronize(), "collision", "workspace growth");
+            check(chx_free(w.ptr), "collision", "workspace growth");
+            w.ptr = nullptr;
+            w.bytes = 0;
+            check(chx_malloc(&w.ptr, need), "collision", "workspace growth");
+            w.bytes = need;
+        }
+        *bytes = w.bytes;
+        return w.ptr;
+    }
+    void* p = nullptr;
+    check(chx_malloc(&p, need), "collision", "workspace");
+    g_ws->push_back(StreamWorkspace{dev, stream, p, need});
+    *bytes = need;
+    return p;
+}
+
+// Runs `call(workspace, bytes)` with the op's workspace, or the stream's
+// library-kept one.
 template <class Call>
 void with_workspace(const ContactSpMVOp& op, int n, std::uintptr_t cuda_stream, const char* what, Call call) {
     const std::size_t need = op.workspace_bytes(n);
@@ -66,10 +115,9 @@ void with_workspace(const ContactSpMVOp& op, int n, std::uintptr_t cuda_stream, 
         call(op.workspace, op.workspace_size);
         return;
     }
-    DeviceArray<unsigned char> tmp;
-    tmp.ensure_device(need);
-    call(static_cast<void*>(tmp.gpu_data()), tmp.gpu_size());
-    check(chx_stream_synchronize(cuda_stream), "collision", what);
+    std::size_t bytes = 0;
+    void* ws = stream_workspace(need, cuda_stream, &bytes);
+    call(ws, bytes);
 }
 
 }  // namespace
@@ -127,8 +175,12 @@ int PCGSolver::solve(const sparse::BlockCSR3& A, DeviceSpan<math::Vec3f> b, Devi
             "PCGSolver::solve: A.diag has fewer than A.num_block_rows() entries; call A.build_topology(...) before "
             "solving");
     initialize(n);
+    last_stream_ = cuda_stream;
+    // With a tolerance the iteration count is only known on the device:
+    // wait for it (an addition); otherwise the call stays asynchronous.
+    const bool wait = params.tolerance > 0.0;
     int it = 0;
-    double rr = 0.0, bb = 0.0;
+    int* itp = wait ? &it : nullptr;
     if (contact.active() && contact.count_dev != nullptr) {
         const chx_contact_op c = collision::to_abi(contact);
         void* cws = contact.workspace;
@@ -144,18 +196,43 @@ int PCGSolver::solve(const sparse::BlockCSR3& A, DeviceSpan<math::Vec3f> b, Devi
                   A.col_indices.gpu_data(), reinterpret_cast<const float*>(A.values.gpu_data()),
                   reinterpret_cast<const float*>(b.data()), reinterpret_cast<float*>(x.data()), params.max_iterations,
                   params.tolerance, params.lanes_per_row, workspace_.gpu_data(), workspace_.gpu_size(), &c, cws,
-                  cbytes, &it, &rr, &bb, cuda_stream),
+                  cbytes, itp, nullptr, nullptr, cuda_stream),
               "solver::PCGSolver", "solve");
     } else {
         check(chx_pcg_solve_bsr3(n, reinterpret_cast<const float*>(A.diag.gpu_data()), A.row_offsets.gpu_data(),
                                  A.col_indices.gpu_data(), reinterpret_cast<const float*>(A.values.gpu_data()),
                                  reinterpret_cast<const float*>(b.data()), reinterpret_cast<float*>(x.data()),
                                  params.max_iterations, params.tolerance, params.lanes_per_row, workspace_.gpu_data(),
-                                 workspace_.gpu_size(), &it, &rr, &bb, cuda_stream),
+                                 workspace_.gpu_size(), itp, nullptr, nullptr, cuda_stream),
               "solver::PCGSolver", "solve");
     }
-    last_rel_residual_ = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    return wait ? it : params.max_iterations;
+}
+
+void PCGSolver::read_stats(int* it, double* rr, double* bb, double* rho) {
+    check(chx_pcg_last_stats(n_, workspace_.gpu_data(), workspace_.gpu_size(), it, rr, bb, rho, last_stream_),
+          "solver::PCGSolver", "statistics read");
+}
+
+float PCGSolver::last_residual() {
+    if (n_ == 0 || workspace_.gpu_size() == 0) return 0.0f;
+    double rho = 0.0;
+    read_stats(nullptr, nullptr, nullptr, &rho);
+    return static_cast<float>(rho);
+}
+
+int PCGSolver::last_iterations() {
+    if (n_ == 0 || workspace_.gpu_size() == 0) return 0;
+    int it = 0;
+    read_stats(&it, nullptr, nullptr, nullptr);
     return it;
+}
+
+double PCGSolver::last_relative_residual() {
+    if (n_ == 0 || workspace_.gpu_size() == 0) return 0.0;
+    double rr = 0.0, bb = 0.0;
+    read_stats(nullptr, &rr, &bb, nullptr);
+    return bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
 }
 
 }  // namespace solver

@@ -18,6 +18,8 @@ public:
     BlockCSR3() = default;
     BlockCSR3(const BlockCSR3&) = delete;
     BlockCSR3& operator=(const BlockCSR3&) = delete;
+    BlockCSR3(BlockCSR3&&) = default;  // move-only (block_csr.h:67-68)
+    BlockCSR3& operator=(BlockCSR3&&) = default;
 
     int num_block_rows() const noexcept { return pattern_.n; }
     int num_off_diag_blocks() const noexcept { return pattern_.nnz(); }
@@ -70,8 +72,9 @@ struct ContactSpMVOp {
     float friction_mu = 0.0f;
     float friction_epsilon = 1.0e-4f;
     // Addition: device workspace of at least workspace_bytes() bytes for the
-    // per-row contact index.  Null: bake / apply allocate a temporary and
-    // wait for `cuda_stream` before freeing it.
+    // per-row contact index.  Null: bake / apply use a workspace the library
+    // keeps per (device, stream), grown on demand and held until the process
+    // ends, so steady-state calls neither allocate nor wait.
     void* workspace = nullptr;
     std::size_t workspace_size = 0;
 
@@ -105,24 +108,35 @@ public:
     PCGSolver() = default;
     PCGSolver(const PCGSolver&) = delete;
     PCGSolver& operator=(const PCGSolver&) = delete;
+    PCGSolver(PCGSolver&&) = default;  // move-only (pcg_solver.h:88-89)
+    PCGSolver& operator=(PCGSolver&&) = default;
 
     void initialize(int num_block_rows);
-    // Solves A x = b in place (x is the warm start); returns the number of
-    // iterations performed (== max_iterations when tolerance <= 0).  An
-    // active `contact` turns every product into (A + C_off) x; its same-slot
-    // terms are the caller's to bake into A.diag (pcg_solver.h:111-131).
+    // Solves A x = b in place (x is the warm start), asynchronously on
+    // `cuda_stream` (pcg_solver.cu:282-318): with tolerance <= 0 it runs
+    // exactly max_iterations and returns that number without waiting.  With
+    // a tolerance (addition) it waits for the solve and returns the
+    // iterations performed.  An active `contact` turns every product into
+    // (A + C_off) x; its same-slot terms are the caller's to bake into
+    // A.diag (pcg_solver.h:111-131).
     int solve(const sparse::BlockCSR3& A, DeviceSpan<math::Vec3f> b, DeviceSpan<math::Vec3f> x,
               const PCGParams& params = PCGParams{}, std::uintptr_t cuda_stream = 0,
               collision::ContactSpMVOp contact = {});
-    // Preconditioner-weighted residual of the last solve: <r, z> is not kept;
-    // this returns ||r||_2 / ||b||_2 of the final iterate (FP64).
-    double last_residual() const noexcept { return last_rel_residual_; }
+    // Preconditioner-weighted residual <r, z> of the last solve's final
+    // iteration, copied to the host on demand (pcg_solver.cu:319-323).
+    float last_residual();
+    // Additions, also read on demand: iterations performed and the final
+    // relative residual ||r||_2 / ||b||_2 (FP64 norms) of the last solve.
+    int last_iterations();
+    double last_relative_residual();
 
 private:
+    void read_stats(int* it, double* rr, double* bb, double* rho);
+
     int n_ = 0;
     DeviceArray<unsigned char> workspace_;
     DeviceArray<unsigned char> contact_workspace_;
-    double last_rel_residual_ = 0.0;
+    std::uintptr_t last_stream_ = 0;
 };
 
 }  // namespace solver

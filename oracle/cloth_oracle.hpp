@@ -631,6 +631,7 @@ template <class T> struct Sim {
     double tol = 0.0;
     int last_iters = 0;
     double last_rr = 0.0, last_bb = 0.0;
+    double last_rho = 0.0;  // <r, z> of the final iterate (pcg_solver.cu:319-323)
 
     // ---- setup (ChysX/src/cloth/cloth_simulator.cu:248-514) -------------
     void set_state(int count, const double* p, const double* v, const double* w) {
@@ -714,6 +715,15 @@ template <class T> struct Sim {
         for (int i = 0; i < count; ++i)
             pin_target[i] = V3<T>(T(targets[3 * i]), T(targets[3 * i + 1]), T(targets[3 * i + 2]));
         topology_dirty = true;
+    }
+
+    // update_pin_targets (cloth_simulator.cu:294-313): same pin set, new
+    // targets; the pattern and the warm start are untouched.
+    void update_pin_targets(const double* targets, int count) {
+        if (count != static_cast<int>(pin_idx.size()))
+            throw std::invalid_argument("update_pin_targets: n_pins does not match the installed pin set");
+        for (int i = 0; i < count; ++i)
+            pin_target[i] = V3<T>(T(targets[3 * i]), T(targets[3 * i + 1]), T(targets[3 * i + 2]));
     }
 
     // Area-weighted lumped mass: cloth_simulator.cu:516-575, 151-190.
@@ -944,6 +954,7 @@ template <class T> struct Sim {
         last_iters = it;
         last_rr = rr;
         last_bb = bb;
+        last_rho = rho;
         return it;
     }
 

@@ -208,6 +208,10 @@ int orc_sim_set_pins(void* hv, const int* idx, const double* targets, int n, dou
     auto* h = static_cast<Handle*>(hv);
     return guarded([&] { DISPATCH(h, { using T = decltype(S.damping); S.set_pins(idx, targets, n, T(float(k))); }); });
 }
+int orc_sim_update_pin_targets(void* hv, const double* targets, int n) {
+    auto* h = static_cast<Handle*>(hv);
+    return guarded([&] { DISPATCH(h, { S.update_pin_targets(targets, n); }); });
+}
 int orc_sim_add_plane(void* hv, const double* nrm, double d) {
     auto* h = static_cast<Handle*>(hv);
     return guarded([&] {
@@ -320,6 +324,15 @@ int orc_sim_get_system(void* hv, double* diag, double* values, double* rhs, doub
         put3(S.rhs, rhs);
         put3(S.dx, dx); }); });
 }
+int orc_sim_get_dx(void* hv, double* dx) {
+    auto* h = static_cast<Handle*>(hv);
+    return guarded([&] { DISPATCH(h, {
+        using T = decltype(S.damping);
+        for (int i = 0; i < S.n; ++i) {
+            const V3<T> d = (static_cast<int>(S.dx.size()) == S.n) ? S.dx[i] : V3<T>();
+            dx[3 * i] = double(d.x); dx[3 * i + 1] = double(d.y); dx[3 * i + 2] = double(d.z);
+        } }); });
+}
 int orc_sim_set_dx(void* hv, const double* dx) {
     auto* h = static_cast<Handle*>(hv);
     return guarded([&] { DISPATCH(h, {
@@ -353,7 +366,7 @@ int orc_sim_hessian(void* hv, double* diag, double* values) {
 // baked the same-slot terms into diag).
 static int pcg_solve(int precision, int n, const int* row_offsets, const int* cols, const double* diag,
                      const double* values, const double* b, double* x, int max_iter, double tol, int* iters,
-                     double* rr, double* bb, const ContactOp* contact) {
+                     double* rr, double* bb, const ContactOp* contact, double* rho = nullptr) {
     auto run = [&](auto tag) {
         using T = decltype(tag);
         Sim<T> S;
@@ -377,6 +390,7 @@ static int pcg_solve(int precision, int n, const int* row_offsets, const int* co
         *iters = S.pcg(bv, xv);
         *rr = S.last_rr;
         *bb = S.last_bb;
+        if (rho) *rho = S.last_rho;
         put3(xv, x);
     };
     return guarded([&] { if (precision == 64) run(double{}); else run(float{}); });
@@ -386,6 +400,13 @@ int orc_pcg_solve(int precision, int n, const int* row_offsets, const int* cols,
                   const double* values, const double* b, double* x, int max_iter, double tol, int* iters,
                   double* rr, double* bb) {
     return pcg_solve(precision, n, row_offsets, cols, diag, values, b, x, max_iter, tol, iters, rr, bb, nullptr);
+}
+
+// orc_pcg_solve plus the final <r, z>.
+int orc_pcg_solve_rho(int precision, int n, const int* row_offsets, const int* cols, const double* diag,
+                      const double* values, const double* b, double* x, int max_iter, double tol, int* iters,
+                      double* rr, double* bb, double* rho) {
+    return pcg_solve(precision, n, row_offsets, cols, diag, values, b, x, max_iter, tol, iters, rr, bb, nullptr, rho);
 }
 
 int orc_pcg_solve_contact(int precision, int n, const int* row_offsets, const int* cols, const double* diag,

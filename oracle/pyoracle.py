@@ -47,7 +47,8 @@ def lib():
             "orc_sim_get_dihedrals", "orc_sim_get_edges", "orc_sim_get_rest_shape", "orc_sim_get_system",
             "orc_sim_set_dx", "orc_sim_energy", "orc_sim_gradient", "orc_sim_hessian", "orc_sim_build_stvk",
             "orc_stvk_element", "orc_box_sdf", "orc_sim_add_plane", "orc_sim_add_box", "orc_sim_set_contact",
-            "orc_contact_bake", "orc_contact_apply", "orc_pcg_solve_contact",
+            "orc_contact_bake", "orc_contact_apply", "orc_pcg_solve_contact", "orc_sim_update_pin_targets",
+            "orc_pcg_solve_rho", "orc_sim_get_dx",
         ):
             getattr(L, name).restype = _I
         L.orc_sim_step.argtypes = [_P, _D]
@@ -154,6 +155,10 @@ class OracleSim:
     def build_bending(self, k):
         _check(self.L.orc_sim_build_bending(self.h, k))
 
+    def update_pin_targets(self, targets):
+        t = _f64(targets).reshape(-1, 3)
+        _check(self.L.orc_sim_update_pin_targets(self.h, _ptr(t), _I(len(t))))
+
     def set_pins(self, idx, targets, k=1.0e6):
         i = _i32(idx)
         t = _f64(targets).reshape(-1, 3)
@@ -251,6 +256,12 @@ class OracleSim:
         _check(self.L.orc_sim_get_system(self.h, _ptr(diag), _ptr(vals), _ptr(rhs), _ptr(dx)))
         return diag, vals[:nnz].copy(), rhs, dx
 
+    def dx(self):
+        """The warm start / last solution dx (zeros before the first step)."""
+        d = np.zeros((self.n, 3))
+        _check(self.L.orc_sim_get_dx(self.h, _ptr(d)))
+        return d
+
     def set_dx(self, dx):
         d = _f64(dx).reshape(-1, 3)
         _check(self.L.orc_sim_set_dx(self.h, _ptr(d)))
@@ -273,18 +284,22 @@ class OracleSim:
         return diag, vals[: s["nnz"]].copy()
 
 
-def pcg_solve(row_offsets, cols, diag, values, b, x0, max_iter, tol=0.0, precision=32):
-    """Standalone block-Jacobi PCG of the oracle; returns (x, iters, rr, bb)."""
+def pcg_solve(row_offsets, cols, diag, values, b, x0, max_iter, tol=0.0, precision=32, return_rho=False):
+    """Standalone block-Jacobi PCG of the oracle; returns (x, iters, rr, bb),
+    plus the final <r, z> with return_rho."""
     ro = _i32(row_offsets)
     ci = _i32(cols) if len(cols) else np.zeros(1, np.int32)
     dg = _f64(diag).reshape(-1, 9)
     vl = _f64(values).reshape(-1, 9) if len(values) else np.zeros((1, 9))
     bb_ = _f64(b).reshape(-1, 3)
     x = _f64(x0).reshape(-1, 3).copy()
-    it, rr, bb = _I(), _D(), _D()
+    it, rr, bb, rho = _I(), _D(), _D(), _D()
     n = len(ro) - 1
-    _check(lib().orc_pcg_solve(_I(precision), _I(n), _ptr(ro), _ptr(ci), _ptr(dg), _ptr(vl), _ptr(bb_), _ptr(x),
-                               _I(max_iter), _D(tol), ctypes.byref(it), ctypes.byref(rr), ctypes.byref(bb)))
+    _check(lib().orc_pcg_solve_rho(_I(precision), _I(n), _ptr(ro), _ptr(ci), _ptr(dg), _ptr(vl), _ptr(bb_), _ptr(x),
+                                   _I(max_iter), _D(tol), ctypes.byref(it), ctypes.byref(rr), ctypes.byref(bb),
+                                   ctypes.byref(rho)))
+    if return_rho:
+        return x, it.value, rr.value, bb.value, rho.value
     return x, it.value, rr.value, bb.value
 
 

@@ -154,13 +154,18 @@ def test_bf16_batch_needs_bf16_instances(gpu):
         batch.step(spec.dt)
 
 
-def _precision_run(n, steps, max_iter):
+def _precision_run(n, steps, max_iter, per_substep=False, floor=False):
     """The precision workload: one n x n grid, E = 1e6 Pa, seeded initial
     shear of up to 5 cm, `steps` substeps, PCG tol 1e-6 and `max_iter`.  Two
-    forms -- FP32 blocks / SpMV with FP64 reductions, and BF16 blocks + outer
-    iterative refinement -- against a full-FP64 oracle run of the same
-    substeps: relative max-norm position error after the last substep, GPU
-    time per substep, PCG iterations."""
+    GPU forms -- FP32 blocks / SpMV with FP64 reductions, and BF16 blocks +
+    outer iterative refinement -- run free against a full-FP64 oracle run of
+    the same substeps, beside the FP32 oracle (the reference algorithm in
+    FP32 on the CPU, the arithmetic baseline).  Reported: relative max-norm
+    position error after the last substep, GPU time per substep, PCG
+    iterations.  per_substep: two more GPU simulators restart every substep
+    from the FP64 oracle's state (positions, velocities, warm start) and
+    their one-substep errors are recorded.  floor: an FP64 oracle at
+    tolerance 1e-10 gives the distance the 1e-6 tolerance alone leaves."""
     import dataclasses
     import json
     import time
@@ -172,79 +177,122 @@ def _precision_run(n, steps, max_iter):
     spec = dataclasses.replace(scenes.precision(n=n), max_iter=max_iter)
     report = {"workload": f"precision grid {n}x{n}, E=1e6 Pa, shear start, {steps} substeps, "
                           f"tol {spec.tol:g}, max_iter {max_iter}", "vertices": spec.n}
-    finals = {}
-    for name, bf16 in (("fp32", False), ("bf16_ir", True)):
-        dc = scenes.DeviceCloth(spec)
+    forms = (("fp32", False), ("bf16_ir", True))
+    free = {name: scenes.DeviceCloth(spec) for name, _ in forms}
+    reset = {name: scenes.DeviceCloth(spec) for name, _ in forms} if per_substep else {}
+    for name, bf16 in forms:
         if bf16:
-            dc.sim.set_bf16_blocks(True)
-        its = []
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        ms = 0.0
-        for s in range(steps):
+            free[name].sim.set_bf16_blocks(True)
+            if per_substep:
+                reset[name].sim.set_bf16_blocks(True)
+    o64 = helpers.oracle_from_spec(spec, precision=64)
+    o32 = helpers.oracle_from_spec(spec, precision=32)
+    o32r = helpers.oracle_from_spec(spec, precision=32) if per_substep else None
+    otight = helpers.oracle_from_spec(spec, precision=64, pcg=(max(max_iter, 40 * n), 1e-10)) if floor else None
+    its = {name: [] for name, _ in forms}
+    ms = {name: 0.0 for name, _ in forms}
+    worst = {name: 0.0 for name, _ in forms}
+    worst32 = 0.0
+    its64, its32 = [], []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        if per_substep:
+            po, vo = o64.state()
+            dxo = o64.dx()
+            for dc in reset.values():
+                dc.pos.copy_(torch.from_numpy(po.astype(np.float32)).cuda())
+                dc.vel.copy_(torch.from_numpy(vo.astype(np.float32)).cuda())
+                dc.sim.debug_set_warm_start(dxo.astype(np.float32))
+                dc.step()
+            # the FP32 oracle from the same state: the reference algorithm's
+            # own one-substep distance from FP64
+            o32r.set_state(po, vo, o32r.inv_mass())
+            o32r.set_dx(dxo)
+            o32r.step(spec.dt)
+        for name, _ in forms:
             e0.record()
-            dc.step()
+            free[name].step()
             e1.record()
             e1.synchronize()
-            ms += e0.elapsed_time(e1)
-            its.append(dc.sim.last_pcg_iterations())
-        finals[name] = dc.state()[0]
-        report[name] = {"ms_per_substep": round(ms / steps, 3), "pcg_iterations_mean": float(np.mean(its)),
-                        "pcg_iterations_max": int(np.max(its)),
-                        "last_relative_residual": float(dc.sim.last_relative_residual())}
-        print(f"{name}: {ms / steps:.3f} ms/substep, iterations mean {np.mean(its):.1f}", flush=True)
-        del dc
-    o = helpers.oracle_from_spec(spec, precision=64)
-    t0 = time.perf_counter()
-    its64 = []
-    for s in range(steps):
-        its64.append(o.step(spec.dt))
+            ms[name] += e0.elapsed_time(e1)
+            its[name].append(free[name].sim.last_pcg_iterations())
+        its64.append(o64.step(spec.dt))
+        its32.append(o32.step(spec.dt))
+        if otight is not None:
+            otight.step(spec.dt)
+        if per_substep:
+            p64 = o64.state()[0]
+            for name, dc in reset.items():
+                worst[name] = max(worst[name], helpers.rel_maxnorm(dc.state()[0], p64))
+            worst32 = max(worst32, helpers.rel_maxnorm(o32r.state()[0], p64))
         if (s + 1) % 10 == 0:
-            print(f"FP64 oracle substep {s + 1}/{steps} ({time.perf_counter() - t0:.0f} s)", flush=True)
-    p64 = o.state()[0]
-    report["fp64_oracle"] = {"pcg_iterations_mean": float(np.mean(its64)), "seconds": round(time.perf_counter() - t0, 1),
-                             "threads": pyoracle_threads()}
-    for name in finals:
-        report[name]["rel_maxnorm_error_vs_fp64"] = helpers.rel_maxnorm(finals[name], p64)
-        report[name]["max_abs_error_m"] = float(np.max(np.abs(finals[name] - p64)))
+            print(f"substep {s + 1}/{steps} ({time.perf_counter() - t0:.0f} s)", flush=True)
+    p64 = o64.state()[0]
+    report["fp64_oracle"] = {"pcg_iterations_mean": float(np.mean(its64)), "threads": pyoracle_threads()}
+    report["fp32_oracle"] = {"pcg_iterations_mean": float(np.mean(its32)),
+                             "rel_maxnorm_error_vs_fp64": helpers.rel_maxnorm(o32.state()[0], p64)}
+    if per_substep:
+        report["fp32_oracle"]["worst_one_substep_error_vs_fp64"] = worst32
+    if otight is not None:
+        report["tolerance_floor"] = {
+            "fp64_tol1e-6_vs_fp64_tol1e-10": helpers.rel_maxnorm(p64, otight.state()[0]),
+            "note": "distance of the FP64 oracle at the workload tolerance from a tightly converged FP64 run"}
+    for name, _ in forms:
+        pf = free[name].state()[0]
+        report[name] = {"ms_per_substep": round(ms[name] / steps, 3), "pcg_iterations_mean": float(np.mean(its[name])),
+                        "pcg_iterations_max": int(np.max(its[name])),
+                        "last_relative_residual": float(free[name].sim.last_relative_residual()),
+                        "rel_maxnorm_error_vs_fp64": helpers.rel_maxnorm(pf, p64),
+                        "max_abs_error_m": float(np.max(np.abs(pf - p64)))}
+        if per_substep:
+            report[name]["worst_one_substep_error_vs_fp64"] = worst[name]
     print("PRECISION " + json.dumps(report), flush=True)
     return report
 
 
-def _precision_size():
-    import os
-
-    return int(os.environ.get("CHYSX_PRECISION_N", "128")), int(os.environ.get("CHYSX_PRECISION_STEPS", "100"))
+def test_precision_config3_capped(gpu):
+    """The precision workload at its own size (512 x 512, 100 substeps,
+    the reference's cap of 100 iterations).  At the cap no form converges
+    on this stiff grid (relative residual ~0.2), so each follows its own
+    truncated Krylov path and even one substep of the reference algorithm in
+    FP32 lands ~1e-4 from FP64 (measured on the MI355X box: 1.1e-4 for the
+    GPU's FP32 form).  The bounds are therefore set against the FP32 oracle,
+    the reference algorithm on the CPU, fixed before any run:
+      * every substep, restarted from the FP64 oracle's state: each form's
+        one-substep distance from the FP64 oracle's next state at most twice
+        the FP32 oracle's (restarted from the same state), plus 1e-6;
+      * free-running for 100 substeps, the FP32 form no further from the
+        FP64 run than twice the FP32 oracle's own distance, the BF16 form
+        within three times it."""
+    rep = _precision_run(512, 100, max_iter=100, per_substep=True)
+    base = rep["fp32_oracle"]["rel_maxnorm_error_vs_fp64"]
+    base1 = rep["fp32_oracle"]["worst_one_substep_error_vs_fp64"]
+    for name in ("fp32", "bf16_ir"):
+        assert rep[name]["worst_one_substep_error_vs_fp64"] <= 2 * base1 + 1e-6, rep
+    assert rep["fp32"]["rel_maxnorm_error_vs_fp64"] <= 2 * base + 1e-5, rep
+    assert rep["bf16_ir"]["rel_maxnorm_error_vs_fp64"] <= 3 * base + 1e-5, rep
 
 
 def test_precision_report_converged(gpu):
-    """Every substep solved to the tolerance (iteration cap far above what
-    the solves need).  Relative residual 1e-6 on this stiff grid still leaves
-    solution differences of ~1e-6 * condition number per substep, so FP32
-    and FP64 end ~1e-4 apart after 100 substeps, and that figure moves by
-    tens of percent with the rounding of the summation (FMA contraction,
-    lanes per row).  Both forms stay under 1e-3; the BF16 form (one outer
-    refinement) within 4x of FP32's error, in at most 1.6x its iterations
-    (1.5x at 128 x 128, 1.53x at 512 x 512).  Default 128 x 128
-    (CHYSX_PRECISION_N / CHYSX_PRECISION_STEPS to change)."""
-    n, steps = _precision_size()
-    rep = _precision_run(n, steps, max_iter=40 * n)
+    """Every substep solved to the tolerance (128 x 128, cap far above the
+    needs).  Bounds fixed before the run: FP32 free-running error after 100
+    substeps at most twice the FP32 oracle's own distance from the FP64 run,
+    BF16 at most four times it; BF16 iterations at most 2 x FP32's + 20 --
+    with one outer refinement the BF16 form runs one inner solve to the
+    tolerance (about the FP32 count) and at most one more from the refined
+    residual, whose norm does not exceed the right-hand side's.  The report
+    also states the tolerance floor: the FP64 oracle at 1e-6 against FP64 at
+    1e-10, the error no arithmetic can remove at this tolerance."""
+    n = 128
+    rep = _precision_run(n, 100, max_iter=40 * n, floor=True)
     f32, b16 = rep["fp32"], rep["bf16_ir"]
+    base = rep["fp32_oracle"]["rel_maxnorm_error_vs_fp64"]
     for name in ("fp32", "bf16_ir"):
         assert rep[name]["pcg_iterations_max"] < 40 * n, (name, rep[name])
-    assert f32["rel_maxnorm_error_vs_fp64"] <= 1e-3, rep
-    assert b16["rel_maxnorm_error_vs_fp64"] <= 1e-3, rep
-    assert b16["rel_maxnorm_error_vs_fp64"] <= 4 * f32["rel_maxnorm_error_vs_fp64"] + 1e-5, rep
-    assert b16["pcg_iterations_mean"] <= 1.6 * f32["pcg_iterations_mean"] + 20, rep
-
-
-def test_precision_report_capped(gpu):
-    """The reference's iteration cap (100): no form converges on this stiff
-    grid, so each follows its own truncated Krylov path and the error
-    against the FP64 run measures that path's sensitivity.  BF16 stays
-    within 10x of FP32's."""
-    n, steps = _precision_size()
-    rep = _precision_run(n, steps, max_iter=100)
-    assert rep["bf16_ir"]["rel_maxnorm_error_vs_fp64"] <= 10 * rep["fp32"]["rel_maxnorm_error_vs_fp64"] + 1e-5, rep
+    assert f32["rel_maxnorm_error_vs_fp64"] <= 2 * base + 1e-5, rep
+    assert b16["rel_maxnorm_error_vs_fp64"] <= 4 * base + 1e-5, rep
+    assert b16["pcg_iterations_mean"] <= 2 * f32["pcg_iterations_mean"] + 20, rep
 
 
 def pyoracle_threads():

@@ -329,3 +329,29 @@ def test_energy_matches_oracle(gpu):
     e_g = dc.sim.compute_energy()
     e_o = o.energy()
     assert abs(e_g - e_o) <= 1e-3 * abs(e_o) + 1e-9
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_pcg_forms_match_oracle(gpu, form):
+    """Both persistent PCG forms of the FP32 solve -- one grid barrier per
+    iteration (the default: sigma and the next rho's recurrence terms in one
+    reduction) and two -- on 60 substeps of the 32x32 workload, each from the
+    oracle's state: 1e-5 relative max-norm and iterations within +-1."""
+    import torch
+
+    from chysx import scenes
+
+    spec = scenes.plain_small(seed=8)
+    dc = _make(spec)
+    dc.sim.set_pcg_form(form)
+    assert dc.sim.pcg_form() == form
+    o = helpers.oracle_from_spec(spec)
+    for s in range(60):
+        po, vo = o.state()
+        dc.pos.copy_(torch.from_numpy(po.astype(np.float32)).cuda())
+        dc.vel.copy_(torch.from_numpy(vo.astype(np.float32)).cuda())
+        dc.sim.debug_set_warm_start(o.dx().astype(np.float32))
+        dc.step()
+        it_o = o.step(spec.dt)
+        assert helpers.rel_maxnorm(dc.state()[0], o.state()[0]) <= 1e-5, (form, s)
+        assert abs(dc.sim.last_pcg_iterations() - it_o) <= 1, (form, s, dc.sim.last_pcg_iterations(), it_o)

@@ -131,3 +131,39 @@ def test_setup_queue_is_bounded():
     eng2 = serving.ServingEngine(SetupExecutor(FakeClock(), prepare_workers=2), hbm_budget_bytes=1e12,
                                  max_preparing=1)
     assert eng2.max_prep == 1
+
+
+def test_horizon_counts_unfinished_jobs_as_censored():
+    """Overload (each substep costs more than the arrivals allow) stopped at
+    a horizon: the censored percentiles cover every arrived job, the
+    unfinished ones at horizon - arrival, so they bound the completed-only
+    percentiles from above, and every arrival is counted."""
+    clock = FakeClock()
+    ex = FakeExecutor(clock, cost_per_vertex=2e-7)
+    reqs = _requests(count=200, seed=3)
+    rep = serving.ServingEngine(ex, hbm_budget_bytes=1e12, clock=clock, sleep=clock.sleep).run(
+        reqs, 1.0, horizon_s=1.0)
+    assert 0 < rep.completed < rep.arrived <= rep.submitted
+    unfinished = [r for r in reqs if r.arrival <= rep.wall_s and r.finished is None]
+    assert len(unfinished) == rep.arrived - rep.completed
+    cens = [r.finished - r.arrival for r in reqs if r.finished is not None] + \
+           [rep.wall_s - r.arrival for r in unfinished]
+    assert rep.p99_latency_censored_s == float(np.percentile(cens, 99))
+    assert rep.p50_latency_censored_s >= rep.p50_latency_s
+
+
+def test_load_shedding_rejects_jobs_that_waited_too_long():
+    """With max_queue_s, a job not admitted within that time of its arrival
+    is rejected: no admitted job waited longer, and rejected + completed
+    account for every job once the stream drains."""
+    clock = FakeClock()
+    ex = FakeExecutor(clock, cost_per_vertex=2e-7)
+    reqs = _requests(count=200, seed=4)
+    rep = serving.ServingEngine(ex, hbm_budget_bytes=50e6, clock=clock, sleep=clock.sleep, max_queue_s=0.2).run(
+        reqs, 1.0)
+    assert rep.rejected > 0 and rep.rejected + rep.completed == rep.submitted
+    for r in reqs:
+        if r.rejected:
+            assert r.admitted is None
+        else:
+            assert r.admitted - r.arrival <= 0.2 + 1e-3
