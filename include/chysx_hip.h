@@ -379,6 +379,24 @@ int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, co
                        int lanes_per_row, void* workspace, size_t workspace_bytes, int* iterations,
                        double* rr, double* bb, uintptr_t stream);
 
+/* Largest system chx_pcg_solve_bsr3_scratch accepts (block rows). */
+#define CHX_PCG_SCRATCH_MAX_ROWS 65536
+
+/* chx_pcg_solve_bsr3 without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The workspace is a device array of the library sized for
+ * CHX_PCG_SCRATCH_MAX_ROWS rows (6.0 MiB, held by each device the library
+ * runs on until the process ends); the call allocates and frees nothing and
+ * rejects n_rows > CHX_PCG_SCRATCH_MAX_ROWS with CHX_ERR_INVALID_ARGUMENT.
+ * All calls on one device share that array, which keeps the last call's
+ * contents (the solve clears what must start at zero): calls that overlap
+ * on one device race (serialise them, e.g. on one stream).  Same arguments
+ * and results as chx_pcg_solve_bsr3 otherwise. */
+int chx_pcg_solve_bsr3_scratch(int n_rows, const float* diag, const int* row_offsets, const int* col_indices,
+                               const float* values, const float* b, float* x, int max_iterations, double tol,
+                               int lanes_per_row, int* iterations, double* rr, double* bb, uintptr_t stream);
+
 /* ---- contact SpMV sidecar ----------------------------------------------------
  * A COO operator added to the block-CSR system the PCG iterates against,
  * replacing collision::ContactSpMVOp / bake_contact_diag / apply_contact_spmv

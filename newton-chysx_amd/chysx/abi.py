@@ -49,6 +49,7 @@ _SIGS = {
     "chx_spmv_bsr3": (_I, [_I, _P, _P, _P, _P, _P, _P, _F, _F, _I, _U]),
     "chx_pcg_workspace_size": (_I, [_I, _P]),
     "chx_pcg_solve_bsr3": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _D, _I, _P, _S, _P, _P, _P, _U]),
+    "chx_pcg_solve_bsr3_scratch": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _D, _I, _P, _P, _P, _U]),
     "chx_contact_workspace_size": (_I, [_I, _I, _P]),
     "chx_contact_bake_diag": (_I, [_I, _P, _P, _F, _P, _S, _U]),
     "chx_contact_spmv": (_I, [_I, _P, _P, _P, _F, _P, _S, _U]),

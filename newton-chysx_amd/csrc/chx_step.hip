@@ -1851,6 +1851,35 @@ void read_workspace_stats(const chx_cloth_batch& B, int* iterations, double* rr,
     if (rho) *rho = rz;
 }
 
+// Scratch of chx_pcg_solve_bsr3_scratch: the workspace of one standalone
+// solve of at most CHX_PCG_SCRATCH_MAX_ROWS block rows as a file-scope
+// device array, so that entry point allocates nothing.  Each device holds
+// its own copy (6.0 MiB) until the process ends; all calls on one device
+// share it, so calls that overlap on one device race.  Host code must not
+// pass the array's name to a kernel: it names a host-side stand-in.  The
+// device address comes from hipGetSymbolAddress, looked up once per device
+// with that device current, and only that address is used.
+constexpr size_t kScratchBytes = pcg_workspace_bytes(CHX_PCG_SCRATCH_MAX_ROWS);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};
+
+void* scratch_address() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    require(dev >= 0 && dev < kScratchDevices, "chx_pcg_solve_bsr3_scratch: device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        require(p != nullptr, "chx_pcg_solve_bsr3_scratch: no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
 // The standalone row-major PCG (pcg_solver.cu:134-215), with the contact
 // sidecar's cross-slot terms in every A x when `op` is active
 // (ContactSpMVOp::active, contact_spmv.h:81-83).
@@ -1907,6 +1936,21 @@ void solve_bsr3(int n_rows, const float* diag, const int* row_offsets, const int
 }  // namespace chx
 
 extern "C" {
+
+int chx_pcg_solve_bsr3_scratch(int n_rows, const float* diag, const int* row_offsets, const int* col_indices,
+                               const float* values, const float* b, float* x, int max_iterations, double tol,
+                               int lanes_per_row, int* iterations, double* rr, double* bb, uintptr_t stream) {
+    return guarded([&] {
+        require(n_rows >= 0, "chx_pcg_solve_bsr3_scratch: negative row count");
+        require(n_rows <= CHX_PCG_SCRATCH_MAX_ROWS,
+                "chx_pcg_solve_bsr3_scratch: n_rows exceeds CHX_PCG_SCRATCH_MAX_ROWS (65536); use "
+                "chx_pcg_solve_bsr3 with a caller workspace");
+        void* ws = nullptr;
+        if (n_rows > 0) ws = scratch_address();
+        solve_bsr3(n_rows, diag, row_offsets, col_indices, values, b, x, max_iterations, tol, lanes_per_row, ws,
+                   kScratchBytes, nullptr, nullptr, 0, iterations, rr, bb, stream);
+    });
+}
 
 int chx_pcg_solve_bsr3(int n_rows, const float* diag, const int* row_offsets, const int* col_indices,
                        const float* values, const float* b, float* x, int max_iterations, double tol, int lanes_per_row,
