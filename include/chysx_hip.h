@@ -55,14 +55,15 @@ enum {
 };
 
 /* Contribution-list entry encoding (gather map of the deterministic
- * assembly): bits 31..30 element kind, 29..28 local row a, 27..26 local
- * column b, 25..0 the element's position in its assembly unit's list of
- * elements of that kind. */
+ * assembly), 16 bits: bits 15..14 element kind, 13..12 local row a, 11..10
+ * local column b, 9..0 the element's position in its assembly unit's list
+ * of elements of that kind (at most CHX_CONTRIB_MAX_LOCAL + 1 elements of a
+ * kind per unit). */
 #define CHX_CONTRIB_PIN 0u
 #define CHX_CONTRIB_TRI 1u
 #define CHX_CONTRIB_DIH 2u
 #define CHX_CONTRIB_STVK 3u
-#define CHX_CONTRIB_MAX_ELEMENT ((1u << 26) - 1u)
+#define CHX_CONTRIB_MAX_LOCAL 1023u
 
 /* Packed element records (device arrays). Vertex indices are local to the
  * element's cloth instance. */
@@ -110,18 +111,25 @@ typedef struct chx_contact_op {
  * (every element with a vertex in the range) it computes into LDS, then
  * every block the unit owns: the diagonal blocks of its rows, then the upper
  * value slots of its rows (slice by slice; block index jj, then row), each
- * summing its contribution list.  Record floats per element in LDS and the
- * LDS capacity a unit's records may use: */
+ * summing its contribution list, which the kernel stages in LDS with the
+ * slots' offsets.  Record floats per element in LDS, and the LDS bytes a
+ * unit's records (4 B a float), contribution entries (2 B each) and slot
+ * offsets (2 B each) may use together: */
 #define CHX_ASM_PIN_REC 4
 #define CHX_ASM_TRI_REC 22
 #define CHX_ASM_DIH_REC 14
 #define CHX_ASM_STVK_REC 31
-#define CHX_ASM_LDS_FLOATS 12288
+#define CHX_ASM_LDS_BYTES 53248
 typedef struct chx_unit {
     int row_begin, row_end;          /* rows [row_begin, row_end) of the instance             */
     int elem_begin;                  /* first entry in unit_elems: pins, triangles, StVK,     */
     int n_pins, n_tris, n_stvks, n_dihs; /* dihedrals, ascending element index per kind      */
-    int slot_begin;                  /* first of the unit's slots in contrib_offsets          */
+    int n_slots;                     /* blocks the unit owns (diagonal, then upper slots)     */
+    int offs_begin;                  /* first of its n_slots + 1 entries in contrib_offsets   */
+                                     /* (a multiple of 8)                                     */
+    int word_begin;                  /* first of its entries in contribs (a multiple of 8)    */
+    int n_words;                     /* its entries, padded with zeros to a multiple of 8     */
+    int pad;
 } chx_unit;
 
 /* Number of float64 partial sums kept per row tile by the PCG kernel. */
@@ -180,8 +188,9 @@ typedef struct chx_instance {
     const chx_dih* dihs;
     const chx_unit* units;           /* [n_units] assembly units, ascending rows             */
     const int* unit_elems;           /* element indices of the units' lists                  */
-    const unsigned* contrib_offsets; /* [slots+1] per unit slot (units in order)             */
-    const unsigned* contribs;        /* gather-map entries                                   */
+    const uint16_t* contrib_offsets; /* per unit: n_slots + 1 offsets of its slots' lists   */
+                                     /* relative to its word_begin, padded to 8 entries      */
+    const uint16_t* contribs;        /* per unit: its slots' contribution entries            */
     float* rhs;                      /* b [n_rows*3]                                         */
     float* x;                        /* dx: warm start in, solution out [n_rows*3]           */
     float* r;

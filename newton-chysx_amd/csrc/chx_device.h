@@ -250,7 +250,7 @@ struct Inst {
     gp<const chx_dih> dihs;
     gp<const chx_unit> units;
     gp<const int> unit_elems;
-    gp<const unsigned> contrib_offsets, contribs;
+    gp<const uint16_t> contrib_offsets, contribs;
     gp<float> rhs, x, r, z, p0, p1, q, q1, minv;
     gp<float> diag_c;
     gp<uint16_t> values16;

@@ -64,31 +64,37 @@ __device__ __forceinline__ void reset_sync(unsigned* sync, bool all) {
 //            element list, ascending per kind -- gathers x_n from the
 //            instance's AoS positions and writes its force / Hessian record
 //            into LDS, field-major (record field f of local element e at
-//            [f * count + e]), so the gather below reads each record from
-//            LDS instead of from memory once per block it touches;
+//            [f * count + e]), so each record is read from LDS instead of
+//            from memory once per block it touches.  The unit's
+//            contribution entries and slot offsets are staged into LDS
+//            alongside with 16-byte loads;
 //   phase 2  one thread per block the unit owns (the diagonal blocks of its
 //            rows, then its upper value slots) sums the block's
-//            contributions in the fixed order of its contribution list, so
-//            the assembled matrix is bitwise reproducible with no float
-//            atomics.  Diagonal blocks also form the right-hand side
-//            b = m/dt^2 (x~ - x_n) - grad E(x_n) and the inertia block
-//            (cloth_simulator.cu:66-142,199-210).
+//            contributions in the fixed order of its contribution list, all
+//            operands in LDS, so the assembled matrix is bitwise
+//            reproducible with no float atomics.  Diagonal blocks also form
+//            the right-hand side b = m/dt^2 (x~ - x_n) - grad E(x_n) and the
+//            inertia block (cloth_simulator.cu:66-142,199-210).
 //
 // Records hold the Hessian in factored form (chx_elements.h): all blocks of
 // an element come from one record.  The element formulas are the
 // reference's (triangle_stretch_constraint.cu:153-311,
 // bending_constraint.cu:180-245, pin_constraint.cu:61-92).
 // ---------------------------------------------------------------------------
-constexpr int kAsmFloats = CHX_ASM_LDS_FLOATS;
 // LDS record fields.  Triangle: Gu, Gv (0..5), Mu, Mv upper triangles
 // (6..17), Dm^-1 (18..21).  StVK: Gu, Gv (0..5), Huu, Hvv upper triangles
 // (6..17), Huv row-major (18..26), Dm^-1 (27..30).  Dihedral: k (theta -
 // theta0), k, a_0..a_3 (2..13).  Pin: k (x - t), k.
 static_assert(CHX_ASM_TRI_REC == 22 && CHX_ASM_STVK_REC == 31 && CHX_ASM_DIH_REC == 14 && CHX_ASM_PIN_REC == 4,
               "record layouts");
+static_assert(CHX_ASM_LDS_BYTES % 16 == 0, "16-byte staging");
 // Floats tri_record / stvk_record write (chx_elements.h): the LDS records
 // keep their first 18 / 27 and Dm^-1 in place of the shape derivatives.
 constexpr int kTriRec = 24, kStvkRec = 36;
+// 16-byte vectors of a unit's entries and offsets each thread holds in
+// registers across phase 1 (the rest is copied after it).
+constexpr int kStageRegs = 2;
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
 
 // Finish of one assembled slot: a diagonal block (row s) with the
 // predictor, right-hand side, inertia and static contact, or an upper value
@@ -213,7 +219,8 @@ __device__ __forceinline__ void finish_slot(const Inst& I, bool is_diag, unsigne
 }
 
 struct AsmLds {
-    float rec[kAsmFloats];
+    // records (floats), then the entries, then the slot offsets (uint16)
+    __align__(16) unsigned char arena[CHX_ASM_LDS_BYTES];
     int sl_first[4];   // slices of the unit: first value slot, lane range, width
     int sl_l0[4];
     int sl_cnt[4];
@@ -230,8 +237,28 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
     const gp<const int> ud = rec_at((gp<const int>)I.units, unsigned(u), unsigned(sizeof(chx_unit) / sizeof(int)));
     const int r0 = ud[0], r1 = ud[1], e0 = ud[2];
     const int np = ud[3], nt = ud[4], nv = ud[5], nd = ud[6];
-    const unsigned slot0 = unsigned(ud[7]);
+    const int n_slots = ud[7], offs0 = ud[8], word0 = ud[9], n_words = ud[10];
     const int tid = int(threadIdx.x);
+    // LDS image: records, then entries at byte wofs, then offsets
+    const int rec_floats = CHX_ASM_PIN_REC * np + CHX_ASM_TRI_REC * nt + CHX_ASM_STVK_REC * nv + CHX_ASM_DIH_REC * nd;
+    const int wofs = (4 * rec_floats + 15) & ~15;
+    const int n_wvec = n_words >> 3, n_ovec = (n_slots + 8) >> 3;
+    v4u* const Wv = reinterpret_cast<v4u*>(L.arena + wofs);
+    v4u* const Ov = Wv + n_wvec;
+    const uint16_t* const Wl = reinterpret_cast<const uint16_t*>(Wv);
+    const uint16_t* const Ol = reinterpret_cast<const uint16_t*>(Ov);
+    // staging loads of entries + offsets (vector i < n_wvec: entries, then
+    // offsets), issued before the element work and stored after it
+    const gp<const v4u> wsrc = (gp<const v4u>)rec_at(I.contribs, unsigned(word0), 1u);
+    const gp<const v4u> osrc = (gp<const v4u>)rec_at(I.contrib_offsets, unsigned(offs0), 1u);
+    const int n_vec = n_wvec + n_ovec;
+    auto stage_src = [&](int i) -> v4u { return i < n_wvec ? wsrc[i] : osrc[i - n_wvec]; };
+    v4u held[kStageRegs];
+#pragma unroll
+    for (int k = 0; k < kStageRegs; ++k) {
+        const int i = tid + k * kBlock;
+        if (i < n_vec) held[k] = stage_src(i);
+    }
     // the unit's slices (whole slices, up to four, or part of one)
     const int s_lo = r0 / int(CHX_SLICE_ROWS), s_hi = (r1 - 1) / int(CHX_SLICE_ROWS);
     const int ns = s_hi - s_lo + 1;
@@ -250,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
         }
     }
     // ---- phase 1: records of the unit's elements into LDS ----
-    float* Rp = L.rec;
+    float* Rp = reinterpret_cast<float*>(L.arena);
     float* Rt = Rp + CHX_ASM_PIN_REC * np;
     float* Rv = Rt + CHX_ASM_TRI_REC * nt;
     float* Rd = Rv + CHX_ASM_STVK_REC * nv;
@@ -305,10 +332,16 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
             }
         }
     }
+    // entries and offsets into LDS: the held vectors, then the rest
+#pragma unroll
+    for (int k = 0; k < kStageRegs; ++k) {
+        const int i = tid + k * kBlock;
+        if (i < n_vec) Wv[i] = held[k];
+    }
+    for (int i = tid + kStageRegs * kBlock; i < n_vec; i += kBlock) Wv[i] = stage_src(i);
     __syncthreads();
     // ---- phase 2: one thread per owned slot ----
     const int R = r1 - r0;
-    const int n_slots = R + L.sl_end[ns - 1];
     for (int t = tid; t < n_slots; t += kBlock) {
         const bool is_diag = t < R;
         unsigned s;  // diagonal: row; upper: value slot
@@ -321,12 +354,12 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
             const int jj = v / L.sl_cnt[q], l = L.sl_l0[q] + v % L.sl_cnt[q];
             s = unsigned(L.sl_first[q] + jj * int(CHX_SLICE_ROWS) + l);
         }
-        const unsigned c0 = ldx(I.contrib_offsets, slot0 + unsigned(t)), c1 = ldx(I.contrib_offsets, slot0 + unsigned(t) + 1u);
+        const unsigned c0 = Ol[t], c1 = Ol[t + 1];
         float h[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         f3 grad = mk(0.f, 0.f, 0.f);
         for (unsigned c = c0; c < c1; ++c) {
-            const unsigned w = ldx(I.contribs, c);
-            const unsigned kind = w >> 30, a = (w >> 28) & 3u, b = (w >> 26) & 3u, l = w & CHX_CONTRIB_MAX_ELEMENT;
+            const unsigned w = Wl[c];
+            const unsigned kind = w >> 14, a = (w >> 12) & 3u, b = (w >> 10) & 3u, l = w & CHX_CONTRIB_MAX_LOCAL;
             if (kind == CHX_CONTRIB_PIN) {
                 const float k = Rp[3 * np + l];
                 h[0] += k;
@@ -521,7 +554,7 @@ __device__ __forceinline__ void spmv_core(Mat mat, int rb, int re, RoAt ro_at, C
 #define CHX_PCG_CI_CAP 4096
 #endif
 #ifndef CHX_PCG_WIN_CAP
-#define CHX_PCG_WIN_CAP 2816
+#define CHX_PCG_WIN_CAP 2048
 #endif
 #ifndef CHX_PCG_COMPACT_MAX
 #define CHX_PCG_COMPACT_MAX 1024
@@ -578,13 +611,19 @@ __device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot
 // indices and source-vector window (every row its blocks reference, from
 // chx_instance::windows) into LDS with independent coalesced loads, so the
 // SpMV's dependent chain row offset -> column -> gathered row runs in LDS and
-// only the block values come from HBM.  Stages beyond the LDS capacity (or
-// without a window) run the same SpMV from global memory.
+// only the block values come from HBM.  The source rows live in a ring of
+// kWinCap rows (row j at j mod kWinCap) that persists across the stages of
+// a tile: consecutive stages' windows overlap by the matrix's bandwidth on
+// each side, and a stage loads (and, in the single-barrier form, computes)
+// only the rows its predecessor did not hold.  Stages beyond the LDS
+// capacity (or without a window) run the same SpMV from global memory.
 // ---------------------------------------------------------------------------
 constexpr int kStage = CHX_STAGE_ROWS;
 static_assert(kStage == kBlock, "one thread per stage row for the row offsets");
 constexpr unsigned kCiCap = CHX_PCG_CI_CAP;   // column indices per staged stage
-constexpr unsigned kWinCap = CHX_PCG_WIN_CAP;  // source rows per staged stage
+constexpr unsigned kWinCap = CHX_PCG_WIN_CAP;  // source rows held (ring capacity)
+static_assert((kWinCap & (kWinCap - 1u)) == 0u, "the ring indexes rows modulo a power of two");
+constexpr unsigned kWinMask = kWinCap - 1u;
 constexpr int kChunk = 16;          // tiles whose instance scalars are summed together
 static_assert(kChunk * 16 == kBlock, "16 lanes per tile in chunk_sums");
 constexpr int kCompactMax = CHX_PCG_COMPACT_MAX;  // instances up to which the active tiles are compacted
@@ -629,12 +668,14 @@ __device__ __forceinline__ f3 lds3(const float* w, unsigned i) { return {w[3 * i
 
 // Loads stage [sb, se) of instance I into S (true), or leaves it to the
 // global path (false: no window, or over capacity).  srcg(j) is the source
-// vector entry j.  Uniform across the workgroup.
+// vector entry j.  The ring holds rows [rlo, rhi] on entry (rlo > rhi:
+// none) and the stage's window [lo, rhi] on return; only the window's rows
+// outside the old range are loaded.  Uniform across the workgroup.
 // SL: the column indices come from the sliced layout (slots of the stage's
 // slices) instead of the CSR order.
 template <int EL, class SrcG>
 __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG srcg, PcgStageLds& S,
-                                           unsigned& beg0, unsigned& lo) {
+                                           unsigned& beg0, unsigned& lo, unsigned& rlo, unsigned& rhi) {
     constexpr bool SL = EL != 0, E16 = EL == 2;
     if (I.windows == nullptr) return false;
     const unsigned rb0 = unsigned(ldx(I.row_offsets, unsigned(sb)));
@@ -647,10 +688,16 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
     const unsigned wlo = unsigned(ldx(I.windows, 2u * st)), whi = unsigned(ldx(I.windows, 2u * st + 1u));
     const unsigned W = whi - wlo + 1u;
     if (nnz_s > kCiCap || W > kWinCap) return false;
+    // window rows not in the ring: [wlo, a_end) below it, [b_beg, whi] above
+    const bool keep = rlo <= rhi && rlo <= whi && wlo <= rhi;
+    const unsigned a_end = keep ? min(whi + 1u, max(rlo, wlo)) : whi + 1u;
+    const unsigned b_beg = keep ? min(whi + 1u, max(wlo, rhi + 1u)) : whi + 1u;
+    const unsigned nA = a_end - wlo;
+    const unsigned nNew = nA + (whi + 1u - b_beg);
     const unsigned R = unsigned(se - sb);
     const unsigned tid = threadIdx.x;
     __syncthreads();  // the previous stage's readers are done
-    const unsigned iters = max(max((nnz_s + 2047u) / 2048u, (W + 1023u) / 1024u), 1u);
+    const unsigned iters = max(max((nnz_s + 2047u) / 2048u, (nNew + 1023u) / 1024u), 1u);
     for (unsigned it = 0; it < iters; ++it) {
         int cv[8];
         f3 wv[4];
@@ -662,10 +709,11 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
                             : ldx(cols, b0 + (k < nnz_s ? k : 0u));
             }
         }
+        auto new_row = [&](unsigned w) { return w < nA ? wlo + w : b_beg + (w - nA); };
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const unsigned w = it * 1024u + unsigned(u) * 256u + tid;
-            wv[u] = srcg(wlo + (w < W ? w : 0u));
+            wv[u] = srcg(new_row(w < nNew ? w : 0u));
         }
         if (it == 0 && tid < R) S.ro[tid] = ldx(I.row_offsets, unsigned(sb) + tid) - int(rb0);
         if (nnz_s > 0) {
@@ -681,10 +729,11 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const unsigned w = it * 1024u + unsigned(u) * 256u + tid;
-            if (w < W) {
-                S.win[3 * w] = wv[u].x;
-                S.win[3 * w + 1] = wv[u].y;
-                S.win[3 * w + 2] = wv[u].z;
+            if (w < nNew) {
+                const unsigned slot = new_row(w) & kWinMask;
+                S.win[3 * slot] = wv[u].x;
+                S.win[3 * slot + 1] = wv[u].y;
+                S.win[3 * slot + 2] = wv[u].z;
             }
         }
     }
@@ -696,6 +745,8 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
     __syncthreads();
     beg0 = b0;
     lo = wlo;
+    rlo = wlo;
+    rhi = whi;
     return true;
 }
 
@@ -728,10 +779,11 @@ __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re
         global_rows(rb, re);
     } else {
         PcgStageLds& T = S.st;
+        unsigned rlo = 1u, rhi = 0u;  // source rows the ring holds (none yet)
         for (int sb = rb; sb < re; sb += kStage) {
             const int se = min(re, sb + kStage);
             unsigned beg0 = 0, lo = 0;
-            if (stage_load<EL>(I, sb, se, srcg, T, beg0, lo)) {
+            if (stage_load<EL>(I, sb, se, srcg, T, beg0, lo, rlo, rhi)) {
                 if constexpr (SL) {
                     const unsigned s_lo = lo / kSlice;
                     spmv_sliced<E16>(
@@ -742,12 +794,12 @@ __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re
                         },
                         [&](unsigned sl) { return unsigned(T.uo[sl - s_lo]); },
                         [&](unsigned r) { return unsigned(T.ro[r - unsigned(sb) + 1u] - T.ro[r - unsigned(sb)]); },
-                        [&](unsigned j) { return lds3(T.win, j - lo); }, fin);
+                        [&](unsigned j) { return lds3(T.win, j & kWinMask); }, fin);
                 } else {
                     spmv_core<L>(
                         mat, sb, se, [&](unsigned r) { return unsigned(T.ro[r - unsigned(sb)]) + beg0; },
                         [&](unsigned k) { return unsigned(T.ci[k - beg0]); },
-                        [&](unsigned j) { return lds3(T.win, j - lo); }, fin);
+                        [&](unsigned j) { return lds3(T.win, j & kWinMask); }, fin);
                 }
             } else {
                 global_rows(sb, se);

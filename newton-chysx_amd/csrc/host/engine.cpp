@@ -10,10 +10,22 @@ namespace engine {
 
 namespace {
 
+// Host-side gather-map entry over global element indices: bits 31..30
+// kind, 29..28 a, 27..26 b, 25..0 the element index.  build_units turns it
+// into the device's 16-bit entry over unit-local positions.
+constexpr unsigned kMaxElement = (1u << 26) - 1u;
 unsigned encode(unsigned kind, int a, int b, std::size_t id) {
-    if (id > CHX_CONTRIB_MAX_ELEMENT)
+    if (id > kMaxElement)
         throw std::runtime_error("chysx::engine: element index exceeds the 26-bit gather-map field");
     return (kind << 30) | (unsigned(a) << 28) | (unsigned(b) << 26) | static_cast<unsigned>(id);
+}
+constexpr long pad8(long v) { return (v + 7) & ~7L; }
+// LDS record floats of an element of kind k (CHX_CONTRIB_*).
+constexpr long asm_rec_floats(int k) {
+    return k == int(CHX_CONTRIB_PIN) ? CHX_ASM_PIN_REC
+                                     : (k == int(CHX_CONTRIB_TRI) ? CHX_ASM_TRI_REC
+                                                                  : (k == int(CHX_CONTRIB_DIH) ? CHX_ASM_DIH_REC
+                                                                                               : CHX_ASM_STVK_REC));
 }
 
 template <class T> void upload(DeviceArray<T>& a, std::size_t& bytes, std::uintptr_t stream) {
@@ -86,23 +98,39 @@ void InstanceData::pack_elements(const InstanceSource& s) {
     }
 }
 
-// Assembly units (chx_unit): the rows are cut into units whose elements'
-// LDS records fit CHX_ASM_LDS_FLOATS -- up to four whole slices, or parts of
-// one slice when a slice alone does not fit -- and every unit gets the list
-// of the elements with a vertex in its rows (ascending per kind) and the
-// contribution lists of its slots (diagonal blocks of its rows, then its
-// upper value slots slice by slice, block index jj, then row), each entry's
-// element index replaced by the element's position in the unit's list of
-// that kind.  goff / gent: the per-slot lists over the slot order
-// [diagonal blocks 0..n) + [value slots n..n+n_upper) with global element
-// indices.
+// Assembly units (chx_unit): the rows are cut into units whose LDS image
+// fits CHX_ASM_LDS_BYTES -- the records of every element with a vertex in
+// the unit's rows, the contribution entries of the unit's slots and the
+// slots' offsets -- with at most CHX_CONTRIB_MAX_LOCAL + 1 elements of a
+// kind: up to four whole slices, or parts of one slice when a slice alone
+// does not fit.  Every unit gets the list of its elements (ascending per
+// kind) and the contribution lists of its slots (diagonal blocks of its
+// rows, then its upper value slots slice by slice, block index jj, then
+// row) as 16-bit entries whose element field is the element's position in
+// the unit's list of that kind; the entries and the offsets are padded to
+// 8-entry (16-byte) runs so the kernel stages them with 16-byte loads.
+// goff / gent: the per-slot lists over the slot order [diagonal blocks
+// 0..n) + [value slots n..n+n_upper) with global element indices.
 void InstanceData::build_units(const std::vector<unsigned>& goff, const std::vector<unsigned>& gent) {
     const int n = n_;
     constexpr int kKinds = 4;  // CHX_CONTRIB_PIN, _TRI, _DIH, _STVK
     static_assert(CHX_CONTRIB_PIN == 0 && CHX_CONTRIB_TRI == 1 && CHX_CONTRIB_DIH == 2 && CHX_CONTRIB_STVK == 3,
                   "kind codes index the per-kind tables");
-    const int rec_floats[kKinds] = {CHX_ASM_PIN_REC, CHX_ASM_TRI_REC, CHX_ASM_DIH_REC, CHX_ASM_STVK_REC};
     const std::size_t counts[kKinds] = {pins_.cpu_size(), tris_.cpu_size(), dihs_.cpu_size(), stvks_.cpu_size()};
+    const int SR = CHX_SLICE_ROWS;
+    const auto& uo = upper_offsets_.host();
+    // slots and contribution entries each row brings to its unit
+    std::vector<long> row_slots(static_cast<std::size_t>(n)), row_words(static_cast<std::size_t>(n));
+    for (int r = 0; r < n; ++r) {
+        const int sl = r / SR, width = (uo[sl + 1] - uo[sl]) / SR;
+        long w = goff[static_cast<std::size_t>(r) + 1] - goff[static_cast<std::size_t>(r)];
+        for (int jj = 0; jj < width; ++jj) {
+            const std::size_t g = static_cast<std::size_t>(n) + static_cast<std::size_t>(uo[sl] + jj * SR + (r - sl * SR));
+            w += goff[g + 1] - goff[g];
+        }
+        row_slots[static_cast<std::size_t>(r)] = 1 + width;
+        row_words[static_cast<std::size_t>(r)] = w;
+    }
     // vertex -> incident elements, per kind (CSR)
     std::vector<int> voff[kKinds], vel[kKinds];
     auto incidence = [&](int kind, int per, auto vertex_of) {
@@ -137,40 +165,59 @@ void InstanceData::build_units(const std::vector<unsigned>& goff, const std::vec
         mark[k].assign(counts[k], -1);
     }
     int token = 0;
-    // LDS floats the records of rows [ra, rb) add to unit uid
+    // What rows [ra, rb) add to unit uid: new elements per kind, slots, entries.
+    struct Need {
+        long elems[kKinds] = {0, 0, 0, 0};
+        long slots = 0, words = 0;
+        void add(const Need& o) {
+            for (int k = 0; k < kKinds; ++k) elems[k] += o.elems[k];
+            slots += o.slots;
+            words += o.words;
+        }
+        bool fits() const {
+            long floats = 0;
+            for (int k = 0; k < kKinds; ++k) {
+                if (elems[k] > static_cast<long>(CHX_CONTRIB_MAX_LOCAL) + 1) return false;
+                floats += elems[k] * asm_rec_floats(k);
+            }
+            return ((4 * floats + 15) & ~15L) + 2 * pad8(words) + 2 * pad8(slots + 1) <= CHX_ASM_LDS_BYTES;
+        }
+    };
     auto added = [&](int ra, int rb, int uid) {
         ++token;
-        long floats = 0;
-        for (int r = ra; r < rb; ++r)
+        Need d;
+        for (int r = ra; r < rb; ++r) {
+            d.slots += row_slots[static_cast<std::size_t>(r)];
+            d.words += row_words[static_cast<std::size_t>(r)];
             for (int k = 0; k < kKinds; ++k)
                 for (int q = voff[k][r]; q < voff[k][r + 1]; ++q) {
                     const int e = vel[k][q];
                     if (stamp[k][e] != uid && mark[k][e] != token) {
                         mark[k][e] = token;
-                        floats += rec_floats[k];
+                        ++d.elems[k];
                     }
                 }
-        return floats;
+        }
+        return d;
     };
     auto commit = [&](int ra, int rb, int uid) {
         for (int r = ra; r < rb; ++r)
             for (int k = 0; k < kKinds; ++k)
                 for (int q = voff[k][r]; q < voff[k][r + 1]; ++q) stamp[k][vel[k][q]] = uid;
     };
-    const long budget = CHX_ASM_LDS_FLOATS;
-    const int SR = CHX_SLICE_ROWS;
     const int slices = (n + SR - 1) / SR;
     std::vector<std::pair<int, int>> ranges;
     int u_r0 = -1, u_slices = 0;
-    long u_floats = 0;
+    Need u_need;
     for (int sl = 0; sl < slices; ++sl) {
         const int sa = sl * SR, sb = std::min(n, sa + SR);
         const int uid = static_cast<int>(ranges.size());
         if (u_r0 >= 0) {
-            const long add = added(sa, sb, uid);
-            if (u_slices < 4 && u_floats + add <= budget) {
+            Need t = u_need;
+            t.add(added(sa, sb, uid));
+            if (u_slices < 4 && t.fits()) {
                 commit(sa, sb, uid);
-                u_floats += add;
+                u_need = t;
                 ++u_slices;
                 continue;
             }
@@ -178,11 +225,11 @@ void InstanceData::build_units(const std::vector<unsigned>& goff, const std::vec
             u_r0 = -1;
         }
         const int nid = static_cast<int>(ranges.size());
-        const long whole = added(sa, sb, nid);
-        if (whole <= budget) {
+        const Need whole = added(sa, sb, nid);
+        if (whole.fits()) {
             commit(sa, sb, nid);
             u_r0 = sa;
-            u_floats = whole;
+            u_need = whole;
             u_slices = 1;
             continue;
         }
@@ -190,33 +237,33 @@ void InstanceData::build_units(const std::vector<unsigned>& goff, const std::vec
         for (int r = sa; r < sb;) {
             const int pid = static_cast<int>(ranges.size());
             const int p0 = r;
-            long pf = 0;
+            Need pn;
             while (r < sb) {
-                const long add = added(r, r + 1, pid);
-                if (pf + add > budget) break;
+                Need t = pn;
+                t.add(added(r, r + 1, pid));
+                if (!t.fits()) break;
                 commit(r, r + 1, pid);
-                pf += add;
+                pn = t;
                 ++r;
             }
             if (r == p0)
                 throw std::invalid_argument(
-                    "chysx::engine: the elements of one vertex exceed an assembly unit's LDS records "
-                    "(CHX_ASM_LDS_FLOATS)");
+                    "chysx::engine: the elements of one vertex exceed an assembly unit's LDS image "
+                    "(CHX_ASM_LDS_BYTES)");
             ranges.emplace_back(p0, r);
         }
     }
     if (u_r0 >= 0) ranges.emplace_back(u_r0, n);
 
-    const auto& uo = upper_offsets_.host();
     auto& units = units_.host();
     auto& uel = unit_elems_.host();
     auto& off = contrib_offsets_.host();
     auto& ent = contribs_.host();
     units.clear();
     uel.clear();
-    off.assign(1, 0u);
+    off.clear();
     ent.clear();
-    ent.reserve(gent.size());
+    ent.reserve(gent.size() + 8 * ranges.size());
     std::vector<int> local[kKinds];
     for (int k = 0; k < kKinds; ++k) local[k].assign(counts[k], -1);
     std::vector<int> list[kKinds];
@@ -226,7 +273,8 @@ void InstanceData::build_units(const std::vector<unsigned>& goff, const std::vec
         U.row_begin = r0;
         U.row_end = r1;
         U.elem_begin = static_cast<int>(uel.size());
-        U.slot_begin = static_cast<int>(off.size() - 1);
+        U.offs_begin = static_cast<int>(off.size());
+        U.word_begin = static_cast<int>(ent.size());
         ++token;
         for (int k = 0; k < kKinds; ++k) list[k].clear();
         for (int r = r0; r < r1; ++r)
@@ -238,30 +286,31 @@ void InstanceData::build_units(const std::vector<unsigned>& goff, const std::vec
                         list[k].push_back(e);
                     }
                 }
-        long floats = 0;
+        Need need;
         const int order[kKinds] = {CHX_CONTRIB_PIN, CHX_CONTRIB_TRI, CHX_CONTRIB_STVK, CHX_CONTRIB_DIH};
         for (int k : order) {
             std::sort(list[k].begin(), list[k].end());
             for (std::size_t i = 0; i < list[k].size(); ++i) local[k][list[k][i]] = static_cast<int>(i);
             uel.insert(uel.end(), list[k].begin(), list[k].end());
-            floats += static_cast<long>(list[k].size()) * rec_floats[k];
+            need.elems[k] = static_cast<long>(list[k].size());
         }
-        if (floats > budget) throw std::logic_error("chysx::engine: assembly unit over its LDS capacity");
         U.n_pins = static_cast<int>(list[CHX_CONTRIB_PIN].size());
         U.n_tris = static_cast<int>(list[CHX_CONTRIB_TRI].size());
         U.n_stvks = static_cast<int>(list[CHX_CONTRIB_STVK].size());
         U.n_dihs = static_cast<int>(list[CHX_CONTRIB_DIH].size());
+        const std::size_t wbase = ent.size();
         auto emit = [&](std::size_t g) {
+            off.push_back(static_cast<uint16_t>(ent.size() - wbase));
             for (unsigned c = goff[g]; c < goff[g + 1]; ++c) {
                 const unsigned w = gent[c];
                 const unsigned kind = w >> 30;
-                const int e = static_cast<int>(w & CHX_CONTRIB_MAX_ELEMENT);
+                const int e = static_cast<int>(w & kMaxElement);
                 const int l = local[kind][static_cast<std::size_t>(e)];
                 if (l < 0 || stamp[kind][static_cast<std::size_t>(e)] < 0)
                     throw std::logic_error("chysx::engine: contribution outside its assembly unit");
-                ent.push_back((w & ~CHX_CONTRIB_MAX_ELEMENT) | static_cast<unsigned>(l));
+                ent.push_back(static_cast<uint16_t>((kind << 14) | (((w >> 28) & 3u) << 12) |
+                                                    (((w >> 26) & 3u) << 10) | static_cast<unsigned>(l)));
             }
-            off.push_back(static_cast<unsigned>(ent.size()));
         };
         for (int r = r0; r < r1; ++r) emit(static_cast<std::size_t>(r));
         for (int sl = r0 / SR; sl <= (r1 - 1) / SR; ++sl) {
@@ -271,6 +320,14 @@ void InstanceData::build_units(const std::vector<unsigned>& goff, const std::vec
                 for (int r = a; r < b; ++r)
                     emit(static_cast<std::size_t>(n) + static_cast<std::size_t>(uo[sl] + jj * SR + (r - sl * SR)));
         }
+        U.n_slots = static_cast<int>(off.size()) - U.offs_begin;
+        need.slots = U.n_slots;
+        need.words = static_cast<long>(ent.size() - wbase);
+        if (!need.fits()) throw std::logic_error("chysx::engine: assembly unit over its LDS capacity");
+        off.push_back(static_cast<uint16_t>(ent.size() - wbase));
+        while (off.size() % 8 != 0) off.push_back(off.back());
+        while ((ent.size() - wbase) % 8 != 0) ent.push_back(0);
+        U.n_words = static_cast<int>(ent.size() - wbase);
         for (int k = 0; k < kKinds; ++k)
             for (int e : list[k]) local[k][e] = -1;
         units.push_back(U);
@@ -459,7 +516,7 @@ void InstanceData::build(const InstanceSource& s, std::uintptr_t stream) {
     if (!fits(static_cast<std::size_t>(n_) + CHX_BLOCK_GROUP, 36) || !fits(static_cast<std::size_t>(nup_), 36) ||
         !fits(static_cast<std::size_t>(npad_), 4) ||
         !fits(tris_.cpu_size(), sizeof(chx_tri)) || !fits(dihs_.cpu_size(), sizeof(chx_dih)) ||
-        !fits(pins_.cpu_size(), sizeof(chx_pin)) || !fits(ent.size(), sizeof(unsigned)) ||
+        !fits(pins_.cpu_size(), sizeof(chx_pin)) || !fits(ent.size(), sizeof(uint16_t)) ||
         !fits(stvks_.cpu_size(), sizeof(chx_stvk)) || !fits(unit_elems_.cpu_size(), sizeof(int)))
         throw std::invalid_argument(
             "chysx::engine: a per-instance array would reach 4 GiB (at most 119,304,647 vertices or off-diagonal "

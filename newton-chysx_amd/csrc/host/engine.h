@@ -108,7 +108,7 @@ private:
     DeviceArray<float> diag_, values_;
     DeviceArray<chx_unit> units_;
     DeviceArray<int> unit_elems_;
-    DeviceArray<unsigned> contrib_offsets_, contribs_;
+    DeviceArray<uint16_t> contrib_offsets_, contribs_;
     DeviceArray<float> rhs_, r_, z_, p0_, p1_, q_, q1_, minv_;
     DeviceArray<float> diag_c_;
     DeviceArray<uint16_t> values16_;
