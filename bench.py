@@ -42,7 +42,7 @@ def b_setup(n, nnz):
     return b_spmv(n, nnz) + 168 * n
 
 
-def kernel_breakdown(cloth, dt, stream, lanes, reps=5, pcg_form=0):
+def kernel_breakdown(cloth, dt, stream, lanes, reps=5, pcg_form=0, tuned=None):
     """Per-kernel times and algorithmic rates of one instance's substep:
     the assembly kernel (element forces and Hessians: x_n 12N + triangle 52T
     + dihedral 20D + pin 16P bytes; assembly: predictor 56N + rhs 28N +
@@ -63,6 +63,11 @@ def kernel_breakdown(cloth, dt, stream, lanes, reps=5, pcg_form=0):
     T, D, P = sim.num_fem_stretch_triangles(), sim.num_bending_dihedrals(), sim.num_pins()
     b = chysx.ClothBatch()
     b.set_lanes_per_row(lanes)
+    if tuned is not None:
+        pcg_form = tuned["pcg_form"]
+        b.set_lds_staging(tuned["lds_staging"])
+        b.set_tile_rows(tuned["tile_rows"])
+        b.set_grid_blocks(tuned["grid_blocks"])
     b.set_pcg_form(pcg_form)
     b.add_instance(sim)
     b.set_trace(1024)
@@ -600,6 +605,18 @@ def main():
         # One large instance (256x256 grid, E ~ U[1e4, 1e6], seed 1) through
         # ClothSimulator.step: the single-instance path at full occupancy.
         big = scenes.DeviceCloth(scenes.plain_large(), lanes_per_row=args.lanes)
+        tuned_big = None
+        if not args.no_autotune:
+            from chysx import autotune
+
+            tuned_big = autotune.tune_simulator(big.sim, dt, stream, log=log)
+            log(f"single-cloth autotune picked staging={int(tuned_big['lds_staging'])} "
+                f"tile_rows={tuned_big['tile_rows']} form={tuned_big['pcg_form']} grid={tuned_big['grid_blocks']}")
+            big = scenes.DeviceCloth(scenes.plain_large(), lanes_per_row=args.lanes)
+            big.sim.set_lds_staging(tuned_big["lds_staging"])
+            big.sim.set_tile_rows(tuned_big["tile_rows"])
+            big.sim.set_pcg_form(tuned_big["pcg_form"])
+            big.sim.set_grid_blocks(tuned_big["grid_blocks"])
         with torch.cuda.stream(stream):
             for _ in range(3):
                 big.sim.step(dt, sh)
@@ -613,7 +630,10 @@ def main():
         plain_large = {"value": big.spec.n * reps / bw, "unit": "vertex-substeps/s", "vertices": big.spec.n,
                        "ms_per_substep": bw / reps * 1e3, "pcg_iterations": big.sim.last_pcg_iterations(sh)}
         log(f"plain 256x256: {plain_large['ms_per_substep']:.2f} ms/substep")
-        plain_large["kernels"] = kernel_breakdown(big, dt, stream, args.lanes, pcg_form=args.pcg_form)
+        if tuned_big is not None:
+            plain_large["autotune"] = tuned_big
+        plain_large["kernels"] = kernel_breakdown(big, dt, stream, args.lanes, pcg_form=args.pcg_form,
+                                                  tuned=tuned_big)
         kb = plain_large["kernels"]
         log("plain 256x256 kernels: " + ", ".join(
             f"{name} {kb[name]['GB_per_s']} GB/s" for name in

@@ -119,3 +119,63 @@ def tune_batch(batch, dt: float, stream=None, lanes=(4,), blocks_per_cu=(1, 2, 3
         except (OSError, ValueError):
             pass
     return dict(result, cached=False)
+
+
+def tune_simulator(sim, dt: float, stream=None, staging=(True, False), tile_rows=(256, 512), forms=(0, 1),
+                   grid_fractions=(1.0, 0.5), warmup: int = 1, steps: int = 3, log=None) -> dict:
+    """The single-cloth path (ClothSimulator.step): sweeps the SpMV form, the
+    rows per PCG tile, the solver form (one or two grid barriers per
+    iteration) and the persistent grid (all tiles, or a fraction of them,
+    within the form's co-residency limit), timing whole substeps with HIP
+    events on the caller's stream, and applies the fastest setting to `sim`.
+    A single cloth has fewer tiles than the device has CUs at most sizes, so
+    the grid is bounded by the tile count rather than by occupancy.  The
+    sweep advances the cloth's state by every trial's substeps.  Returns
+    {"lds_staging", "tile_rows", "pcg_form", "grid_blocks", "ms_per_substep",
+    "table"}."""
+    import torch
+
+    from . import abi
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    sh = s.cuda_stream
+    with torch.cuda.stream(s):
+        sim.step(dt, sh)  # builds the solver topology: num_particles() is 0 before it
+    n = sim.num_particles()
+    table = []
+
+    def apply(r):
+        sim.set_lds_staging(r["lds_staging"])
+        sim.set_tile_rows(r["tile_rows"])
+        sim.set_pcg_form(r["pcg_form"])
+        sim.set_grid_blocks(r["grid_blocks"])
+
+    for form in forms:
+        for st in staging:
+            limit = abi.pcg_grid_limit(4, False, st)
+            for T in tile_rows:
+                tiles = max(1, (n + T - 1) // T)
+                for f in grid_fractions:
+                    g = max(1, min(limit, int(round(tiles * f))))
+                    if any(r["lds_staging"] == st and r["tile_rows"] == T and r["pcg_form"] == form and
+                           r["grid_blocks"] == g for r in table):
+                        continue
+                    r = {"lds_staging": st, "tile_rows": T, "pcg_form": form, "grid_blocks": g}
+                    apply(r)
+                    with torch.cuda.stream(s):
+                        for _ in range(warmup):
+                            sim.step(dt, sh)
+                        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        e0.record(s)
+                        for _ in range(steps):
+                            sim.step(dt, sh)
+                        e1.record(s)
+                        e1.synchronize()
+                    r["ms_per_substep"] = e0.elapsed_time(e1) / steps
+                    table.append(r)
+                    if log:
+                        log(f"autotune (single cloth) staging={int(st)} tile_rows={T} form={form} grid={g}: "
+                            f"{r['ms_per_substep']:.3f} ms/substep")
+    best = min(table, key=lambda r: r["ms_per_substep"])
+    apply(best)
+    return dict(best, table=table)

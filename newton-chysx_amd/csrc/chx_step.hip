@@ -94,6 +94,11 @@ constexpr int kTriRec = 24, kStvkRec = 36;
 // 16-byte vectors of a unit's entries and offsets each thread holds in
 // registers across phase 1 (the rest is copied after it).
 constexpr int kStageRegs = 2;
+// Lanes per diagonal block in the assembly's block phase (a power of two
+// dividing the wavefront, so a block's lanes share a wavefront).
+constexpr int kDiagLanes = 2;
+static_assert((kDiagLanes & (kDiagLanes - 1)) == 0 && kWave % kDiagLanes == 0 && kBlock % kDiagLanes == 0,
+              "diagonal lane groups stay inside a wavefront");
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
 
 // Finish of one assembled slot: a diagonal block (row s) with the
@@ -282,54 +287,80 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
     float* Rv = Rt + CHX_ASM_TRI_REC * nt;
     float* Rd = Rv + CHX_ASM_STVK_REC * nv;
     const gp<const float> pos = I.pos;
-    for (int e = tid; e < np + nt + nv + nd; e += kBlock) {
-        const unsigned id = unsigned(ldx(I.unit_elems, unsigned(e0 + e)));
-        if (e < np) {
-            const chx_pin pn = ldg(I.pins, id);
-            const f3 g = mul(sub(ld3g(pos, unsigned(pn.v)), ld3g(I.pin_targets, id)), pn.k);  // pin_constraint.cu:61-75
-            Rp[0 * np + e] = g.x;
-            Rp[1 * np + e] = g.y;
-            Rp[2 * np + e] = g.z;
-            Rp[3 * np + e] = pn.k;
-        } else if (e < np + nt) {
-            const int l = e - np;
-            const chx_tri t = ldg(I.tris, id);
-            float rec[kTriRec];
-            tri_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00,
-                       t.dm01, t.dm10, t.dm11, t.kA, rec);
+    // pins and StVK triangles (a few per cloth, or StVK cloths only): one
+    // element per thread
+    for (int l = tid; l < np; l += kBlock) {
+        const unsigned id = unsigned(ldx(I.unit_elems, unsigned(e0 + l)));
+        const chx_pin pn = ldg(I.pins, id);
+        const f3 g = mul(sub(ld3g(pos, unsigned(pn.v)), ld3g(I.pin_targets, id)), pn.k);  // pin_constraint.cu:61-75
+        Rp[0 * np + l] = g.x;
+        Rp[1 * np + l] = g.y;
+        Rp[2 * np + l] = g.z;
+        Rp[3 * np + l] = pn.k;
+    }
+    for (int l = tid; l < nv; l += kBlock) {
+        const unsigned id = unsigned(ldx(I.unit_elems, unsigned(e0 + np + nt + l)));
+        const chx_stvk t = ldg(I.stvks, id);
+        float rec[kStvkRec];
+        stvk_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00, t.dm01,
+                    t.dm10, t.dm11, t.muA, t.lamA, rec);
 #pragma unroll
-            for (int f = 0; f < 18; ++f) Rt[f * nt + l] = rec[f];
-            Rt[18 * nt + l] = t.dm00;
-            Rt[19 * nt + l] = t.dm01;
-            Rt[20 * nt + l] = t.dm10;
-            Rt[21 * nt + l] = t.dm11;
-        } else if (e < np + nt + nv) {
-            const int l = e - np - nt;
-            const chx_stvk t = ldg(I.stvks, id);
-            float rec[kStvkRec];
-            stvk_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00,
-                        t.dm01, t.dm10, t.dm11, t.muA, t.lamA, rec);
+        for (int f = 0; f < 27; ++f) Rv[f * nv + l] = rec[f];
+        Rv[27 * nv + l] = t.dm00;
+        Rv[28 * nv + l] = t.dm01;
+        Rv[29 * nv + l] = t.dm10;
+        Rv[30 * nv + l] = t.dm11;
+    }
+    // BW98 triangles and dihedrals
+    auto tri_out = [&](int l, const chx_tri& t, f3 x0, f3 x1, f3 x2) {
+        float rec[kTriRec];
+        tri_record(x0, x1, x2, t.dm00, t.dm01, t.dm10, t.dm11, t.kA, rec);
 #pragma unroll
-            for (int f = 0; f < 27; ++f) Rv[f * nv + l] = rec[f];
-            Rv[27 * nv + l] = t.dm00;
-            Rv[28 * nv + l] = t.dm01;
-            Rv[29 * nv + l] = t.dm10;
-            Rv[30 * nv + l] = t.dm11;
-        } else {
-            const int l = e - np - nt - nv;
-            const chx_dih d = ldg(I.dihs, id);
-            const DihFrame F = dihedral_frame(ld3g(pos, unsigned(d.v0)), ld3g(pos, unsigned(d.v1)),
-                                              ld3g(pos, unsigned(d.v2)), ld3g(pos, unsigned(d.v3)));
-            // An invalid frame contributes nothing (bending_constraint.cu:190-191,227-228).
-            Rd[0 * nd + l] = F.valid ? d.k * (F.theta - d.rest) : 0.0f;
-            Rd[1 * nd + l] = F.valid ? d.k : 0.0f;
+        for (int f = 0; f < 18; ++f) Rt[f * nt + l] = rec[f];
+        Rt[18 * nt + l] = t.dm00;
+        Rt[19 * nt + l] = t.dm01;
+        Rt[20 * nt + l] = t.dm10;
+        Rt[21 * nt + l] = t.dm11;
+    };
+    auto dih_out = [&](int l, const chx_dih& d, f3 x0, f3 x1, f3 x2, f3 x3) {
+        const DihFrame F = dihedral_frame(x0, x1, x2, x3);
+        // An invalid frame contributes nothing (bending_constraint.cu:190-191,227-228).
+        Rd[0 * nd + l] = F.valid ? d.k * (F.theta - d.rest) : 0.0f;
+        Rd[1 * nd + l] = F.valid ? d.k : 0.0f;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const f3 a = F.valid ? F.a[j] : mk(0.0f, 0.0f, 0.0f);
-                Rd[(2 + 3 * j) * nd + l] = a.x;
-                Rd[(3 + 3 * j) * nd + l] = a.y;
-                Rd[(4 + 3 * j) * nd + l] = a.z;
-            }
+        for (int j = 0; j < 4; ++j) {
+            const f3 a = F.valid ? F.a[j] : mk(0.0f, 0.0f, 0.0f);
+            Rd[(2 + 3 * j) * nd + l] = a.x;
+            Rd[(3 + 3 * j) * nd + l] = a.y;
+            Rd[(4 + 3 * j) * nd + l] = a.z;
+        }
+    };
+    if (nt > 0 && nd > 0) {
+        // one of each per thread and round, the two dependent load chains
+        // (list entry -> element -> positions) issued side by side; lanes
+        // past a kind's count redo its last element and store the same bits
+        // to its slot, so no branch separates (or sinks) the two chains
+        for (int l = tid; l < max(nt, nd); l += kBlock) {
+            const int lt = min(l, nt - 1), ld = min(l, nd - 1);
+            const unsigned idt = unsigned(ldx(I.unit_elems, unsigned(e0 + np + lt)));
+            const unsigned idd = unsigned(ldx(I.unit_elems, unsigned(e0 + np + nt + nv + ld)));
+            const chx_tri t = ldg(I.tris, idt);
+            const chx_dih d = ldg(I.dihs, idd);
+            const f3 t0 = ld3g(pos, unsigned(t.v0)), t1 = ld3g(pos, unsigned(t.v1)), t2 = ld3g(pos, unsigned(t.v2));
+            const f3 d0 = ld3g(pos, unsigned(d.v0)), d1 = ld3g(pos, unsigned(d.v1));
+            const f3 d2 = ld3g(pos, unsigned(d.v2)), d3 = ld3g(pos, unsigned(d.v3));
+            tri_out(lt, t, t0, t1, t2);
+            dih_out(ld, d, d0, d1, d2, d3);
+        }
+    } else {
+        for (int l = tid; l < nt; l += kBlock) {
+            const chx_tri t = ldg(I.tris, unsigned(ldx(I.unit_elems, unsigned(e0 + np + l))));
+            tri_out(l, t, ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)));
+        }
+        for (int l = tid; l < nd; l += kBlock) {
+            const chx_dih d = ldg(I.dihs, unsigned(ldx(I.unit_elems, unsigned(e0 + np + nt + nv + l))));
+            dih_out(l, d, ld3g(pos, unsigned(d.v0)), ld3g(pos, unsigned(d.v1)), ld3g(pos, unsigned(d.v2)),
+                    ld3g(pos, unsigned(d.v3)));
         }
     }
     // entries and offsets into LDS: the held vectors, then the rest
@@ -340,10 +371,17 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
     }
     for (int i = tid + kStageRegs * kBlock; i < n_vec; i += kBlock) Wv[i] = stage_src(i);
     __syncthreads();
-    // ---- phase 2: one thread per owned slot ----
+    // ---- phase 2: one work item per owned block, a diagonal block's list
+    // (pins, triangles and dihedrals of its vertex: the longest lists) split
+    // over kDiagLanes adjacent lanes -- lane q sums entries c0 + q,
+    // c0 + q + kDiagLanes, ... -- and combined by a fixed xor butterfly ----
     const int R = r1 - r0;
-    for (int t = tid; t < n_slots; t += kBlock) {
-        const bool is_diag = t < R;
+    const int n_items = kDiagLanes * R + (n_slots - R);
+    for (int it = tid; it < n_items; it += kBlock) {
+        const bool is_diag = it < kDiagLanes * R;
+        const int t = is_diag ? it / kDiagLanes : it - (kDiagLanes - 1) * R;  // the unit's slot
+        const unsigned lane_q = is_diag ? unsigned(it % kDiagLanes) : 0u;
+        const unsigned stride = is_diag ? unsigned(kDiagLanes) : 1u;
         unsigned s;  // diagonal: row; upper: value slot
         if (is_diag) {
             s = unsigned(r0 + t);
@@ -354,10 +392,10 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
             const int jj = v / L.sl_cnt[q], l = L.sl_l0[q] + v % L.sl_cnt[q];
             s = unsigned(L.sl_first[q] + jj * int(CHX_SLICE_ROWS) + l);
         }
-        const unsigned c0 = Ol[t], c1 = Ol[t + 1];
+        const unsigned c0 = Ol[t] + lane_q, c1 = Ol[t + 1];
         float h[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         f3 grad = mk(0.f, 0.f, 0.f);
-        for (unsigned c = c0; c < c1; ++c) {
+        for (unsigned c = c0; c < c1; c += stride) {
             const unsigned w = Wl[c];
             const unsigned kind = w >> 14, a = (w >> 12) & 3u, b = (w >> 10) & 3u, l = w & CHX_CONTRIB_MAX_LOCAL;
             if (kind == CHX_CONTRIB_PIN) {
@@ -408,6 +446,17 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
                 h[6] += (ai.z * aj.x) * k; h[7] += (ai.z * aj.y) * k; h[8] += (ai.z * aj.z) * k;
                 if (is_diag && a == b) grad = add(grad, mul(ai, r(0)));
             }
+        }
+        if (is_diag) {
+            // the diagonal group's lanes are adjacent and all active
+#pragma unroll
+            for (int m = 1; m < kDiagLanes; m <<= 1) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) h[k] += __shfl_xor(h[k], m);
+                grad = mk(grad.x + __shfl_xor(grad.x, m), grad.y + __shfl_xor(grad.y, m),
+                          grad.z + __shfl_xor(grad.z, m));
+            }
+            if (lane_q != 0u) continue;
         }
         finish_slot(I, is_diag, s, h, grad);
     }
