@@ -150,6 +150,25 @@ def tune_simulator(sim, dt: float, stream=None, staging=(True, False), tile_rows
         sim.set_pcg_form(r["pcg_form"])
         sim.set_grid_blocks(r["grid_blocks"])
 
+    def trial(r):
+        apply(r)
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                sim.step(dt, sh)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(steps):
+                sim.step(dt, sh)
+            e1.record(s)
+            e1.synchronize()
+        r["ms_per_substep"] = e0.elapsed_time(e1) / steps
+        table.append(r)
+        if log:
+            log(f"autotune (single cloth) staging={int(r['lds_staging'])} tile_rows={r['tile_rows']} "
+                f"form={r['pcg_form']} grid={r['grid_blocks']}: "
+                f"{r['ms_per_substep']:.3f} ms/substep")
+
+    seen = set()
     for form in forms:
         for st in staging:
             limit = abi.pcg_grid_limit(4, False, st)
@@ -157,25 +176,10 @@ def tune_simulator(sim, dt: float, stream=None, staging=(True, False), tile_rows
                 tiles = max(1, (n + T - 1) // T)
                 for f in grid_fractions:
                     g = max(1, min(limit, int(round(tiles * f))))
-                    if any(r["lds_staging"] == st and r["tile_rows"] == T and r["pcg_form"] == form and
-                           r["grid_blocks"] == g for r in table):
+                    if (form, st, T, g) in seen:
                         continue
-                    r = {"lds_staging": st, "tile_rows": T, "pcg_form": form, "grid_blocks": g}
-                    apply(r)
-                    with torch.cuda.stream(s):
-                        for _ in range(warmup):
-                            sim.step(dt, sh)
-                        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                        e0.record(s)
-                        for _ in range(steps):
-                            sim.step(dt, sh)
-                        e1.record(s)
-                        e1.synchronize()
-                    r["ms_per_substep"] = e0.elapsed_time(e1) / steps
-                    table.append(r)
-                    if log:
-                        log(f"autotune (single cloth) staging={int(st)} tile_rows={T} form={form} grid={g}: "
-                            f"{r['ms_per_substep']:.3f} ms/substep")
+                    seen.add((form, st, T, g))
+                    trial({"lds_staging": st, "tile_rows": T, "pcg_form": form, "grid_blocks": g})
     best = min(table, key=lambda r: r["ms_per_substep"])
     apply(best)
     return dict(best, table=table)
