@@ -116,9 +116,9 @@ typedef struct chx_contact_op {
  * unit's records (4 B a float), contribution entries (2 B each) and slot
  * offsets (2 B each) may use together: */
 #define CHX_ASM_PIN_REC 4
-#define CHX_ASM_TRI_REC 22
-#define CHX_ASM_DIH_REC 14
-#define CHX_ASM_STVK_REC 31
+#define CHX_ASM_TRI_REC 24
+#define CHX_ASM_DIH_REC 16
+#define CHX_ASM_STVK_REC 32
 #define CHX_ASM_LDS_BYTES 53248
 typedef struct chx_unit {
     int row_begin, row_end;          /* rows [row_begin, row_end) of the instance             */

@@ -81,11 +81,13 @@ __device__ __forceinline__ void reset_sync(unsigned* sync, bool all) {
 // reference's (triangle_stretch_constraint.cu:153-311,
 // bending_constraint.cu:180-245, pin_constraint.cu:61-92).
 // ---------------------------------------------------------------------------
-// LDS record fields.  Triangle: Gu, Gv (0..5), Mu, Mv upper triangles
-// (6..17), Dm^-1 (18..21).  StVK: Gu, Gv (0..5), Huu, Hvv upper triangles
-// (6..17), Huv row-major (18..26), Dm^-1 (27..30).  Dihedral: k (theta -
-// theta0), k, a_0..a_3 (2..13).  Pin: k (x - t), k.
-static_assert(CHX_ASM_TRI_REC == 22 && CHX_ASM_STVK_REC == 31 && CHX_ASM_DIH_REC == 14 && CHX_ASM_PIN_REC == 4,
+// LDS records, element-major (record of local element e at [e * REC]),
+// 16-byte aligned so they are written and read as 16-byte vectors.
+// Triangle (24): Dm^-1 (0..3), Mu, Mv upper triangles (4..15), Gu, Gv
+// (16..21).  StVK (32): Dm^-1 (0..3), Gu, Gv (4..9), Huu, Hvv upper
+// triangles (10..21), Huv row-major (22..30).  Dihedral (16): k (theta -
+// theta0), k (0, 1), a_0..a_3 (4..15).  Pin (4): k (x - t), k.
+static_assert(CHX_ASM_TRI_REC == 24 && CHX_ASM_STVK_REC == 32 && CHX_ASM_DIH_REC == 16 && CHX_ASM_PIN_REC == 4,
               "record layouts");
 static_assert(CHX_ASM_LDS_BYTES % 16 == 0, "16-byte staging");
 // Floats tri_record / stvk_record write (chx_elements.h): the LDS records
@@ -286,6 +288,7 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
     float* Rt = Rp + CHX_ASM_PIN_REC * np;
     float* Rv = Rt + CHX_ASM_TRI_REC * nt;
     float* Rd = Rv + CHX_ASM_STVK_REC * nv;
+    auto st4 = [](float* p, float a, float b, float c, float d) { *reinterpret_cast<float4*>(p) = make_float4(a, b, c, d); };
     const gp<const float> pos = I.pos;
     // pins and StVK triangles (a few per cloth, or StVK cloths only): one
     // element per thread
@@ -293,10 +296,7 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
         const unsigned id = unsigned(ldx(I.unit_elems, unsigned(e0 + l)));
         const chx_pin pn = ldg(I.pins, id);
         const f3 g = mul(sub(ld3g(pos, unsigned(pn.v)), ld3g(I.pin_targets, id)), pn.k);  // pin_constraint.cu:61-75
-        Rp[0 * np + l] = g.x;
-        Rp[1 * np + l] = g.y;
-        Rp[2 * np + l] = g.z;
-        Rp[3 * np + l] = pn.k;
+        st4(Rp + CHX_ASM_PIN_REC * l, g.x, g.y, g.z, pn.k);
     }
     for (int l = tid; l < nv; l += kBlock) {
         const unsigned id = unsigned(ldx(I.unit_elems, unsigned(e0 + np + nt + l)));
@@ -304,36 +304,40 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
         float rec[kStvkRec];
         stvk_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00, t.dm01,
                     t.dm10, t.dm11, t.muA, t.lamA, rec);
+        float* o = Rv + CHX_ASM_STVK_REC * l;
+        st4(o, t.dm00, t.dm01, t.dm10, t.dm11);
 #pragma unroll
-        for (int f = 0; f < 27; ++f) Rv[f * nv + l] = rec[f];
-        Rv[27 * nv + l] = t.dm00;
-        Rv[28 * nv + l] = t.dm01;
-        Rv[29 * nv + l] = t.dm10;
-        Rv[30 * nv + l] = t.dm11;
+        for (int q = 0; q < 7; ++q)
+            st4(o + 4 + 4 * q, rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], q < 6 ? rec[4 * q + 3] : 0.0f);
     }
     // BW98 triangles and dihedrals
     auto tri_out = [&](int l, const chx_tri& t, f3 x0, f3 x1, f3 x2) {
         float rec[kTriRec];
         tri_record(x0, x1, x2, t.dm00, t.dm01, t.dm10, t.dm11, t.kA, rec);
-#pragma unroll
-        for (int f = 0; f < 18; ++f) Rt[f * nt + l] = rec[f];
-        Rt[18 * nt + l] = t.dm00;
-        Rt[19 * nt + l] = t.dm01;
-        Rt[20 * nt + l] = t.dm10;
-        Rt[21 * nt + l] = t.dm11;
+        float* o = Rt + CHX_ASM_TRI_REC * l;
+        st4(o, t.dm00, t.dm01, t.dm10, t.dm11);
+        st4(o + 4, rec[6], rec[7], rec[8], rec[9]);
+        st4(o + 8, rec[10], rec[11], rec[12], rec[13]);
+        st4(o + 12, rec[14], rec[15], rec[16], rec[17]);
+        st4(o + 16, rec[0], rec[1], rec[2], rec[3]);
+        st4(o + 20, rec[4], rec[5], 0.0f, 0.0f);
     };
     auto dih_out = [&](int l, const chx_dih& d, f3 x0, f3 x1, f3 x2, f3 x3) {
         const DihFrame F = dihedral_frame(x0, x1, x2, x3);
         // An invalid frame contributes nothing (bending_constraint.cu:190-191,227-228).
-        Rd[0 * nd + l] = F.valid ? d.k * (F.theta - d.rest) : 0.0f;
-        Rd[1 * nd + l] = F.valid ? d.k : 0.0f;
+        float a[12];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const f3 a = F.valid ? F.a[j] : mk(0.0f, 0.0f, 0.0f);
-            Rd[(2 + 3 * j) * nd + l] = a.x;
-            Rd[(3 + 3 * j) * nd + l] = a.y;
-            Rd[(4 + 3 * j) * nd + l] = a.z;
+            const f3 v = F.valid ? F.a[j] : mk(0.0f, 0.0f, 0.0f);
+            a[3 * j] = v.x;
+            a[3 * j + 1] = v.y;
+            a[3 * j + 2] = v.z;
         }
+        float* o = Rd + CHX_ASM_DIH_REC * l;
+        st4(o, F.valid ? d.k * (F.theta - d.rest) : 0.0f, F.valid ? d.k : 0.0f, 0.0f, 0.0f);
+        st4(o + 4, a[0], a[1], a[2], a[3]);
+        st4(o + 8, a[4], a[5], a[6], a[7]);
+        st4(o + 12, a[8], a[9], a[10], a[11]);
     };
     if (nt > 0 && nd > 0) {
         // one of each per thread and round, the two dependent load chains
@@ -399,30 +403,33 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
             const unsigned w = Wl[c];
             const unsigned kind = w >> 14, a = (w >> 12) & 3u, b = (w >> 10) & 3u, l = w & CHX_CONTRIB_MAX_LOCAL;
             if (kind == CHX_CONTRIB_PIN) {
-                const float k = Rp[3 * np + l];
-                h[0] += k;
-                h[4] += k;
-                h[8] += k;
-                grad = add(grad, mk(Rp[l], Rp[np + l], Rp[2 * np + l]));
+                const float4 r = *reinterpret_cast<const float4*>(Rp + CHX_ASM_PIN_REC * l);
+                h[0] += r.w;
+                h[4] += r.w;
+                h[8] += r.w;
+                grad = add(grad, mk(r.x, r.y, r.z));
             } else if (kind == CHX_CONTRIB_TRI) {
-                auto r = [&](int f) { return Rt[f * nt + int(l)]; };
-                const float d00 = r(18), d01 = r(19), d10 = r(20), d11 = r(21);
-                const float dwu[3] = {-d00 - d10, d00, d10}, dwv[3] = {-d01 - d11, d01, d11};
+                const float4* r = reinterpret_cast<const float4*>(Rt + CHX_ASM_TRI_REC * l);
+                const float4 dm = r[0], q1 = r[1], q2 = r[2], q3 = r[3];
+                // Mu = (q1.x q1.y q1.z q1.w q2.x q2.y), Mv = (q2.z q2.w q3.x q3.y q3.z q3.w)
+                const float dwu[3] = {-dm.x - dm.z, dm.x, dm.z}, dwv[3] = {-dm.y - dm.w, dm.y, dm.w};
                 const float su = dwu[a] * dwu[b];
                 const float sv = dwv[a] * dwv[b];
-                const float m0 = su * r(6) + sv * r(12), m1 = su * r(7) + sv * r(13), m2 = su * r(8) + sv * r(14);
-                const float m3 = su * r(9) + sv * r(15), m4 = su * r(10) + sv * r(16), m5 = su * r(11) + sv * r(17);
+                const float m0 = su * q1.x + sv * q2.z, m1 = su * q1.y + sv * q2.w, m2 = su * q1.z + sv * q3.x;
+                const float m3 = su * q1.w + sv * q3.y, m4 = su * q2.x + sv * q3.z, m5 = su * q2.y + sv * q3.w;
                 h[0] += m0; h[1] += m1; h[2] += m2;
                 h[3] += m1; h[4] += m3; h[5] += m4;
                 h[6] += m2; h[7] += m4; h[8] += m5;
                 if (is_diag && a == b) {
+                    const float4 g0 = r[4], g1 = r[5];  // Gu = g0.xyz, Gv = (g0.w, g1.x, g1.y)
                     const float ua = dwu[a], va = dwv[a];
-                    grad = add(grad, mk(ua * r(0) + va * r(3), ua * r(1) + va * r(4), ua * r(2) + va * r(5)));
+                    grad = add(grad, mk(ua * g0.x + va * g0.w, ua * g0.y + va * g1.x, ua * g0.z + va * g1.y));
                 }
             } else if (kind == CHX_CONTRIB_STVK) {
-                auto r = [&](int f) { return Rv[f * nv + int(l)]; };
-                const float d00 = r(27), d01 = r(28), d10 = r(29), d11 = r(30);
-                const float dwu[3] = {-d00 - d10, d00, d10}, dwv[3] = {-d01 - d11, d01, d11};
+                const float* rv = Rv + CHX_ASM_STVK_REC * l;
+                auto r = [&](int f) { return rv[4 + f]; };
+                const float4 dm = *reinterpret_cast<const float4*>(rv);
+                const float dwu[3] = {-dm.x - dm.z, dm.x, dm.z}, dwv[3] = {-dm.y - dm.w, dm.y, dm.w};
                 const float wua = dwu[a], wva = dwv[a], wub = dwu[b], wvb = dwv[b];
                 const float su = wua * wub, sv = wva * wvb, suv = wua * wvb, svu = wva * wub;
                 // Huu / Hvv symmetric (xx xy xz yy yz zz), Huv full; Hvu = Huv^T.
@@ -437,14 +444,16 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
                 if (is_diag && a == b)
                     grad = add(grad, mk(wua * r(0) + wva * r(3), wua * r(1) + wva * r(4), wua * r(2) + wva * r(5)));
             } else {
-                auto r = [&](int f) { return Rd[f * nd + int(l)]; };
-                const float k = r(1);
-                const f3 ai = mk(r(2 + 3 * a), r(3 + 3 * a), r(4 + 3 * a));
-                const f3 aj = mk(r(2 + 3 * b), r(3 + 3 * b), r(4 + 3 * b));
+                const float* rd = Rd + CHX_ASM_DIH_REC * l;
+                const float k = rd[1];
+                const float* pa = rd + 4 + 3 * a;
+                const float* pb = rd + 4 + 3 * b;
+                const f3 ai = mk(pa[0], pa[1], pa[2]);
+                const f3 aj = mk(pb[0], pb[1], pb[2]);
                 h[0] += (ai.x * aj.x) * k; h[1] += (ai.x * aj.y) * k; h[2] += (ai.x * aj.z) * k;
                 h[3] += (ai.y * aj.x) * k; h[4] += (ai.y * aj.y) * k; h[5] += (ai.y * aj.z) * k;
                 h[6] += (ai.z * aj.x) * k; h[7] += (ai.z * aj.y) * k; h[8] += (ai.z * aj.z) * k;
-                if (is_diag && a == b) grad = add(grad, mul(ai, r(0)));
+                if (is_diag && a == b) grad = add(grad, mul(ai, rd[0]));
             }
         }
         if (is_diag) {
