@@ -2,6 +2,8 @@
 the inner PCG iterations (FP32 arithmetic) with a diagonal that absorbs
 their rounding, outer iterative refinement against the FP32 true residual.
 Checked against the full-FP64 oracle."""
+import os
+
 import numpy as np
 import pytest
 
@@ -284,8 +286,11 @@ def test_precision_report_converged(gpu):
     residual, whose norm does not exceed the right-hand side's.  The report
     also states the tolerance floor: the FP64 oracle at 1e-6 against FP64 at
     1e-10, the error no arithmetic can remove at this tolerance."""
-    n = 128
-    rep = _precision_run(n, 100, max_iter=40 * n, floor=True)
+    # CHYSX_PRECISION_N / CHYSX_PRECISION_STEPS select another size for a
+    # one-off report (the 512 x 512 floor run: N=512, STEPS=30).
+    n = int(os.environ.get("CHYSX_PRECISION_N", "128"))
+    steps = int(os.environ.get("CHYSX_PRECISION_STEPS", "100"))
+    rep = _precision_run(n, steps, max_iter=40 * n, floor=True)
     f32, b16 = rep["fp32"], rep["bf16_ir"]
     base = rep["fp32_oracle"]["rel_maxnorm_error_vs_fp64"]
     for name in ("fp32", "bf16_ir"):
