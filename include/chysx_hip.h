@@ -335,6 +335,12 @@ int chx_memcpy_d2d(void* dst, const void* src, size_t bytes, uintptr_t stream);
 /* Zero-fill with a kernel (safe inside stream capture). */
 int chx_zero(void* dst, size_t bytes, uintptr_t stream);
 int chx_stream_synchronize(uintptr_t stream);
+/* Host-side profiler ranges (roctx; shown by rocprofv3 --marker-trace),
+ * the role of the reference's NVTX ranges (ChysX/src/profile/nvtx_range.h:
+ * 58-100): a nested range per calling thread.  chx_cloth_step itself marks
+ * "step::hessian" (the assembly launch) and "step::pcg" (the PCG launch). */
+int chx_profile_range_push(const char* name);
+int chx_profile_range_pop(void);
 int chx_device_synchronize(void);
 
 /* ---- setup kernels -------------------------------------------------------- */

@@ -75,7 +75,9 @@ def build(verbose: bool = True) -> None:
         hip_objs = list(ex.map(_compile_hip, hip_srcs))
         host_objs = list(ex.map(lambda s: _compile_host(s, py_inc if s.endswith("bindings.cpp") else []), host_srcs))
     tmp = HIP_LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *hip_objs])
+    # roctx (host-side profiler ranges) from the ROCm install
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *hip_objs, f"-L{ROCM}/lib",
+          "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, HIP_LIB)
     tmp = NATIVE_LIB + ".tmp"
     _run([CXX, "-shared", "-fPIC", "-o", tmp, *host_objs, "-L", PKG, "-lchysx_hip",

@@ -41,6 +41,8 @@ _SIGS = {
     "chx_memcpy_d2d": (_I, [_P, _P, _S, _U]),
     "chx_zero": (_I, [_P, _S, _U]),
     "chx_stream_synchronize": (_I, [_U]),
+    "chx_profile_range_push": (_I, [ctypes.c_char_p]),
+    "chx_profile_range_pop": (_I, []),
     "chx_device_synchronize": (_I, []),
     "chx_triangle_rest_shape": (_I, [_P, _P, _I, _F, _F, _P, _P, _U]),
     "chx_dihedral_rest_angle": (_I, [_P, _P, _I, _P, _U]),

@@ -5,6 +5,8 @@
 #include <cstring>
 #include <string>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "chx_device.h"
 #include "chx_internal.h"
 
@@ -159,6 +161,17 @@ int chx_zero(void* dst, size_t bytes, uintptr_t stream) {
         }
         hip_check(hipGetLastError(), "zero kernel launch");
     });
+}
+
+int chx_profile_range_push(const char* name) {
+    return guarded([&] {
+        require(name != nullptr, "chx_profile_range_push: null name");
+        roctxRangePushA(name);
+    });
+}
+
+int chx_profile_range_pop(void) {
+    return guarded([&] { roctxRangePop(); });
 }
 
 int chx_stream_synchronize(uintptr_t stream) {

@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <mutex>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "chx_contact.h"
 #include "chx_device.h"
 #include "chx_elements.h"
@@ -2126,8 +2128,16 @@ int chx_cloth_step(const chx_cloth_batch* batch, uintptr_t stream) {
     return guarded([&] {
         validate_batch(batch);
         const hipStream_t s = as_stream(stream);
+        // host-side ranges named after the reference's step phases
+        // (cloth_simulator.cu:887-1253): the assembly kernel is its
+        // predictor, gradient and Hessian phases, the PCG launch its PCG and
+        // finalize phases
+        roctxRangePushA("step::hessian");
         assemble(*batch, s);
+        roctxRangePop();
+        roctxRangePushA("step::pcg");
         launch_pcg(*batch, 1, s);
+        roctxRangePop();
     });
 }
 

@@ -143,6 +143,17 @@ private:
     std::vector<T> host_;
 };
 
+// Host-side profiler range for the life of a scope (the reference's
+// CHYSX_NVTX_RANGE, ChysX/src/profile/nvtx_range.h:58-100; roctx here, shown
+// by rocprofv3 --marker-trace).  Never throws.
+class ProfileRange {
+public:
+    explicit ProfileRange(const char* name) noexcept { (void)chx_profile_range_push(name); }
+    ~ProfileRange() { (void)chx_profile_range_pop(); }
+    ProfileRange(const ProfileRange&) = delete;
+    ProfileRange& operator=(const ProfileRange&) = delete;
+};
+
 // Page-locked staging for small asynchronous host-to-device uploads (the
 // reference's pinned host mirrors, cuda_array.cu:41-62): a ring of two
 // buffers, each reused only after the copy that read it has run on its

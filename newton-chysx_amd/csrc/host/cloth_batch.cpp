@@ -48,6 +48,7 @@ void ClothBatch::finalize(float dt, std::uintptr_t cuda_stream) {
 
 void ClothBatch::step(float dt, std::uintptr_t cuda_stream) {
     if (!(dt > 0.0f)) throw std::invalid_argument("ClothBatch::step: dt must be positive");
+    ProfileRange range("chysx::cloth::batch_step");
     if (!bound_) finalize(dt, cuda_stream);
     else runner_.update_many(descriptors(dt, cuda_stream), cuda_stream);
     if (graph_) runner_.step_graph(cuda_stream);

@@ -335,9 +335,14 @@ void ClothSimulator::step(float dt, std::uintptr_t cuda_stream) {
             "chysx::cloth::ClothSimulator::step: pos / vel spans must be set via set_external_buffers() before "
             "stepping.");
     if (!(dt > 0.0f)) throw std::invalid_argument("chysx::cloth::ClothSimulator::step: dt must be positive");
+    ProfileRange range("chysx::cloth::step");  // cloth_simulator.cu:815
     resize_work_buffers(n, cuda_stream);
-    if (topology_dirty_ || H_num_block_rows_ != n) ensure_hessian_topology(cuda_stream);
-    else check_elements_current(cuda_stream);
+    if (topology_dirty_ || H_num_block_rows_ != n) {
+        ProfileRange rebuild("step::topology_rebuild");  // cloth_simulator.cu:1119
+        ensure_hessian_topology(cuda_stream);
+    } else {
+        check_elements_current(cuda_stream);
+    }
     engine::InstanceSource s = source();
     s.dt = dt;
     runner_.update_single(inst_.descriptor(s));

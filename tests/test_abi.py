@@ -33,3 +33,14 @@ def test_validation_errors_without_device():
 
 def test_pcg_workspace_size_monotone():
     assert abi.pcg_workspace_size(0) <= abi.pcg_workspace_size(1) < abi.pcg_workspace_size(100000)
+
+
+def test_profile_ranges_without_device():
+    """The roctx ranges (the reference's NVTX ranges) need no GPU: nested
+    push / pop succeed, a null name is an argument error."""
+    L = abi.lib()
+    assert L.chx_profile_range_push(b"chysx::cloth::step") == 0
+    assert L.chx_profile_range_push(b"step::hessian") == 0
+    assert L.chx_profile_range_pop() == 0
+    assert L.chx_profile_range_pop() == 0
+    assert L.chx_profile_range_push(None) == 1 and b"null name" in L.chx_last_error_message()
