@@ -687,6 +687,10 @@ constexpr unsigned kWinMask = kWinCap - 1u;
 constexpr int kChunk = 16;          // tiles whose instance scalars are summed together
 static_assert(kChunk * 16 == kBlock, "16 lanes per tile in chunk_sums");
 constexpr int kCompactMax = CHX_PCG_COMPACT_MAX;  // instances up to which the active tiles are compacted
+#ifndef CHX_PCG_SUM_DEPTH
+#define CHX_PCG_SUM_DEPTH 8
+#endif
+constexpr int kSumDepth = CHX_PCG_SUM_DEPTH;  // tiles per lane whose partials load together (instance sums)
 static_assert(kCompactMax <= 4 * kBlock, "build_active scans 4 instances per thread");
 
 // Shared by both SpMV forms.
@@ -757,7 +761,12 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
     const unsigned R = unsigned(se - sb);
     const unsigned tid = threadIdx.x;
     __syncthreads();  // the previous stage's readers are done
-    const unsigned iters = max(max((nnz_s + 2047u) / 2048u, (nNew + 1023u) / 1024u), 1u);
+    // 16-bit entries move as 32-bit pairs (a stage's entries are whole
+    // slices: an even count from an even index), 4096 per pass
+    constexpr unsigned kEntPass = E16 ? 4096u : 2048u;
+    const unsigned iters = max(max((nnz_s + kEntPass - 1u) / kEntPass, (nNew + 1023u) / 1024u), 1u);
+    const unsigned nw16 = nnz_s / 2u;
+    const gp<const unsigned> ent32 = (gp<const unsigned>)(gp<const uint16_t>(I.slice_entries16) + b0);
     for (unsigned it = 0; it < iters; ++it) {
         int cv[8];
         f3 wv[4];
@@ -765,8 +774,8 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const unsigned k = it * 2048u + unsigned(u) * 256u + tid;
-                cv[u] = E16 ? int(ldx(I.slice_entries16, b0 + (k < nnz_s ? k : 0u)))
-                            : ldx(cols, b0 + (k < nnz_s ? k : 0u));
+                if constexpr (E16) cv[u] = int(ldx(ent32, k < nw16 ? k : 0u));
+                else cv[u] = ldx(cols, b0 + (k < nnz_s ? k : 0u));
             }
         }
         auto new_row = [&](unsigned w) { return w < nA ? wlo + w : b_beg + (w - nA); };
@@ -780,9 +789,10 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const unsigned k = it * 2048u + unsigned(u) * 256u + tid;
-                if (k < nnz_s) {
-                    if (E16) reinterpret_cast<uint16_t*>(S.ci)[k] = uint16_t(cv[u]);
-                    else S.ci[k] = cv[u];
+                if constexpr (E16) {
+                    if (k < nw16) reinterpret_cast<unsigned*>(S.ci)[k] = unsigned(cv[u]);
+                } else {
+                    if (k < nnz_s) S.ci[k] = cv[u];
                 }
             }
         }
@@ -1447,10 +1457,14 @@ __device__ __forceinline__ void chunk_sums_n(const chx_cloth_batch& B, gp<const 
             tb = I.tile_begin;
             te = I.tile_end;
         }
-        for (int u0 = tb + l; u0 < te; u0 += 32) {
-            double v[2][NF];
+        // kSumDepth of the lane's tiles per pass: their loads are issued
+        // together (the partials were written on other XCDs, so each pass
+        // waits for the Infinity Cache), then added in ascending tile order
+        // -- the same sums as one tile at a time.
+        for (int u0 = tb + l; u0 < te; u0 += 16 * kSumDepth) {
+            double v[kSumDepth][NF];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < kSumDepth; ++h) {
                 const int u = u0 + 16 * h;
                 if (u < te) {
                     const gp<const double> p = rec_at(part, unsigned(u), unsigned(CHX_TILE_PARTIALS));
@@ -1459,7 +1473,7 @@ __device__ __forceinline__ void chunk_sums_n(const chx_cloth_batch& B, gp<const 
                 }
             }
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+            for (int h = 0; h < kSumDepth; ++h)
                 if (u0 + 16 * h < te) {
 #pragma unroll
                     for (int q = 0; q < NF; ++q) sum[q] += v[h][q];
