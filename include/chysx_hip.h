@@ -27,14 +27,17 @@
  * 119,304,647 block rows or off-diagonal blocks per matrix.
  * The matrices a cloth instance assembles for itself (chx_instance diag,
  * values, minv, diag_c, values16) and the preconditioner of
- * chx_pcg_solve_bsr3's workspace are stored grouped planar instead: blocks in
- * groups of CHX_BLOCK_GROUP (64), each group as nine planes of 64 words, i.e.
- * component c (row-major r*3+col) of block i at
- * [((i / 64) * 9 + c) * 64 + i % 64], arrays rounded up to whole groups.  A
- * wavefront reads one component of 64 consecutive blocks as 256 contiguous
- * bytes, and the nine components of a group are one contiguous run.  A cloth
- * instance's diagonal blocks and preconditioner are symmetric and keep six
- * planes per group (xx, xy, xz, yy, yz, zz).
+ * chx_pcg_solve_bsr3's workspace are stored in grouped paired planes
+ * instead: blocks in groups of CHX_BLOCK_GROUP (64), each group one
+ * contiguous run of components (row-major r*3+col) in planes of pairs --
+ * (c0,c1), (c2,c3), (c4,c5), (c6,c7) as 64 pairs each, then c8 as 64 words:
+ * component c < 8 of block i at [(i / 64) * 576 + (c / 2) * 128 +
+ * 2 * (i % 64) + c % 2], c8 at [(i / 64) * 576 + 512 + i % 64]; arrays
+ * rounded up to whole groups.  A wavefront reads one pair of 64 consecutive
+ * blocks as 512 contiguous bytes (five loads per block).  A cloth instance's
+ * diagonal blocks and preconditioner are symmetric and keep three planes of
+ * pairs per group, (xx,xy), (xz,yy), (yz,zz): component q at
+ * [(i / 64) * 384 + (q / 2) * 128 + 2 * (i % 64) + q % 2].
  */
 #ifndef CHYSX_HIP_H_
 #define CHYSX_HIP_H_
@@ -149,7 +152,7 @@ typedef struct chx_unit {
 
 /* Rows per slice of the sliced layout of the off-diagonal blocks. */
 #define CHX_SLICE_ROWS 64
-/* Blocks per group of the grouped planar block layout. */
+/* Blocks per group of the grouped block layout. */
 #define CHX_BLOCK_GROUP 64
 
 /* One cloth instance: every array is the instance's own device memory, with
@@ -178,10 +181,11 @@ typedef struct chx_instance {
     const float* inv_mass;           /* external inverse masses, or NULL (mass 1 kg)         */
     const int* row_offsets;          /* [n_rows+1]                                           */
     const int* col_indices;          /* [nnz]                                                */
-    float* diag;                     /* symmetric: upper triangles (xx xy xz yy yz zz) in    */
-                                     /* groups of 64 blocks, six planes each; room for 9 *   */
-                                     /* n_rows rounded up to 64 blocks                       */
-    float* values;                   /* grouped planar over the n_upper upper value slots    */
+    float* diag;                     /* symmetric: upper triangles in groups of 64 blocks,  */
+                                     /* three planes of pairs each; room for 9 * n_rows     */
+                                     /* rounded up to 64 blocks                              */
+    float* values;                   /* grouped paired planes over the n_upper upper value   */
+                                     /* slots                                                */
     const chx_pin* pins;
     const float* pin_targets;        /* [n_pins*3] AoS float3 pin targets                    */
     const chx_tri* tris;             /* stretch elements, then shear elements                */
@@ -201,9 +205,9 @@ typedef struct chx_instance {
     float* q1;                       /* second q buffer of the single-barrier solve          */
     float* minv;                     /* block-Jacobi preconditioner, symmetric like diag     */
     float* diag_c;                   /* BF16 precision mode, else NULL: the inner SpMV's     */
-    uint16_t* values16;              /* matrix, grouped planar FP32 compensated diagonal     */
-                                     /* D + sum_j (V_ij - bf16(V_ij)) and grouped planar     */
-                                     /* BF16 upper-block copies                              */
+    uint16_t* values16;              /* matrix, grouped paired-plane FP32 compensated        */
+                                     /* diagonal D + sum_j (V_ij - bf16(V_ij)) and BF16      */
+                                     /* upper-block copies in the same layout               */
     const chx_stvk* stvks;           /* StVK membrane elements, or NULL                      */
     const int* windows;              /* [ceil(n_rows/CHX_STAGE_ROWS)][2]: smallest and       */
                                      /* largest column (own rows included) of each stage of  */

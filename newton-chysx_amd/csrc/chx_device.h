@@ -150,32 +150,47 @@ struct Mat32G {
     }
 };
 
-// Grouped planar blocks: 3x3 blocks in groups of 64, each group stored as
-// nine planes of 64 words — component c (row-major r*3+col) of block i at
-// p[((i / 64) * 9 + c) * 64 + i % 64] (CHX_BLOCK_GROUP).  One load
-// instruction of a wavefront reads one component of 64 consecutive blocks
-// (256 contiguous bytes, not a 36-byte stride over ~9x more cache lines), and
-// the nine components of a group are one contiguous 2304-byte run, so the
-// nine loads of a block land in one DRAM page instead of nine planes far
-// apart.  Arrays hold whole groups (the block count rounded up to 64).
+// Grouped paired-plane blocks: 3x3 blocks in groups of 64, each group one
+// contiguous run of 2304 bytes holding the components in planes of pairs —
+// (c0,c1), (c2,c3), (c4,c5), (c6,c7) as 64 two-word pairs each, then c8 as
+// 64 words (component c row-major r*3+col; CHX_BLOCK_GROUP).  One load
+// instruction of a wavefront reads one pair of 64 consecutive blocks (512
+// contiguous bytes), so a block takes five loads instead of nine, and the
+// nine components of a group stay in one DRAM page.  Arrays hold whole
+// groups (the block count rounded up to 64).
 static_assert(CHX_BLOCK_GROUP == 64, "bp() indexes groups of 64 blocks");
 __device__ __forceinline__ unsigned bp(unsigned c, unsigned i) {
-    return (((i >> 6) * 9u + c) << 6) | (i & 63u);
+    const unsigned g = (i >> 6) * 576u, l = i & 63u;
+    return c < 8u ? g + (c >> 1) * 128u + 2u * l + (c & 1u) : g + 512u + l;
 }
 // Symmetric blocks (the cloth instances' diagonal blocks and block-Jacobi
 // preconditioner, symmetric by construction) keep only their upper
-// triangle: groups of 64 blocks as six planes xx, xy, xz, yy, yz, zz,
-// component q of block i at p[((i / 64) * 6 + q) * 64 + i % 64].
+// triangle: groups of 64 blocks as three planes of pairs (xx,xy), (xz,yy),
+// (yz,zz); component q of block i at p[(i / 64) * 384 + (q / 2) * 128 +
+// 2 * (i % 64) + q % 2].
 __device__ __forceinline__ unsigned bp6(unsigned q, unsigned i) {
-    return (((i >> 6) * 6u + q) << 6) | (i & 63u);
+    return (i >> 6) * 384u + (q >> 1) * 128u + 2u * (i & 63u) + (q & 1u);
 }
 // Row-major position of upper-triangle component q.
 __device__ __forceinline__ unsigned sym6_pos(unsigned q) { return q < 3u ? q : (q < 5u ? q + 1u : 8u); }
+// Two consecutive components (c even) of block i as one load: two floats,
+// or two BF16 words in one 32-bit load.
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ld_pair(gp<const float> m, unsigned off, float& a, float& b) {
+    const f2v v = *(gp<const f2v>)rec_at(m, off, 1u);
+    a = v.x;
+    b = v.y;
+}
+__device__ __forceinline__ void ld_pair(gp<const uint16_t> m, unsigned off, float& a, float& b) {
+    const unsigned v = *(gp<const unsigned>)rec_at(m, off, 1u);
+    a = bf16_to_f32(uint16_t(v & 0xffffu));
+    b = bf16_to_f32(uint16_t(v >> 16));
+}
 template <class T>
 __device__ __forceinline__ void ld6p(gp<const T> m, unsigned i, float (&a)[9]) {
     float u[6];
 #pragma unroll
-    for (unsigned q = 0; q < 6; ++q) u[q] = to_f32(ldx(m, bp6(q, i)));
+    for (unsigned q = 0; q < 6; q += 2) ld_pair(m, bp6(q, i), u[q], u[q + 1]);
     a[0] = u[0];
     a[1] = u[1];
     a[2] = u[2];
@@ -192,12 +207,16 @@ __device__ __forceinline__ f3 mv6p(gp<const T> m, unsigned i, f3 v) {
     ld6p(m, i, a);
     return mv9(a, v);
 }
-
+template <class T>
+__device__ __forceinline__ void ld9p(gp<const T> m, unsigned i, float (&a)[9]) {
+#pragma unroll
+    for (unsigned c = 0; c < 8; c += 2) ld_pair(m, bp(c, i), a[c], a[c + 1]);
+    a[8] = to_f32(ldx(m, bp(8u, i)));
+}
 template <class T>
 __device__ __forceinline__ f3 mv9p(gp<const T> m, unsigned i, f3 v) {
     float a[9];
-#pragma unroll
-    for (unsigned c = 0; c < 9; ++c) a[c] = to_f32(ldx(m, bp(c, i)));
+    ld9p(m, i, a);
     return mv9(a, v);
 }
 // Block i, or its transpose when tr (lower blocks of the symmetric layout
@@ -205,8 +224,7 @@ __device__ __forceinline__ f3 mv9p(gp<const T> m, unsigned i, f3 v) {
 template <class T>
 __device__ __forceinline__ f3 mv9p_t(gp<const T> m, unsigned i, f3 v, bool tr) {
     float a[9];
-#pragma unroll
-    for (unsigned c = 0; c < 9; ++c) a[c] = to_f32(ldx(m, bp(c, i)));
+    ld9p(m, i, a);
     const float b[9] = {a[0], tr ? a[3] : a[1], tr ? a[6] : a[2], tr ? a[1] : a[3], a[4],
                         tr ? a[7] : a[5], tr ? a[2] : a[6], tr ? a[5] : a[7], a[8]};
     return mv9(b, v);

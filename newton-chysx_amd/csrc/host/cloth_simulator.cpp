@@ -380,12 +380,13 @@ double ClothSimulator::last_relative_residual(std::uintptr_t cuda_stream) {
 }
 
 namespace {
-// Device blocks are grouped planar (component c of block i at
-// p[((i / 64) * 9 + c) * 64 + i % 64], whole groups of CHX_BLOCK_GROUP); the
-// reference's debug taps return row-major 3x3 blocks.
+// Device blocks are grouped paired planes (per group of CHX_BLOCK_GROUP
+// blocks: components (c0,c1) .. (c6,c7) as 64 pairs each, then c8 as 64
+// words); the reference's debug taps return row-major 3x3 blocks.
 inline std::size_t grouped(int c, std::size_t i) {
-    const std::size_t G = CHX_BLOCK_GROUP;
-    return ((i / G) * 9 + static_cast<std::size_t>(c)) * G + i % G;
+    const std::size_t G = CHX_BLOCK_GROUP, g = (i / G) * 9 * G, l = i % G;
+    const std::size_t cc = static_cast<std::size_t>(c);
+    return cc < 8 ? g + (cc / 2) * 2 * G + 2 * l + cc % 2 : g + 8 * G + l;
 }
 inline std::size_t group_floats(std::size_t count) {
     return 9 * ((count + CHX_BLOCK_GROUP - 1) / CHX_BLOCK_GROUP * CHX_BLOCK_GROUP);
@@ -399,7 +400,7 @@ void copy_planar_blocks(T* out, const T* dev, int count, const char* what) {
             out[static_cast<std::size_t>(9) * i + c] = planes[grouped(c, static_cast<std::size_t>(i))];
 }
 // Symmetric blocks (the diagonal) keep their upper triangle: groups of 64
-// blocks as six planes xx, xy, xz, yy, yz, zz.
+// blocks as three planes of pairs (xx,xy), (xz,yy), (yz,zz).
 template <class T>
 void copy_sym_blocks(T* out, const T* dev, int count, const char* what) {
     const std::size_t G = CHX_BLOCK_GROUP;
@@ -410,7 +411,8 @@ void copy_sym_blocks(T* out, const T* dev, int count, const char* what) {
     for (int i = 0; i < count; ++i) {
         const std::size_t ii = static_cast<std::size_t>(i);
         for (int q = 0; q < 6; ++q) {
-            const T v = planes[((ii / G) * 6 + static_cast<std::size_t>(q)) * G + ii % G];
+            const std::size_t qq = static_cast<std::size_t>(q);
+            const T v = planes[(ii / G) * 6 * G + (qq / 2) * 2 * G + 2 * (ii % G) + qq % 2];
             out[9 * ii + 3 * pos[q][0] + pos[q][1]] = v;
             out[9 * ii + 3 * pos[q][1] + pos[q][0]] = v;
         }
