@@ -186,6 +186,16 @@ __device__ __forceinline__ void ld_pair(gp<const uint16_t> m, unsigned off, floa
     a = bf16_to_f32(uint16_t(v & 0xffffu));
     b = bf16_to_f32(uint16_t(v >> 16));
 }
+// Stores of the same pairs (assembly).
+__device__ __forceinline__ void st_pair(gp<float> m, unsigned off, float a, float b) {
+    f2v v;
+    v.x = a;
+    v.y = b;
+    *(gp<f2v>)rec_at(m, off, 1u) = v;
+}
+__device__ __forceinline__ void st_pair(gp<uint16_t> m, unsigned off, uint16_t a, uint16_t b) {
+    *(gp<unsigned>)rec_at(m, off, 1u) = unsigned(a) | (unsigned(b) << 16);
+}
 template <class T>
 __device__ __forceinline__ void ld6p(gp<const T> m, unsigned i, float (&a)[9]) {
     float u[6];

@@ -215,14 +215,17 @@ __device__ __forceinline__ void finish_slot(const Inst& I, bool is_diag, unsigne
         // Symmetric grouped planar blocks (bp6): the upper triangle of the
         // block row's diagonal block.
 #pragma unroll
-        for (unsigned q = 0; q < 6; ++q) stx(I.diag, bp6(q, row), h[sym6_pos(q)]);
+        for (unsigned q = 0; q < 6; q += 2) st_pair(I.diag, bp6(q, row), h[sym6_pos(q)], h[sym6_pos(q + 1)]);
     } else {
         const unsigned slot = s;
 #pragma unroll
-        for (unsigned k = 0; k < 9; ++k) stx(I.values, bp(k, slot), h[k]);
+        for (unsigned k = 0; k < 8; k += 2) st_pair(I.values, bp(k, slot), h[k], h[k + 1]);
+        stx(I.values, bp(8u, slot), h[8]);
         if (I.values16 != nullptr) {
 #pragma unroll
-            for (unsigned k = 0; k < 9; ++k) stx(I.values16, bp(k, slot), f32_to_bf16(h[k]));
+            for (unsigned k = 0; k < 8; k += 2)
+                st_pair(I.values16, bp(k, slot), f32_to_bf16(h[k]), f32_to_bf16(h[k + 1]));
+            stx(I.values16, bp(8u, slot), f32_to_bf16(h[8]));
         }
     }
 }
@@ -1136,7 +1139,8 @@ __device__ __forceinline__ void residual_tile(const chx_cloth_batch& B, const In
             inverse9(d9, mi);
             if constexpr (SL) {  // symmetric: the upper triangle
 #pragma unroll
-                for (unsigned q = 0; q < 6; ++q) stx(I.minv, bp6(q, row), mi[sym6_pos(q)]);
+                for (unsigned q = 0; q < 6; q += 2)
+                    st_pair(I.minv, bp6(q, row), mi[sym6_pos(q)], mi[sym6_pos(q + 1)]);
             } else {
 #pragma unroll
                 for (unsigned k = 0; k < 9; ++k) stx(I.minv, bp(k, row), mi[k]);
