@@ -1593,10 +1593,13 @@ __global__ __launch_bounds__(kBlock) void pcg1_kernel(chx_cloth_batch B, int fin
                         rn = mk(fmaf(-alpha, q.x, r.x), fmaf(-alpha, q.y, r.y), fmaf(-alpha, q.z, r.z));
                         zn = mv6p(minv, j, rn);
                     };
-                    // own-row update: r_k, x_k; returns z_k
-                    auto own = [&](unsigned row, f3& zn) {
-                        f3 rn;
-                        rz(row, rn, zn);
+                    // own-row update: r_k, x_k; returns z_k and the row's
+                    // M^-1 block (the row finish reuses it for M^-1 q_k)
+                    auto own = [&](unsigned row, f3& zn, float (&mi)[9]) {
+                        const f3 r = ld3g(r_prev, row), q = ld3g(q_prev, row);
+                        const f3 rn = mk(fmaf(-alpha, q.x, r.x), fmaf(-alpha, q.y, r.y), fmaf(-alpha, q.z, r.z));
+                        ld6p(minv, row, mi);
+                        zn = mv9(mi, rn);
                         const f3 pp = ld3g(p_prev, row);
                         f3 x = ld3g(I.x, row);
                         x = mk(fmaf(alpha, pp.x, x.x), fmaf(alpha, pp.y, x.y), fmaf(alpha, pp.z, x.z));
@@ -1614,18 +1617,20 @@ __global__ __launch_bounds__(kBlock) void pcg1_kernel(chx_cloth_batch B, int fin
                         };
                         tile_spmv<4, ST, EL>(I, PcgMats<1>::a(I), rb, re, S, pj, [&](unsigned row, f3 Ap, f3 pr) {
                             f3 zn;
-                            own(row, zn);
+                            float mi[9];
+                            own(row, zn, mi);
                             st3g(p_cur, row, pr);
                             st3g(q_cur, row, Ap);
                             acc[2] += dot64(pr, Ap);
                             acc[3] += dot64(Ap, zn);
-                            acc[4] += dot64(Ap, mv6p(minv, row, Ap));
+                            acc[4] += dot64(Ap, mv9(mi, Ap));
                         });
                     } else {
                         // at the cap: the last update only
                         for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
                             f3 zn;
-                            own(row, zn);
+                            float mi[9];
+                            own(row, zn, mi);
                         }
                     }
                 }
