@@ -95,11 +95,14 @@ __device__ __forceinline__ void st3(P p, long i, f3 v) {
     p[3 * i + 2] = v.z;
 }
 
-// 3x3 row-major block times vector: m[r*3+c].
+// 3x3 row-major block times vector: m[r*3+c].  The multiply-adds are
+// spelled out (no contraction left to the compiler), so every kernel that
+// applies a block -- staged or global SpMV, batch or single instance --
+// rounds it the same way.
 template <class P>
 __device__ __forceinline__ f3 mv9(P m, f3 v) {
-    return {m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
-            m[6] * v.x + m[7] * v.y + m[8] * v.z};
+    return {fmaf(m[2], v.z, fmaf(m[1], v.y, m[0] * v.x)), fmaf(m[5], v.z, fmaf(m[4], v.y, m[3] * v.x)),
+            fmaf(m[8], v.z, fmaf(m[7], v.y, m[6] * v.x))};
 }
 
 // Adjugate / determinant inverse without a singular guard
