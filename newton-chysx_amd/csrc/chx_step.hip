@@ -718,7 +718,7 @@ constexpr unsigned kWinSlices = kWinCap / CHX_SLICE_ROWS + 2;  // slices a windo
 struct PcgStageLds {
     int ro[kStage + 1];
     int ci[kCiCap];
-    float win[3 * kWinCap];
+    float4 win[kWinCap];  // source rows, one 16-byte slot each (one LDS access per row)
     int uo[kWinSlices];  // sliced layout: first value slot of the window's slices
 };
 
@@ -731,7 +731,10 @@ struct PcgLds : PcgShared {
 template <>
 struct PcgLds<false> : PcgShared {};
 
-__device__ __forceinline__ f3 lds3(const float* w, unsigned i) { return {w[3 * i], w[3 * i + 1], w[3 * i + 2]}; }
+__device__ __forceinline__ f3 lds3(const float4* w, unsigned i) {
+    const float4 v = w[i];
+    return {v.x, v.y, v.z};
+}
 
 // Loads stage [sb, se) of instance I into S (true), or leaves it to the
 // global path (false: no window, or over capacity).  srcg(j) is the source
@@ -804,9 +807,7 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
             const unsigned w = it * 1024u + unsigned(u) * 256u + tid;
             if (w < nNew) {
                 const unsigned slot = new_row(w) & kWinMask;
-                S.win[3 * slot] = wv[u].x;
-                S.win[3 * slot + 1] = wv[u].y;
-                S.win[3 * slot + 2] = wv[u].z;
+                S.win[slot] = make_float4(wv[u].x, wv[u].y, wv[u].z, 0.0f);
             }
         }
     }
