@@ -83,12 +83,17 @@ __device__ __forceinline__ void reset_sync(unsigned* sync, bool all) {
 // reference's (triangle_stretch_constraint.cu:153-311,
 // bending_constraint.cu:180-245, pin_constraint.cu:61-92).
 // ---------------------------------------------------------------------------
-// LDS records, element-major (record of local element e at [e * REC]),
-// 16-byte aligned so they are written and read as 16-byte vectors.
+// LDS records of a unit, kind by kind, quad-major: 16-byte quad q of
+// local element e of a kind with n elements at quad q * n + e, so a wave's
+// lanes writing quad q of consecutive elements hit consecutive quads (every
+// bank once) and lanes reading quad q of scattered elements spread over
+// all banks (an element-major stride of 4, 6 or 8 quads would keep them on
+// a quarter, half or eighth of the banks).
 // Triangle (24): Dm^-1 (0..3), Mu, Mv upper triangles (4..15), Gu, Gv
 // (16..21).  StVK (32): Dm^-1 (0..3), Gu, Gv (4..9), Huu, Hvv upper
-// triangles (10..21), Huv row-major (22..30).  Dihedral (16): k (theta -
-// theta0), k (0, 1), a_0..a_3 (4..15).  Pin (4): k (x - t), k.
+// triangles (10..21), Huv row-major (22..30).  Dihedral (16): quad j holds
+// a_j and, in its last float, k (theta - theta0) (j = 0), k (j = 1) or 0,
+// so a block (a, b) reads quads a and b whole.  Pin (4): k (x - t), k.
 static_assert(CHX_ASM_TRI_REC == 24 && CHX_ASM_STVK_REC == 32 && CHX_ASM_DIH_REC == 16 && CHX_ASM_PIN_REC == 4,
               "record layouts");
 static_assert(CHX_ASM_LDS_BYTES % 16 == 0, "16-byte staging");
@@ -289,11 +294,15 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
         }
     }
     // ---- phase 1: records of the unit's elements into LDS ----
-    float* Rp = reinterpret_cast<float*>(L.arena);
-    float* Rt = Rp + CHX_ASM_PIN_REC * np;
-    float* Rv = Rt + CHX_ASM_TRI_REC * nt;
-    float* Rd = Rv + CHX_ASM_STVK_REC * nv;
-    auto st4 = [](float* p, float a, float b, float c, float d) { *reinterpret_cast<float4*>(p) = make_float4(a, b, c, d); };
+    float4* const Rp = reinterpret_cast<float4*>(L.arena);
+    float4* const Rt = Rp + (CHX_ASM_PIN_REC / 4) * np;
+    float4* const Rv = Rt + (CHX_ASM_TRI_REC / 4) * nt;
+    float4* const Rd = Rv + (CHX_ASM_STVK_REC / 4) * nv;
+    auto st4 = [](float4* p, float a, float b, float c, float d) { *p = make_float4(a, b, c, d); };
+    // float f of a record (quad f / 4 at f / 4 * n + l)
+    auto rf = [](const float4* R, int n, unsigned l, unsigned f) {
+        return reinterpret_cast<const float*>(R + (f >> 2) * unsigned(n) + l)[f & 3u];
+    };
     const gp<const float> pos = I.pos;
     // pins and StVK triangles (a few per cloth, or StVK cloths only): one
     // element per thread
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
         const unsigned id = unsigned(ldx(I.unit_elems, unsigned(e0 + l)));
         const chx_pin pn = ldg(I.pins, id);
         const f3 g = mul(sub(ld3g(pos, unsigned(pn.v)), ld3g(I.pin_targets, id)), pn.k);  // pin_constraint.cu:61-75
-        st4(Rp + CHX_ASM_PIN_REC * l, g.x, g.y, g.z, pn.k);
+        st4(Rp + l, g.x, g.y, g.z, pn.k);
     }
     for (int l = tid; l < nv; l += kBlock) {
         const unsigned id = unsigned(ldx(I.unit_elems, unsigned(e0 + np + nt + l)));
@@ -309,23 +318,22 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
         float rec[kStvkRec];
         stvk_record(ld3g(pos, unsigned(t.v0)), ld3g(pos, unsigned(t.v1)), ld3g(pos, unsigned(t.v2)), t.dm00, t.dm01,
                     t.dm10, t.dm11, t.muA, t.lamA, rec);
-        float* o = Rv + CHX_ASM_STVK_REC * l;
-        st4(o, t.dm00, t.dm01, t.dm10, t.dm11);
+        st4(Rv + l, t.dm00, t.dm01, t.dm10, t.dm11);
 #pragma unroll
         for (int q = 0; q < 7; ++q)
-            st4(o + 4 + 4 * q, rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], q < 6 ? rec[4 * q + 3] : 0.0f);
+            st4(Rv + (q + 1) * nv + l, rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], q < 6 ? rec[4 * q + 3] : 0.0f);
     }
     // BW98 triangles and dihedrals
     auto tri_out = [&](int l, const chx_tri& t, f3 x0, f3 x1, f3 x2) {
         float rec[kTriRec];
         tri_record(x0, x1, x2, t.dm00, t.dm01, t.dm10, t.dm11, t.kA, rec);
-        float* o = Rt + CHX_ASM_TRI_REC * l;
+        float4* o = Rt + l;
         st4(o, t.dm00, t.dm01, t.dm10, t.dm11);
-        st4(o + 4, rec[6], rec[7], rec[8], rec[9]);
-        st4(o + 8, rec[10], rec[11], rec[12], rec[13]);
-        st4(o + 12, rec[14], rec[15], rec[16], rec[17]);
-        st4(o + 16, rec[0], rec[1], rec[2], rec[3]);
-        st4(o + 20, rec[4], rec[5], 0.0f, 0.0f);
+        st4(o + nt, rec[6], rec[7], rec[8], rec[9]);
+        st4(o + 2 * nt, rec[10], rec[11], rec[12], rec[13]);
+        st4(o + 3 * nt, rec[14], rec[15], rec[16], rec[17]);
+        st4(o + 4 * nt, rec[0], rec[1], rec[2], rec[3]);
+        st4(o + 5 * nt, rec[4], rec[5], 0.0f, 0.0f);
     };
     auto dih_out = [&](int l, const chx_dih& d, f3 x0, f3 x1, f3 x2, f3 x3) {
         const DihFrame F = dihedral_frame(x0, x1, x2, x3);
@@ -338,11 +346,11 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
             a[3 * j + 1] = v.y;
             a[3 * j + 2] = v.z;
         }
-        float* o = Rd + CHX_ASM_DIH_REC * l;
-        st4(o, F.valid ? d.k * (F.theta - d.rest) : 0.0f, F.valid ? d.k : 0.0f, 0.0f, 0.0f);
-        st4(o + 4, a[0], a[1], a[2], a[3]);
-        st4(o + 8, a[4], a[5], a[6], a[7]);
-        st4(o + 12, a[8], a[9], a[10], a[11]);
+        float4* o = Rd + l;
+        st4(o, a[0], a[1], a[2], F.valid ? d.k * (F.theta - d.rest) : 0.0f);
+        st4(o + nd, a[3], a[4], a[5], F.valid ? d.k : 0.0f);
+        st4(o + 2 * nd, a[6], a[7], a[8], 0.0f);
+        st4(o + 3 * nd, a[9], a[10], a[11], 0.0f);
     };
     if (nt > 0 && nd > 0) {
         // one of each per thread and round, the two dependent load chains
@@ -408,14 +416,14 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
             const unsigned w = Wl[c];
             const unsigned kind = w >> 14, a = (w >> 12) & 3u, b = (w >> 10) & 3u, l = w & CHX_CONTRIB_MAX_LOCAL;
             if (kind == CHX_CONTRIB_PIN) {
-                const float4 r = *reinterpret_cast<const float4*>(Rp + CHX_ASM_PIN_REC * l);
+                const float4 r = Rp[l];
                 h[0] += r.w;
                 h[4] += r.w;
                 h[8] += r.w;
                 grad = add(grad, mk(r.x, r.y, r.z));
             } else if (kind == CHX_CONTRIB_TRI) {
-                const float4* r = reinterpret_cast<const float4*>(Rt + CHX_ASM_TRI_REC * l);
-                const float4 dm = r[0], q1 = r[1], q2 = r[2], q3 = r[3];
+                const float4* r = Rt + l;
+                const float4 dm = r[0], q1 = r[nt], q2 = r[2 * nt], q3 = r[3 * nt];
                 // Mu = (q1.x q1.y q1.z q1.w q2.x q2.y), Mv = (q2.z q2.w q3.x q3.y q3.z q3.w)
                 const float dwu[3] = {-dm.x - dm.z, dm.x, dm.z}, dwv[3] = {-dm.y - dm.w, dm.y, dm.w};
                 const float su = dwu[a] * dwu[b];
@@ -426,14 +434,13 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
                 h[3] += m1; h[4] += m3; h[5] += m4;
                 h[6] += m2; h[7] += m4; h[8] += m5;
                 if (is_diag && a == b) {
-                    const float4 g0 = r[4], g1 = r[5];  // Gu = g0.xyz, Gv = (g0.w, g1.x, g1.y)
+                    const float4 g0 = r[4 * nt], g1 = r[5 * nt];  // Gu = g0.xyz, Gv = (g0.w, g1.x, g1.y)
                     const float ua = dwu[a], va = dwv[a];
                     grad = add(grad, mk(ua * g0.x + va * g0.w, ua * g0.y + va * g1.x, ua * g0.z + va * g1.y));
                 }
             } else if (kind == CHX_CONTRIB_STVK) {
-                const float* rv = Rv + CHX_ASM_STVK_REC * l;
-                auto r = [&](int f) { return rv[4 + f]; };
-                const float4 dm = *reinterpret_cast<const float4*>(rv);
+                auto r = [&](int f) { return rf(Rv, nv, l, unsigned(4 + f)); };
+                const float4 dm = Rv[l];
                 const float dwu[3] = {-dm.x - dm.z, dm.x, dm.z}, dwv[3] = {-dm.y - dm.w, dm.y, dm.w};
                 const float wua = dwu[a], wva = dwv[a], wub = dwu[b], wvb = dwv[b];
                 const float su = wua * wub, sv = wva * wvb, suv = wua * wvb, svu = wva * wub;
@@ -449,16 +456,14 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(chx_cloth_batch B) {
                 if (is_diag && a == b)
                     grad = add(grad, mk(wua * r(0) + wva * r(3), wua * r(1) + wva * r(4), wua * r(2) + wva * r(5)));
             } else {
-                const float* rd = Rd + CHX_ASM_DIH_REC * l;
-                const float k = rd[1];
-                const float* pa = rd + 4 + 3 * a;
-                const float* pb = rd + 4 + 3 * b;
-                const f3 ai = mk(pa[0], pa[1], pa[2]);
-                const f3 aj = mk(pb[0], pb[1], pb[2]);
+                const float4 qa = Rd[a * unsigned(nd) + l], qb = Rd[b * unsigned(nd) + l];
+                const float k = rf(Rd, nd, l, 7u);
+                const f3 ai = mk(qa.x, qa.y, qa.z);
+                const f3 aj = mk(qb.x, qb.y, qb.z);
                 h[0] += (ai.x * aj.x) * k; h[1] += (ai.x * aj.y) * k; h[2] += (ai.x * aj.z) * k;
                 h[3] += (ai.y * aj.x) * k; h[4] += (ai.y * aj.y) * k; h[5] += (ai.y * aj.z) * k;
                 h[6] += (ai.z * aj.x) * k; h[7] += (ai.z * aj.y) * k; h[8] += (ai.z * aj.z) * k;
-                if (is_diag && a == b) grad = add(grad, mul(ai, rd[0]));
+                if (is_diag && a == b) grad = add(grad, mul(ai, rf(Rd, nd, l, 3u)));
             }
         }
         if (is_diag) {
