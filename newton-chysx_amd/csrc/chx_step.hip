@@ -23,6 +23,7 @@
 // two-barrier form, Q_* for the single-barrier form).
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include <rocprofiler-sdk-roctx/roctx.h>
 
@@ -636,9 +637,15 @@ constexpr unsigned kSlice = CHX_SLICE_ROWS;
 static_assert(CHX_SLICE_ROWS == kWave, "one lane per row of a slice");
 static_assert(CHX_STAGE_ROWS % CHX_SLICE_ROWS == 0, "stages hold whole slices");
 
-template <bool E16, class Mat, class SlotAt, class EntAt, class UoffAt, class LenAt, class Src, class Fin>
+// No per-row prefetch: fin(row, Ax, x_row).
+struct NoPre {
+    __device__ int operator()(unsigned) const { return 0; }
+};
+
+template <bool E16, class Mat, class SlotAt, class EntAt, class UoffAt, class LenAt, class Src, class Fin,
+          class Pre = NoPre>
 __device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot_at, EntAt ent_at, UoffAt uoff_at,
-                                            LenAt len_at, Src src, Fin fin) {
+                                            LenAt len_at, Src src, Fin fin, Pre pre = Pre{}) {
     constexpr int KS = kSliceBatch;
     const unsigned lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x >> 6;
@@ -652,6 +659,7 @@ __device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot
         const bool valid = row < unsigned(re);
         const unsigned rowc = valid ? row : unsigned(s0);
         const unsigned len = valid ? len_at(rowc) : 0u;
+        const auto pv = pre(rowc);  // in flight with the row's block loads
         const f3 xr = src(rowc);
         f3 acc = mat.diag(rowc, xr);
         for (unsigned j0 = 0; j0 < width; j0 += KS) {
@@ -673,7 +681,10 @@ __device__ __forceinline__ void spmv_sliced(Mat mat, int rb, int re, SlotAt slot
             for (int u = 0; u < KS; ++u)
                 if (j0 + unsigned(u) < len) acc = add(acc, part[u]);
         }
-        if (valid) fin(rowc, acc, xr);
+        if (valid) {
+            if constexpr (std::is_same_v<Pre, NoPre>) fin(rowc, acc, xr);
+            else fin(rowc, acc, xr, pv);
+        }
     }
 }
 
@@ -837,9 +848,11 @@ __device__ __forceinline__ bool stage_load(const Inst& I, int sb, int se, SrcG s
 // time, or straight from global memory; SL: sliced layout (cloth
 // instances), else CSR with L lanes per row (caller-provided row-major
 // blocks).
-template <int L, bool ST, int EL, class Mat, class SrcG, class Fin>
+// pre(row) (sliced layout only): the row's loads for fin, issued before
+// its block loads; fin then takes it as a fourth argument.
+template <int L, bool ST, int EL, class Mat, class SrcG, class Fin, class Pre = NoPre>
 __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re, PcgLds<ST>& S, SrcG srcg,
-                                          Fin fin) {
+                                          Fin fin, Pre pre = Pre{}) {
     constexpr bool SL = EL != 0, E16 = EL == 2;
     auto global_rows = [&](int b, int e) {
         if constexpr (SL) {
@@ -851,7 +864,7 @@ __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re
                 },
                 [&](unsigned sl) { return unsigned(ldx(I.upper_offsets, sl)); },
                 [&](unsigned r) { return unsigned(ldx(I.row_offsets, r + 1u) - ldx(I.row_offsets, r)); }, srcg,
-                fin);
+                fin, pre);
         } else {
             spmv_core<L>(
                 mat, b, e, [&](unsigned r) { return unsigned(ldx(I.row_offsets, r)); },
@@ -877,7 +890,7 @@ __device__ __forceinline__ void tile_spmv(const Inst& I, Mat mat, int rb, int re
                         },
                         [&](unsigned sl) { return unsigned(T.uo[sl - s_lo]); },
                         [&](unsigned r) { return unsigned(T.ro[r - unsigned(sb) + 1u] - T.ro[r - unsigned(sb)]); },
-                        [&](unsigned j) { return lds3(T.win, j & kWinMask); }, fin);
+                        [&](unsigned j) { return lds3(T.win, j & kWinMask); }, fin, pre);
                 } else {
                     spmv_core<L>(
                         mat, sb, se, [&](unsigned r) { return unsigned(T.ro[r - unsigned(sb)]) + beg0; },
@@ -1505,6 +1518,13 @@ __device__ __forceinline__ void chunk_sums_n(const chx_cloth_batch& B, gp<const 
     __syncthreads();
 }
 
+// Loads of a row's own update in the single-barrier iteration: r, q, p and x
+// of the previous parity and the row's M^-1 block.
+struct OwnRow {
+    f3 r, q, p, x;
+    float mi[9];
+};
+
 template <int EL, bool ST>
 __global__ __launch_bounds__(kBlock) void pcg1_kernel(chx_cloth_batch B, int finalize) {
     __shared__ PcgLds<ST> S;
@@ -1603,16 +1623,23 @@ __global__ __launch_bounds__(kBlock) void pcg1_kernel(chx_cloth_batch B, int fin
                         rn = mk(fmaf(-alpha, q.x, r.x), fmaf(-alpha, q.y, r.y), fmaf(-alpha, q.z, r.z));
                         zn = mv6p(minv, j, rn);
                     };
-                    // own-row update: r_k, x_k; returns z_k and the row's
-                    // M^-1 block (the row finish reuses it for M^-1 q_k)
-                    auto own = [&](unsigned row, f3& zn, float (&mi)[9]) {
-                        const f3 r = ld3g(r_prev, row), q = ld3g(q_prev, row);
-                        const f3 rn = mk(fmaf(-alpha, q.x, r.x), fmaf(-alpha, q.y, r.y), fmaf(-alpha, q.z, r.z));
-                        ld6p(minv, row, mi);
-                        zn = mv9(mi, rn);
-                        const f3 pp = ld3g(p_prev, row);
-                        f3 x = ld3g(I.x, row);
-                        x = mk(fmaf(alpha, pp.x, x.x), fmaf(alpha, pp.y, x.y), fmaf(alpha, pp.z, x.z));
+                    // the own row's loads (issued with its block loads) and
+                    // its update: r_k, x_k; returns z_k (the row finish
+                    // reuses the row's M^-1 block for M^-1 q_k)
+                    auto own_loads = [&](unsigned row) {
+                        OwnRow o;
+                        o.r = ld3g(r_prev, row);
+                        o.q = ld3g(q_prev, row);
+                        ld6p(minv, row, o.mi);
+                        o.p = ld3g(p_prev, row);
+                        o.x = ld3g(I.x, row);
+                        return o;
+                    };
+                    auto own = [&](unsigned row, const OwnRow& o, f3& zn) {
+                        const f3 rn =
+                            mk(fmaf(-alpha, o.q.x, o.r.x), fmaf(-alpha, o.q.y, o.r.y), fmaf(-alpha, o.q.z, o.r.z));
+                        zn = mv9(o.mi, rn);
+                        const f3 x = mk(fmaf(alpha, o.p.x, o.x.x), fmaf(alpha, o.p.y, o.x.y), fmaf(alpha, o.p.z, o.x.z));
                         st3g(I.x, row, x);
                         st3g(r_cur, row, rn);
                         acc[0] += dot64(rn, zn);
@@ -1625,22 +1652,23 @@ __global__ __launch_bounds__(kBlock) void pcg1_kernel(chx_cloth_batch B, int fin
                             const f3 pp = ld3g(p_prev, j);
                             return mk(fmaf(beta, pp.x, zn.x), fmaf(beta, pp.y, zn.y), fmaf(beta, pp.z, zn.z));
                         };
-                        tile_spmv<4, ST, EL>(I, PcgMats<1>::a(I), rb, re, S, pj, [&](unsigned row, f3 Ap, f3 pr) {
-                            f3 zn;
-                            float mi[9];
-                            own(row, zn, mi);
-                            st3g(p_cur, row, pr);
-                            st3g(q_cur, row, Ap);
-                            acc[2] += dot64(pr, Ap);
-                            acc[3] += dot64(Ap, zn);
-                            acc[4] += dot64(Ap, mv9(mi, Ap));
-                        });
+                        tile_spmv<4, ST, EL>(
+                            I, PcgMats<1>::a(I), rb, re, S, pj,
+                            [&](unsigned row, f3 Ap, f3 pr, const OwnRow& o) {
+                                f3 zn;
+                                own(row, o, zn);
+                                st3g(p_cur, row, pr);
+                                st3g(q_cur, row, Ap);
+                                acc[2] += dot64(pr, Ap);
+                                acc[3] += dot64(Ap, zn);
+                                acc[4] += dot64(Ap, mv9(o.mi, Ap));
+                            },
+                            own_loads);
                     } else {
                         // at the cap: the last update only
                         for (unsigned row = unsigned(rb) + threadIdx.x; row < unsigned(re); row += blockDim.x) {
                             f3 zn;
-                            float mi[9];
-                            own(row, zn, mi);
+                            own(row, own_loads(row), zn);
                         }
                     }
                 }
