@@ -139,7 +139,7 @@ typedef struct chx_unit {
 #define CHX_TILE_PARTIALS 16
 
 /* 32-bit words of the grid-barrier / control area `sync` of a batch. */
-#define CHX_SYNC_WORDS 960
+#define CHX_SYNC_WORDS 3008
 
 /* Default rows per PCG row tile (the unit of workgroup assignment and of the
  * FP64 partial sums; the batch may use any multiple of CHX_STAGE_ROWS),

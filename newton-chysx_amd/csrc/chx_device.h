@@ -494,10 +494,13 @@ enum : int {
     SYNC_WAITING = 259,
     SYNC_SNAP = 320,    // two snapshots of SYNC_ACTIVE, 64 words apart
     SYNC_XCOUNT = 448,  // arrival counters of the 8 workgroup groups, 64 words apart
+    SYNC_FLAGS = 960,   // release flags (the generation), one per 128-byte line
     SYNC_WORDS = CHX_SYNC_WORDS
 };
 constexpr unsigned kSyncGroups = 8;  // workgroups g = x (mod 8): one XCD each
-static_assert(SYNC_SNAP + 64 < SYNC_XCOUNT && SYNC_XCOUNT + 64 * kSyncGroups <= CHX_SYNC_WORDS,
+constexpr unsigned kSyncFlags = 64, kFlagStride = 32;  // workgroup g polls flag g mod 64
+static_assert(SYNC_FLAGS + kFlagStride * kSyncFlags <= CHX_SYNC_WORDS && kSyncFlags == 64, "release flags");
+static_assert(SYNC_SNAP + 64 < SYNC_XCOUNT && SYNC_XCOUNT + 64 * kSyncGroups <= SYNC_FLAGS,
               "sync area too small");
 // The barrier is self-resetting: the last arriving workgroup zeroes the
 // count and bumps the generation, so consecutive launches reuse it.  Every
@@ -553,6 +556,15 @@ __device__ __forceinline__ void backoff(unsigned& level) {
 // counter; the last of the groups acquires all of them before it releases
 // the generation, so every workgroup's writes reach every other one.
 //
+// Release fans out the same way: the releasing workgroup's first wave writes
+// the new generation into 64 flags, one per 128-byte line, after its
+// release fence, and workgroup g polls flag g mod 64 -- four pollers a line
+// instead of the whole grid on one word, whose loads the memory side also
+// serves one after another (with one word the last workgroup saw the
+// release ~2 us after the first on a grid of 256).  The wait bound is
+// checked on every poll, the abort word (one line for the grid) on every
+// 16th.
+//
 // With snap = 0 or 1 the last arriver also copies SYNC_ACTIVE into snapshot
 // slot `snap` before it releases the generation, and every workgroup reads
 // that slot back into *active (uniform across the workgroup).  The live
@@ -566,40 +578,50 @@ __device__ __forceinline__ bool grid_sync(const Barrier& bar, unsigned nblocks, 
     __shared__ unsigned s_active;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < kWave) {
+        const unsigned lane = threadIdx.x;
+        unsigned* const flag = &sync[SYNC_FLAGS + kFlagStride * (blockIdx.x % kSyncFlags)];
         int ok = 1;
-        // The generation is read before the release fence, whose wait for all
-        // outstanding memory operations completes the read before this
-        // workgroup arrives.  (A read issued after the fence could still be in
-        // flight -- queued behind the pollers of the same word -- when the
-        // last arrival bumps the generation; it would return the next
-        // generation and this workgroup would wait one barrier too long,
-        // stalling all others at the next barrier.)
-        const unsigned gen = ld_relaxed(&sync[SYNC_GEN]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned x = blockIdx.x % kSyncGroups;
-        const unsigned groups = nblocks < kSyncGroups ? nblocks : kSyncGroups;
-        const unsigned gsize = (nblocks - x + kSyncGroups - 1u) / kSyncGroups;  // workgroups of group x
-        unsigned* xc = &sync[SYNC_XCOUNT + 64 * x];
         bool last = false;
-        if (__hip_atomic_fetch_add(xc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1u) {
-            st_relaxed(xc, 0u);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        unsigned gen = 0;
+        if (lane == 0) {
+            // The generation is read before the release fence, whose wait for
+            // all outstanding memory operations completes the read before this
+            // workgroup arrives.  (A read issued after the fence could still
+            // be in flight when the last arrival bumps the generation; it
+            // would return the next generation and this workgroup would wait
+            // one barrier too long, stalling all others at the next barrier.)
+            gen = ld_relaxed(flag);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            last = __hip_atomic_fetch_add(&sync[SYNC_COUNT], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-                   groups - 1u;
+            const unsigned x = blockIdx.x % kSyncGroups;
+            const unsigned groups = nblocks < kSyncGroups ? nblocks : kSyncGroups;
+            const unsigned gsize = (nblocks - x + kSyncGroups - 1u) / kSyncGroups;  // workgroups of group x
+            unsigned* xc = &sync[SYNC_XCOUNT + 64 * x];
+            if (__hip_atomic_fetch_add(xc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1u) {
+                st_relaxed(xc, 0u);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                last = __hip_atomic_fetch_add(&sync[SYNC_COUNT], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                       groups - 1u;
+            }
+            if (last) {
+                st_relaxed(&sync[SYNC_COUNT], 0u);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (snap >= 0) st_relaxed(&sync[SYNC_SNAP + 64 * snap], ld_relaxed(&sync[SYNC_ACTIVE]));
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                st_relaxed(&sync[SYNC_GEN], gen + 1u);
+            }
         }
+        last = __builtin_amdgcn_readfirstlane(last ? 1 : 0) != 0;
+        gen = __builtin_amdgcn_readfirstlane(gen);
         if (last) {
-            st_relaxed(&sync[SYNC_COUNT], 0u);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            if (snap >= 0) st_relaxed(&sync[SYNC_SNAP + 64 * snap], ld_relaxed(&sync[SYNC_ACTIVE]));
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_fetch_add(&sync[SYNC_GEN], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+            // after lane 0's release fence in program order
+            st_relaxed(&sync[SYNC_FLAGS + kFlagStride * lane], gen + 1u);
+        } else if (lane == 0) {
             const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-            unsigned level = 0;
-            while (ld_relaxed(&sync[SYNC_GEN]) == gen) {
-                if (ld_relaxed(&sync[SYNC_ABORT]) != 0u) {
+            unsigned level = 0, polls = 0;
+            while (ld_relaxed(flag) == gen) {
+                if ((++polls & 15u) == 0u && ld_relaxed(&sync[SYNC_ABORT]) != 0u) {
                     ok = 0;
                     break;
                 }
@@ -620,8 +642,10 @@ __device__ __forceinline__ bool grid_sync(const Barrier& bar, unsigned nblocks, 
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        s_ok = ok;
-        if (snap >= 0) s_active = ld_relaxed(&sync[SYNC_SNAP + 64 * snap]);
+        if (lane == 0) {
+            s_ok = ok;
+            if (snap >= 0) s_active = ld_relaxed(&sync[SYNC_SNAP + 64 * snap]);
+        }
     }
     __syncthreads();
     if (active != nullptr) *active = s_active;
