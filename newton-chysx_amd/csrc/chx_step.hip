@@ -66,15 +66,16 @@ __device__ __forceinline__ void reset_sync(unsigned* sync, bool all) {
 //   phase 1  every element with a vertex in the unit's rows -- the unit's
 //            element list, ascending per kind -- gathers x_n from the
 //            instance's AoS positions and writes its force / Hessian record
-//            into LDS, field-major (record field f of local element e at
-//            [f * count + e]), so each record is read from LDS instead of
-//            from memory once per block it touches.  The unit's
+//            into LDS as 16-byte quads, quad-major (quad q of local element
+//            e at q * count + e), so each record is read from LDS instead
+//            of from memory once per block it touches.  The unit's
 //            contribution entries and slot offsets are staged into LDS
 //            alongside with 16-byte loads;
-//   phase 2  one thread per block the unit owns (the diagonal blocks of its
-//            rows, then its upper value slots) sums the block's
-//            contributions in the fixed order of its contribution list, all
-//            operands in LDS, so the assembled matrix is bitwise
+//   phase 2  one work item per block the unit owns (the diagonal blocks of
+//            its rows on kDiagLanes lanes each, then its upper value slots
+//            on one) sums the block's contributions in the fixed order of
+//            its contribution list, all operands in LDS, so the assembled
+//            matrix is bitwise
 //            reproducible with no float atomics.  Diagonal blocks also form
 //            the right-hand side b = m/dt^2 (x~ - x_n) - grad E(x_n) and the
 //            inertia block (cloth_simulator.cu:66-142,199-210).
