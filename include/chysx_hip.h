@@ -247,8 +247,9 @@ typedef struct chx_instance {
  * CHX_ITEM_SIZE each; energy only) and unit items {instance, unit, 0, 0}.  A
  * batch with inst == NULL holds one instance in `single` (passed by value in
  * the kernel arguments); its element and unit lists are implied by the
- * counts, and so is its tile list when `tiles` is NULL (CHX_TILE_ROWS rows
- * per tile). */
+ * counts, and so is its tile list when `tiles` is NULL (tile_rows rows per
+ * tile, CHX_TILE_ROWS when tile_rows is 0; n_tiles = ceil(n_rows / that)),
+ * which saves the PCG a load of the tile record per tile and phase. */
 typedef struct chx_cloth_batch {
     int n_instances;
     int n_tiles;
@@ -262,7 +263,9 @@ typedef struct chx_cloth_batch {
                                      /* the tolerance                                 */
     int entries16;                   /* 1: every instance has slice_entries16 and the PCG   */
                                      /* reads entries in 16 bits                         */
-    int tile_rows;                   /* rows per tile of the tile list (informational) */
+    int tile_rows;                   /* rows per tile: informational with a tile list, */
+                                     /* the implied tiles' size without one (0:       */
+                                     /* CHX_TILE_ROWS; a multiple of CHX_STAGE_ROWS)  */
     int no_lds_staging;              /* 1: the PCG SpMV gathers from global memory;   */
                                      /* 0: stages of CHX_STAGE_ROWS rows go through   */
                                      /* LDS (instances with windows)                  */

@@ -380,10 +380,11 @@ __device__ __forceinline__ void get_tile(const chx_cloth_batch& B, int t, int& i
         inst = p[0];
         rb = p[1];
         re = p[2];
-    } else {
+    } else {  // a single instance's implied list: tile_rows rows per tile
+        const int tr = B.tile_rows > 0 ? B.tile_rows : CHX_TILE_ROWS;
         inst = 0;
-        rb = t * CHX_TILE_ROWS;
-        re = min(B.single.n_rows, rb + CHX_TILE_ROWS);
+        rb = t * tr;
+        re = min(B.single.n_rows, rb + tr);
     }
 }
 

@@ -1847,6 +1847,12 @@ void validate_batch(const chx_cloth_batch* b) {
         require(b->elem_items == nullptr && b->unit_items == nullptr,
                 "chysx: single-instance element and unit lists are implied");
         const chx_instance& I = b->single;
+        if (b->tiles == nullptr) {
+            const int tr = b->tile_rows > 0 ? b->tile_rows : CHX_TILE_ROWS;
+            require(b->tile_rows >= 0 && tr % CHX_STAGE_ROWS == 0 &&
+                        b->n_tiles == (I.n_rows + tr - 1) / tr && I.tile_begin == 0 && I.tile_end == b->n_tiles,
+                    "chysx: implied tile list does not match n_tiles / tile_rows");
+        }
         require(I.n_rows >= 0 && I.nnz >= 0 && I.n_pins >= 0 && I.n_tris >= 0 && I.n_dihs >= 0 && I.n_stvks >= 0,
                 "chysx: negative size in instance");
         if (I.n_rows > 0)

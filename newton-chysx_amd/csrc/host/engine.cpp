@@ -804,6 +804,7 @@ void BatchRunner::set_tile_rows(int rows) {
         std::vector<chx_instance> one{desc_.single};
         build_tiles(one);
         desc_.single = one[0];
+        desc_.tiles = nullptr;  // implied by tile_rows
     } else {
         build_tiles(table_host_);
         table_.host() = table_host_;
@@ -842,6 +843,7 @@ static int entries16_all(const std::vector<chx_instance>& ds) {
 void BatchRunner::bind_single(const chx_instance& d, std::uintptr_t stream) {
     std::vector<chx_instance> one{d};
     build_tiles(one, stream);
+    desc_.tiles = nullptr;  // implied by tile_rows: no tile record loads
     desc_.n_instances = 1;
     desc_.n_elem_items = 0;
     desc_.n_unit_items = 0;
@@ -859,6 +861,13 @@ void BatchRunner::bind_single(const chx_instance& d, std::uintptr_t stream) {
 
 void BatchRunner::bind_many(const std::vector<chx_instance>& ds, std::uintptr_t stream) {
     if (ds.empty()) throw std::invalid_argument("chysx::engine: empty batch");
+    if (ds.size() == 1) {
+        // one instance runs as a single-instance batch (descriptor in the
+        // kernel arguments, implied work lists): the same kernels and bits
+        bind_single(ds[0], stream);
+        table_host_ = {desc_.single};
+        return;
+    }
     table_host_ = ds;
     build_tiles(table_host_, stream);
     auto& ei = elem_items_.host();
@@ -896,6 +905,11 @@ void BatchRunner::bind_many(const std::vector<chx_instance>& ds, std::uintptr_t 
 
 void BatchRunner::update_many(const std::vector<chx_instance>& ds, std::uintptr_t stream) {
     if (ds.size() != table_host_.size()) throw std::invalid_argument("chysx::engine: instance count changed");
+    if (desc_.inst == nullptr) {
+        update_single(ds[0]);
+        table_host_[0] = desc_.single;
+        return;
+    }
     bool changed = false;
     for (std::size_t i = 0; i < ds.size(); ++i) {
         chx_instance d = ds[i];
