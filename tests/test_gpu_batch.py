@@ -49,6 +49,37 @@ def test_batch_equals_alone_bitwise(gpu):
         assert np.array_equal(pa, pm) and np.array_equal(va, vm)
 
 
+def test_one_instance_batch_at_512_row_tiles_bitwise(gpu):
+    """A one-instance ClothBatch runs as a single-instance batch (descriptor
+    in the kernel arguments, tile list implied by the tile size); at 512-row
+    tiles it matches the same cloth stepped alone and the same cloth inside
+    a two-instance batch (instance table, explicit tile list) bit for bit."""
+    import chysx
+    from chysx import scenes
+
+    spec, other = scenes.batched(seed=5, count=2, lo=40, hi=48)
+    alone, one, two = (scenes.DeviceCloth(spec) for _ in range(3))
+    partner = scenes.DeviceCloth(other)
+    alone.sim.set_tile_rows(512)
+    b1, b2 = chysx.ClothBatch(), chysx.ClothBatch()
+    b1.set_tile_rows(512)
+    b2.set_tile_rows(512)
+    b1.add_instance(one.sim)
+    b2.add_instance(two.sim)
+    b2.add_instance(partner.sim)
+    print(f"one-instance batch: {spec.n} rows, 512-row tiles", flush=True)
+    for step in range(6):
+        alone.step()
+        b1.step(spec.dt)
+        b2.step(spec.dt)
+        its = (alone.sim.last_pcg_iterations(), b1.last_pcg_iterations()[0], b2.last_pcg_iterations()[0])
+        assert its[0] == its[1] == its[2], (step, its)
+    pa, va = alone.state()
+    for c in (one, two):
+        pc, vc = c.state()
+        assert np.array_equal(pa, pc) and np.array_equal(va, vc)
+
+
 def test_32bit_entries_equal_16bit_bitwise(gpu):
     """A row-numbered grid solved alone reads 16-bit sliced entries; batched
     with a randomly renumbered grid (columns up to 4095 rows away) the whole
