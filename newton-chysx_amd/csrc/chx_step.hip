@@ -728,6 +728,9 @@ struct PcgShared {
     int pre[kCompactMax];
     int wsum[kBlock / kWave];
     int nact;
+    // the instance descriptors of the chunk's tiles (instance-table batches):
+    // one load round per chunk instead of one per tile
+    chx_instance cdesc[kChunk];
     int ctile[kChunk];  // the tiles of the chunk at hand, -1 past the end
     int cinst[kChunk];  // their instances and row ranges (tile records)
     int crb[kChunk];
@@ -1088,7 +1091,26 @@ __device__ __forceinline__ bool next_chunk(const chx_cloth_batch& B, PcgShared& 
         }
     }
     __syncthreads();
+    if (B.inst != nullptr && S.ctile[0] >= 0) {
+        constexpr int kWords = int(sizeof(chx_instance) / sizeof(unsigned long long));
+        static_assert(sizeof(chx_instance) % sizeof(unsigned long long) == 0, "descriptor copy in 8-byte words");
+        for (int k = int(threadIdx.x); k < kChunk * kWords; k += int(blockDim.x)) {
+            const int m = k / kWords, w = k - m * kWords;
+            if (S.ctile[m] >= 0) {
+                const gp<const unsigned long long> src = (gp<const unsigned long long>)rec_at(
+                    glb((const chx_instance*)B.inst), unsigned(S.cinst[m]), 1u);
+                reinterpret_cast<unsigned long long*>(&S.cdesc[m])[w] = src[w];
+            }
+        }
+        __syncthreads();
+    }
     return S.ctile[0] >= 0;
+}
+
+// Descriptor of chunk tile m's instance: the LDS copy next_chunk made, or
+// the single instance's.
+__device__ __forceinline__ Inst chunk_inst(const chx_cloth_batch& B, const PcgShared& S, int m) {
+    return B.inst != nullptr ? make_inst(&S.cdesc[m]) : make_inst(&B.single);
 }
 
 // Tile record m of the chunk at hand (uniform values in scalar registers).
@@ -1312,7 +1334,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 if (t < 0) break;
                 int inst, rb, re;
                 chunk_tile(S, m, inst, rb, re);
-                const Inst I = inst_at(B, inst);
+                const Inst I = chunk_inst(B, S, m);
                 const double rho = S.sc[m][0], rho_prev = S.sc[m][1], rr = S.sc[m][2], bb = S.sc[m][3];
                 const bool finished = S.sc[m][4] != 0.0;
                 const int rstate = BF16 ? int(S.sc[m][5] + 0.5) : 1;
@@ -1389,7 +1411,7 @@ __global__ __launch_bounds__(kBlock) void pcg_kernel(chx_cloth_batch B, int fina
                 if (t < 0) break;
                 int inst, rb, re;
                 chunk_tile(S, m, inst, rb, re);
-                const Inst I = inst_at(B, inst);
+                const Inst I = chunk_inst(B, S, m);
                 const double rho = S.sc[m][0], sigma = S.sc[m][1], rr = S.sc[m][2], bb = S.sc[m][3];
                 const int rstate = BF16 ? int(S.sc[m][5] + 0.5) : 1;
                 if (instance_slot(I, k, rr, bb, (rstate & 1) != 0, rstate >> 1, S.sc[m][4] != 0.0) != ITERATE)
@@ -1570,7 +1592,7 @@ __global__ __launch_bounds__(kBlock) void pcg1_kernel(chx_cloth_batch B, int fin
                 if (t < 0) break;
                 int inst, rb, re;
                 chunk_tile(S, m, inst, rb, re);
-                const Inst I = inst_at(B, inst);
+                const Inst I = chunk_inst(B, S, m);
                 const double rho = S.sc[m][0], sig = S.sc[m][1], del = S.sc[m][2], gam = S.sc[m][3];
                 const double rr = S.sc[m][4], bb = S.sc[m][5];
                 const int d = int(S.sc[m][6]);
